@@ -1,0 +1,294 @@
+/*
+ * msbg_gpu.h -- C ABI of the MI355X-native MSBG sparse-block stencil path.
+ *
+ * The library (libmsbg_gpu.so) owns all device memory of a grid: the block
+ * tables, the particle buffer and the page pools of every channel. Callers
+ * pass plain host pointers and sizes. Every entry point returns 0 on success
+ * or a negative MSBG_E* code; msbg_last_error() describes the last failure of
+ * the calling thread.
+ *
+ * Each declaration names the reference interface it replaces. Reference
+ * paths are relative to the reference's src/ directory. The reference is a
+ * C++ class library (MultiresSparseGrid, SparseGrid<T>, HaloBlockSet); this
+ * ABI flattens the calls the sweep/halo/splat path makes into C functions
+ * on an opaque grid handle.
+ */
+#ifndef MSBG_GPU_H
+#define MSBG_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct msbg_grid msbg_grid;
+
+/* ---- error codes (the reference's rcThis/MI_E* pattern, globdef.h:216) ---- */
+enum {
+  MSBG_OK = 0,
+  MSBG_EINVAL = -1,   /* invalid argument (MI_EINVARG) */
+  MSBG_ENOMEM = -2,   /* device or host allocation failed (MI_ENOMEM) */
+  MSBG_EDEVICE = -3,  /* HIP runtime error or no device */
+  MSBG_ESTATE = -4,   /* call order violated (e.g. sweep before allocation) */
+  MSBG_EUNSUPP = -5   /* option outside the implemented path */
+};
+
+/* ---- storage formats of a channel's pages ---- */
+enum {
+  MSBG_STORAGE_U16 = 1,  /* UNORM16: value u/65535 (RenderDensity, msbgcell.h:68) */
+  MSBG_STORAGE_F32 = 2,
+  MSBG_STORAGE_F16 = 3,  /* IEEE binary16, FP32 arithmetic */
+  MSBG_STORAGE_BF16 = 4, /* bfloat16, FP32 arithmetic */
+  MSBG_STORAGE_F64 = 5   /* FP64 storage and arithmetic (oracle run) */
+};
+
+/* ---- option bits and ids with the reference's values (msbg.h:101-391) ---- */
+#define MSBG_OPT_INTERPOLATE (1u << 7)
+#define MSBG_OPT_BC_NEUMANN (1u << 10)
+#define MSBG_OPT_SINGLE_LEVEL (1u << 14)
+#define MSBG_OPT_BC_COARSE_LEVEL (1u << 26)
+#define MSBG_OPT_8_COLOR_SCHEME (1u << 29)
+#define MSBG_PDE_MEAN_CURVATURE 4
+#define MSBG_CH_NULL 0
+#define MSBG_CH_FLOAT_1 1
+#define MSBG_CH_UINT16_1 30
+
+/* block flags (msbg.h:557-575) */
+#define MSBG_BLK_DOM_BORDER (1u << 1)
+#define MSBG_BLK_COARSE_FINE (1u << 2)
+#define MSBG_BLK_FINE_COARSE (1u << 3)
+#define MSBG_BLK_EXISTS (1u << 12)
+#define MSBG_BLK_UNIFORM_EMPTY (1u << 13)
+
+/* block map codes returned by msbg_channel_get_pagemap / halo tables */
+#define MSBG_BLOCK_NULL (-1)  /* no block at this level (_blockmap[bid]==NULL) */
+#define MSBG_BLOCK_EMPTY (-2) /* const EMPTY block (sbg.h:2042-2047) */
+#define MSBG_BLOCK_FULL (-3)  /* const FULL block */
+#define MSBG_BLOCK_OUT (-4)   /* outside the domain (halo tables only) */
+
+/* block kinds accepted by msbg_channel_define */
+#define MSBG_KIND_NULL 0
+#define MSBG_KIND_EMPTY 1
+#define MSBG_KIND_FULL 2
+#define MSBG_KIND_VALUE 3
+
+/* arithmetic mode of the sweep (msbg_set_arith) */
+#define MSBG_ARITH_EXACT 0 /* reference op order, no FMA contraction, IEEE divide */
+#define MSBG_ARITH_FAST 1  /* FMA-contracted, reciprocal divide */
+
+/* ======================================================================== */
+/* grid lifetime                                                            */
+/* ======================================================================== */
+
+/* MultiresSparseGrid::create(name, sx, sy, sz, blockSize0, dTransRes,
+ * maxSizeMB, initialSizeMB, options, ..., nLevels) (msbg.h:834-850,
+ * msbg.cpp:117-1106). Block-aligned domain: sx,sy,sz must be multiples of
+ * blockSize0 (sbg.cpp:155-163); blockSize0 in {8,16,32,64}; nLevels in
+ * [1,4] (MSBG_MAXRESLEVELS, msbgcell.h:147), -1 = 3 (msbg.cpp:137).
+ * options: MSBG_OPT_SINGLE_LEVEL or 0. */
+int msbg_create(const char *name, int sx, int sy, int sz, int blockSize0,
+                unsigned options, int nLevels, msbg_grid **out);
+
+/* MultiresSparseGrid::destroy (msbg.h:851-852). */
+int msbg_destroy(msbg_grid *g);
+
+/* Thread-local description of the last error (TRCERR log lines). */
+const char *msbg_last_error(void);
+
+/* dims[0..7] = sx, sy, sz, bsx0, nbx, nby, nbz, nLevels; nBlocks() in *nblocks
+ * (msbg.h:3425-3429, sbg.h nbx()/nby()/nbz()/bsx()). */
+int msbg_get_dims(const msbg_grid *g, int dims[8], int64_t *nblocks);
+
+/* Select the sweep arithmetic (MSBG_ARITH_*). Default MSBG_ARITH_FAST. */
+int msbg_set_arith(msbg_grid *g, int mode);
+
+/* Device synchronisation of the grid's stream. */
+int msbg_synchronize(msbg_grid *g);
+
+/* ======================================================================== */
+/* particles, activation, refinement map (msbg_demo.cpp:339-441)            */
+/* ======================================================================== */
+
+/* Copy n points (xyz float triples, host memory) into the grid's device
+ * particle buffer, dropping those outside the domain with the reference's
+ * filter: isInDomainRange(pos) (msbg.h:2900-2908, bounds sbg.cpp:262-316)
+ * and inRange(truncate_to_int(pos)) (msbg_demo.cpp:356-360).
+ * *nkept receives the number of particles kept (nActParticles). */
+int msbg_points_upload(msbg_grid *g, const float *xyz, int64_t n,
+                       int64_t *nkept);
+
+/* Download the kept particles (in upload order) into xyz[3*nkept]. */
+int msbg_points_download(msbg_grid *g, float *xyz, int64_t cap);
+
+/* Block activation (msbg_demo.cpp:368-382): for each kept particle,
+ * blockActive[b] += 1 for every block b in
+ * [max(trunc(pos/bsx - rScan/bsx),0), min(trunc(pos/bsx + rScan/bsx),nb-1)]^3.
+ * counts (optional, host [nBlocks]) receives blockActive[]; *nactive the
+ * number of blocks with a nonzero count. */
+int msbg_activate_from_points(msbg_grid *g, float rScan, uint32_t *counts,
+                              int64_t *nactive);
+
+/* Demo level rule on the GPU (msbg_demo.cpp:418-428), generalised to
+ * distance bands: level = 0 for blocks activated with rScan[0], else 1 for
+ * blocks activated with rScan[1] (if nBands > 1), ..., else nLevels-1.
+ * Runs one activation pass per band. Writes blockLevels (host, optional). */
+int msbg_levels_from_points(msbg_grid *g, const float *rScan, int nBands,
+                            int *blockLevels);
+
+/* MultiresSparseGrid::regularizeRefinementMap(blockmapIn, blockmapOut)
+ * (msbg.h:897-901, msbg.cpp:1202-1423): passes ll = 1..nLevels-2 on the
+ * GPU. blockmapOut may equal blockmapIn; with nLevels == 2 it is left
+ * untouched, like the reference. Host arrays of nBlocks ints. */
+int msbg_regularize_refinement_map(msbg_grid *g, const int *blockmapIn,
+                                   int *blockmapOut);
+
+/* MultiresSparseGrid::setRefinementMap(blockLevels, blockFlags, ...)
+ * (msbg.h:903-908, msbg.cpp:1428-1745) at levelMg 0: sets BlockInfo.level
+ * and flags (BLK_DOM_BORDER, BLK_FINE_COARSE, BLK_COARSE_FINE,
+ * BLK_UNIFORM_EMPTY; BLK_EXISTS from blockFlags) on the GPU and builds the
+ * per-level fine-coarse block lists (unless OPT_SINGLE_LEVEL).
+ * blockLevels == NULL uses the levels left by msbg_levels_from_points.
+ * blockFlags may be NULL. */
+int msbg_set_refinement_map(msbg_grid *g, const int *blockLevels,
+                            const uint32_t *blockFlags);
+
+/* getBlockInfo(bid)->level / ->flags for all blocks (msbg.h:945-950,
+ * 673-678). Either pointer may be NULL. */
+int msbg_get_block_info(msbg_grid *g, uint16_t *levels, uint16_t *flags);
+
+/* Set or clear BLK_EXISTS for every block (the demo's allocation loop,
+ * msbg_demo.cpp:482-497, does FLAG_RESET then FLAG_SET for level-0 blocks).
+ * exists == NULL: set it exactly for level-0 blocks. */
+int msbg_set_exists(msbg_grid *g, const uint8_t *exists);
+
+/* _blocksFineCoarsePerLevel[level] (msbg.cpp:1720-1727), ascending bid.
+ * *n receives the count; out (optional) receives up to cap block ids. */
+int msbg_get_fine_coarse_blocks(msbg_grid *g, int level, int32_t *out,
+                                int64_t cap, int64_t *n);
+
+/* ======================================================================== */
+/* channels (getUint16Channel / getFloatChannel, msbg.h:3048-3077)          */
+/* ======================================================================== */
+
+/* Define the block map of channel `chan` at resolution `level`: the
+ * equivalent of reset() + prepareDataAccess() + setEmptyValue/FullValue +
+ * per block getBlockDataPtr(bid,1) / setEmptyBlock / setFullBlock
+ * (sbg.h:1937-1940, 2042-2047, 2165-2168). kind[bid] is MSBG_KIND_*;
+ * value blocks get pages in ascending bid order, filled with fill_raw.
+ * Raw values are in the storage's own domain (UNORM16: 0..65535).
+ * All levels of one channel share one storage format. */
+int msbg_channel_define(msbg_grid *g, int chan, int storage, int level,
+                        const uint8_t *kind, double empty_raw,
+                        double full_raw, double fill_raw);
+
+/* The demo's allocation step (msbg_demo.cpp:451-511) for level 0:
+ * empty value 0, full value 1, level-0 blocks become value blocks filled
+ * with the full value, all other blocks the const EMPTY block; BLK_EXISTS
+ * is set for level-0 blocks and cleared elsewhere. The ascending active
+ * block list goes to activeBlocks (optional, cap entries); *nactive gets
+ * its length. */
+int msbg_channel_alloc_demo(msbg_grid *g, int chan, int storage,
+                            int32_t *activeBlocks, int64_t cap,
+                            int64_t *nactive);
+
+/* Page map of (chan, level): codes MSBG_BLOCK_* or page index per block. */
+int msbg_channel_get_pagemap(msbg_grid *g, int chan, int level,
+                             int32_t *pagemap, int64_t *npages);
+
+/* Bulk page transfer of (chan, level): npages * bsx_level^3 values of the
+ * channel's storage type, page-major, vid = (vz*bsx + vy)*bsx + vx. */
+int msbg_channel_download(msbg_grid *g, int chan, int level, void *pages,
+                          int64_t bytes);
+int msbg_channel_upload(msbg_grid *g, int chan, int level, const void *pages,
+                        int64_t bytes);
+
+/* SparseGrid::getBlockDataPtr(bid) read/write of one block (sbg.h:912-977). */
+int msbg_download_block(msbg_grid *g, int chan, int level, int bid,
+                        void *data);
+int msbg_upload_block(msbg_grid *g, int chan, int level, int bid,
+                      const void *data);
+
+/* Halo index table of (chan, level): for each value page p (in page order)
+ * 27 entries k = (dz+1)*9 + (dy+1)*3 + (dx+1) holding the neighbour's page
+ * or a MSBG_BLOCK_* code (OUT outside the domain). out: npages*27 ints. */
+int msbg_get_halo_table(msbg_grid *g, int chan, int level, int32_t *out,
+                        int64_t cap);
+
+/* ======================================================================== */
+/* hot path                                                                 */
+/* ======================================================================== */
+
+/* 8-colour min-distance splat of the kept particles into the UNORM16
+ * distance field of `chan` (msbg_demo.cpp:541-636): for every voxel of the
+ * footprint [ceil(p-rScan-.5), floor(p+rScan-.5)] with d^2 <= rScan^2,
+ * value = min(value, roundf(d^2/rScan^2 * 65535)); a particle is abandoned
+ * at its first in-sphere voxel (z,y,x order) whose block is not a value
+ * block. rScan = rParticle + nbDist. For FP16/BF16 storage the distance
+ * code is held in the page bits until msbg_finalize_density; for FP32/FP64
+ * in the low 16 bits of each element. */
+int msbg_splat_points(msbg_grid *g, int chan, float rParticle, float nbDist);
+
+/* Finalize (msbg_demo.cpp:655-680): q = u/65535; q < 0.999 ->
+ * 1 - linstep(-rParticle, rParticle+nbDist, sqrt(q*rScan^2) - rParticle),
+ * else 0; stored as roundf(f*65535) for UNORM16, or converted with
+ * round-to-nearest-even for the other storages. */
+int msbg_finalize_density(msbg_grid *g, int chan, float rParticle,
+                          float nbDist);
+
+/* HaloBlockSet::fillHaloBlock_<T>(chan, bid, levelMg=0, tid, opt)
+ * (halo.h:382-388, halo.cpp:267-479) for one block at its own level:
+ * writes (bsx_L+2*dmax)^3 floats to out (host). opt: MSBG_OPT_BC_NEUMANN,
+ * MSBG_OPT_BC_COARSE_LEVEL [| MSBG_OPT_INTERPOLATE], or 0 (empty value). */
+int msbg_halo_fill(msbg_grid *g, int chan, int bid, int dmax, unsigned opt,
+                   float *out);
+
+/* MultiresSparseGrid::prepareGhostBlocks(chan) (msbg.h:878,
+ * msbg.cpp:3059-3170): for each BLK_FINE_COARSE block of levels
+ * 0..nLevels-2 with BLK_EXISTS, fill its halo with OPT_BC_COARSE_LEVEL and
+ * downsample 4x4x4 ([1,3,3,1]/8 separable, msbg.cpp:298-343, 2902-2974)
+ * into the level+1 block slot (allocated as needed); blocks without
+ * BLK_EXISTS get the const EMPTY block at level+1. */
+int msbg_prepare_ghost_blocks(msbg_grid *g, int chan);
+
+/* MultiresSparseGrid::applyChannelPdeFast<T>(chPhi, chTmp, chTmp2,
+ * blockList, laplTyp, doConstrZeroOne, densThr, densThrTol1, densThrTol2,
+ * numIter, gridSpacing, timeStep, ...) (msbg.h:2150-2172,
+ * msbg4.cpp:60-957) for laplTyp = -(PDE_MEAN_CURVATURE|OPT_8_COLOR_SCHEME):
+ * numIter rounded up to even sweeps of 8 colour passes, in place, level-0
+ * blocks of blockList only, Neumann halo. blockList == NULL sweeps every
+ * level-0 value block. Thresholds must be 0 (no hard/soft constraints) and
+ * gridSpacing 1 (msbg4.cpp:326-330). residual (optional, host
+ * [sweeps]) receives the FP64 RMS of each sweep's update over the swept
+ * voxels. Returns MSBG_EUNSUPP for other laplTyp values. */
+int msbg_apply_channel_pde_fast(msbg_grid *g, int chPhi, int chTmp,
+                                int chTmp2, const int32_t *blockList,
+                                int64_t nList, int laplTyp,
+                                int doConstrZeroOne, float densThr,
+                                float densThrTol1, float densThrTol2,
+                                int numIter, float gridSpacing,
+                                float timeStep, double *residual);
+
+/* Device-time of the last msbg_apply_channel_pde_fast call (HIP events on
+ * the grid's stream around the sweep kernels): total ms and per-launch
+ * average ms of the colour-pass kernel. */
+int msbg_last_sweep_timing(msbg_grid *g, double *total_ms,
+                           double *avg_pass_ms, int64_t *npasses);
+
+/* ======================================================================== */
+/* synthetic scenes (host)                                                  */
+/* ======================================================================== */
+
+/* Seeded sphere/ellipsoid surface-point scene: nObj objects, centres
+ * uniform in [cmin,cmax]^3, radii (or semi-axes) uniform in [rmin,rmax],
+ * nPts jittered surface points each (Gaussian sigma). ellipsoidFrac of the
+ * objects are ellipsoids with a random rotation. Deterministic for a given
+ * seed, independent of thread count. xyz: 3*nObj*nPts floats (host). */
+int msbg_scene_spheres(uint64_t seed, int nObj, float cmin, float cmax,
+                       float rmin, float rmax, int64_t nPts, float sigma,
+                       float ellipsoidFrac, float *xyz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

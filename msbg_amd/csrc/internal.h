@@ -1,0 +1,162 @@
+// Internal types of libmsbg_gpu: grid, channels, device buffers, launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/msbg_gpu.h"
+
+namespace msbg {
+
+constexpr int kMaxLevels = 4;  // MSBG_MAXRESLEVELS (msbgcell.h:147)
+
+void set_error(const char *fmt, ...);
+
+#define MSBG_HIP(call)                                                      \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      ::msbg::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,          \
+                        hipGetErrorString(e_));                             \
+      return MSBG_EDEVICE;                                                  \
+    }                                                                       \
+  } while (0)
+
+#define MSBG_CHECK(cond, code, ...)        \
+  do {                                     \
+    if (!(cond)) {                         \
+      ::msbg::set_error(__VA_ARGS__);      \
+      return (code);                       \
+    }                                      \
+  } while (0)
+
+#define MSBG_TRY(expr)      \
+  do {                      \
+    int rc_ = (expr);       \
+    if (rc_) return rc_;    \
+  } while (0)
+
+int storage_bytes(int storage);
+
+// Run body(i) for i in [0, n) on up to hardware_concurrency() host threads,
+// in contiguous chunks.
+void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)> &body);
+
+// Device buffer with explicit ownership.
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count) {
+    free();
+    if (count == 0) return 0;
+    if (hipMalloc((void **)&p, count * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      set_error("hipMalloc of %zu bytes failed", count * sizeof(T));
+      return MSBG_ENOMEM;
+    }
+    n = count;
+    return 0;
+  }
+  int ensure(size_t count) { return count <= n ? 0 : alloc(count); }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { free(); }
+};
+
+// One resolution level of one channel.
+struct ChanLevel {
+  bool defined = false;
+  int bsx = 0;                       // block edge at this level
+  std::vector<int32_t> h_page;       // block map codes (host mirror)
+  DevBuf<int32_t> d_page;            // block map codes (device)
+  DevBuf<uint8_t> d_pages;           // page pool
+  int64_t npages = 0;
+  std::vector<int32_t> h_page_bid;   // page -> block id
+  DevBuf<int32_t> d_page_bid;
+  DevBuf<int32_t> d_nbr;             // halo index table [npages][27]
+  bool nbr_valid = false;
+  double empty_raw = 0, full_raw = 0;
+};
+
+struct Channel {
+  int id = 0;
+  int storage = 0;
+  ChanLevel lv[kMaxLevels];
+};
+
+}  // namespace msbg
+
+struct msbg_grid {
+  std::string name;
+  int sx = 0, sy = 0, sz = 0, bsx0 = 0, lg0 = 0;
+  int nbx = 0, nby = 0, nbz = 0;
+  int64_t nblocks = 0;
+  int nlevels = 0;
+  unsigned options = 0;
+  int arith = MSBG_ARITH_FAST;
+  hipStream_t stream = nullptr;
+
+  // block info (levelMg 0)
+  std::vector<uint16_t> h_level, h_flags;
+  msbg::DevBuf<uint16_t> d_level, d_flags;
+  bool have_levels = false;
+  std::vector<int32_t> fc_list[msbg::kMaxLevels];  // fine-coarse blocks per level
+
+  // particles
+  msbg::DevBuf<float> d_xyz;
+  int64_t npoints = 0;
+  msbg::DevBuf<uint32_t> d_count;
+  msbg::DevBuf<int32_t> d_tmp_levels;
+
+  std::map<int, msbg::Channel> channels;
+
+  // sweep scratch
+  msbg::DevBuf<int32_t> d_sweep_list;   // colour-sorted page indices
+  msbg::DevBuf<double> d_partial;       // per-block residual partials
+  msbg::DevBuf<double> d_resid;         // per-sweep residuals
+  uint64_t topo_version = 1;             // bumped when block maps or levels change
+  uint64_t sweep_key = 0;                // key of the cached colour lists
+  int64_t sweep_off[9] = {0};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  double last_total_ms = 0, last_avg_pass_ms = 0;
+  int64_t last_npasses = 0;
+};
+
+namespace msbg {
+
+// ---- launchers implemented in the .hip files ----
+// activation: counts[bid] += 1 over each particle's trunc box
+int launch_activate(msbg_grid *g, float rScan, uint32_t *d_count);
+// level bands: level = first band whose count is nonzero, else nLevels-1
+int launch_level_band(msbg_grid *g, const uint32_t *d_count, int band,
+                      int nbands, int32_t *d_levels);
+int launch_regularize(msbg_grid *g, const int32_t *d_in, int32_t *d_out,
+                      int ll);
+int launch_flags(msbg_grid *g, const int32_t *d_levels,
+                 const uint32_t *d_in_flags);
+int launch_build_nbr(msbg_grid *g, ChanLevel &cl, int level);
+int launch_fill_pages(msbg_grid *g, ChanLevel &cl, int storage,
+                      double fill_raw, int64_t first_page, int64_t npages);
+int launch_splat(msbg_grid *g, Channel &ch, float rScan);
+int launch_finalize(msbg_grid *g, Channel &ch, float rParticle, float nbDist);
+int launch_halo_fill(msbg_grid *g, Channel &ch, int level, int bid, int dmax,
+                     unsigned opt, float *d_out);
+int launch_ghost(msbg_grid *g, Channel &ch, int level, const int32_t *d_bids,
+                 int64_t n);
+int launch_sweep_pass(msbg_grid *g, Channel &ch, const int32_t *d_list,
+                      int64_t n, float dt, int doConstr01, double *d_partial);
+int launch_reduce_residual(msbg_grid *g, const double *d_partial, int64_t n,
+                           double *d_out);
+int launch_domain_filter(msbg_grid *g, const float *d_in, int64_t n,
+                         float *d_out, int64_t *nkept);
+
+}  // namespace msbg

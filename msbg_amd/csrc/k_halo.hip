@@ -1,0 +1,295 @@
+// Generic halo fill (test hook) and the cross-level ghost downsample.
+//
+// halo_value() is the per-voxel rule of HaloBlockSet::fillHaloBlock_ /
+// getOutOfBlockValue (halo.cpp:267-479, halo.h:267-355) on device-side block
+// maps of one channel level and its coarser neighbour level.
+//
+// Reference paths are relative to the reference's src/ directory.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "internal.h"
+#include "storage.h"
+
+namespace msbg {
+
+namespace {
+
+struct LevelView {
+  int sx, sy, sz, bsx, lg, nbx, nby, nbz;
+  const int32_t *page;
+  const uint8_t *pages;
+  int storage;
+  double empty_raw, full_raw;
+};
+
+struct GhostWeights {
+  float w[64];
+};
+
+__device__ inline int code_at(const LevelView &L, int x, int y, int z) {
+  return L.page[((int64_t)(z >> L.lg) * L.nby + (y >> L.lg)) * L.nbx + (x >> L.lg)];
+}
+
+__device__ inline int vid_at(const LevelView &L, int x, int y, int z) {
+  int m = L.bsx - 1;
+  return ((((z & m) << L.lg) + (y & m)) << L.lg) + (x & m);
+}
+
+__device__ inline bool in_range(const LevelView &L, int x, int y, int z) {
+  return x >= 0 && x < L.sx && y >= 0 && y < L.sy && z >= 0 && z < L.sz;
+}
+
+__device__ inline void clip3(const LevelView &L, int &x, int &y, int &z) {
+  x = x < 0 ? 0 : (x > L.sx - 1 ? L.sx - 1 : x);
+  y = y < 0 ? 0 : (y > L.sy - 1 ? L.sy - 1 : y);
+  z = z < 0 ? 0 : (z > L.sz - 1 ? L.sz - 1 : z);
+}
+
+__device__ inline double page_raw(const LevelView &L, int64_t page, int vid) {
+  int64_t i = page * ((int64_t)1 << (3 * L.lg)) + vid;
+  switch (L.storage) {
+    case MSBG_STORAGE_U16: return ((const uint16_t *)L.pages)[i];
+    case MSBG_STORAGE_F32: return ((const float *)L.pages)[i];
+    case MSBG_STORAGE_F16: return f16_bits_to_float(((const uint16_t *)L.pages)[i]);
+    case MSBG_STORAGE_BF16: return bf16_bits_to_float(((const uint16_t *)L.pages)[i]);
+    default: return ((const double *)L.pages)[i];
+  }
+}
+
+__device__ inline double block_raw(const LevelView &L, int c, int vid) {
+  if (c >= 0) return page_raw(L, c, vid);
+  return c == MSBG_BLOCK_EMPTY ? L.empty_raw : L.full_raw;
+}
+
+// transferValue: UNORM16 -> u*(1.f/65535) (halo.h:484-501), floats as is
+__device__ inline float raw_to_float(int storage, double raw) {
+  if (storage == MSBG_STORAGE_U16) return (float)(uint16_t)raw * (1.0f / 65535.0f);
+  return (float)raw;
+}
+
+// SparseGrid::getValue(int) (sbg.h:2622-2648)
+__device__ inline double get_value_raw(const LevelView &L, int x, int y, int z) {
+  int c = code_at(L, x, y, z);
+  if (c == MSBG_BLOCK_NULL) return L.empty_raw;
+  return block_raw(L, c, vid_at(L, x, y, z));
+}
+
+__device__ inline double to_raw_type(int storage, float v) {
+  if (storage == MSBG_STORAGE_U16) return (double)(uint16_t)v;  // Data_T = uint16_t truncates
+  return (double)v;
+}
+
+// SparseGrid<Data_T>::interpolateGhost (sbg.h:3002-3095), options 0.
+__device__ double interpolate_ghost_raw(const LevelView &Lo, float px, float py, float pz) {
+#pragma clang fp contract(off)
+  const float po = 0.5f;
+  float x = px - po, y = py - po, z = pz - po;
+  int ix = (int)floorf(x), iy = (int)floorf(y), iz = (int)floorf(z);
+  float u = x - (float)ix, v = y - (float)iy, w = z - (float)iz;
+  double bx[2], by[2], bzv[2];
+  double val0 = Lo.empty_raw;
+  bool have0 = false;
+  for (int k = 0; k < 2; k++) {
+    for (int j = 0; j < 2; j++) {
+      for (int i = 0; i < 2; i++) {
+        int X = ix + i, Y = iy + j, Z = iz + k;
+        double val;
+        if (!in_range(Lo, X, Y, Z)) {
+          clip3(Lo, X, Y, Z);
+          val = get_value_raw(Lo, X, Y, Z);
+        } else if (code_at(Lo, X, Y, Z) == MSBG_BLOCK_NULL) {
+          if (!have0) {
+            int qx = (int)floorf(px), qy = (int)floorf(py), qz = (int)floorf(pz);
+            clip3(Lo, qx, qy, qz);
+            val0 = get_value_raw(Lo, qx, qy, qz);
+            have0 = true;
+          }
+          val = val0;
+        } else {
+          val = block_raw(Lo, code_at(Lo, X, Y, Z), vid_at(Lo, X, Y, Z));
+        }
+        bx[i] = val;
+      }
+      by[j] = to_raw_type(Lo.storage, (float)bx[1] * u + (float)bx[0] * (1.0f - u));
+    }
+    bzv[k] = to_raw_type(Lo.storage, (float)by[1] * v + (float)by[0] * (1.0f - v));
+  }
+  return to_raw_type(Lo.storage, (float)bzv[1] * w + (float)bzv[0] * (1.0f - w));
+}
+
+// fillHaloBlock_ per-voxel rule + getOutOfBlockValue.
+__device__ float halo_value(const LevelView &L, const LevelView &Lo, int x, int y, int z,
+                            int px0, int py0, int pz0, int own, unsigned opt) {
+#pragma clang fp contract(off)
+  bool inR = in_range(L, x, y, z);
+  if (inR) {
+    int c = code_at(L, x, y, z);
+    if (c != MSBG_BLOCK_NULL) return raw_to_float(L.storage, block_raw(L, c, vid_at(L, x, y, z)));
+  }
+  if (opt & MSBG_OPT_BC_NEUMANN) {
+    int b = L.bsx;
+    int cx = x < px0 ? px0 : (x > px0 + b - 1 ? px0 + b - 1 : x);
+    int cy = y < py0 ? py0 : (y > py0 + b - 1 ? py0 + b - 1 : y);
+    int cz = z < pz0 ? pz0 : (z > pz0 + b - 1 ? pz0 + b - 1 : z);
+    return raw_to_float(L.storage, block_raw(L, own, vid_at(L, cx, cy, cz)));
+  }
+  if (opt & MSBG_OPT_BC_COARSE_LEVEL) {
+    if (!inR) {
+      clip3(L, x, y, z);
+      int c = code_at(L, x, y, z);
+      if (c != MSBG_BLOCK_NULL) return raw_to_float(L.storage, block_raw(L, c, vid_at(L, x, y, z)));
+    }
+    if (opt & MSBG_OPT_INTERPOLATE) {
+      float qx = ((float)x + 0.5f) / 2, qy = ((float)y + 0.5f) / 2, qz = ((float)z + 0.5f) / 2;
+      return raw_to_float(Lo.storage, interpolate_ghost_raw(Lo, qx, qy, qz));
+    }
+    int xl = x / 2, yl = y / 2, zl = z / 2;
+    clip3(Lo, xl, yl, zl);
+    // a NULL coarse block is an assertion failure in the reference
+    // (UT_ASSERT2(pVal), halo.h:347); its empty value is read here
+    return raw_to_float(Lo.storage, get_value_raw(Lo, xl, yl, zl));
+  }
+  return raw_to_float(L.storage, L.empty_raw);
+}
+
+__global__ void k_halo_fill(LevelView L, LevelView Lo, int bid, int dmax, unsigned opt,
+                            float *__restrict__ out) {
+  int hsx = L.bsx + 2 * dmax;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= hsx * hsx * hsx) return;
+  int hx = t % hsx, hy = (t / hsx) % hsx, hz = t / (hsx * hsx);
+  int bx = bid % L.nbx, by = (bid / L.nbx) % L.nby, bz = bid / (L.nbx * L.nby);
+  int px0 = bx * L.bsx, py0 = by * L.bsx, pz0 = bz * L.bsx;
+  int own = L.page[bid];
+  out[t] = halo_value(L, Lo, px0 - dmax + hx, py0 - dmax + hy, pz0 - dmax + hz, px0, py0, pz0, own,
+                      opt);
+}
+
+// downsampleGhostBlockFloat (msbg.cpp:2902-2974): coarse voxel (vx,vy,vz)
+// sums the 4x4x4 halo window at halo index (2vx,2vy,2vz) with four row
+// accumulators S0..S3 over k, then horizontal_add((S0+S1)+S2)+S3) =
+// (a0+a2)+(a1+a3) (vectorf128.h:870-883).
+template <int ST>
+__global__ void k_ghost(LevelView L, LevelView Lo, GhostWeights W, const int32_t *__restrict__ bids,
+                        uint8_t *__restrict__ loPages) {
+#pragma clang fp contract(off)
+  int bid = bids[blockIdx.y];
+  int bl = L.bsx / 2;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= bl * bl * bl) return;
+  int vx = t % bl, vy = (t / bl) % bl, vz = t / (bl * bl);
+  int bx = bid % L.nbx, by = (bid / L.nbx) % L.nby, bz = bid / (L.nbx * L.nby);
+  int px0 = bx * L.bsx, py0 = by * L.bsx, pz0 = bz * L.bsx;
+  int own = L.page[bid];
+  float S[4][4];
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int l = 0; l < 4; l++) S[j][l] = 0.0f;
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int l = 0; l < 4; l++) {
+        // halo index (2v + l) -> grid coordinate pos0 - 1 + 2v + l
+        float h = halo_value(L, Lo, px0 - 1 + 2 * vx + l, py0 - 1 + 2 * vy + j,
+                             pz0 - 1 + 2 * vz + k, px0, py0, pz0, own, MSBG_OPT_BC_COARSE_LEVEL);
+        S[j][l] = S[j][l] + W.w[16 * k + 4 * j + l] * h;
+      }
+  float T[4];
+#pragma unroll
+  for (int l = 0; l < 4; l++) T[l] = ((S[0][l] + S[1][l]) + S[2][l]) + S[3][l];
+  float val = (T[0] + T[2]) + (T[1] + T[3]);
+  int pg = Lo.page[bid];
+  int64_t idx = (int64_t)pg * bl * bl * bl + t;
+  if constexpr (ST == MSBG_STORAGE_F32)
+    ((float *)loPages)[idx] = val;
+  else
+    ((uint16_t *)loPages)[idx] = Stor<ST>::enc(val);
+}
+
+LevelView make_view(msbg_grid *g, Channel &ch, int level) {
+  LevelView v{};
+  if (level >= g->nlevels || !ch.lv[level].defined) return v;
+  ChanLevel &cl = ch.lv[level];
+  v.sx = g->sx >> level;
+  v.sy = g->sy >> level;
+  v.sz = g->sz >> level;
+  v.bsx = cl.bsx;
+  v.lg = g->lg0 - level;
+  v.nbx = g->nbx;
+  v.nby = g->nby;
+  v.nbz = g->nbz;
+  v.page = cl.d_page.p;
+  v.pages = cl.d_pages.p;
+  v.storage = ch.storage;
+  v.empty_raw = cl.empty_raw;
+  v.full_raw = cl.full_raw;
+  return v;
+}
+
+// cell-centred 4x4x4 kernel weights (msbg.cpp:298-343): per axis
+// 1 - 0.5|1 - (d + 0.5)| for d = -1..2, product in double stored as float,
+// then divided by the double sum.
+GhostWeights ghost_weights() {
+  GhostWeights W;
+  double wd[64], wsum = 0;
+  for (int dz = -1, ik = 0; dz <= 2; dz++)
+    for (int dy = -1; dy <= 2; dy++)
+      for (int dx = -1; dx <= 2; dx++, ik++) {
+        double v = (1 - 0.5 * std::fabs(1.0 - (dx + 0.5))) * (1 - 0.5 * std::fabs(1.0 - (dy + 0.5))) *
+                   (1 - 0.5 * std::fabs(1.0 - (dz + 0.5)));
+        wd[ik] = v;
+        wsum += v;
+      }
+  for (int i = 0; i < 64; i++) W.w[i] = (float)((double)(float)wd[i] / wsum);
+  return W;
+}
+
+}  // namespace
+
+int launch_halo_fill(msbg_grid *g, Channel &ch, int level, int bid, int dmax, unsigned opt,
+                     float *d_out) {
+  LevelView L = make_view(g, ch, level);
+  LevelView Lo = make_view(g, ch, level + 1);
+  int hsx = L.bsx + 2 * dmax;
+  int n = hsx * hsx * hsx;
+  hipLaunchKernelGGL(k_halo_fill, dim3((n + 255) / 256), dim3(256), 0, g->stream, L, Lo, bid, dmax,
+                     opt, d_out);
+  MSBG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_ghost(msbg_grid *g, Channel &ch, int level, const int32_t *d_bids, int64_t n) {
+  LevelView L = make_view(g, ch, level);
+  LevelView Lo = make_view(g, ch, level + 1);
+  GhostWeights W = ghost_weights();
+  int bl = L.bsx / 2;
+  int nv = bl * bl * bl;
+  for (int64_t off = 0; off < n; off += 65535) {
+    unsigned cnt = (unsigned)(n - off < 65535 ? n - off : 65535);
+    dim3 grid((nv + 255) / 256, cnt);
+    switch (ch.storage) {
+      case MSBG_STORAGE_F32:
+        hipLaunchKernelGGL(k_ghost<MSBG_STORAGE_F32>, grid, dim3(256), 0, g->stream, L, Lo, W,
+                           d_bids + off, ch.lv[level + 1].d_pages.p);
+        break;
+      case MSBG_STORAGE_F16:
+        hipLaunchKernelGGL(k_ghost<MSBG_STORAGE_F16>, grid, dim3(256), 0, g->stream, L, Lo, W,
+                           d_bids + off, ch.lv[level + 1].d_pages.p);
+        break;
+      case MSBG_STORAGE_BF16:
+        hipLaunchKernelGGL(k_ghost<MSBG_STORAGE_BF16>, grid, dim3(256), 0, g->stream, L, Lo, W,
+                           d_bids + off, ch.lv[level + 1].d_pages.p);
+        break;
+      default: set_error("ghost: unsupported storage %d", ch.storage); return MSBG_EUNSUPP;
+    }
+    MSBG_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+}  // namespace msbg

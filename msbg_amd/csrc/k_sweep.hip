@@ -1,0 +1,439 @@
+// 8-colour in-place mean-curvature-flow sweep: one colour pass per launch.
+//
+// Reference: MultiresSparseGrid::applyChannelPdeFast<T>, laplTyp 4 in colour
+// mode (msbg4.cpp:60-957): per block, fill a (bsx+2)^3 float halo
+// (fillHaloBlock_ with OPT_BC_NEUMANN, halo.cpp:267-479), evaluate the
+// 19-point MCF stencil for every voxel from that halo (Jacobi inside the
+// block), clamp to [0,1] and store in place. Blocks of one colour never
+// touch each other's halos, so a pass is race-free.
+//
+// GPU mapping: one workgroup per block. The workgroup marches through the
+// block's z-planes; each step stages one halo plane ((bsx+2)^2 values,
+// interior from the block's page with 4-wide coalesced loads, the ring from
+// the neighbour pages named by the halo index table) into a double-buffered
+// LDS plane, and every thread pulls the 4x4 patch around its 2x2 voxel
+// column into registers. Three such patches (z-1, z, z+1) rotate in
+// registers, so each halo value is read from HBM once and from LDS once per
+// thread that needs it. All loads of a plane happen before the plane is
+// overwritten, which keeps the in-place update exact.
+//
+// Reference paths are relative to the reference's src/ directory.
+
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+#include "storage.h"
+
+namespace msbg {
+
+namespace {
+
+// ---- vector raw loads/stores -------------------------------------------
+__device__ inline void load4(const uint16_t *p, uint16_t v[4]) {
+  uint2 u = *(const uint2 *)p;
+  v[0] = (uint16_t)(u.x & 0xffffu);
+  v[1] = (uint16_t)(u.x >> 16);
+  v[2] = (uint16_t)(u.y & 0xffffu);
+  v[3] = (uint16_t)(u.y >> 16);
+}
+__device__ inline void load4(const float *p, float v[4]) {
+  float4 u = *(const float4 *)p;
+  v[0] = u.x;
+  v[1] = u.y;
+  v[2] = u.z;
+  v[3] = u.w;
+}
+__device__ inline void load4(const double *p, double v[4]) {
+  double2 a = ((const double2 *)p)[0], b = ((const double2 *)p)[1];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+__device__ inline void store2(uint16_t *p, uint16_t a, uint16_t b) {
+  *(uint32_t *)p = (uint32_t)a | ((uint32_t)b << 16);
+}
+__device__ inline void store2(float *p, float a, float b) { *(float2 *)p = make_float2(a, b); }
+__device__ inline void store2(double *p, double a, double b) {
+  *(double2 *)p = make_double2(a, b);
+}
+
+// ---- the MCF update --------------------------------------------------------
+// Patches: 4x4 values of one plane around the thread's 2x2 column, index
+// j*4+i with halo-local x = i-1, y = j-1 relative to the column origin.
+
+// Reference evaluation order (msbg4.cpp:559-637, 813-825), one rounding per
+// operation: bit-identical to the strict CPU restatement.
+template <typename C>
+__device__ inline C mcf_exact(const C *D, const C *M, const C *U, int c, C dt, bool c01) {
+#pragma clang fp contract(off)
+  const C h = (C)0.5, two = (C)2;
+  C f0 = M[c], f1 = M[c - 1], f2 = M[c + 1], f3 = M[c - 4], f4 = M[c + 4], f5 = D[c], f6 = U[c];
+  C fx = h * (f2 - f1), fy = h * (f4 - f3), fz = h * (f6 - f5);
+  C fxx = f1 + f2 - two * f0, fyy = f3 + f4 - two * f0, fzz = f5 + f6 - two * f0;
+  C fy_right = (M[c + 5] - M[c - 3]) * h, fy_left = (M[c + 3] - M[c - 5]) * h;
+  C fz_right = (U[c + 1] - D[c + 1]) * h, fz_left = (U[c - 1] - D[c - 1]) * h;
+  C fz_up = (U[c + 4] - D[c + 4]) * h, fz_down = (U[c - 4] - D[c - 4]) * h;
+  C fyx = (fy_right - fy_left) * h, fzx = (fz_right - fz_left) * h, fzy = (fz_up - fz_down) * h;
+  C H = (fy * fy + fz * fz) * fxx + (fx * fx + fz * fz) * fyy + (fx * fx + fy * fy) * fzz -
+        two * (fx * fy * fyx + fx * fz * fzx + fy * fz * fzy);
+  C g = fx * fx + fy * fy + fz * fz;
+  H = g > (C)1e-7f ? H / g : (C)0;
+  C Dv = dt * H;
+  Dv = Dv > (C)-0.1f ? Dv : (C)-0.1f;  // Vec8f max(a,b) = a > b ? a : b
+  Dv = Dv < (C)0.1f ? Dv : (C)0.1f;    // Vec8f min(a,b) = a < b ? a : b
+  C phi = f0 + Dv;
+  if (c01) {
+    phi = phi > (C)0 ? phi : (C)0;
+    phi = phi < (C)1 ? phi : (C)1;
+  }
+  return phi;
+}
+
+template <>
+__device__ inline double mcf_exact<double>(const double *D, const double *M, const double *U, int c,
+                                           double dt, bool c01) {
+#pragma clang fp contract(off)
+  double f0 = M[c], f1 = M[c - 1], f2 = M[c + 1], f3 = M[c - 4], f4 = M[c + 4], f5 = D[c],
+         f6 = U[c];
+  double fx = 0.5 * (f2 - f1), fy = 0.5 * (f4 - f3), fz = 0.5 * (f6 - f5);
+  double fxx = f1 + f2 - 2.0 * f0, fyy = f3 + f4 - 2.0 * f0, fzz = f5 + f6 - 2.0 * f0;
+  double fyx = ((M[c + 5] - M[c - 3]) * 0.5 - (M[c + 3] - M[c - 5]) * 0.5) * 0.5;
+  double fzx = ((U[c + 1] - D[c + 1]) * 0.5 - (U[c - 1] - D[c - 1]) * 0.5) * 0.5;
+  double fzy = ((U[c + 4] - D[c + 4]) * 0.5 - (U[c - 4] - D[c - 4]) * 0.5) * 0.5;
+  double H = (fy * fy + fz * fz) * fxx + (fx * fx + fz * fz) * fyy + (fx * fx + fy * fy) * fzz -
+             2.0 * (fx * fy * fyx + fx * fz * fzx + fy * fz * fzy);
+  double g = fx * fx + fy * fy + fz * fz;
+  H = g > 1e-7 ? H / g : 0.0;
+  double Dv = dt * H;
+  Dv = Dv > -0.1 ? Dv : -0.1;
+  Dv = Dv < 0.1 ? Dv : 0.1;
+  double phi = f0 + Dv;
+  if (c01) {
+    phi = phi > 0.0 ? phi : 0.0;
+    phi = phi < 1.0 ? phi : 1.0;
+  }
+  return phi;
+}
+
+// Fast evaluation of the same formula for one 2x2 column step. With
+// d* = 2 f*, c** = 4 f**, the reference's H / |grad|^2 equals
+//   N / G,  N = (dy^2+dz^2) fxx + (dx^2+dz^2) fyy + (dx^2+dy^2) fzz
+//               - 0.5 (dx dy cyx + dx dz czx + dy dz czy),
+//           G = dx^2 + dy^2 + dz^2,
+// and the threshold |grad|^2 > 1e-7 becomes G > 4e-7. Central differences
+// shared by neighbouring voxels of the column are formed once.
+__device__ inline void mcf_fast4(const float *D, const float *M, const float *U, float dt, bool c01,
+                                 float out[4]) {
+  float colY[4][4], dzc[4][4];
+#pragma unroll
+  for (int j = 1; j <= 2; j++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) colY[j][i] = M[(j + 1) * 4 + i] - M[(j - 1) * 4 + i];
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      if ((j == 1 || j == 2) || (i == 1 || i == 2)) dzc[j][i] = U[j * 4 + i] - D[j * 4 + i];
+#pragma unroll
+  for (int b = 0; b < 2; b++)
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      const int i = a + 1, j = b + 1, c = j * 4 + i;
+      float f0 = M[c];
+      float dx = M[c + 1] - M[c - 1];
+      float dy = colY[j][i];
+      float dz = dzc[j][i];
+      float fxx = __builtin_fmaf(-2.0f, f0, M[c - 1] + M[c + 1]);
+      float fyy = __builtin_fmaf(-2.0f, f0, M[c - 4] + M[c + 4]);
+      float fzz = __builtin_fmaf(-2.0f, f0, D[c] + U[c]);
+      float cyx = colY[j][i + 1] - colY[j][i - 1];
+      float czx = dzc[j][i + 1] - dzc[j][i - 1];
+      float czy = dzc[j + 1][i] - dzc[j - 1][i];
+      float dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+      float A = dy2 + dz2, B = dx2 + dz2, Cq = dx2 + dy2;
+      float G = A + dx2;
+      float N = A * fxx;
+      N = __builtin_fmaf(B, fyy, N);
+      N = __builtin_fmaf(Cq, fzz, N);
+      float p1 = dy * cyx;
+      p1 = __builtin_fmaf(dz, czx, p1);
+      float t = (dy * dz) * czy;
+      t = __builtin_fmaf(dx, p1, t);
+      N = __builtin_fmaf(-0.5f, t, N);
+      float r = __builtin_amdgcn_rcpf(G);
+      float Dv = (N * dt) * r;
+      Dv = G > 4.0f * 1e-7f ? Dv : 0.0f;
+      Dv = __builtin_amdgcn_fmed3f(Dv, -0.1f, 0.1f);
+      float phi = f0 + Dv;
+      if (c01) phi = __builtin_amdgcn_fmed3f(phi, 0.0f, 1.0f);
+      out[b * 2 + a] = phi;
+    }
+}
+
+// ---- the pass kernel ----------------------------------------------------
+template <int BSX, int ST, bool EXACT>
+__global__ __launch_bounds__((BSX / 2) * (BSX / 2)) void k_sweep(
+    const int32_t *__restrict__ list, const int32_t *__restrict__ nbrTab,
+    uint8_t *__restrict__ pagesRaw, float dtf, int c01i, float emptyv, float fullv,
+    double *__restrict__ partial) {
+  using S = Stor<ST>;
+  using R = typename S::raw;
+  using C = typename S::ct;
+  constexpr int NT = (BSX / 2) * (BSX / 2);
+  constexpr int P = BSX + 2;  // LDS row pitch (halo coords 0..BSX+1)
+  constexpr int PS = BSX * BSX;
+  constexpr int64_t PE = (int64_t)BSX * BSX * BSX;
+  constexpr int NR = 4 * BSX + 4;  // ring cells of one halo plane
+  constexpr int RPT = (NR + NT - 1) / NT;
+  constexpr int M = BSX - 1;
+
+  __shared__ C buf[2][P * P];
+  __shared__ int codes[27];
+  __shared__ double red[(NT + 63) / 64];
+
+  const int tid = threadIdx.x;
+  const int page = list[blockIdx.x];
+  if (tid < 27) codes[tid] = nbrTab[(int64_t)page * 27 + tid];
+  __syncthreads();
+
+  R *pages = (R *)pagesRaw;
+  R *own = pages + (int64_t)page * PE;
+  const C dt = (C)dtf;
+  const bool c01 = c01i != 0;
+
+  // load mapping: 4 consecutive x of one row
+  const int lx = 4 * (tid % (BSX / 4)), ly = tid / (BSX / 4);
+  // compute mapping: 2x2 column
+  const int x0 = 2 * (tid % (BSX / 2)), y0 = 2 * (tid / (BSX / 2));
+
+  C vin[4];
+  C vring[RPT];
+
+  auto dec = [](R r) -> C { return (C)S::dec(r); };
+
+  // global loads of halo plane zz (zz in [-1, BSX]) into vin/vring
+  auto load_plane = [&](int zz) {
+    // interior
+    const R *src = nullptr;
+    C cval = 0;
+    if (zz >= 0 && zz < BSX) {
+      src = own + zz * PS;
+    } else {
+      int c = codes[zz < 0 ? 4 : 22];
+      if (c >= 0)
+        src = pages + (int64_t)c * PE + (zz < 0 ? M : 0) * PS;
+      else if (c == MSBG_BLOCK_EMPTY)
+        cval = (C)emptyv;
+      else if (c == MSBG_BLOCK_FULL)
+        cval = (C)fullv;
+      else
+        src = own + (zz < 0 ? 0 : M) * PS;  // Neumann: own boundary plane
+    }
+    if (src) {
+      R r[4];
+      load4(src + ly * BSX + lx, r);
+#pragma unroll
+      for (int k = 0; k < 4; k++) vin[k] = dec(r[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) vin[k] = cval;
+    }
+    // ring
+    const int dzi = zz < 0 ? 0 : (zz >= BSX ? 2 : 1);
+    const int zc = zz < 0 ? 0 : (zz > M ? M : zz);  // clamped (Neumann)
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      int r = tid + q * NT;
+      if (r < NR) {
+        int x, y;
+        if (r < BSX) {
+          x = -1;
+          y = r;
+        } else if (r < 2 * BSX) {
+          x = BSX;
+          y = r - BSX;
+        } else if (r < 3 * BSX) {
+          x = r - 2 * BSX;
+          y = -1;
+        } else if (r < 4 * BSX) {
+          x = r - 3 * BSX;
+          y = BSX;
+        } else {
+          int k = r - 4 * BSX;
+          x = (k & 1) ? BSX : -1;
+          y = (k & 2) ? BSX : -1;
+        }
+        int dxi = x < 0 ? 0 : (x >= BSX ? 2 : 1), dyi = y < 0 ? 0 : (y >= BSX ? 2 : 1);
+        int c = codes[dzi * 9 + dyi * 3 + dxi];
+        C v;
+        if (c >= 0) {
+          v = dec(pages[(int64_t)c * PE + (zz & M) * PS + (y & M) * BSX + (x & M)]);
+        } else if (c == MSBG_BLOCK_EMPTY) {
+          v = (C)emptyv;
+        } else if (c == MSBG_BLOCK_FULL) {
+          v = (C)fullv;
+        } else {
+          int xc = x < 0 ? 0 : (x > M ? M : x), yc = y < 0 ? 0 : (y > M ? M : y);
+          v = dec(own[zc * PS + yc * BSX + xc]);
+        }
+        vring[q] = v;
+      }
+    }
+  };
+
+  auto put_plane = [&](C *b) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) b[(ly + 1) * P + lx + 1 + k] = vin[k];
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      int r = tid + q * NT;
+      if (r < NR) {
+        int x, y;
+        if (r < BSX) {
+          x = -1;
+          y = r;
+        } else if (r < 2 * BSX) {
+          x = BSX;
+          y = r - BSX;
+        } else if (r < 3 * BSX) {
+          x = r - 2 * BSX;
+          y = -1;
+        } else if (r < 4 * BSX) {
+          x = r - 3 * BSX;
+          y = BSX;
+        } else {
+          int k = r - 4 * BSX;
+          x = (k & 1) ? BSX : -1;
+          y = (k & 2) ? BSX : -1;
+        }
+        b[(y + 1) * P + x + 1] = vring[q];
+      }
+    }
+  };
+
+  auto get_patch = [&](const C *b, C (&p)[16]) {
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) p[j * 4 + i] = b[(y0 + j) * P + x0 + i];
+  };
+
+  double acc = 0.0;
+
+  auto compute = [&](int z, const C (&Dp)[16], const C (&Mp)[16], const C (&Up)[16]) {
+    C o[4];
+    if constexpr (EXACT || sizeof(C) == 8) {
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int a = 0; a < 2; a++) o[b * 2 + a] = mcf_exact<C>(Dp, Mp, Up, (b + 1) * 4 + a + 1, dt, c01);
+    } else {
+      mcf_fast4(Dp, Mp, Up, dt, c01, o);
+    }
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      R e0 = S::enc(o[b * 2]), e1 = S::enc(o[b * 2 + 1]);
+      store2(own + z * PS + (y0 + b) * BSX + x0, e0, e1);
+      if (partial) {
+        double d0 = (double)S::dec(e0) - (double)Mp[(b + 1) * 4 + 1];
+        double d1 = (double)S::dec(e1) - (double)Mp[(b + 1) * 4 + 2];
+        acc += d0 * d0;
+        acc += d1 * d1;
+      }
+    }
+  };
+
+  C pA[16], pB[16], pC[16];
+
+  // prologue: planes -1 and 0 into pA, pB; plane 1 prefetched
+  load_plane(-1);
+  put_plane(buf[1]);
+  load_plane(0);
+  __syncthreads();
+  get_patch(buf[1], pA);
+  put_plane(buf[0]);
+  load_plane(1);
+  __syncthreads();
+  get_patch(buf[0], pB);
+
+  // step z: stage plane z+1 (prefetched), prefetch z+2, compute plane z
+  auto step = [&](int z, C (&Dp)[16], C (&Mp)[16], C (&Up)[16]) {
+    put_plane(buf[(z + 1) & 1]);
+    if (z + 2 <= BSX) load_plane(z + 2);
+    __syncthreads();
+    get_patch(buf[(z + 1) & 1], Up);
+    compute(z, Dp, Mp, Up);
+  };
+
+  for (int z = 0; z < BSX; z += 3) {
+    step(z, pA, pB, pC);
+    if (z + 1 < BSX) step(z + 1, pB, pC, pA);
+    if (z + 2 < BSX) step(z + 2, pC, pA, pB);
+  }
+
+  if (partial) {
+    // deterministic block reduction: wave shuffles, then waves in order
+    constexpr int W = NT < 64 ? NT : 64;
+    for (int off = W / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, W);
+    if ((tid & 63) == 0) red[tid >> 6] = acc;
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+      for (int w = 0; w < (NT + 63) / 64; w++) s += red[w];
+      partial[blockIdx.x] = s;
+    }
+  }
+}
+
+template <int BSX, int ST>
+int launch_t(msbg_grid *g, ChanLevel &cl, const int32_t *d_list, int64_t n, float dt, int c01,
+             double *d_partial) {
+  constexpr int NT = (BSX / 2) * (BSX / 2);
+  float ev = raw_to_float_host(ST, cl.empty_raw), fv = raw_to_float_host(ST, cl.full_raw);
+  for (int64_t off = 0; off < n; off += 0x7fffffff) {
+    unsigned cnt = (unsigned)(n - off < 0x7fffffff ? n - off : 0x7fffffff);
+    bool exact = g->arith == MSBG_ARITH_EXACT;
+    if constexpr (ST == MSBG_STORAGE_F64) exact = true;  // the FP64 run is always exact
+    if (exact)
+      hipLaunchKernelGGL((k_sweep<BSX, ST, true>), dim3(cnt), dim3(NT), 0, g->stream, d_list + off,
+                         cl.d_nbr.p, cl.d_pages.p, dt, c01, ev, fv,
+                         d_partial ? d_partial + off : nullptr);
+    else if constexpr (ST != MSBG_STORAGE_F64)
+      hipLaunchKernelGGL((k_sweep<BSX, ST, false>), dim3(cnt), dim3(NT), 0, g->stream, d_list + off,
+                         cl.d_nbr.p, cl.d_pages.p, dt, c01, ev, fv,
+                         d_partial ? d_partial + off : nullptr);
+    MSBG_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+template <int BSX>
+int launch_bsx(msbg_grid *g, Channel &ch, const int32_t *d_list, int64_t n, float dt, int c01,
+               double *d_partial) {
+  ChanLevel &cl = ch.lv[0];
+  switch (ch.storage) {
+    case MSBG_STORAGE_U16: return launch_t<BSX, MSBG_STORAGE_U16>(g, cl, d_list, n, dt, c01, d_partial);
+    case MSBG_STORAGE_F32: return launch_t<BSX, MSBG_STORAGE_F32>(g, cl, d_list, n, dt, c01, d_partial);
+    case MSBG_STORAGE_F16: return launch_t<BSX, MSBG_STORAGE_F16>(g, cl, d_list, n, dt, c01, d_partial);
+    case MSBG_STORAGE_BF16: return launch_t<BSX, MSBG_STORAGE_BF16>(g, cl, d_list, n, dt, c01, d_partial);
+    case MSBG_STORAGE_F64: return launch_t<BSX, MSBG_STORAGE_F64>(g, cl, d_list, n, dt, c01, d_partial);
+  }
+  set_error("sweep: bad storage %d", ch.storage);
+  return MSBG_EINVAL;
+}
+
+}  // namespace
+
+int launch_sweep_pass(msbg_grid *g, Channel &ch, const int32_t *d_list, int64_t n, float dt,
+                      int doConstr01, double *d_partial) {
+  switch (g->bsx0) {
+    case 8: return launch_bsx<8>(g, ch, d_list, n, dt, doConstr01, d_partial);
+    case 16: return launch_bsx<16>(g, ch, d_list, n, dt, doConstr01, d_partial);
+    case 32: return launch_bsx<32>(g, ch, d_list, n, dt, doConstr01, d_partial);
+  }
+  set_error("sweep: block size %d not supported by the sweep kernel (8, 16, 32)", g->bsx0);
+  return MSBG_EUNSUPP;
+}
+
+}  // namespace msbg

@@ -1,0 +1,251 @@
+"""Host-side mirror of the reference's MultiresSparseGrid interface for the
+sparse-block stencil path, over the C ABI of libmsbg_gpu.so.
+
+Method names and argument meanings follow the reference (msbg.h,
+msbg_demo.cpp); reference paths are relative to the reference's src/
+directory. All device memory is owned by the library; arrays crossing this
+boundary are host numpy arrays.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, lib
+
+_NP_DTYPE = {L.U16: np.uint16, L.F32: np.float32, L.F16: np.float16, L.BF16: np.uint16,
+             L.F64: np.float64}
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def storage_dtype(storage):
+    return _NP_DTYPE[storage]
+
+
+def getBlockColor8(bx, by, bz):
+    """msbg.h:642-646."""
+    return (bx & 1) + ((by & 1) << 1) + ((bz & 1) << 2)
+
+
+def scene_spheres(seed, nObj, cmin, cmax, rmin, rmax, nPts, sigma=0.5, ellipsoidFrac=0.0):
+    """Seeded jittered sphere/ellipsoid surface points, (nObj*nPts, 3) float32."""
+    out = np.empty((nObj * nPts, 3), dtype=np.float32)
+    check(lib().msbg_scene_spheres(seed, nObj, cmin, cmax, rmin, rmax, nPts, sigma,
+                                   ellipsoidFrac, _ptr(out)))
+    return out
+
+
+class MultiresSparseGrid:
+    """MultiresSparseGrid (msbg.h:834-3907) restricted to the stencil path."""
+
+    def __init__(self, handle):
+        self._h = handle
+        dims = np.zeros(8, dtype=np.int32)
+        nb = C.c_int64()
+        check(lib().msbg_get_dims(self._h, _ptr(dims), C.byref(nb)))
+        self.sx, self.sy, self.sz, self.bsx0, self.nbx, self.nby, self.nbz, self.nLevels = (
+            int(v) for v in dims)
+        self._nblocks = nb.value
+        self._storage = {}
+
+    # -- lifetime ----------------------------------------------------------
+    @classmethod
+    def create(cls, name, sx, sy, sz, blockSize0=16, dTransRes=-1, maxSizeMB=0,
+               initialSizeMB=-1, options=0, miscConf=None, pSolverConf=None, nLevels=-1):
+        """MultiresSparseGrid::create (msbg.h:834-850). dTransRes, maxSizeMB,
+        initialSizeMB and the solver configs do not affect this path."""
+        h = C.c_void_p()
+        check(lib().msbg_create(name.encode(), sx, sy, sz, blockSize0, options, nLevels,
+                                C.byref(h)))
+        return cls(h)
+
+    def destroy(self):
+        if self._h:
+            lib().msbg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.destroy()
+
+    # -- geometry ----------------------------------------------------------
+    def nBlocks(self):
+        return self._nblocks
+
+    def getNumLevels(self):
+        return self.nLevels
+
+    def bsx(self, level=0):
+        return self.bsx0 >> level
+
+    def getBlockCoordsById(self, bid):
+        """sbg.h:1789-1798."""
+        bz, rem = divmod(bid, self.nbx * self.nby)
+        by, bx = divmod(rem, self.nbx)
+        return bx, by, bz
+
+    def getBlockIndex(self, bx, by, bz):
+        """sbg.h:1744-1749."""
+        return (bz * self.nby + by) * self.nbx + bx
+
+    def set_arith(self, mode):
+        check(lib().msbg_set_arith(self._h, mode))
+
+    def synchronize(self):
+        check(lib().msbg_synchronize(self._h))
+
+    # -- particles / refinement map (msbg_demo.cpp:339-441) -----------------
+    def points_upload(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        nk = C.c_int64()
+        check(lib().msbg_points_upload(self._h, _ptr(xyz), xyz.shape[0], C.byref(nk)))
+        self._npoints = nk.value
+        return nk.value
+
+    def points_download(self):
+        out = np.empty((self._npoints, 3), dtype=np.float32)
+        check(lib().msbg_points_download(self._h, _ptr(out), out.shape[0]))
+        return out
+
+    def activate(self, rScan):
+        """blockActive[] counts of msbg_demo.cpp:368-382."""
+        counts = np.empty(self._nblocks, dtype=np.uint32)
+        na = C.c_int64()
+        check(lib().msbg_activate_from_points(self._h, rScan, _ptr(counts), C.byref(na)))
+        return counts
+
+    def levels_from_points(self, rScans):
+        r = np.ascontiguousarray(rScans, dtype=np.float32)
+        out = np.empty(self._nblocks, dtype=np.int32)
+        check(lib().msbg_levels_from_points(self._h, _ptr(r), r.size, _ptr(out)))
+        return out
+
+    def regularizeRefinementMap(self, blockmapIn, blockmapOut=None):
+        """msbg.cpp:1202-1423. Returns blockmapOut (a copy when None is given)."""
+        a = np.ascontiguousarray(blockmapIn, dtype=np.int32)
+        out = np.array(a, copy=True) if blockmapOut is None else blockmapOut
+        check(lib().msbg_regularize_refinement_map(self._h, _ptr(a), _ptr(out)))
+        return out
+
+    def setRefinementMap(self, blockLevels=None, blockFlags=None):
+        """msbg.cpp:1428-1745 at levelMg 0."""
+        lv = None if blockLevels is None else np.ascontiguousarray(blockLevels, dtype=np.int32)
+        fl = None if blockFlags is None else np.ascontiguousarray(blockFlags, dtype=np.uint32)
+        check(lib().msbg_set_refinement_map(self._h, _ptr(lv), _ptr(fl)))
+
+    def getBlockInfo(self, bid=None):
+        """(levels, flags) of all blocks, or of block bid (msbg.h:945-950)."""
+        lv = np.empty(self._nblocks, dtype=np.uint16)
+        fl = np.empty(self._nblocks, dtype=np.uint16)
+        check(lib().msbg_get_block_info(self._h, _ptr(lv), _ptr(fl)))
+        if bid is None:
+            return lv, fl
+        return int(lv[bid]), int(fl[bid])
+
+    def set_exists(self, exists=None):
+        e = None if exists is None else np.ascontiguousarray(exists, dtype=np.uint8)
+        check(lib().msbg_set_exists(self._h, _ptr(e)))
+
+    def blocksFineCoarse(self, level):
+        n = C.c_int64()
+        check(lib().msbg_get_fine_coarse_blocks(self._h, level, None, 0, C.byref(n)))
+        out = np.empty(n.value, dtype=np.int32)
+        check(lib().msbg_get_fine_coarse_blocks(self._h, level, _ptr(out), n.value, None))
+        return out
+
+    # -- channels ------------------------------------------------------------
+    def channel_define(self, chan, storage, level, kind, empty_raw=0.0, full_raw=None,
+                       fill_raw=None):
+        if full_raw is None:
+            full_raw = 65535.0 if storage == L.U16 else 1.0
+        if fill_raw is None:
+            fill_raw = full_raw
+        k = np.ascontiguousarray(kind, dtype=np.uint8)
+        check(lib().msbg_channel_define(self._h, chan, storage, level, _ptr(k), empty_raw,
+                                        full_raw, fill_raw))
+        self._storage[chan] = storage
+
+    def alloc_demo(self, chan, storage):
+        """Allocation step of msbg_demo.cpp:451-511; returns activeBlocks."""
+        out = np.empty(self._nblocks, dtype=np.int32)
+        n = C.c_int64()
+        check(lib().msbg_channel_alloc_demo(self._h, chan, storage, _ptr(out), out.size,
+                                            C.byref(n)))
+        self._storage[chan] = storage
+        return out[:n.value].copy()
+
+    def pagemap(self, chan, level=0):
+        pm = np.empty(self._nblocks, dtype=np.int32)
+        n = C.c_int64()
+        check(lib().msbg_channel_get_pagemap(self._h, chan, level, _ptr(pm), C.byref(n)))
+        return pm, n.value
+
+    def download(self, chan, level=0):
+        """All pages of (chan, level) as [npages, bsx, bsx, bsx] (z, y, x)."""
+        st = self._storage[chan]
+        _, np_ = self.pagemap(chan, level)
+        b = self.bsx(level)
+        out = np.empty((np_, b, b, b), dtype=_NP_DTYPE[st])
+        check(lib().msbg_channel_download(self._h, chan, level, _ptr(out), out.nbytes))
+        return out
+
+    def upload(self, chan, level, pages):
+        st = self._storage[chan]
+        a = np.ascontiguousarray(pages, dtype=_NP_DTYPE[st])
+        check(lib().msbg_channel_upload(self._h, chan, level, _ptr(a), a.nbytes))
+
+    def halo_table(self, chan, level=0):
+        _, np_ = self.pagemap(chan, level)
+        out = np.empty((np_, 27), dtype=np.int32)
+        check(lib().msbg_get_halo_table(self._h, chan, level, _ptr(out), out.size))
+        return out
+
+    # -- hot path --------------------------------------------------------------
+    def splat(self, chan, rParticle=2.0, nbDist=2.0):
+        check(lib().msbg_splat_points(self._h, chan, rParticle, nbDist))
+
+    def finalize(self, chan, rParticle=2.0, nbDist=2.0):
+        check(lib().msbg_finalize_density(self._h, chan, rParticle, nbDist))
+
+    def fillHaloBlock(self, chan, bid, dmax=1, opt=L.OPT_BC_NEUMANN):
+        """HaloBlockSet::fillHaloBlock_ (halo.cpp:267-479): (hsx,hsx,hsx) float32."""
+        level = int(self.getBlockInfo(bid)[0])
+        hsx = self.bsx(level) + 2 * dmax
+        out = np.empty((hsx, hsx, hsx), dtype=np.float32)
+        check(lib().msbg_halo_fill(self._h, chan, bid, dmax, opt, _ptr(out)))
+        return out
+
+    def prepareGhostBlocks(self, chan):
+        check(lib().msbg_prepare_ghost_blocks(self._h, chan))
+
+    def applyChannelPdeFast(self, chPhi, chTmp, chTmp2, blockList, laplTyp, doConstrZeroOne,
+                            densThr, densThrTol1, densThrTol2, numIter, gridSpacing=1.0,
+                            timeStep=0.025, residual=False):
+        """msbg4.cpp:60-957. Returns the per-sweep FP64 residuals if asked."""
+        bl = None if blockList is None else np.ascontiguousarray(blockList, dtype=np.int32)
+        nsweeps = numIter + (numIter % 2)
+        res = np.zeros(max(nsweeps, 1), dtype=np.float64) if residual else None
+        check(lib().msbg_apply_channel_pde_fast(
+            self._h, chPhi, chTmp, chTmp2, _ptr(bl), 0 if bl is None else bl.size, laplTyp,
+            doConstrZeroOne, densThr, densThrTol1, densThrTol2, numIter, gridSpacing, timeStep,
+            _ptr(res)))
+        return res[:nsweeps] if residual else None
+
+    def last_sweep_timing(self):
+        t, a, n = C.c_double(), C.c_double(), C.c_int64()
+        check(lib().msbg_last_sweep_timing(self._h, C.byref(t), C.byref(a), C.byref(n)))
+        return t.value, a.value, n.value
+
+
+MCF_8COLOR = -(L.PDE_MEAN_CURVATURE + L.OPT_8_COLOR_SCHEME)

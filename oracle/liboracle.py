@@ -1,0 +1,212 @@
+"""ctypes binding of the CPU oracle (liboracle_strict.so / liboracle_fast.so).
+
+Test infrastructure only: imported by tests/, the smoke check and the
+benchmark's CPU-baseline leg. Each wrapper names the C function it calls;
+the C sources cite the reference lines they restate.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+U16, F32, F16, BF16, F64 = 1, 2, 3, 4, 5
+OPT_INTERPOLATE, OPT_BC_NEUMANN, OPT_BC_COARSE_LEVEL = 1 << 7, 1 << 10, 1 << 26
+NULL, EMPTY, FULL = -1, -2, -3
+_NP = {U16: np.uint16, F32: np.float32, F16: np.uint16, BF16: np.uint16, F64: np.float64}
+
+
+class OrLevel(C.Structure):
+    _fields_ = [("sx", C.c_int), ("sy", C.c_int), ("sz", C.c_int), ("bsx", C.c_int),
+                ("nbx", C.c_int), ("nby", C.c_int), ("nbz", C.c_int),
+                ("page", C.c_void_p), ("pages", C.c_void_p), ("storage", C.c_int),
+                ("empty_raw", C.c_double), ("full_raw", C.c_double)]
+
+
+_libs = {}
+
+
+def load(kind="strict"):
+    if kind not in _libs:
+        path = os.path.join(_HERE, f"liboracle_{kind}.so")
+        if not os.path.exists(path):
+            raise OSError(f"{path} not built; run `make -C oracle`")
+        L = C.CDLL(path)
+        P, I, I64, F, D = C.c_void_p, C.c_int, C.c_int64, C.c_float, C.c_double
+        sig = {
+            "or_u16_to_float": (F, [C.c_uint16]),
+            "or_u16_from_float_roundf": (C.c_uint16, [F]),
+            "or_u16_from_float_rne": (C.c_uint16, [F]),
+            "or_f16_from_float": (C.c_uint16, [F]),
+            "or_f16_to_float": (F, [C.c_uint16]),
+            "or_bf16_from_float": (C.c_uint16, [F]),
+            "or_bf16_to_float": (F, [C.c_uint16]),
+            "or_domain_bounds": (None, [I, I, I, P, P]),
+            "or_domain_filter": (I64, [I, I, I, P, I64, P]),
+            "or_activate": (None, [I, I, I, I, P, P, I64, F, P, P]),
+            "or_levels_from_activity": (None, [P, I64, I, P]),
+            "or_regularize": (I, [I, I, I, I, P, P]),
+            "or_block_flags": (None, [I, I, I, I, P, P, P, P]),
+            "or_color8": (I, [I, I, I]),
+            "or_color_lists": (None, [I, I, P, I, P, P]),
+            "or_splat_u16": (None, [P, P, P, I64, F]),
+            "or_finalize_value": (C.c_uint16, [C.c_uint16, F, F]),
+            "or_finalize_u16": (None, [P, I64, F, F]),
+            "or_finalize_density_f": (F, [C.c_uint16, F, F]),
+            "or_halo_fill": (None, [P, P, I, I, C.c_uint, P]),
+            "or_halo_fill_f64": (None, [P, I, I, C.c_uint, P]),
+            "or_mcf_sweeps": (I, [P, P, I, P, I, F, I, P, I]),
+            "or_mcf_voxel": (F, [P, I, I, F, I]),
+            "or_mcf_voxel_f64": (D, [P, I, I, D, I]),
+            "or_ghost_downsample": (None, [P, I, P]),
+            "or_prepare_ghost_blocks": (None, [P, I, P, P, P]),
+            "or_version": (C.c_uint64, []),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(L, n)
+            f.restype = r
+            f.argtypes = a
+        _libs[kind] = L
+    return _libs[kind]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Level:
+    """One level of one channel: block map + pages (numpy), as or_level."""
+
+    def __init__(self, sx, sy, sz, bsx, pagemap, pages, storage, empty_raw=0.0, full_raw=None):
+        self.sx, self.sy, self.sz, self.bsx = sx, sy, sz, bsx
+        self.nbx, self.nby, self.nbz = sx // bsx, sy // bsx, sz // bsx
+        self.page = np.ascontiguousarray(pagemap, dtype=np.int32)
+        self.pages = np.ascontiguousarray(pages, dtype=_NP[storage])
+        self.storage = storage
+        self.empty_raw = empty_raw
+        self.full_raw = (65535.0 if storage == U16 else 1.0) if full_raw is None else full_raw
+
+    def struct(self):
+        return OrLevel(self.sx, self.sy, self.sz, self.bsx, self.nbx, self.nby, self.nbz,
+                       _p(self.page), _p(self.pages), self.storage, self.empty_raw,
+                       self.full_raw)
+
+
+def domain_bounds(sx, sy, sz):
+    a, b = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    load().or_domain_bounds(sx, sy, sz, _p(a), _p(b))
+    return a, b
+
+
+def domain_filter(sx, sy, sz, xyz):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+    keep = np.zeros(xyz.shape[0], np.uint8)
+    load().or_domain_filter(sx, sy, sz, _p(xyz), xyz.shape[0], _p(keep))
+    return keep
+
+
+def activate(bsx, nbx, nby, nbz, xyz, rScan, keep=None):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+    cnt = np.zeros(nbx * nby * nbz, np.uint32)
+    home = np.zeros(xyz.shape[0], np.int32)
+    k = None if keep is None else np.ascontiguousarray(keep, dtype=np.uint8)
+    load().or_activate(bsx, nbx, nby, nbz, _p(xyz), _p(k), xyz.shape[0], rScan, _p(cnt), _p(home))
+    return cnt, home
+
+
+def levels_from_activity(count, nLevels):
+    count = np.ascontiguousarray(count, dtype=np.uint32)
+    out = np.zeros(count.size, np.int32)
+    load().or_levels_from_activity(_p(count), count.size, nLevels, _p(out))
+    return out
+
+
+def regularize(nbx, nby, nbz, nLevels, levels):
+    a = np.ascontiguousarray(levels, dtype=np.int32)
+    out = a.copy()
+    rc = load().or_regularize(nbx, nby, nbz, nLevels, _p(a), _p(out))
+    return out, rc
+
+
+def block_flags(nbx, nby, nbz, nLevels, levels, inFlags=None):
+    a = np.ascontiguousarray(levels, dtype=np.int32)
+    f = None if inFlags is None else np.ascontiguousarray(inFlags, dtype=np.uint32)
+    lv = np.zeros(a.size, np.uint16)
+    fl = np.zeros(a.size, np.uint16)
+    load().or_block_flags(nbx, nby, nbz, nLevels, _p(a), _p(f), _p(lv), _p(fl))
+    return lv, fl
+
+
+def color_lists(nbx, nby, blocks):
+    b = np.ascontiguousarray(blocks, dtype=np.int32)
+    out = np.zeros(b.size, np.int32)
+    off = np.zeros(9, np.int32)
+    load().or_color_lists(nbx, nby, _p(b), b.size, _p(out), _p(off))
+    return out, off
+
+
+def splat_u16(level, xyz, rScan, home=None):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+    s = level.struct()
+    h = None if home is None else np.ascontiguousarray(home, dtype=np.int32)
+    load().or_splat_u16(C.byref(s), _p(xyz), _p(h), xyz.shape[0], rScan)
+
+
+def finalize_u16(pages, rParticle=2.0, nbDist=2.0):
+    load().or_finalize_u16(_p(pages), pages.size, rParticle, nbDist)
+
+
+def finalize_table(rParticle=2.0, nbDist=2.0):
+    L = load()
+    return np.array([L.or_finalize_value(u, rParticle, nbDist) for u in range(65536)], np.uint16)
+
+
+def finalize_density_table(rParticle=2.0, nbDist=2.0):
+    L = load()
+    return np.array([L.or_finalize_density_f(u, rParticle, nbDist) for u in range(65536)],
+                    np.float32)
+
+
+def halo_fill(level, bid, dmax=1, opt=OPT_BC_NEUMANN, lo=None):
+    hsx = level.bsx + 2 * dmax
+    out = np.zeros((hsx, hsx, hsx), np.float32)
+    s = level.struct()
+    so = lo.struct() if lo is not None else None
+    load().or_halo_fill(C.byref(s), C.byref(so) if so is not None else None, bid, dmax, opt,
+                        _p(out))
+    return out
+
+
+def mcf_sweeps(level, blocks, levels, numIter, dt, doConstrZeroOne=1, residual=False,
+               nthreads=0, kind="strict"):
+    b = np.ascontiguousarray(blocks, dtype=np.int32)
+    lv = None if levels is None else np.ascontiguousarray(levels, dtype=np.uint16)
+    n = numIter + (numIter % 2)
+    res = np.zeros(max(n, 1), np.float64) if residual else None
+    s = level.struct()
+    load(kind).or_mcf_sweeps(C.byref(s), _p(b), b.size, _p(lv), numIter, dt, doConstrZeroOne,
+                             _p(res), nthreads)
+    return res[:n] if residual else None
+
+
+def mcf_voxel(h, hid, dt, doConstrZeroOne=1):
+    h = np.ascontiguousarray(h, dtype=np.float32)
+    return load().or_mcf_voxel(_p(h), h.shape[-1], hid, dt, doConstrZeroOne)
+
+
+def ghost_downsample(halo, bsxHi):
+    halo = np.ascontiguousarray(halo, dtype=np.float32)
+    out = np.zeros((bsxHi // 2,) * 3, np.float32)
+    load().or_ghost_downsample(_p(halo), bsxHi, _p(out))
+    return out
+
+
+def prepare_ghost_blocks(levels, fc_lists, flags):
+    """levels: list of Level (one per resolution level, all defined)."""
+    arr = (OrLevel * len(levels))(*[lv.struct() for lv in levels])
+    fc = np.concatenate([np.asarray(x, np.int32) for x in fc_lists] + [np.zeros(0, np.int32)])
+    off = np.zeros(len(fc_lists) + 1, np.int32)
+    off[1:] = np.cumsum([len(x) for x in fc_lists])
+    fl = np.ascontiguousarray(flags, dtype=np.uint16)
+    load().or_prepare_ghost_blocks(arr, len(levels), _p(fc), _p(off), _p(fl))

@@ -1,0 +1,154 @@
+/*
+ * msbg_oracle.h -- CPU restatement of MSBG's sparse-block stencil hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is the checker, not the product: only the
+ * test suite, the smoke check and the benchmark's CPU-baseline leg may load
+ * it. The product path (libmsbg_gpu.so) never calls into it.
+ *
+ * Every function restates the behaviour of a reference function; the
+ * reference file:line it follows is cited at each declaration (paths are
+ * relative to the reference's src/ directory).
+ *
+ * Storage codes (shared with the product's C-ABI):
+ *   1 = UNORM16 (uint16, value = u/65535, the reference's RenderDensity)
+ *   2 = FP32, 3 = FP16 (IEEE binary16), 4 = BF16, 5 = FP64
+ *
+ * Block map codes (per block id of one resolution level):
+ *   >= 0  page index of an owned value block
+ *   -1    NULL block (no data at this level)
+ *   -2    const EMPTY block (reads as the level's empty value everywhere)
+ *   -3    const FULL block (reads as the level's full value everywhere)
+ *
+ * Page layout: pages[page * bsx^3 + vid], vid = (vz*bsx + vy)*bsx + vx
+ * (sbg.h:1870-1889, getVoxelInBlockIndex).
+ */
+#ifndef MSBG_ORACLE_H
+#define MSBG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_U16 = 1, OR_F32 = 2, OR_F16 = 3, OR_BF16 = 4, OR_F64 = 5 };
+enum { OR_NULL = -1, OR_EMPTY = -2, OR_FULL = -3 };
+
+/* halo options, values of msbg.h:110,113,126 */
+enum { OR_OPT_INTERPOLATE = 1 << 7, OR_OPT_BC_NEUMANN = 1 << 10,
+       OR_OPT_BC_COARSE_LEVEL = 1 << 26 };
+
+/* block flags, values of msbg.h:561-573 */
+enum { OR_BLK_DOM_BORDER = 1 << 1, OR_BLK_COARSE_FINE = 1 << 2,
+       OR_BLK_FINE_COARSE = 1 << 3, OR_BLK_EXISTS = 1 << 12,
+       OR_BLK_UNIFORM_EMPTY = 1 << 13 };
+
+/* One resolution level of one channel. */
+typedef struct {
+  int sx, sy, sz;        /* voxel resolution of this level */
+  int bsx;               /* block edge at this level (power of two) */
+  int nbx, nby, nbz;     /* block grid (identical on every level) */
+  const int32_t *page;   /* [nbx*nby*nbz] block map codes */
+  void *pages;           /* page pool of this level */
+  int storage;           /* OR_U16 .. OR_F64 */
+  double empty_raw;      /* raw storage value of the EMPTY block */
+  double full_raw;       /* raw storage value of the FULL block */
+} or_level;
+
+/* ---- scalar conversions (sbg.h:225-366, vectorf256.h:1064-1068) ---- */
+float or_u16_to_float(uint16_t u);                 /* u * (1.f/65535) */
+uint16_t or_u16_from_float_roundf(float f);        /* roundf(f*65535), scalar path */
+uint16_t or_u16_from_float_rne(float f);           /* round-half-even, SIMD store path */
+uint16_t or_f16_from_float(float f);               /* IEEE binary16, RNE */
+float or_f16_to_float(uint16_t h);
+uint16_t or_bf16_from_float(float f);              /* bfloat16, RNE */
+float or_bf16_to_float(uint16_t h);
+
+/* ---- A1: domain filter (sbg.cpp:289-316, msbg_demo.cpp:356-360) ---- */
+void or_domain_bounds(int sx, int sy, int sz, float dmin[3], float dmax[3]);
+/* keep[i] = 1 iff point i passes isInDomainRange and trunc(pos) inRange */
+int64_t or_domain_filter(int sx, int sy, int sz, const float *xyz, int64_t n,
+                         uint8_t *keep);
+
+/* ---- A2/A3: block activation and home blocks (msbg_demo.cpp:368-389) ----
+ * count[bid] += 1 for every block in the trunc box of each kept point;
+ * home[i] = home block id of point i (-1 if the point is not kept). */
+void or_activate(int bsx, int nbx, int nby, int nbz, const float *xyz,
+                 const uint8_t *keep, int64_t n, float rScan, uint32_t *count,
+                 int32_t *home);
+
+/* ---- A4: demo level rule + regularizeRefinementMap (msbg_demo.cpp:418-428,
+ * msbg.cpp:1202-1423). in and out may alias. */
+void or_levels_from_activity(const uint32_t *count, int64_t nblocks,
+                             int nLevels, int *levels);
+int or_regularize(int nbx, int nby, int nbz, int nLevels, const int *in,
+                  int *out);
+
+/* ---- A5: setRefinementMap block flags (msbg.cpp:1615-1745), levelMg 0.
+ * inFlags (may be NULL) is the caller's blockFlags[] input. */
+void or_block_flags(int nbx, int nby, int nbz, int nLevels, const int *levels,
+                    const uint32_t *inFlags, uint16_t *level_out,
+                    uint16_t *flags_out);
+
+/* ---- A7: colour (msbg.h:642-646) and per-colour lists (msbg_demo.cpp:527-539)
+ * lists: out[] holds the blocks of colour 0, then 1, ..., in list order;
+ * off[c]..off[c+1] delimits colour c. */
+int or_color8(int bx, int by, int bz);
+void or_color_lists(int nbx, int nby, const int32_t *blocks, int n,
+                    int32_t *out, int32_t off[9]);
+
+/* ---- A8: 8-colour min-distance splat into UNORM16 pages
+ * (msbg_demo.cpp:541-636). Points are visited per colour, per home block in
+ * the order of `blocks`, in list order inside a block -- the result does not
+ * depend on that order (min), except through the non-value-block abandon. */
+void or_splat_u16(const or_level *L, const float *xyz, const int32_t *home,
+                  int64_t n, float rScan);
+
+/* ---- A9: finalize distances to density (msbg_demo.cpp:655-680) ---- */
+uint16_t or_finalize_value(uint16_t u, float rParticle, float nbDist);
+float or_finalize_density_f(uint16_t u, float rParticle, float nbDist);
+void or_finalize_u16(uint16_t *pages, int64_t nvox, float rParticle,
+                     float nbDist);
+
+/* ---- A11/A13: halo fill of block bid into a float (or double) buffer of
+ * (bsx+2*dmax)^3 (halo.cpp:267-479, halo.h:267-355,422-501,
+ * sbg.h:3002-3095). Lo may be NULL unless OPT_BC_COARSE_LEVEL is set. */
+void or_halo_fill(const or_level *L, const or_level *Lo, int bid, int dmax,
+                  unsigned opt, float *out);
+void or_halo_fill_f64(const or_level *L, int bid, int dmax, unsigned opt,
+                      double *out);
+
+/* ---- A10/A12: 8-colour in-place mean-curvature-flow sweeps
+ * (msbg4.cpp:60-957). `blocks` is the caller's block list, `levels` the
+ * per-block resolution level (only level-0 blocks are updated,
+ * msbg4.cpp:467-470). numIter is rounded up to even (msbg4.cpp:150-152).
+ * Arithmetic is FP32 for every storage except FP64, which runs the whole
+ * sweep in double. residual (optional, [nSweeps]) receives the FP64 RMS of
+ * the per-sweep update over the swept voxels. Returns the sweep count. */
+int or_mcf_sweeps(const or_level *L, const int32_t *blocks, int n,
+                  const uint16_t *levels, int numIter, float dt,
+                  int doConstrZeroOne, double *residual, int nthreads);
+/* The per-voxel MCF update from a halo buffer (msbg4.cpp:559-637,813-825). */
+float or_mcf_voxel(const float *h, int hsx, int hid, float dt,
+                   int doConstrZeroOne);
+double or_mcf_voxel_f64(const double *h, int hsx, int hid, double dt,
+                        int doConstrZeroOne);
+
+/* ---- A14: 4x4x4 ghost downsample of one fine halo block into a coarse
+ * block of bsxHi/2 edge (msbg.cpp:298-343, 2902-2974). */
+void or_ghost_downsample(const float *halo, int bsxHi, float *lo);
+/* prepareGhostBlocks (msbg.cpp:3059-3170) for FP32 channels on levels
+ * 0..nLevels-2, given the fine-coarse block lists per level. lvl[l] are the
+ * channel's levels; ghost pages are written into lvl[l+1]'s page map,
+ * which must already own a page for each fine-coarse block of level l
+ * that has BLK_EXISTS, and hold OR_EMPTY for the others. */
+void or_prepare_ghost_blocks(or_level *lvl, int nLevels, const int32_t *fc,
+                             const int32_t *fc_off, const uint16_t *flags);
+
+/* ---- misc ---- */
+uint64_t or_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,334 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Integer/index outputs (activation counts, levels, flags, colour lists, halo
+index tables, UNORM16 splat/finalize) must be bit-exact. The sweep in
+MSBG_ARITH_EXACT mode must be bit-exact for every storage; the fast mode is
+checked against the FP32 oracle with the tolerance stated in each test.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import liboracle as O
+from tests.helpers import RSCAN, dense_halo, edge_points, gpu_pipeline, oracle_pipeline, small_scene
+
+pytestmark = pytest.mark.gpu
+
+M = pytest.importorskip("msbg_amd")
+
+
+def note(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _scene(S, seed=7, nObj=5, nPts=3000):
+    return np.concatenate([small_scene(S, nObj=nObj, nPts=nPts, seed=seed), edge_points(S, 300, seed)])
+
+
+# --------------------------------------------------------------------------
+# block tables
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("S,bsx,nLevels", [(128, 16, 3), (128, 32, 4), (96, 8, 3), (64, 16, 2)])
+def test_tables_bitexact(S, bsx, nLevels):
+    note("tables", S, bsx, nLevels)
+    xyz = _scene(S)
+    o = oracle_pipeline(S, bsx, xyz, nLevels=nLevels, do_sweep=False)
+    gp = gpu_pipeline(S, bsx, xyz, nLevels=nLevels, do_sweep=False)
+    assert gp["nk"] == int(o["keep"].sum())
+    assert np.array_equal(gp["grid"].points_download(), o["xk"])
+    assert np.array_equal(gp["cnt"], o["cnt"])
+    assert np.array_equal(gp["lv0"], o["lv0"])
+    assert np.array_equal(gp["lv"], o["lv"])
+    assert np.array_equal(gp["blv"], o["blv"])
+    ofl = o["bfl"].copy()
+    # the demo's allocation loop then sets BLK_EXISTS for level-0 blocks
+    _, gfl = gp["grid"].getBlockInfo()
+    ofl = np.where(o["blv"] == 0, ofl | M._lib.BLK_EXISTS, ofl & ~M._lib.BLK_EXISTS).astype(np.uint16)
+    assert np.array_equal(gfl, ofl)
+    assert np.array_equal(gp["active"], o["active"])
+
+
+def test_regularize_multilevel_bitexact():
+    note("regularize nLevels=4")
+    nb = 16
+    rng = np.random.default_rng(3)
+    m = np.where(rng.random(nb ** 3) < 0.03, 0, 3).astype(np.int32)
+    m[rng.random(nb ** 3) < 0.02] = 2
+    with M.MultiresSparseGrid.create("r", nb * 16, nb * 16, nb * 16, 16, nLevels=4) as g:
+        got = g.regularizeRefinementMap(m)
+        want, rc = O.regularize(nb, nb, nb, 4, m)
+        assert np.array_equal(got, want)
+        flags_in = (rng.random(nb ** 3) < 0.5).astype(np.uint32) * (1 << 12) | 1
+        g.setRefinementMap(got, flags_in)
+        lv, fl = g.getBlockInfo()
+        wl, wf = O.block_flags(nb, nb, nb, 4, want, flags_in)
+        assert np.array_equal(lv, wl) and np.array_equal(fl, wf)
+        for level in range(4):
+            fc = g.blocksFineCoarse(level)
+            want_fc = np.nonzero((wl == level) & ((wf & (1 << 3)) != 0))[0]
+            assert np.array_equal(fc, want_fc)
+
+
+def test_halo_table_matches_pagemap():
+    note("halo table")
+    S, bsx = 128, 16
+    gp = gpu_pipeline(S, bsx, _scene(S), do_sweep=False)
+    g = gp["grid"]
+    pm, npg = g.pagemap(M.CH_UINT16_1)
+    tab = g.halo_table(M.CH_UINT16_1)
+    nb = S // bsx
+    pages_bid = np.nonzero(pm >= 0)[0]
+    assert np.array_equal(pm[pages_bid], np.arange(npg))
+    for p, bid in enumerate(pages_bid):
+        bz, rem = divmod(bid, nb * nb)
+        by, bx = divmod(rem, nb)
+        for k in range(27):
+            dx, dy, dz = k % 3 - 1, (k // 3) % 3 - 1, k // 9 - 1
+            x, y, z = bx + dx, by + dy, bz + dz
+            want = pm[(z * nb + y) * nb + x] if 0 <= min(x, y, z) and max(x, y, z) < nb else -4
+            assert tab[p, k] == want
+
+
+# --------------------------------------------------------------------------
+# splat / finalize
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("S,bsx", [(128, 16), (128, 32), (64, 8)])
+def test_splat_finalize_u16_bitexact(S, bsx):
+    note("splat", S, bsx)
+    if 2 * (RSCAN + 0.5) > bsx:
+        pytest.skip("footprint wider than a block")
+    xyz = _scene(S, seed=S + bsx, nObj=8, nPts=4000)
+    o = oracle_pipeline(S, bsx, xyz, do_sweep=False)
+    gp = gpu_pipeline(S, bsx, xyz, do_sweep=False)
+    assert np.array_equal(gp["splat"], o["splat"])
+
+
+def test_splat_abandon_quirk_gpu():
+    note("splat goto")
+    S, bsx = 32, 16
+    with M.MultiresSparseGrid.create("g", S, S, S, bsx, options=M.OPT_SINGLE_LEVEL, nLevels=2) as g:
+        kind = np.full(8, 1, np.uint8)
+        kind[0] = 3
+        g.channel_define(M.CH_UINT16_1, M.U16, 0, kind, 0.0, 65535.0, 65535.0)
+        pts = np.array([[14.5, 10.5, 10.5], [5.25, 5.5, 12.75], [9.5, 14.25, 2.0]], np.float32)
+        g.points_upload(pts)
+        g.splat(M.CH_UINT16_1, 2.0, 2.0)
+        got = g.download(M.CH_UINT16_1)
+        pm = np.full(8, O.EMPTY, np.int32)
+        pm[0] = 0
+        L = O.Level(S, S, S, bsx, pm, np.full((1, bsx, bsx, bsx), 65535, np.uint16), O.U16)
+        O.splat_u16(L, pts, 4.0)
+        assert np.array_equal(got, L.pages)
+
+
+@pytest.mark.parametrize("storage", [M.U16, M.F32, M.F16, M.BF16, M.F64])
+def test_finalize_exhaustive(storage):
+    note("finalize exhaustive", storage)
+    S, bsx = 64, 32  # 2 value pages = 65536 voxels
+    with M.MultiresSparseGrid.create("f", S, S, S, bsx, options=M.OPT_SINGLE_LEVEL, nLevels=2) as g:
+        kind = np.full(8, 1, np.uint8)
+        kind[:2] = 3
+        g.channel_define(M.CH_UINT16_1, storage, 0, kind, 0.0, None, 0.0)
+        codes = np.arange(65536, dtype=np.uint16)
+        if storage in (M.U16,):
+            g.upload(M.CH_UINT16_1, 0, codes.reshape(2, bsx, bsx, bsx))
+        elif storage in (M.F16, M.BF16):
+            g.upload(M.CH_UINT16_1, 0, codes.view(np.float16 if storage == M.F16 else np.uint16)
+                     .reshape(2, bsx, bsx, bsx))
+        elif storage == M.F32:
+            g.upload(M.CH_UINT16_1, 0, codes.astype(np.uint32).view(np.float32).reshape(2, bsx, bsx, bsx))
+        else:
+            g.upload(M.CH_UINT16_1, 0, codes.astype(np.uint64).view(np.float64).reshape(2, bsx, bsx, bsx))
+        g.finalize(M.CH_UINT16_1, 2.0, 2.0)
+        got = g.download(M.CH_UINT16_1).reshape(-1)
+        if storage == M.U16:
+            assert np.array_equal(got, O.finalize_table())
+        else:
+            f = O.finalize_density_table()
+            if storage == M.F32:
+                assert np.array_equal(got.view(np.uint32), f.view(np.uint32))
+            elif storage == M.F64:
+                assert np.array_equal(got, f.astype(np.float64))
+            elif storage == M.F16:
+                assert np.array_equal(got.view(np.uint16), f.astype(np.float16).view(np.uint16))
+            else:
+                Lb = O.load()
+                want = np.array([Lb.or_bf16_from_float(float(v)) for v in f], np.uint16)
+                assert np.array_equal(got.view(np.uint16), want)
+
+
+# --------------------------------------------------------------------------
+# sweep
+# --------------------------------------------------------------------------
+def _init_field(o, S, bsx, storage):
+    """The oracle's finalized UNORM16 field converted to `storage`."""
+    vals = o["splat"].astype(np.float32) * np.float32(1.0 / 65535.0)
+    if storage == O.U16:
+        return o["splat"].copy(), O.Level(S, S, S, bsx, o["pagemap"], o["splat"].copy(), O.U16)
+    from tests.helpers import to_storage_level
+    L = to_storage_level(S, bsx, o["pagemap"], vals, storage)
+    return L.pages.copy(), L
+
+
+def _np_view(pages, storage):
+    if storage == M.F16:
+        return pages.view(np.float16)
+    return pages
+
+
+@pytest.mark.parametrize("storage", [M.U16, M.F32, M.F16, M.BF16, M.F64])
+@pytest.mark.parametrize("S,bsx", [(128, 16), (128, 32), (64, 8)])
+def test_sweep_exact_bitexact(S, bsx, storage):
+    note("sweep exact", S, bsx, storage)
+    xyz = _scene(S, seed=11 + bsx, nObj=6, nPts=3000)
+    o = oracle_pipeline(S, bsx, xyz, do_sweep=False)
+    init, L = _init_field(o, S, bsx, storage)
+    res_o = O.mcf_sweeps(L, o["active"], o["blv"], 3, 0.1, 1, residual=True)
+    with M.MultiresSparseGrid.create("s", S, S, S, bsx, options=M.OPT_SINGLE_LEVEL, nLevels=3) as g:
+        g.set_arith(M.ARITH_EXACT)
+        g.setRefinementMap(o["lv"])
+        active = g.alloc_demo(M.CH_UINT16_1, storage)
+        assert np.array_equal(active, o["active"])
+        g.upload(M.CH_UINT16_1, 0, _np_view(init, storage))
+        res_g = g.applyChannelPdeFast(M.CH_UINT16_1, 0, 0, active, M.MCF_8COLOR, 1, 0, 0, 0, 3, 1.0,
+                                      0.1, residual=True)
+        got = g.download(M.CH_UINT16_1)
+        want = L.pages
+        if storage == M.F16:
+            got = got.view(np.uint16)
+        assert np.array_equal(got.view(np.uint8), want.view(np.uint8).reshape(got.view(np.uint8).shape))
+        assert res_g.shape == res_o.shape == (4,)
+        assert np.allclose(res_g, res_o, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("S,bsx", [(128, 16), (128, 32)])
+def test_sweep_fast_close_to_exact(S, bsx):
+    """FMA + reciprocal divide: within 2e-5 of the strict FP32 oracle on
+    all but a tiny fraction of voxels (threshold flips near |grad|^2=1e-7)."""
+    note("sweep fast", S, bsx)
+    xyz = _scene(S, seed=21, nObj=6, nPts=3000)
+    o = oracle_pipeline(S, bsx, xyz, do_sweep=False)
+    init, L = _init_field(o, S, bsx, O.F32)
+    O.mcf_sweeps(L, o["active"], o["blv"], 4, 0.1, 1)
+    with M.MultiresSparseGrid.create("s", S, S, S, bsx, options=M.OPT_SINGLE_LEVEL, nLevels=3) as g:
+        g.set_arith(M.ARITH_FAST)
+        g.setRefinementMap(o["lv"])
+        active = g.alloc_demo(M.CH_FLOAT_1, M.F32)
+        g.upload(M.CH_FLOAT_1, 0, init)
+        g.applyChannelPdeFast(M.CH_FLOAT_1, 0, 0, active, M.MCF_8COLOR, 1, 0, 0, 0, 4, 1.0, 0.1)
+        got = g.download(M.CH_FLOAT_1)
+    err = np.abs(got.astype(np.float64) - L.pages)
+    note("fast max err", err.max(), "frac>2e-5", (err > 2e-5).mean())
+    assert (err > 2e-5).mean() < 1e-4
+    assert np.sqrt((err ** 2).mean()) < 1e-6
+
+
+def test_sweep_skips_non_level0_and_rejects_bad_lists():
+    note("sweep list semantics")
+    S, bsx = 64, 16
+    xyz = _scene(S, seed=5, nObj=3)
+    gp = gpu_pipeline(S, bsx, xyz, do_sweep=False)
+    g = gp["grid"]
+    before = g.download(M.CH_UINT16_1)
+    lv, _ = g.getBlockInfo()
+    empty_bid = int(np.nonzero(lv != 0)[0][0])
+    with pytest.raises(M.MsbgError):
+        g.applyChannelPdeFast(M.CH_UINT16_1, 0, 0, np.array([empty_bid]), M.MCF_8COLOR, 1, 0, 0, 0,
+                              2, 1.0, 0.1)
+    with pytest.raises(M.MsbgError):
+        g.applyChannelPdeFast(M.CH_UINT16_1, 0, 0, None, -M.PDE_MEAN_CURVATURE, 1, 0, 0, 0, 2, 1.0, 0.1)
+    g.applyChannelPdeFast(M.CH_UINT16_1, 0, 0, np.zeros(0, np.int32), M.MCF_8COLOR, 1, 0, 0, 0, 2,
+                          1.0, 0.1)
+    assert np.array_equal(g.download(M.CH_UINT16_1), before)
+
+
+def test_sweep_deterministic_and_constant_field():
+    note("determinism")
+    S, bsx = 256, 32
+    xyz = _scene(S, seed=2, nObj=10, nPts=8000)
+    a = gpu_pipeline(S, bsx, xyz, storage=M.F16, numIter=6, arith=M.ARITH_FAST)["pages"]
+    b = gpu_pipeline(S, bsx, xyz, storage=M.F16, numIter=6, arith=M.ARITH_FAST)["pages"]
+    assert np.array_equal(a.view(np.uint16), b.view(np.uint16))
+    with M.MultiresSparseGrid.create("c", S, S, S, bsx, options=M.OPT_SINGLE_LEVEL, nLevels=2) as g:
+        kind = np.full(g.nBlocks(), 3, np.uint8)
+        g.channel_define(M.CH_FLOAT_1, M.F32, 0, kind, 0.0, 1.0, 0.375)
+        g.setRefinementMap(np.zeros(g.nBlocks(), np.int32))
+        g.applyChannelPdeFast(M.CH_FLOAT_1, 0, 0, None, M.MCF_8COLOR, 1, 0, 0, 0, 4, 1.0, 0.1)
+        assert np.all(g.download(M.CH_FLOAT_1) == np.float32(0.375))
+
+
+# --------------------------------------------------------------------------
+# halo fill (test hook) and cross-level ghosts
+# --------------------------------------------------------------------------
+def _multilevel_grid(S=64, bsx=16, seed=1, storage=M.F32):
+    nb = S // bsx
+    rng = np.random.default_rng(seed)
+    m = np.where(rng.random(nb ** 3) < 0.15, 0, 2).astype(np.int32)
+    g = M.MultiresSparseGrid.create("m", S, S, S, bsx, nLevels=3)
+    lv = g.regularizeRefinementMap(m)
+    g.setRefinementMap(lv)
+    g.set_exists(None)
+    levels, flags = g.getBlockInfo()
+    pages = []
+    pms = []
+    for level in range(3):
+        kind = np.where(levels == level, 3, 0).astype(np.uint8)
+        if level > 0:
+            kind[(levels < level)] = 0
+        g.channel_define(M.CH_FLOAT_1, storage, level, kind, 0.125, 0.875, 0.0)
+        pm, npg = g.pagemap(M.CH_FLOAT_1, level)
+        b = bsx >> level
+        data = rng.random((npg, b, b, b)).astype(np.float32)
+        g.upload(M.CH_FLOAT_1, level, data)
+        pages.append(data)
+        pms.append(pm)
+    return g, levels, flags, pms, pages
+
+
+def _olevel(S, bsx, level, pm, pages, storage=O.F32):
+    return O.Level(S >> level, S >> level, S >> level, bsx >> level, pm, pages, storage, 0.125, 0.875)
+
+
+@pytest.mark.parametrize("opt", [M.OPT_BC_NEUMANN, M.OPT_BC_COARSE_LEVEL,
+                                 M.OPT_BC_COARSE_LEVEL | M.OPT_INTERPOLATE, 0])
+def test_halo_fill_hook_bitexact(opt):
+    note("halo fill", opt)
+    S, bsx = 64, 16
+    g, levels, flags, pms, pages = _multilevel_grid(S, bsx)
+    with g:
+        for level in range(2):
+            L = _olevel(S, bsx, level, pms[level], pages[level])
+            Lo = _olevel(S, bsx, level + 1, pms[level + 1], pages[level + 1])
+            bids = np.nonzero(levels == level)[0][:12]
+            for bid in bids:
+                got = g.fillHaloBlock(M.CH_FLOAT_1, int(bid), 1, opt)
+                want = O.halo_fill(L, int(bid), 1, opt, lo=Lo)
+                assert np.array_equal(got, want), (level, bid)
+                if opt == M.OPT_BC_NEUMANN:
+                    dense = dense_halo(pms[level], pages[level], S >> level, bsx >> level, int(bid),
+                                       0.125, 0.875)
+                    assert np.array_equal(got, dense)
+
+
+def test_prepare_ghost_blocks_bitexact():
+    note("ghost blocks")
+    S, bsx = 64, 16
+    g, levels, flags, pms, pages = _multilevel_grid(S, bsx, seed=4)
+    with g:
+        g.prepareGhostBlocks(M.CH_FLOAT_1)
+        _, fl = g.getBlockInfo()
+        post_pm = [g.pagemap(M.CH_FLOAT_1, l)[0] for l in range(3)]
+        post = [g.download(M.CH_FLOAT_1, l) for l in range(3)]
+        olv = []
+        for l in range(3):
+            b = bsx >> l
+            pg = np.zeros((int((post_pm[l] >= 0).sum()), b, b, b), np.float32)
+            pg[:pages[l].shape[0]] = pages[l]
+            olv.append(_olevel(S, bsx, l, post_pm[l], pg))
+        fcs = [g.blocksFineCoarse(l) for l in range(3)]
+        O.prepare_ghost_blocks(olv, 3, fcs, fl)
+        for l in range(3):
+            assert np.array_equal(post[l], olv[l].pages), l
+        assert sum(len(f) for f in fcs) > 0

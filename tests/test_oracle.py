@@ -1,0 +1,467 @@
+"""Pins of the CPU oracle against closed forms, brute force and library
+routines (no GPU). The reference ships no golden vectors for this path, so
+every oracle function is checked here against something that does not
+restate its formula.
+"""
+import numpy as np
+import pytest
+import scipy.ndimage as ndi
+
+from oracle import liboracle as O
+from tests.helpers import dense_halo, small_scene
+
+L = O.load()
+
+
+# --------------------------------------------------------------------------
+# conversions (sbg.h:225-366)
+# --------------------------------------------------------------------------
+def test_u16_roundings_differ_only_at_even_ties():
+    """The splat/finalize path rounds half away from zero (roundf), the sweep
+    store rounds half to even (cvtps2dq): find exact .5 products and check
+    both modes."""
+    seen_even = 0
+    for k in range(0, 4000):
+        f = np.float32(k + 0.5) / np.float32(65535)
+        if np.float32(f) * np.float32(65535) != np.float32(k + 0.5):
+            continue
+        assert L.or_u16_from_float_roundf(float(f)) == k + 1
+        assert L.or_u16_from_float_rne(float(f)) == (k if k % 2 == 0 else k + 1)
+        seen_even += k % 2 == 0
+    assert seen_even > 10
+    assert L.or_u16_to_float(0) == 0.0
+    assert abs(L.or_u16_to_float(65535) - 1.0) <= 2 ** -23
+
+
+def test_f16_matches_numpy_rne():
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(-2, 2, 20000), rng.uniform(0, 1, 20000),
+                        np.float32(2.0) ** rng.uniform(-30, 17, 5000) * rng.choice([-1, 1], 5000),
+                        [0.0, -0.0, 65504.0, 65520.0, 65519.99, 6e-8, 3e-8, 2.98e-8]]
+                       ).astype(np.float32)
+    ref = x.astype(np.float16).view(np.uint16)
+    got = np.array([L.or_f16_from_float(float(v)) for v in x], np.uint16)
+    assert np.array_equal(got, ref)
+    allh = np.arange(65536, dtype=np.uint16)
+    finite = np.isfinite(allh.view(np.float16))
+    dec = np.array([L.or_f16_to_float(int(h)) for h in allh[finite]], np.float32)
+    assert np.array_equal(dec, allh[finite].view(np.float16).astype(np.float32))
+
+
+def test_bf16_matches_torch_rne():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.uniform(-2, 2, 20000), rng.uniform(0, 1, 20000)]).astype(np.float32)
+    ref = torch.from_numpy(x).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    got = np.array([L.or_bf16_from_float(float(v)) for v in x], np.uint16)
+    assert np.array_equal(got, ref)
+
+
+# --------------------------------------------------------------------------
+# A1 domain bounds (sbg.cpp:262-316)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("S,beps", [(512, 1e-4), (4096, 2e-4), (64, 1e-4), (2048, 1e-4)])
+def test_domain_bounds_closed_form(S, beps):
+    lo, hi = O.domain_bounds(S, S, S)
+    b = np.float32(beps)
+    # beps doubles until S - beps rounds below S in float32
+    assert np.all(lo == np.float32(0) + b)
+    assert np.all(hi == np.float32(S) - b)
+    assert np.all(hi < S) and np.all(hi == np.float32(S) - b)
+
+
+def test_domain_filter_truncation():
+    S = 64
+    lo, hi = O.domain_bounds(S, S, S)
+    p = np.array([[lo[0], 1, 1], [np.nextafter(lo[0], 0), 1, 1], [hi[0], 1, 1],
+                  [np.nextafter(hi[0], 100), 1, 1], [np.nan, 1, 1], [63.999, 0.5, 1]], np.float32)
+    keep = O.domain_filter(S, S, S, p)
+    assert keep.tolist() == [1, 0, 1, 0, 0, 1]
+
+
+# --------------------------------------------------------------------------
+# A2 activation (msbg_demo.cpp:368-389)
+# --------------------------------------------------------------------------
+def test_activation_hand_kat():
+    # bpos = p/16 +- 0.25 truncated toward zero, clamped to [0, nb-1]
+    p = np.array([[15.9, 0.2, 31.99]], np.float32)
+    cnt, home = O.activate(16, 2, 4, 3, p, 4.0)
+    c = cnt.reshape(3, 4, 2)  # z, y, x
+    want = np.zeros((3, 4, 2), np.uint32)
+    want[1:3, 0, 0:2] = 1  # x: trunc(0.74375)..trunc(1.24375) = 0..1; y: 0..0; z: 1..2
+    assert np.array_equal(c, want)
+    assert home[0] == (1 * 4 + 0) * 2 + 0  # (15>>4, 0>>4, 31>>4) = (0, 0, 1)
+
+
+def test_activation_covers_footprint():
+    """Invariant: every in-sphere voxel of a point's splat footprint lies in
+    a block activated by that point (same rScan)."""
+    S, bsx = 64, 16
+    nb = S // bsx
+    rng = np.random.default_rng(5)
+    pts = rng.uniform(0.01, S - 0.01, size=(300, 3)).astype(np.float32)
+    for p in pts:
+        cnt, _ = O.activate(bsx, nb, nb, nb, p[None], 4.0)
+        act = cnt.reshape(nb, nb, nb) > 0
+        lo = np.maximum(np.ceil(p - 4.5).astype(int), 0)
+        hi = np.minimum(np.floor(p + 3.5).astype(int), S - 1)
+        zz, yy, xx = np.meshgrid(*[np.arange(lo[k], hi[k] + 1) for k in (2, 1, 0)], indexing="ij")
+        d2 = (xx + 0.5 - p[0]) ** 2 + (yy + 0.5 - p[1]) ** 2 + (zz + 0.5 - p[2]) ** 2
+        m = d2 <= 16 - 1e-3
+        assert act[zz[m] // bsx, yy[m] // bsx, xx[m] // bsx].all()
+
+
+# --------------------------------------------------------------------------
+# A4 regularizeRefinementMap (msbg.cpp:1202-1423)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("nLevels", [3, 4, 5])
+def test_regularize_is_clamped_chebyshev_distance(nLevels):
+    """For a {0, nLevels-1} input map, pass ll sets blocks at Chebyshev
+    distance ll from level 0 to ll, so the result is
+    min(nLevels-1, chessboard distance to the nearest level-0 block)."""
+    rng = np.random.default_rng(nLevels)
+    nb = 12
+    m = np.where(rng.random((nb, nb, nb)) < 0.02, 0, nLevels - 1).astype(np.int32)
+    out, rc = O.regularize(nb, nb, nb, nLevels, m.ravel())
+    assert rc == 0
+    dist = ndi.distance_transform_cdt(m != 0, metric="chessboard")
+    want = np.minimum(dist, nLevels - 1)
+    assert np.array_equal(out.reshape(nb, nb, nb), want)
+    out2, _ = O.regularize(nb, nb, nb, nLevels, out)
+    assert np.array_equal(out2, out)  # idempotent
+
+
+def test_regularize_two_levels_untouched():
+    m = np.array([0, 1, 1, 0, 1, 1, 1, 1], np.int32)
+    out, rc = O.regularize(2, 2, 2, 2, m)
+    assert rc == 0 and np.array_equal(out, m)
+
+
+# --------------------------------------------------------------------------
+# A5 block flags (msbg.cpp:1649-1745)
+# --------------------------------------------------------------------------
+def test_block_flags_vs_filters():
+    rng = np.random.default_rng(2)
+    nb = 10
+    m = np.where(rng.random((nb, nb, nb)) < 0.05, 0, 2).astype(np.int32)
+    lv, _ = O.regularize(nb, nb, nb, 3, m.ravel())
+    lv = lv.reshape(nb, nb, nb)
+    inflags = (rng.random(nb ** 3) < 0.3).astype(np.uint32) << 12
+    blv, bfl = O.block_flags(nb, nb, nb, 3, lv.ravel(), inflags)
+    fp = np.ones((3, 3, 3), bool)
+    fp[1, 1, 1] = False
+    nmax = ndi.maximum_filter(lv, footprint=fp, mode="constant", cval=-1)
+    nmin = ndi.minimum_filter(lv, footprint=fp, mode="constant", cval=99)
+    border = np.zeros((nb, nb, nb), bool)
+    border[[0, -1], :, :] = border[:, [0, -1], :] = border[:, :, [0, -1]] = True
+    fc = nmax > lv
+    cf = nmin < lv
+    ex = (inflags.reshape(nb, nb, nb) & (1 << 12)) != 0
+    want = (ex << 12) | (border << 1) | (cf << 2) | (fc << 3) | ((~ex & ~fc & ~cf) << 13)
+    assert np.array_equal(bfl.reshape(nb, nb, nb), want.astype(np.uint16))
+    assert np.array_equal(blv.reshape(nb, nb, nb), lv)
+
+
+# --------------------------------------------------------------------------
+# A7 colours (msbg.h:642-646)
+# --------------------------------------------------------------------------
+def test_colours_disjoint_halos():
+    nb = 6
+    b = np.arange(nb ** 3, dtype=np.int32)
+    out, off = O.color_lists(nb, nb, b)
+    assert off[-1] == nb ** 3
+    for c in range(8):
+        blk = out[off[c]:off[c + 1]]
+        assert np.all(np.diff(blk) > 0)  # list order kept (ascending)
+        z, rem = np.divmod(blk, nb * nb)
+        y, x = np.divmod(rem, nb)
+        assert np.all((x & 1) + 2 * (y & 1) + 4 * (z & 1) == c)
+        # same-colour blocks never share a face, edge or corner
+        pts = np.stack([x, y, z], 1)
+        d = np.abs(pts[:, None, :] - pts[None, :, :]).max(-1)
+        np.fill_diagonal(d, 99)
+        assert d.min() >= 2
+
+
+# --------------------------------------------------------------------------
+# A8 splat (msbg_demo.cpp:541-636)
+# --------------------------------------------------------------------------
+def _single_block_level(S, bsx, code=0):
+    nb = S // bsx
+    pm = np.full(nb ** 3, code, np.int32)
+    npg = 0
+    if code == 0:
+        pm = np.arange(nb ** 3, dtype=np.int32)
+        npg = nb ** 3
+    pages = np.full((max(npg, 1), bsx, bsx, bsx), 65535, np.uint16)
+    return O.Level(S, S, S, bsx, pm, pages, O.U16)
+
+
+def _dense(level, S):
+    bsx = level.bsx
+    nb = S // bsx
+    d = np.zeros((S, S, S), np.uint16)
+    for bid in range(nb ** 3):
+        bz, rem = divmod(bid, nb * nb)
+        by, bx = divmod(rem, nb)
+        c = level.page[bid]
+        d[bz * bsx:(bz + 1) * bsx, by * bsx:(by + 1) * bsx, bx * bsx:(bx + 1) * bsx] = (
+            level.pages[c] if c >= 0 else 0)
+    return d
+
+
+@pytest.mark.parametrize("p", [(10.25, 11.5, 9.75), (16.0, 16.0, 16.0), (3.125, 28.5, 17.375)])
+def test_splat_single_point_closed_form(p):
+    """Dyadic positions make every FP32 step exact, so the code equals the
+    float64 closed form roundf(d^2/16*65535) voxel by voxel."""
+    S, bsx = 32, 16
+    lvl = _single_block_level(S, bsx)
+    pt = np.array([p], np.float32)
+    O.splat_u16(lvl, pt, 4.0)
+    d = _dense(lvl, S)
+    z, y, x = np.meshgrid(np.arange(S), np.arange(S), np.arange(S), indexing="ij")
+    d2 = (x + 0.5 - p[0]) ** 2 + (y + 0.5 - p[1]) ** 2 + (z + 0.5 - p[2]) ** 2
+    want = np.where(d2 <= 16, np.floor(d2 / 16 * 65535 + 0.5), 65535).astype(np.uint16)
+    assert np.array_equal(d, want)
+
+
+def test_splat_abandons_at_first_non_value_block():
+    """goto loop_exit (msbg_demo.cpp:615-632): the first in-sphere voxel (z,y,x
+    order) in a non-value block ends the particle; earlier voxels keep
+    their update, later ones in value blocks are not touched."""
+    S, bsx = 32, 16
+    nb = 2
+    pm = np.full(nb ** 3, O.EMPTY, np.int32)
+    pm[0] = 0  # only block (0,0,0) holds values
+    lvl = O.Level(S, S, S, bsx, pm, np.full((1, bsx, bsx, bsx), 65535, np.uint16), O.U16)
+    p = np.array([[14.5, 10.5, 10.5]], np.float32)  # footprint x in 10..17 crosses into bx=1
+    O.splat_u16(lvl, p, 4.0)
+    got = lvl.pages[0]
+    # walk the footprint in reference order and stop at the first voxel in bx=1
+    want = np.full((bsx, bsx, bsx), 65535, np.uint16)
+    lo = np.maximum(np.ceil(p[0] - 4.5).astype(int), 0)
+    hi = np.minimum(np.floor(p[0] + 3.5).astype(int), S - 1)
+    done = False
+    for iz in range(lo[2], hi[2] + 1):
+        for iy in range(lo[1], hi[1] + 1):
+            for ix in range(lo[0], hi[0] + 1):
+                d2 = (ix + 0.5 - p[0, 0]) ** 2 + (iy + 0.5 - p[0, 1]) ** 2 + (iz + 0.5 - p[0, 2]) ** 2
+                if d2 > 16:
+                    continue
+                if ix >= 16:
+                    done = True
+                    break
+                want[iz, iy, ix] = min(want[iz, iy, ix], int(np.floor(d2 / 16 * 65535 + 0.5)))
+            if done:
+                break
+        if done:
+            break
+    assert np.array_equal(got, want)
+    assert (got < 65535).sum() > 0 and done
+
+
+def test_splat_order_independent():
+    S, bsx = 64, 16
+    pts = small_scene(S, nObj=3, nPts=2000, seed=11)
+    keep = O.domain_filter(S, S, S, pts)
+    pts = pts[keep == 1]
+    a = _single_block_level(S, bsx)
+    b = _single_block_level(S, bsx)
+    O.splat_u16(a, pts, 4.0)
+    O.splat_u16(b, pts[np.random.default_rng(0).permutation(len(pts))], 4.0)
+    assert np.array_equal(a.pages, b.pages)
+
+
+# --------------------------------------------------------------------------
+# A9 finalize (msbg_demo.cpp:655-680), exhaustive
+# --------------------------------------------------------------------------
+def test_finalize_table_closed_form():
+    t = O.finalize_table(2.0, 2.0).astype(np.int64)
+    u = np.arange(65536)
+    q = u / 65535.0
+    d = np.sqrt(q * 16) - 2
+    f = 1 - np.clip((d + 2) / 6, 0, 1)
+    want = np.where(q < 0.999, np.floor(f * 65535 + 0.5), 0)
+    assert np.abs(t - want).max() <= 1
+    assert t[0] == 65535 and t[65535] == 0
+    # q >= 0.999 gives 0; the cut sits where u/65535 crosses 0.999 in FP32
+    cut = np.nonzero((u.astype(np.float32) * np.float32(1 / 65535)) >= np.float32(0.999))[0][0]
+    assert np.all(t[cut:] == 0) and t[cut - 1] > 0
+
+
+# --------------------------------------------------------------------------
+# A11 halo (halo.cpp:267-479): tag test against a dense embedding
+# --------------------------------------------------------------------------
+def _random_grid(S, bsx, seed, storage=O.F32):
+    rng = np.random.default_rng(seed)
+    nb = S // bsx
+    kinds = rng.choice([O.NULL, O.EMPTY, O.FULL, 0], size=nb ** 3, p=[0.25, 0.15, 0.1, 0.5])
+    pm = np.full(nb ** 3, O.NULL, np.int32)
+    pm[kinds == O.EMPTY] = O.EMPTY
+    pm[kinds == O.FULL] = O.FULL
+    vb = np.nonzero(kinds == 0)[0]
+    pm[vb] = np.arange(vb.size)
+    # tag each voxel with its global linear index (exact in float32 < 2^24)
+    pages = np.zeros((vb.size, bsx, bsx, bsx), np.float32)
+    for i, bid in enumerate(vb):
+        bz, rem = divmod(bid, nb * nb)
+        by, bx = divmod(rem, nb)
+        z, y, x = np.meshgrid(*[np.arange(bsx)] * 3, indexing="ij")
+        pages[i] = ((bz * bsx + z) * S + by * bsx + y) * S + bx * bsx + x + 7
+    return O.Level(S, S, S, bsx, pm, pages, O.F32, 0.5, 0.25), vb
+
+
+def test_halo_neumann_tag_dense_embedding():
+    S, bsx = 32, 8
+    lvl, vb = _random_grid(S, bsx, 4)
+    for bid in vb:
+        h = O.halo_fill(lvl, int(bid), 1, O.OPT_BC_NEUMANN)
+        want = dense_halo(lvl.page, lvl.pages, S, bsx, int(bid), empty=0.5, full=0.25)
+        assert np.array_equal(h, want), bid
+
+
+# --------------------------------------------------------------------------
+# A12 MCF voxel closed forms (msbg4.cpp:559-637)
+# --------------------------------------------------------------------------
+def _halo_of(fn, n=5):
+    z, y, x = np.meshgrid(*[np.arange(n, dtype=np.float64)] * 3, indexing="ij")
+    return fn(x, y, z).astype(np.float32)
+
+
+def _centre(n=5):
+    c = n // 2
+    return (c * n + c) * n + c
+
+
+def test_mcf_constant_and_linear_fields_unchanged():
+    for fn in [lambda x, y, z: 0 * x + 0.375, lambda x, y, z: 0.5 + 0.01 * x - 0.02 * y + 0.005 * z]:
+        h = _halo_of(fn)
+        assert O.mcf_voxel(h, _centre(), 0.1) == h.ravel()[_centre()]
+
+
+def test_mcf_quadratic_radial_exact():
+    """phi = s*r^2 + c: central differences are exact for quadratics and
+    H/|grad|^2 = |grad phi| * (2/r) = 4s, so the update is dt*4s."""
+    s, c = 0.001, 0.25
+    cx, cy, cz = 0.5, -1.25, 0.75
+    h = _halo_of(lambda x, y, z: s * ((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2) + c)
+    f0 = h.ravel()[_centre()]
+    got = O.mcf_voxel(h, _centre(), 0.1)
+    assert abs((got - f0) - 0.1 * 4 * s) < 2e-6
+
+
+@pytest.mark.parametrize("axes", [(0, 1), (0, 2), (1, 2)])
+def test_mcf_cross_terms(axes):
+    """phi = a + s*u*v (u, v two axes): H/|grad|^2 = -2 s^2 u v / (s^2 (u^2+v^2))
+    * s = -2 s u v/(u^2+v^2), exact for the quadratic."""
+    s, a = 0.01, 0.5
+    off = np.array([3.0, -2.0, 5.0])
+
+    def fn(x, y, z):
+        g = [x + off[0], y + off[1], z + off[2]]
+        return a + s * g[axes[0]] * g[axes[1]]
+
+    h = _halo_of(fn)
+    u, v = off[axes[0]] + 2, off[axes[1]] + 2
+    want = 0.1 * (-2 * s * u * v / (u * u + v * v))
+    got = O.mcf_voxel(h, _centre(), 0.1) - h.ravel()[_centre()]
+    assert abs(got - want) < 2e-6
+
+
+def test_mcf_threshold_limiter_clamp():
+    # |grad|^2 <= 1e-7 -> no update
+    h = _halo_of(lambda x, y, z: 0.5 + 1e-4 * x * x)
+    assert O.mcf_voxel(h, _centre(), 0.1) == h.ravel()[_centre()]
+    # limiter: |D| <= 0.1 and the clamp keeps [0,1]
+    h = _halo_of(lambda x, y, z: np.where((x == 2) & (y == 2) & (z == 2), 0.95, 0.0) + 0.001 * x)
+    got = O.mcf_voxel(h, _centre(), 10.0)
+    assert abs(got - 0.95) <= 0.1 + 1e-7
+    h2 = _halo_of(lambda x, y, z: 0.02 + 0.001 * x * x + 0.0 * y)
+    assert O.mcf_voxel(h2, _centre(), -100.0, 1) >= 0.0
+
+
+def test_mcf_axis_permutation_symmetry():
+    rng = np.random.default_rng(9)
+    h = rng.random((5, 5, 5)).astype(np.float32)
+    a = O.mcf_voxel(h, _centre(), 0.1)
+    for perm in [(0, 2, 1), (1, 0, 2), (2, 1, 0)]:
+        b = O.mcf_voxel(np.ascontiguousarray(h.transpose(perm)), _centre(), 0.1)
+        assert abs(a - b) <= 4e-7
+
+
+# --------------------------------------------------------------------------
+# A10 sweep driver vs a dense numpy Gauss-Seidel-across-colours restatement
+# --------------------------------------------------------------------------
+def _np_mcf(h, dt):
+    """Vectorised MCF of all interior voxels of a halo block, float32,
+    reference operation order."""
+    f = lambda dx, dy, dz: h[1 + dz:h.shape[0] - 1 + dz, 1 + dy:h.shape[1] - 1 + dy,
+                             1 + dx:h.shape[2] - 1 + dx]
+    hf = np.float32(0.5)
+    f0, f1, f2, f3, f4, f5, f6 = (f(0, 0, 0), f(-1, 0, 0), f(1, 0, 0), f(0, -1, 0), f(0, 1, 0),
+                                  f(0, 0, -1), f(0, 0, 1))
+    fx, fy, fz = hf * (f2 - f1), hf * (f4 - f3), hf * (f6 - f5)
+    two = np.float32(2)
+    fxx, fyy, fzz = f1 + f2 - two * f0, f3 + f4 - two * f0, f5 + f6 - two * f0
+    fyx = ((f(1, 1, 0) - f(1, -1, 0)) * hf - (f(-1, 1, 0) - f(-1, -1, 0)) * hf) * hf
+    fzx = ((f(1, 0, 1) - f(1, 0, -1)) * hf - (f(-1, 0, 1) - f(-1, 0, -1)) * hf) * hf
+    fzy = ((f(0, 1, 1) - f(0, 1, -1)) * hf - (f(0, -1, 1) - f(0, -1, -1)) * hf) * hf
+    H = ((fy * fy + fz * fz) * fxx + (fx * fx + fz * fz) * fyy + (fx * fx + fy * fy) * fzz
+         - two * (fx * fy * fyx + fx * fz * fzx + fy * fz * fzy))
+    g = fx * fx + fy * fy + fz * fz
+    with np.errstate(divide="ignore", invalid="ignore"):
+        H = np.where(g > np.float32(1e-7), H / g, np.float32(0))
+    D = np.float32(dt) * H
+    D = np.where(D > np.float32(-0.1), D, np.float32(-0.1))
+    D = np.where(D < np.float32(0.1), D, np.float32(0.1))
+    p = f0 + D
+    p = np.where(p > 0, p, np.float32(0))
+    return np.where(p < 1, p, np.float32(1)).astype(np.float32)
+
+
+def test_sweep_driver_matches_dense_colour_restatement():
+    S, bsx = 32, 8
+    nb = S // bsx
+    rng = np.random.default_rng(3)
+    kinds = rng.random(nb ** 3)
+    pm = np.full(nb ** 3, O.EMPTY, np.int32)
+    pm[kinds < 0.15] = O.NULL
+    vb = np.nonzero(kinds >= 0.4)[0].astype(np.int32)
+    pm[vb] = np.arange(vb.size)
+    z, y, x = np.meshgrid(*[np.arange(S)] * 3, indexing="ij")
+    field = (0.5 + 0.45 * np.sin(x / 5.0) * np.cos(y / 7.0 + z / 9.0)).astype(np.float32)
+    pages = np.zeros((vb.size, bsx, bsx, bsx), np.float32)
+    for i, bid in enumerate(vb):
+        bz, rem = divmod(bid, nb * nb)
+        by, bx = divmod(rem, nb)
+        pages[i] = field[bz * bsx:(bz + 1) * bsx, by * bsx:(by + 1) * bsx, bx * bsx:(bx + 1) * bsx]
+    levels = np.zeros(nb ** 3, np.uint16)
+    lvl = O.Level(S, S, S, bsx, pm, pages.copy(), O.F32)
+    n = O.mcf_sweeps(lvl, vb, levels, 3, 0.1, 1, residual=True)
+    assert n.size == 4  # numIter 3 rounds up to 4 sweeps
+    ref = pages.copy()
+    for _ in range(4):
+        for c in range(8):
+            for bid in vb:
+                bz, rem = divmod(int(bid), nb * nb)
+                by, bx = divmod(rem, nb)
+                if (bx & 1) + 2 * (by & 1) + 4 * (bz & 1) != c:
+                    continue
+                h = dense_halo(pm, ref, S, bsx, int(bid))
+                ref[pm[bid]] = _np_mcf(h, 0.1)
+    assert np.array_equal(lvl.pages, ref)
+
+
+def test_ghost_downsample_closed_forms():
+    bsx = 8
+    z, y, x = np.meshgrid(*[np.arange(bsx + 2, dtype=np.float64)] * 3, indexing="ij")
+    const = np.full((bsx + 2,) * 3, 0.625, np.float32)
+    assert np.all(O.ghost_downsample(const, bsx) == np.float32(0.625))
+    a, b, c, d = 1 / 64, -1 / 128, 1 / 32, 0.25
+    lin = (a * x + b * y + c * z + d).astype(np.float32)
+    lo = O.ghost_downsample(lin, bsx)
+    v = np.arange(bsx // 2)
+    cz, cy, cx = np.meshgrid(v, v, v, indexing="ij")
+    # window of halo indices 2v..2v+3 is centred at halo coordinate 2v+1.5
+    want = a * (2 * cx + 1.5) + b * (2 * cy + 1.5) + c * (2 * cz + 1.5) + d
+    assert np.abs(lo - want).max() < 1e-6
