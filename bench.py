@@ -166,6 +166,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calib", action="store_true",
+                    help="more copy-kernel launches for profiler byte-counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
@@ -252,6 +254,11 @@ def main():
         "cpu_baseline": cpu,
     }
     g.destroy()
+    # measured copy bandwidth on this box (1 GiB read + 1 GiB written per
+    # launch): the reference for the profiler's byte counters
+    copy_gbs = M.copy_bandwidth(1 << 30, 5 if args.calib else 3)
+    out["roofline"]["copy_peak_measured"] = copy_gbs
+    out["roofline"]["frac_of_copy"] = achieved / copy_gbs
     print(json.dumps(out))
 
 

@@ -1,0 +1,178 @@
+// msbg.hpp -- header-only C++ facade over the C ABI of libmsbg_gpu.so with
+// the reference's MultiresSparseGrid names for the sparse-block stencil path.
+//
+// A caller of the reference's
+//   MSBG::MultiresSparseGrid::create(...), regularizeRefinementMap(...),
+//   setRefinementMap(...), applyChannelPdeFast<T>(...), prepareGhostBlocks(...)
+// (msbg.h:834-3907; paths relative to the reference's src/ directory) swaps
+// the include and keeps the call sites. Errors throw msbg::Error carrying the
+// C ABI's code and message (the reference's rcThis/raiseRc pattern,
+// globdef.h:216, becomes an exception here).
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../msbg_gpu.h"
+
+namespace MSBG {
+
+enum : unsigned {
+  OPT_INTERPOLATE = MSBG_OPT_INTERPOLATE,
+  OPT_BC_NEUMANN = MSBG_OPT_BC_NEUMANN,
+  OPT_SINGLE_LEVEL = MSBG_OPT_SINGLE_LEVEL,
+  OPT_BC_COARSE_LEVEL = MSBG_OPT_BC_COARSE_LEVEL,
+  OPT_8_COLOR_SCHEME = MSBG_OPT_8_COLOR_SCHEME,
+};
+enum { PDE_MEAN_CURVATURE = MSBG_PDE_MEAN_CURVATURE };
+enum { CH_NULL = MSBG_CH_NULL, CH_FLOAT_1 = MSBG_CH_FLOAT_1, CH_UINT16_1 = MSBG_CH_UINT16_1 };
+
+struct BlockInfo {  // msbg.h:673-678
+  uint16_t level, flags;
+};
+
+class Error : public std::runtime_error {
+ public:
+  Error(int code, const std::string &msg) : std::runtime_error(msg), code(code) {}
+  int code;
+};
+
+inline void check(int rc) {
+  if (rc) throw Error(rc, msbg_last_error());
+}
+
+template <typename T>
+struct StorageOf;
+template <>
+struct StorageOf<uint16_t> {
+  static constexpr int value = MSBG_STORAGE_U16;
+};
+template <>
+struct StorageOf<float> {
+  static constexpr int value = MSBG_STORAGE_F32;
+};
+template <>
+struct StorageOf<double> {
+  static constexpr int value = MSBG_STORAGE_F64;
+};
+
+// getBlockColor8 (msbg.h:642-646)
+static inline int getBlockColor8(int bx, int by, int bz) {
+  return (bx & 1) + ((by & 1) << 1) + ((bz & 1) << 2);
+}
+
+class MultiresSparseGrid {
+ public:
+  // MultiresSparseGrid::create (msbg.h:834-850). dTransRes, maxSizeMB and
+  // initialSizeMB do not change this path; the solver configs are unused.
+  static MultiresSparseGrid *create(const char *name, int sx, int sy, int sz, int blockSize0 = 16,
+                                    int /*dTransRes*/ = -1, int /*maxSizeMB*/ = 0,
+                                    int /*initialSizeMB*/ = -1, unsigned options = 0,
+                                    void * /*miscConf*/ = nullptr, void * /*pConf*/ = nullptr,
+                                    int nLevels = -1) {
+    msbg_grid *g = nullptr;
+    check(msbg_create(name, sx, sy, sz, blockSize0, options, nLevels, &g));
+    return new MultiresSparseGrid(g);
+  }
+  // MultiresSparseGrid::destroy (msbg.h:851-852)
+  static void destroy(MultiresSparseGrid *&m) {
+    delete m;
+    m = nullptr;
+  }
+  ~MultiresSparseGrid() { msbg_destroy(g_); }
+  MultiresSparseGrid(const MultiresSparseGrid &) = delete;
+  MultiresSparseGrid &operator=(const MultiresSparseGrid &) = delete;
+
+  int nBlocks() const { return (int)nblocks_; }
+  int getNumLevels() const { return dims_[7]; }
+  int nbx() const { return dims_[4]; }
+  int nby() const { return dims_[5]; }
+  int nbz() const { return dims_[6]; }
+  int bsx0() const { return dims_[3]; }
+  msbg_grid *handle() const { return g_; }
+
+  void regularizeRefinementMap(int *blockmapIn, int *blockmapOut = nullptr,
+                               unsigned * /*blockFlags*/ = nullptr, int /*rbzWidthBlk*/ = 0) {
+    check(msbg_regularize_refinement_map(g_, blockmapIn, blockmapOut ? blockmapOut : blockmapIn));
+  }
+  void setRefinementMap(int *blockLevels, unsigned *blockFlags = nullptr, int /*levelMgMax*/ = -1,
+                        MultiresSparseGrid * /*msbgObstacles*/ = nullptr,
+                        bool /*initCellFlags*/ = true, bool /*resetChannels*/ = true) {
+    check(msbg_set_refinement_map(g_, blockLevels, reinterpret_cast<const uint32_t *>(blockFlags)));
+    refreshBlockInfo();
+  }
+  BlockInfo *getBlockInfo(int bid, int /*levelMg*/ = 0) { return &info_[bid]; }
+
+  void prepareGhostBlocks(int chan) { check(msbg_prepare_ghost_blocks(g_, chan)); }
+
+  // applyChannelPdeFast<T> (msbg.h:2150-2172); laplTyp must be
+  // -(PDE_MEAN_CURVATURE + OPT_8_COLOR_SCHEME).
+  template <typename Data_T>
+  void applyChannelPdeFast(int chPhi, int chTmp, int chTmp2, std::vector<int> *blockList,
+                           int laplTyp, int doConstrZeroOne, float densThr, float densThrTol1,
+                           float densThrTol2, int numIter, float gridSpacing = 1,
+                           float timeStep = 0.025f, float /*velMagThr*/ = 0,
+                           MultiresSparseGrid * /*msbgVel*/ = nullptr, float /*lambda*/ = 0,
+                           float /*lambdaSDF*/ = 0, float /*taubin*/ = 0, float /*redist*/ = 0,
+                           float /*anisDiffAlpha*/ = 0, int /*veldep_on*/ = 0,
+                           float * /*veldep_fit*/ = nullptr, int /*visOpt*/ = 0,
+                           double *residual = nullptr) {
+    (void)StorageOf<Data_T>::value;
+    check(msbg_apply_channel_pde_fast(
+        g_, chPhi, chTmp, chTmp2, blockList ? blockList->data() : nullptr,
+        blockList ? (int64_t)blockList->size() : 0, laplTyp, doConstrZeroOne, densThr,
+        densThrTol1, densThrTol2, numIter, gridSpacing, timeStep, residual));
+  }
+
+  // ---- demo phases (msbg_demo.cpp:339-680) ----
+  int64_t uploadParticles(const float *xyz, int64_t n) {
+    int64_t kept = 0;
+    check(msbg_points_upload(g_, xyz, n, &kept));
+    return kept;
+  }
+  std::vector<uint32_t> activateBlocks(float rScan) {
+    std::vector<uint32_t> c((size_t)nblocks_);
+    check(msbg_activate_from_points(g_, rScan, c.data(), nullptr));
+    return c;
+  }
+  template <typename Data_T>
+  std::vector<int> allocateLevel0Blocks(int chan) {
+    std::vector<int> active((size_t)nblocks_);
+    int64_t n = 0;
+    check(msbg_channel_alloc_demo(g_, chan, StorageOf<Data_T>::value, active.data(),
+                                  (int64_t)active.size(), &n));
+    active.resize((size_t)n);
+    refreshBlockInfo();
+    return active;
+  }
+  void splatParticles(int chan, float rParticle, float nbDist) {
+    check(msbg_splat_points(g_, chan, rParticle, nbDist));
+  }
+  void finalizeDensity(int chan, float rParticle, float nbDist) {
+    check(msbg_finalize_density(g_, chan, rParticle, nbDist));
+  }
+  template <typename Data_T>
+  void getBlockData(int chan, int bid, Data_T *out, int level = 0) {
+    check(msbg_download_block(g_, chan, level, bid, out));
+  }
+
+ private:
+  explicit MultiresSparseGrid(msbg_grid *g) : g_(g) {
+    check(msbg_get_dims(g_, dims_, &nblocks_));
+    info_.resize((size_t)nblocks_);
+    refreshBlockInfo();
+  }
+  void refreshBlockInfo() {
+    std::vector<uint16_t> l((size_t)nblocks_), f((size_t)nblocks_);
+    check(msbg_get_block_info(g_, l.data(), f.data()));
+    for (size_t i = 0; i < info_.size(); i++) info_[i] = BlockInfo{l[i], f[i]};
+  }
+  msbg_grid *g_;
+  int dims_[8] = {0};
+  int64_t nblocks_ = 0;
+  std::vector<BlockInfo> info_;
+};
+
+}  // namespace MSBG

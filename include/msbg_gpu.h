@@ -275,6 +275,13 @@ int msbg_apply_channel_pde_fast(msbg_grid *g, int chPhi, int chTmp,
 int msbg_last_sweep_timing(msbg_grid *g, double *total_ms,
                            double *avg_pass_ms, int64_t *npasses);
 
+/* Read+write bandwidth of a plain streaming copy of `bytes` between two
+ * fresh device buffers, averaged over `reps` launches (GB/s, counting the
+ * bytes read plus the bytes written). No counterpart in the reference: it
+ * is the measured bandwidth reference the sweep is reported against, and a
+ * kernel of known traffic for scaling profiler byte counters. */
+int msbg_copy_bandwidth(int64_t bytes, int reps, double *gbps);
+
 /* ======================================================================== */
 /* synthetic scenes (host)                                                  */
 /* ======================================================================== */

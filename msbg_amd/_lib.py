@@ -82,6 +82,7 @@ SIGNATURES = {
                                          _F, _P]),
     "msbg_last_sweep_timing": (_I, [_P, _P, _P, _P]),
     "msbg_scene_spheres": (_I, [_U64, _I, _F, _F, _F, _F, _I64, _F, _F, _P]),
+    "msbg_copy_bandwidth": (_I, [_I64, _I, _P]),
 }
 
 _lib = None

@@ -64,9 +64,23 @@ def build_oracle(verbose=True):
         print(f"built {odir}/liboracle_strict.so, liboracle_fast.so")
 
 
+def build_facade_demo(verbose=True):
+    """The C++ facade example used by the tests (host code only)."""
+    src = os.path.join(ROOT, "tests", "cpp", "facade_demo.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "facade_demo")
+    cmd = ["g++", "-O2", "-std=c++17", "-o", out, src, f"-L{PKG}", "-lmsbg_gpu",
+           f"-Wl,-rpath,$ORIGIN/../../msbg_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"facade demo build failed: {' '.join(cmd)}\n{r.stderr}")
+    if verbose:
+        print(f"built {out}")
+
+
 def main():
     build_oracle()
     build_lib()
+    build_facade_demo()
 
 
 if __name__ == "__main__":

@@ -112,6 +112,7 @@ static int upload_pagemap(msbg_grid *g, ChanLevel &cl) {
   }
   MSBG_HIP(hipStreamSynchronize(g->stream));
   cl.nbr_valid = false;
+  cl.side_valid = false;
   g->topo_version++;
   return 0;
 }
@@ -540,6 +541,7 @@ int msbg_channel_upload(msbg_grid *g, int chan, int level, const void *pages, in
              (long long)need);
   MSBG_HIP(hipStreamSynchronize(g->stream));
   if (need) MSBG_HIP(hipMemcpy(cl->d_pages.p, pages, (size_t)need, hipMemcpyHostToDevice));
+  cl->side_valid = false;
   return 0;
 }
 
@@ -553,10 +555,12 @@ static int block_io(msbg_grid *g, int chan, int level, int bid, void *data, bool
   size_t sz = (size_t)cl->bsx * cl->bsx * cl->bsx * storage_bytes(ch->storage);
   uint8_t *dev = cl->d_pages.p + (size_t)pg * sz;
   MSBG_HIP(hipStreamSynchronize(g->stream));
-  if (down)
+  if (down) {
     MSBG_HIP(hipMemcpy(data, dev, sz, hipMemcpyDeviceToHost));
-  else
+  } else {
     MSBG_HIP(hipMemcpy(dev, data, sz, hipMemcpyHostToDevice));
+    cl->side_valid = false;
+  }
   return 0;
 }
 
@@ -596,6 +600,7 @@ int msbg_splat_points(msbg_grid *g, int chan, float rParticle, float nbDist) {
              "(the reference's lock-free colour scheme needs it to fit)",
              2 * (rScan + 0.5f), g->bsx0);
   MSBG_TRY(launch_splat(g, *ch, rScan));
+  cl->side_valid = false;
   MSBG_HIP(hipStreamSynchronize(g->stream));
   return 0;
 }
@@ -605,6 +610,7 @@ int msbg_finalize_density(msbg_grid *g, int chan, float rParticle, float nbDist)
   ChanLevel *cl;
   MSBG_TRY(get_level(g, chan, 0, &ch, &cl));
   MSBG_TRY(launch_finalize(g, *ch, rParticle, nbDist));
+  cl->side_valid = false;
   MSBG_HIP(hipStreamSynchronize(g->stream));
   return 0;
 }
@@ -681,6 +687,7 @@ int msbg_prepare_ghost_blocks(msbg_grid *g, int chan) {
       MSBG_HIP(hipMemcpyAsync(d.p, todo.data(), sizeof(int32_t) * todo.size(),
                               hipMemcpyHostToDevice, g->stream));
       MSBG_TRY(launch_ghost(g, *ch, l, d.p, (int64_t)todo.size()));
+      lo.side_valid = false;
       MSBG_HIP(hipStreamSynchronize(g->stream));
     }
   }

@@ -84,6 +84,12 @@ struct ChanLevel {
   DevBuf<int32_t> d_page_bid;
   DevBuf<int32_t> d_nbr;             // halo index table [npages][27]
   bool nbr_valid = false;
+  // side-car copies of each page's two x-faces [page][2][z][y] followed by
+  // its four z-edges [page][4][z]: the sweep reads neighbour x-faces from
+  // here contiguously instead of with a bsx-strided gather from the page
+  DevBuf<uint8_t> d_side;
+  DevBuf<uint8_t> d_consts;          // 4 x empty value, 4 x full value (raw)
+  bool side_valid = false;
   double empty_raw = 0, full_raw = 0;
 };
 

@@ -123,8 +123,8 @@ __device__ inline double mcf_exact<double>(const double *D, const double *M, con
 //           G = dx^2 + dy^2 + dz^2,
 // and the threshold |grad|^2 > 1e-7 becomes G > 4e-7. Central differences
 // shared by neighbouring voxels of the column are formed once.
-__device__ inline void mcf_fast4(const float *D, const float *M, const float *U, float dt, bool c01,
-                                 float out[4]) {
+__device__ inline void mcf_fast4(const float *D, const float *M, const float *U, float dt, float lo,
+                                 float hi, float out[4]) {
   float colY[4][4], dzc[4][4];
 #pragma unroll
   for (int j = 1; j <= 2; j++)
@@ -165,17 +165,53 @@ __device__ inline void mcf_fast4(const float *D, const float *M, const float *U,
       float Dv = (N * dt) * r;
       Dv = G > 4.0f * 1e-7f ? Dv : 0.0f;
       Dv = __builtin_amdgcn_fmed3f(Dv, -0.1f, 0.1f);
-      float phi = f0 + Dv;
-      if (c01) phi = __builtin_amdgcn_fmed3f(phi, 0.0f, 1.0f);
-      out[b * 2 + a] = phi;
+      // doConstrZeroOne: [lo,hi] = [0,1], else (-inf,inf)
+      out[b * 2 + a] = __builtin_amdgcn_fmed3f(f0 + Dv, lo, hi);
     }
 }
 
+// ---- side-car refresh ---------------------------------------------------
+// faces [page][2][z][y] (x = 0 and x = bsx-1), then edges [page][4][z]
+// (edge k at x = (k&1) ? bsx-1 : 0, y = (k&2) ? bsx-1 : 0)
+template <int BSX, typename R>
+__global__ void k_refresh_side(const R *__restrict__ pages, R *__restrict__ side, int64_t npages) {
+  constexpr int M = BSX - 1, PS = BSX * BSX, FS = 2 * PS, ES = 4 * BSX;
+  constexpr int64_t PE = (int64_t)PS * BSX;
+  const int64_t p = blockIdx.x;
+  const R *pg = pages + p * PE;
+  R *faces = side + p * FS;
+  R *edges = side + npages * FS + p * ES;
+  for (int t = threadIdx.x; t < FS + ES; t += blockDim.x) {
+    if (t < FS) {
+      int sd = t / PS, z = (t % PS) / BSX, y = t % BSX;
+      faces[t] = pg[z * PS + y * BSX + (sd ? M : 0)];
+    } else {
+      int u = t - FS, k = u / BSX, z = u % BSX;
+      edges[u] = pg[z * PS + ((k & 2) ? M : 0) * BSX + ((k & 1) ? M : 0)];
+    }
+  }
+}
+
 // ---- the pass kernel ----------------------------------------------------
+// One source of halo values for one ring cell (or the interior of the
+// planes outside the block): pointers for the plane below the block
+// (zz = -1), for plane 0 of the block (advanced by `stride` per plane) and
+// for the plane above (zz = bsx). Const blocks point into a small buffer of
+// the const value with stride 0.
+template <typename R>
+struct Src {
+  const R *m, *p0, *p;
+  int stride;
+  __device__ inline const R *at(int zz, int n) const {
+    return zz < 0 ? m : (zz >= n ? p : p0 + zz * stride);
+  }
+};
+
 template <int BSX, int ST, bool EXACT>
 __global__ __launch_bounds__((BSX / 2) * (BSX / 2)) void k_sweep(
     const int32_t *__restrict__ list, const int32_t *__restrict__ nbrTab,
-    uint8_t *__restrict__ pagesRaw, float dtf, int c01i, float emptyv, float fullv,
+    uint8_t *__restrict__ pagesRaw, uint8_t *__restrict__ sideRaw,
+    const uint8_t *__restrict__ constsRaw, int64_t npages, float dtf, float lo01, float hi01,
     double *__restrict__ partial) {
   using S = Stor<ST>;
   using R = typename S::raw;
@@ -184,6 +220,7 @@ __global__ __launch_bounds__((BSX / 2) * (BSX / 2)) void k_sweep(
   constexpr int P = BSX + 2;  // LDS row pitch (halo coords 0..BSX+1)
   constexpr int PS = BSX * BSX;
   constexpr int64_t PE = (int64_t)BSX * BSX * BSX;
+  constexpr int FS = 2 * PS, ES = 4 * BSX;
   constexpr int NR = 4 * BSX + 4;  // ring cells of one halo plane
   constexpr int RPT = (NR + NT - 1) / NT;
   constexpr int M = BSX - 1;
@@ -194,122 +231,131 @@ __global__ __launch_bounds__((BSX / 2) * (BSX / 2)) void k_sweep(
 
   const int tid = threadIdx.x;
   const int page = list[blockIdx.x];
-  if (tid < 27) codes[tid] = nbrTab[(int64_t)page * 27 + tid];
+  // 27 codes; with 8^3 blocks the workgroup has only 16 threads
+  for (int k = tid; k < 27; k += NT) codes[k] = nbrTab[(int64_t)page * 27 + k];
   __syncthreads();
 
   R *pages = (R *)pagesRaw;
   R *own = pages + (int64_t)page * PE;
+  R *faces = (R *)sideRaw;                 // [npages][2][z][y]
+  R *edges = faces + npages * FS;          // [npages][4][z]
+  R *ownFaces = faces + (int64_t)page * FS;
+  R *ownEdges = edges + (int64_t)page * ES;
+  const R *consts = (const R *)constsRaw;  // [0..3] empty, [4..7] full
   const C dt = (C)dtf;
-  const bool c01 = c01i != 0;
 
   // load mapping: 4 consecutive x of one row
   const int lx = 4 * (tid % (BSX / 4)), ly = tid / (BSX / 4);
   // compute mapping: 2x2 column
   const int x0 = 2 * (tid % (BSX / 2)), y0 = 2 * (tid / (BSX / 2));
 
+  // interior of the planes below/above the block (uniform over the block)
+  auto zsrc = [&](int dzi, bool top) -> const R * {
+    int c = codes[dzi * 9 + 4];
+    if (c >= 0) return pages + (int64_t)c * PE + (top ? 0 : M) * PS;
+    if (c == MSBG_BLOCK_EMPTY) return consts;
+    if (c == MSBG_BLOCK_FULL) return consts + 4;
+    return own + (top ? M : 0) * PS;  // Neumann: the block's own boundary plane
+  };
+  const R *zsrcM = zsrc(0, false), *zsrcP = zsrc(2, true);
+  const int moffM = (codes[4] == MSBG_BLOCK_EMPTY || codes[4] == MSBG_BLOCK_FULL) ? 0 : ly * BSX + lx;
+  const int moffP = (codes[22] == MSBG_BLOCK_EMPTY || codes[22] == MSBG_BLOCK_FULL) ? 0 : ly * BSX + lx;
+
+  // ring cells of this thread and their sources
+  Src<R> rs[RPT];
+  int rlds[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    int r = tid + q * NT;
+    if (r >= NR) r = NR - 1;  // duplicate of the last cell, never stored twice differently
+    int x, y;
+    if (r < BSX) {
+      x = -1;
+      y = r;
+    } else if (r < 2 * BSX) {
+      x = BSX;
+      y = r - BSX;
+    } else if (r < 3 * BSX) {
+      x = r - 2 * BSX;
+      y = -1;
+    } else if (r < 4 * BSX) {
+      x = r - 3 * BSX;
+      y = BSX;
+    } else {
+      int k = r - 4 * BSX;
+      x = (k & 1) ? BSX : -1;
+      y = (k & 2) ? BSX : -1;
+    }
+    rlds[q] = (y + 1) * P + x + 1;
+    const int dxi = x < 0 ? 0 : (x >= BSX ? 2 : 1), dyi = y < 0 ? 0 : (y >= BSX ? 2 : 1);
+    const bool xside = dxi != 1, yside = dyi != 1;
+    // source pointer for neighbour code c at plane zz (zz = -1 / 0 / BSX)
+    auto one = [&](int dzi, int zz, int *stride) -> const R * {
+      int c = codes[dzi * 9 + dyi * 3 + dxi];
+      int zc = zz < 0 ? 0 : (zz > M ? M : zz);
+      if (c == MSBG_BLOCK_EMPTY) {
+        *stride = 0;
+        return consts;
+      }
+      if (c == MSBG_BLOCK_FULL) {
+        *stride = 0;
+        return consts + 4;
+      }
+      if (c >= 0) {
+        int zn = zz & M;  // plane inside the neighbour
+        if (xside && yside) {  // corner: the neighbour's opposite z-edge
+          int k = (x < 0 ? 1 : 0) + (y < 0 ? 2 : 0);
+          *stride = 1;
+          return edges + (int64_t)c * ES + k * BSX + zn;
+        }
+        if (xside) {  // x-face: the neighbour's opposite face, [z][y]
+          *stride = BSX;
+          return faces + (int64_t)c * FS + (x < 0 ? 1 : 0) * PS + zn * BSX + y;
+        }
+        *stride = PS;  // y-face: a contiguous row of the neighbour page
+        return pages + (int64_t)c * PE + zn * PS + (y < 0 ? M : 0) * BSX + x;
+      }
+      // NULL block or outside the domain: the block's own clamped voxel
+      if (xside && yside) {
+        int k = (x < 0 ? 0 : 1) + (y < 0 ? 0 : 2);
+        *stride = 1;
+        return ownEdges + k * BSX + zc;
+      }
+      if (xside) {
+        *stride = BSX;
+        return ownFaces + (x < 0 ? 0 : 1) * PS + zc * BSX + y;
+      }
+      *stride = PS;
+      return own + zc * PS + (y < 0 ? 0 : M) * BSX + x;
+    };
+    int sdummy, st0;
+    rs[q].m = one(0, -1, &sdummy);
+    rs[q].p0 = one(1, 0, &st0);
+    rs[q].p = one(2, BSX, &sdummy);
+    rs[q].stride = st0;
+  }
+
   C vin[4];
   C vring[RPT];
-
   auto dec = [](R r) -> C { return (C)S::dec(r); };
 
-  // global loads of halo plane zz (zz in [-1, BSX]) into vin/vring
   auto load_plane = [&](int zz) {
-    // interior
-    const R *src = nullptr;
-    C cval = 0;
-    if (zz >= 0 && zz < BSX) {
-      src = own + zz * PS;
-    } else {
-      int c = codes[zz < 0 ? 4 : 22];
-      if (c >= 0)
-        src = pages + (int64_t)c * PE + (zz < 0 ? M : 0) * PS;
-      else if (c == MSBG_BLOCK_EMPTY)
-        cval = (C)emptyv;
-      else if (c == MSBG_BLOCK_FULL)
-        cval = (C)fullv;
-      else
-        src = own + (zz < 0 ? 0 : M) * PS;  // Neumann: own boundary plane
-    }
-    if (src) {
-      R r[4];
-      load4(src + ly * BSX + lx, r);
+    const R *src = zz < 0 ? zsrcM + moffM : (zz >= BSX ? zsrcP + moffP : own + zz * PS + ly * BSX + lx);
+    R r[4];
+    load4(src, r);
 #pragma unroll
-      for (int k = 0; k < 4; k++) vin[k] = dec(r[k]);
-    } else {
+    for (int k = 0; k < 4; k++) vin[k] = dec(r[k]);
 #pragma unroll
-      for (int k = 0; k < 4; k++) vin[k] = cval;
-    }
-    // ring
-    const int dzi = zz < 0 ? 0 : (zz >= BSX ? 2 : 1);
-    const int zc = zz < 0 ? 0 : (zz > M ? M : zz);  // clamped (Neumann)
-#pragma unroll
-    for (int q = 0; q < RPT; q++) {
-      int r = tid + q * NT;
-      if (r < NR) {
-        int x, y;
-        if (r < BSX) {
-          x = -1;
-          y = r;
-        } else if (r < 2 * BSX) {
-          x = BSX;
-          y = r - BSX;
-        } else if (r < 3 * BSX) {
-          x = r - 2 * BSX;
-          y = -1;
-        } else if (r < 4 * BSX) {
-          x = r - 3 * BSX;
-          y = BSX;
-        } else {
-          int k = r - 4 * BSX;
-          x = (k & 1) ? BSX : -1;
-          y = (k & 2) ? BSX : -1;
-        }
-        int dxi = x < 0 ? 0 : (x >= BSX ? 2 : 1), dyi = y < 0 ? 0 : (y >= BSX ? 2 : 1);
-        int c = codes[dzi * 9 + dyi * 3 + dxi];
-        C v;
-        if (c >= 0) {
-          v = dec(pages[(int64_t)c * PE + (zz & M) * PS + (y & M) * BSX + (x & M)]);
-        } else if (c == MSBG_BLOCK_EMPTY) {
-          v = (C)emptyv;
-        } else if (c == MSBG_BLOCK_FULL) {
-          v = (C)fullv;
-        } else {
-          int xc = x < 0 ? 0 : (x > M ? M : x), yc = y < 0 ? 0 : (y > M ? M : y);
-          v = dec(own[zc * PS + yc * BSX + xc]);
-        }
-        vring[q] = v;
-      }
-    }
+    for (int q = 0; q < RPT; q++) vring[q] = dec(*rs[q].at(zz, BSX));
   };
 
   auto put_plane = [&](C *b) {
+    C *row = b + (ly + 1) * P + lx + 1;
 #pragma unroll
-    for (int k = 0; k < 4; k++) b[(ly + 1) * P + lx + 1 + k] = vin[k];
+    for (int k = 0; k < 4; k++) row[k] = vin[k];
 #pragma unroll
-    for (int q = 0; q < RPT; q++) {
-      int r = tid + q * NT;
-      if (r < NR) {
-        int x, y;
-        if (r < BSX) {
-          x = -1;
-          y = r;
-        } else if (r < 2 * BSX) {
-          x = BSX;
-          y = r - BSX;
-        } else if (r < 3 * BSX) {
-          x = r - 2 * BSX;
-          y = -1;
-        } else if (r < 4 * BSX) {
-          x = r - 3 * BSX;
-          y = BSX;
-        } else {
-          int k = r - 4 * BSX;
-          x = (k & 1) ? BSX : -1;
-          y = (k & 2) ? BSX : -1;
-        }
-        b[(y + 1) * P + x + 1] = vring[q];
-      }
-    }
+    for (int q = 0; q < RPT; q++)
+      if (tid + q * NT < NR) b[rlds[q]] = vring[q];
   };
 
   auto get_patch = [&](const C *b, C (&p)[16]) {
@@ -320,27 +366,46 @@ __global__ __launch_bounds__((BSX / 2) * (BSX / 2)) void k_sweep(
   };
 
   double acc = 0.0;
+  const bool wFace0 = x0 == 0, wFace1 = x0 == BSX - 2, wY0 = y0 == 0, wY1 = y0 == BSX - 2;
 
   auto compute = [&](int z, const C (&Dp)[16], const C (&Mp)[16], const C (&Up)[16]) {
     C o[4];
     if constexpr (EXACT || sizeof(C) == 8) {
+      const bool c01 = lo01 == 0.0f;
 #pragma unroll
       for (int b = 0; b < 2; b++)
 #pragma unroll
-        for (int a = 0; a < 2; a++) o[b * 2 + a] = mcf_exact<C>(Dp, Mp, Up, (b + 1) * 4 + a + 1, dt, c01);
+        for (int a = 0; a < 2; a++)
+          o[b * 2 + a] = mcf_exact<C>(Dp, Mp, Up, (b + 1) * 4 + a + 1, dt, c01);
     } else {
-      mcf_fast4(Dp, Mp, Up, dt, c01, o);
+      mcf_fast4(Dp, Mp, Up, dt, lo01, hi01, o);
     }
+    R e[4];
 #pragma unroll
-    for (int b = 0; b < 2; b++) {
-      R e0 = S::enc(o[b * 2]), e1 = S::enc(o[b * 2 + 1]);
-      store2(own + z * PS + (y0 + b) * BSX + x0, e0, e1);
-      if (partial) {
-        double d0 = (double)S::dec(e0) - (double)Mp[(b + 1) * 4 + 1];
-        double d1 = (double)S::dec(e1) - (double)Mp[(b + 1) * 4 + 2];
-        acc += d0 * d0;
-        acc += d1 * d1;
-      }
+    for (int k = 0; k < 4; k++) e[k] = S::enc(o[k]);
+#pragma unroll
+    for (int b = 0; b < 2; b++) store2(own + z * PS + (y0 + b) * BSX + x0, e[b * 2], e[b * 2 + 1]);
+    // keep the side-car faces and edges equal to the page
+    if (wFace0) {
+      ownFaces[z * BSX + y0] = e[0];
+      ownFaces[z * BSX + y0 + 1] = e[2];
+    }
+    if (wFace1) {
+      ownFaces[PS + z * BSX + y0] = e[1];
+      ownFaces[PS + z * BSX + y0 + 1] = e[3];
+    }
+    if (wFace0 && wY0) ownEdges[0 * BSX + z] = e[0];
+    if (wFace1 && wY0) ownEdges[1 * BSX + z] = e[1];
+    if (wFace0 && wY1) ownEdges[2 * BSX + z] = e[2];
+    if (wFace1 && wY1) ownEdges[3 * BSX + z] = e[3];
+    if (partial) {
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+          double d = (double)S::dec(e[b * 2 + a]) - (double)Mp[(b + 1) * 4 + a + 1];
+          acc += d * d;
+        }
     }
   };
 
@@ -379,30 +444,53 @@ __global__ __launch_bounds__((BSX / 2) * (BSX / 2)) void k_sweep(
     if ((tid & 63) == 0) red[tid >> 6] = acc;
     __syncthreads();
     if (tid == 0) {
-      double s = 0.0;
-      for (int w = 0; w < (NT + 63) / 64; w++) s += red[w];
-      partial[blockIdx.x] = s;
+      double sum = 0.0;
+      for (int w = 0; w < (NT + 63) / 64; w++) sum += red[w];
+      partial[blockIdx.x] = sum;
     }
   }
+}
+
+template <int BSX, int ST>
+int ensure_side(msbg_grid *g, ChanLevel &cl) {
+  using R = typename Stor<ST>::raw;
+  if (cl.side_valid) return 0;
+  const int64_t per = 2 * BSX * BSX + 4 * BSX;
+  MSBG_TRY(cl.d_side.ensure((size_t)(cl.npages * per * (int64_t)sizeof(R))));
+  MSBG_TRY(cl.d_consts.ensure(8 * sizeof(R)));
+  R c[8];
+  for (int k = 0; k < 4; k++) {
+    c[k] = raw_value<ST>(cl.empty_raw);
+    c[4 + k] = raw_value<ST>(cl.full_raw);
+  }
+  MSBG_HIP(hipMemcpyAsync(cl.d_consts.p, c, sizeof c, hipMemcpyHostToDevice, g->stream));
+  if (cl.npages)
+    hipLaunchKernelGGL((k_refresh_side<BSX, R>), dim3((unsigned)cl.npages), dim3(256), 0, g->stream,
+                       (const R *)cl.d_pages.p, (R *)cl.d_side.p, cl.npages);
+  MSBG_HIP(hipGetLastError());
+  MSBG_HIP(hipStreamSynchronize(g->stream));
+  cl.side_valid = true;
+  return 0;
 }
 
 template <int BSX, int ST>
 int launch_t(msbg_grid *g, ChanLevel &cl, const int32_t *d_list, int64_t n, float dt, int c01,
              double *d_partial) {
   constexpr int NT = (BSX / 2) * (BSX / 2);
-  float ev = raw_to_float_host(ST, cl.empty_raw), fv = raw_to_float_host(ST, cl.full_raw);
+  MSBG_TRY((ensure_side<BSX, ST>(g, cl)));
+  float lo = c01 ? 0.0f : -__builtin_inff(), hi = c01 ? 1.0f : __builtin_inff();
   for (int64_t off = 0; off < n; off += 0x7fffffff) {
     unsigned cnt = (unsigned)(n - off < 0x7fffffff ? n - off : 0x7fffffff);
     bool exact = g->arith == MSBG_ARITH_EXACT;
     if constexpr (ST == MSBG_STORAGE_F64) exact = true;  // the FP64 run is always exact
     if (exact)
       hipLaunchKernelGGL((k_sweep<BSX, ST, true>), dim3(cnt), dim3(NT), 0, g->stream, d_list + off,
-                         cl.d_nbr.p, cl.d_pages.p, dt, c01, ev, fv,
-                         d_partial ? d_partial + off : nullptr);
+                         cl.d_nbr.p, cl.d_pages.p, cl.d_side.p, cl.d_consts.p, cl.npages, dt, lo,
+                         hi, d_partial ? d_partial + off : nullptr);
     else if constexpr (ST != MSBG_STORAGE_F64)
       hipLaunchKernelGGL((k_sweep<BSX, ST, false>), dim3(cnt), dim3(NT), 0, g->stream, d_list + off,
-                         cl.d_nbr.p, cl.d_pages.p, dt, c01, ev, fv,
-                         d_partial ? d_partial + off : nullptr);
+                         cl.d_nbr.p, cl.d_pages.p, cl.d_side.p, cl.d_consts.p, cl.npages, dt, lo,
+                         hi, d_partial ? d_partial + off : nullptr);
     MSBG_HIP(hipGetLastError());
   }
   return 0;

@@ -119,6 +119,16 @@ struct Stor<MSBG_STORAGE_F64> {
   __device__ static inline raw enc(double f) { return f; }
 };
 
+// raw storage value (as double) -> the storage's raw element, on the host
+template <int ST>
+inline typename Stor<ST>::raw raw_value(double raw) {
+  if constexpr (ST == MSBG_STORAGE_U16) return (uint16_t)raw;
+  else if constexpr (ST == MSBG_STORAGE_F32) return (float)raw;
+  else if constexpr (ST == MSBG_STORAGE_F16) return f16_bits_from_float((float)raw);
+  else if constexpr (ST == MSBG_STORAGE_BF16) return bf16_bits_from_float((float)raw);
+  else return raw;
+}
+
 // raw storage value (as double) -> float, for const block values on the host
 inline float raw_to_float_host(int storage, double raw) {
   if (storage == MSBG_STORAGE_U16) return (float)(uint16_t)raw * (1.0f / 65535.0f);

@@ -38,6 +38,13 @@ def scene_spheres(seed, nObj, cmin, cmax, rmin, rmax, nPts, sigma=0.5, ellipsoid
     return out
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=5):
+    """Measured read+write GB/s of a plain device copy of nbytes."""
+    v = C.c_double()
+    check(lib().msbg_copy_bandwidth(nbytes, reps, C.byref(v)))
+    return v.value
+
+
 class MultiresSparseGrid:
     """MultiresSparseGrid (msbg.h:834-3907) restricted to the stencil path."""
 

@@ -86,7 +86,7 @@ def to_storage_level(S, bsx, pagemap, vals_f32, storage):
 
 
 def gpu_pipeline(S, bsx, xyz, nLevels=3, rScan=RSCAN, storage=1, numIter=2, dt=0.1,
-                 residual=False, arith=0, do_sweep=True, chan=30):
+                 residual=False, arith=0, do_sweep=True, chan=30, do_splat=True):
     import msbg_amd as M
     g = M.MultiresSparseGrid.create("test", S, S, S, bsx, options=M.OPT_SINGLE_LEVEL,
                                     nLevels=nLevels)
@@ -98,9 +98,11 @@ def gpu_pipeline(S, bsx, xyz, nLevels=3, rScan=RSCAN, storage=1, numIter=2, dt=0
     g.setRefinementMap(lv)
     blv, bfl = g.getBlockInfo()
     active = g.alloc_demo(chan, storage)
-    g.splat(chan, rScan / 2, rScan / 2)
-    g.finalize(chan, rScan / 2, rScan / 2)
-    splat = g.download(chan)
+    splat = None
+    if do_splat:
+        g.splat(chan, rScan / 2, rScan / 2)
+        g.finalize(chan, rScan / 2, rScan / 2)
+        splat = g.download(chan)
     res = None
     if do_sweep:
         res = g.applyChannelPdeFast(chan, 0, 0, active, M.MCF_8COLOR, 1, 0, 0, 0, numIter, 1.0,

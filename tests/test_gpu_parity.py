@@ -35,7 +35,8 @@ def test_tables_bitexact(S, bsx, nLevels):
     note("tables", S, bsx, nLevels)
     xyz = _scene(S)
     o = oracle_pipeline(S, bsx, xyz, nLevels=nLevels, do_sweep=False)
-    gp = gpu_pipeline(S, bsx, xyz, nLevels=nLevels, do_sweep=False)
+    # the splat needs 2*(rScan+0.5) <= bsx (msbg_demo.cpp lock-free scheme)
+    gp = gpu_pipeline(S, bsx, xyz, nLevels=nLevels, do_sweep=False, do_splat=bsx >= 9)
     assert gp["nk"] == int(o["keep"].sum())
     assert np.array_equal(gp["grid"].points_download(), o["xk"])
     assert np.array_equal(gp["cnt"], o["cnt"])
@@ -45,7 +46,8 @@ def test_tables_bitexact(S, bsx, nLevels):
     ofl = o["bfl"].copy()
     # the demo's allocation loop then sets BLK_EXISTS for level-0 blocks
     _, gfl = gp["grid"].getBlockInfo()
-    ofl = np.where(o["blv"] == 0, ofl | M._lib.BLK_EXISTS, ofl & ~M._lib.BLK_EXISTS).astype(np.uint16)
+    ex = np.uint16(M._lib.BLK_EXISTS)
+    ofl = np.where(o["blv"] == 0, ofl | ex, ofl & ~ex).astype(np.uint16)
     assert np.array_equal(gfl, ofl)
     assert np.array_equal(gp["active"], o["active"])
 
@@ -227,8 +229,8 @@ def test_sweep_fast_close_to_exact(S, bsx):
 
 def test_sweep_skips_non_level0_and_rejects_bad_lists():
     note("sweep list semantics")
-    S, bsx = 64, 16
-    xyz = _scene(S, seed=5, nObj=3)
+    S, bsx = 128, 16
+    xyz = small_scene(S, nObj=2, nPts=2000, seed=5, rmin=8, rmax=12)
     gp = gpu_pipeline(S, bsx, xyz, do_sweep=False)
     g = gp["grid"]
     before = g.download(M.CH_UINT16_1)
@@ -328,7 +330,27 @@ def test_prepare_ghost_blocks_bitexact():
             pg[:pages[l].shape[0]] = pages[l]
             olv.append(_olevel(S, bsx, l, post_pm[l], pg))
         fcs = [g.blocksFineCoarse(l) for l in range(3)]
-        O.prepare_ghost_blocks(olv, 3, fcs, fl)
+        O.prepare_ghost_blocks(olv, fcs, fl)
         for l in range(3):
             assert np.array_equal(post[l], olv[l].pages), l
         assert sum(len(f) for f in fcs) > 0
+
+
+# --------------------------------------------------------------------------
+# the C++ facade (include/msbg/msbg.hpp) drives the same path
+# --------------------------------------------------------------------------
+def test_cpp_facade_demo_matches_binding(tmp_path):
+    import os
+    import subprocess
+    note("facade demo")
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "facade_demo")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/facade_demo not built (run the build step)")
+    out = tmp_path / "pages.bin"
+    r = subprocess.run([exe, str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(out, dtype=np.uint16)
+    S, bsx = 128, 16
+    xyz = M.scene_spheres(99, 6, 30, 98, 10, 25, 4000, 0.5, 0.3)
+    gp = gpu_pipeline(S, bsx, xyz, numIter=5, dt=0.1, arith=M.ARITH_FAST)
+    assert np.array_equal(got, gp["pages"].reshape(-1))
