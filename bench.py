@@ -98,7 +98,8 @@ def setup_grid(cfg, xyz, storage, chan=30):
     g.finalize(chan, 2.0, 2.0)
     g.synchronize()
     t["finalize_s"] = time.perf_counter() - t0
-    if multi:
+    if multi and storage in (2, 3, 4):
+        # the reference's prepareGhostBlocks handles float channels only
         t0 = time.perf_counter()
         g.prepareGhostBlocks(chan)
         g.synchronize()

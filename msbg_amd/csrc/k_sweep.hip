@@ -10,12 +10,12 @@
 // GPU mapping: one workgroup per block. The workgroup marches through the
 // block's z-planes; each step stages one halo plane ((bsx+2)^2 values,
 // interior from the block's page with 4-wide coalesced loads, the ring from
-// the neighbour pages named by the halo index table) into a double-buffered
-// LDS plane, and every thread pulls the 4x4 patch around its 2x2 voxel
-// column into registers. Three such patches (z-1, z, z+1) rotate in
-// registers, so each halo value is read from HBM once and from LDS once per
-// thread that needs it. All loads of a plane happen before the plane is
-// overwritten, which keeps the in-place update exact.
+// the neighbour pages and side-car faces named by the halo index table)
+// into a double-buffered LDS plane, and every thread pulls the patch around
+// its voxel column (4x2 for 32^3 blocks) into registers. Three patches
+// (z-1, z, z+1) rotate in registers, and plane loads run two planes ahead.
+// All loads of a plane happen before the plane is overwritten, which keeps
+// the in-place update exact.
 //
 // Reference paths are relative to the reference's src/ directory.
 
@@ -27,6 +27,24 @@
 namespace msbg {
 
 namespace {
+
+// compile-time configuration of the sweep kernel
+#ifndef MSBG_SWEEP_ALG
+#define MSBG_SWEEP_ALG 1  // fold -2 f0 of fxx/fyy/fzz into H = Np/G - 4 f0
+#endif
+#ifndef MSBG_SWEEP_PD
+#define MSBG_SWEEP_PD 1  // plane prefetch distance (1 or 2)
+#endif
+#ifndef MSBG_SWEEP_PX32
+#define MSBG_SWEEP_PX32 4  // voxel column width for 32^3 blocks
+#endif
+#ifndef MSBG_SWEEP_PY32
+#define MSBG_SWEEP_PY32 2  // voxel column height for 32^3 blocks
+#endif
+#ifndef MSBG_SWEEP_WAVES
+#define MSBG_SWEEP_WAVES 0  // minimum waves per SIMD requested (0 = none)
+#endif
+
 
 // ---- vector raw loads/stores -------------------------------------------
 __device__ inline void load4(const uint16_t *p, uint16_t v[4]) {
@@ -59,25 +77,26 @@ __device__ inline void store2(double *p, double a, double b) {
 }
 
 // ---- the MCF update --------------------------------------------------------
-// Patches: 4x4 values of one plane around the thread's 2x2 column, index
-// j*4+i with halo-local x = i-1, y = j-1 relative to the column origin.
+// Patches: (PX+2) x (PY+2) values of one plane around the thread's PX x PY
+// voxel column, row stride W = PX+2; halo-local x = i-1, y = j-1.
 
 // Reference evaluation order (msbg4.cpp:559-637, 813-825), one rounding per
 // operation: bit-identical to the strict CPU restatement.
-template <typename C>
+template <typename C, int W>
 __device__ inline C mcf_exact(const C *D, const C *M, const C *U, int c, C dt, bool c01) {
 #pragma clang fp contract(off)
   const C h = (C)0.5, two = (C)2;
-  C f0 = M[c], f1 = M[c - 1], f2 = M[c + 1], f3 = M[c - 4], f4 = M[c + 4], f5 = D[c], f6 = U[c];
+  C f0 = M[c], f1 = M[c - 1], f2 = M[c + 1], f3 = M[c - W], f4 = M[c + W], f5 = D[c], f6 = U[c];
   C fx = h * (f2 - f1), fy = h * (f4 - f3), fz = h * (f6 - f5);
   C fxx = f1 + f2 - two * f0, fyy = f3 + f4 - two * f0, fzz = f5 + f6 - two * f0;
-  C fy_right = (M[c + 5] - M[c - 3]) * h, fy_left = (M[c + 3] - M[c - 5]) * h;
+  C fy_right = (M[c + W + 1] - M[c - W + 1]) * h, fy_left = (M[c + W - 1] - M[c - W - 1]) * h;
   C fz_right = (U[c + 1] - D[c + 1]) * h, fz_left = (U[c - 1] - D[c - 1]) * h;
-  C fz_up = (U[c + 4] - D[c + 4]) * h, fz_down = (U[c - 4] - D[c - 4]) * h;
+  C fz_up = (U[c + W] - D[c + W]) * h, fz_down = (U[c - W] - D[c - W]) * h;
   C fyx = (fy_right - fy_left) * h, fzx = (fz_right - fz_left) * h, fzy = (fz_up - fz_down) * h;
   C H = (fy * fy + fz * fz) * fxx + (fx * fx + fz * fz) * fyy + (fx * fx + fy * fy) * fzz -
         two * (fx * fy * fyx + fx * fz * fzx + fy * fz * fzy);
   C g = fx * fx + fy * fy + fz * fz;
+  // the threshold is the float literal 1e-7f, also for the FP64 run
   H = g > (C)1e-7f ? H / g : (C)0;
   C Dv = dt * H;
   Dv = Dv > (C)-0.1f ? Dv : (C)-0.1f;  // Vec8f max(a,b) = a > b ? a : b
@@ -90,83 +109,66 @@ __device__ inline C mcf_exact(const C *D, const C *M, const C *U, int c, C dt, b
   return phi;
 }
 
-template <>
-__device__ inline double mcf_exact<double>(const double *D, const double *M, const double *U, int c,
-                                           double dt, bool c01) {
-#pragma clang fp contract(off)
-  double f0 = M[c], f1 = M[c - 1], f2 = M[c + 1], f3 = M[c - 4], f4 = M[c + 4], f5 = D[c],
-         f6 = U[c];
-  double fx = 0.5 * (f2 - f1), fy = 0.5 * (f4 - f3), fz = 0.5 * (f6 - f5);
-  double fxx = f1 + f2 - 2.0 * f0, fyy = f3 + f4 - 2.0 * f0, fzz = f5 + f6 - 2.0 * f0;
-  double fyx = ((M[c + 5] - M[c - 3]) * 0.5 - (M[c + 3] - M[c - 5]) * 0.5) * 0.5;
-  double fzx = ((U[c + 1] - D[c + 1]) * 0.5 - (U[c - 1] - D[c - 1]) * 0.5) * 0.5;
-  double fzy = ((U[c + 4] - D[c + 4]) * 0.5 - (U[c - 4] - D[c - 4]) * 0.5) * 0.5;
-  double H = (fy * fy + fz * fz) * fxx + (fx * fx + fz * fz) * fyy + (fx * fx + fy * fy) * fzz -
-             2.0 * (fx * fy * fyx + fx * fz * fzx + fy * fz * fzy);
-  double g = fx * fx + fy * fy + fz * fz;
-  H = g > 1e-7 ? H / g : 0.0;
-  double Dv = dt * H;
-  Dv = Dv > -0.1 ? Dv : -0.1;
-  Dv = Dv < 0.1 ? Dv : 0.1;
-  double phi = f0 + Dv;
-  if (c01) {
-    phi = phi > 0.0 ? phi : 0.0;
-    phi = phi < 1.0 ? phi : 1.0;
-  }
-  return phi;
-}
-
-// Fast evaluation of the same formula for one 2x2 column step. With
-// d* = 2 f*, c** = 4 f**, the reference's H / |grad|^2 equals
-//   N / G,  N = (dy^2+dz^2) fxx + (dx^2+dz^2) fyy + (dx^2+dy^2) fzz
-//               - 0.5 (dx dy cyx + dx dz czx + dy dz czy),
-//           G = dx^2 + dy^2 + dz^2,
-// and the threshold |grad|^2 > 1e-7 becomes G > 4e-7. Central differences
-// shared by neighbouring voxels of the column are formed once.
-__device__ inline void mcf_fast4(const float *D, const float *M, const float *U, float dt, float lo,
-                                 float hi, float out[4]) {
-  float colY[4][4], dzc[4][4];
+// Fast evaluation of the same formula for a PX x PY column step. With
+// d* = 2 f* (central differences), c** = 4 f** (mixed derivatives) and
+// s** = the two neighbours' sum along an axis, the reference's
+// H / |grad|^2 equals
+//   Np / G - 4 f0,  Np = A sxx + B syy + C szz - 0.5 (dx dy cyx + dx dz czx + dy dz czy),
+//   A = dy^2+dz^2, B = dx^2+dz^2, C = dx^2+dy^2, G = dx^2+dy^2+dz^2,
+// because A+B+C = 2G folds the -2 f0 of fxx, fyy, fzz into one term; the
+// threshold |grad|^2 > 1e-7 becomes G > 4e-7. Central differences shared by
+// neighbouring voxels of the column are formed once.
+template <int PX, int PY>
+__device__ inline void mcf_fast(const float *D, const float *M, const float *U, float dt, float lo,
+                                float hi, float *out) {
+  constexpr int W = PX + 2;
+  float colY[PY + 2][PX + 2], dzc[PY + 2][PX + 2];
 #pragma unroll
-  for (int j = 1; j <= 2; j++)
+  for (int j = 1; j <= PY; j++)
 #pragma unroll
-    for (int i = 0; i < 4; i++) colY[j][i] = M[(j + 1) * 4 + i] - M[(j - 1) * 4 + i];
+    for (int i = 0; i < W; i++) colY[j][i] = M[(j + 1) * W + i] - M[(j - 1) * W + i];
 #pragma unroll
-  for (int j = 0; j < 4; j++)
+  for (int j = 0; j < PY + 2; j++)
 #pragma unroll
-    for (int i = 0; i < 4; i++)
-      if ((j == 1 || j == 2) || (i == 1 || i == 2)) dzc[j][i] = U[j * 4 + i] - D[j * 4 + i];
+    for (int i = 0; i < W; i++)
+      if ((j >= 1 && j <= PY) || (i >= 1 && i <= PX)) dzc[j][i] = U[j * W + i] - D[j * W + i];
+  const float m4dt = -4.0f * dt;
 #pragma unroll
-  for (int b = 0; b < 2; b++)
+  for (int b = 0; b < PY; b++)
 #pragma unroll
-    for (int a = 0; a < 2; a++) {
-      const int i = a + 1, j = b + 1, c = j * 4 + i;
-      float f0 = M[c];
-      float dx = M[c + 1] - M[c - 1];
-      float dy = colY[j][i];
-      float dz = dzc[j][i];
-      float fxx = __builtin_fmaf(-2.0f, f0, M[c - 1] + M[c + 1]);
-      float fyy = __builtin_fmaf(-2.0f, f0, M[c - 4] + M[c + 4]);
-      float fzz = __builtin_fmaf(-2.0f, f0, D[c] + U[c]);
-      float cyx = colY[j][i + 1] - colY[j][i - 1];
-      float czx = dzc[j][i + 1] - dzc[j][i - 1];
-      float czy = dzc[j + 1][i] - dzc[j - 1][i];
-      float dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
-      float A = dy2 + dz2, B = dx2 + dz2, Cq = dx2 + dy2;
-      float G = A + dx2;
-      float N = A * fxx;
-      N = __builtin_fmaf(B, fyy, N);
-      N = __builtin_fmaf(Cq, fzz, N);
+    for (int a = 0; a < PX; a++) {
+      const int i = a + 1, j = b + 1, c = j * W + i;
+      const float f0 = M[c];
+      const float dx = M[c + 1] - M[c - 1], dy = colY[j][i], dz = dzc[j][i];
+      const float sxx = M[c - 1] + M[c + 1], syy = M[c - W] + M[c + W], szz = D[c] + U[c];
+      const float cyx = colY[j][i + 1] - colY[j][i - 1];
+      const float czx = dzc[j][i + 1] - dzc[j][i - 1];
+      const float czy = dzc[j + 1][i] - dzc[j - 1][i];
+      const float dz2 = dz * dz, dx2 = dx * dx;
+      const float A = __builtin_fmaf(dy, dy, dz2), B = dx2 + dz2, Cq = __builtin_fmaf(dy, dy, dx2);
+      const float G = A + dx2;
       float p1 = dy * cyx;
       p1 = __builtin_fmaf(dz, czx, p1);
       float t = (dy * dz) * czy;
       t = __builtin_fmaf(dx, p1, t);
+      const float r = __builtin_amdgcn_rcpf(G);
+#if MSBG_SWEEP_ALG
+      float Np = A * sxx;
+      Np = __builtin_fmaf(B, syy, Np);
+      Np = __builtin_fmaf(Cq, szz, Np);
+      Np = __builtin_fmaf(-0.5f, t, Np);
+      float Dv = __builtin_fmaf(Np * dt, r, m4dt * f0);  // dt * (Np/G - 4 f0)
+#else
+      float N = A * __builtin_fmaf(-2.0f, f0, sxx);
+      N = __builtin_fmaf(B, __builtin_fmaf(-2.0f, f0, syy), N);
+      N = __builtin_fmaf(Cq, __builtin_fmaf(-2.0f, f0, szz), N);
       N = __builtin_fmaf(-0.5f, t, N);
-      float r = __builtin_amdgcn_rcpf(G);
       float Dv = (N * dt) * r;
+#endif
       Dv = G > 4.0f * 1e-7f ? Dv : 0.0f;
       Dv = __builtin_amdgcn_fmed3f(Dv, -0.1f, 0.1f);
       // doConstrZeroOne: [lo,hi] = [0,1], else (-inf,inf)
-      out[b * 2 + a] = __builtin_amdgcn_fmed3f(f0 + Dv, lo, hi);
+      out[b * PX + a] = __builtin_amdgcn_fmed3f(f0 + Dv, lo, hi);
     }
 }
 
@@ -207,8 +209,23 @@ struct Src {
   }
 };
 
+// Voxel column per thread: 4 x 2 for 32^3 blocks (128 threads), 2 x 2 for
+// smaller blocks. Measured on MI355X (FP16 pages): 4x2 with one plane of
+// prefetch ran 10% faster than 2x2; 2x4, 4x4, 8x2 (register pressure) and a
+// two-plane prefetch (lower occupancy) were slower.
+template <int BSX>
+struct Tile {
+  static constexpr int PX = BSX >= 32 ? MSBG_SWEEP_PX32 : 2, PY = BSX >= 32 ? MSBG_SWEEP_PY32 : 2;
+  static constexpr int NT = (BSX / PX) * (BSX / PY);
+};
+
 template <int BSX, int ST, bool EXACT>
-__global__ __launch_bounds__((BSX / 2) * (BSX / 2)) void k_sweep(
+#if MSBG_SWEEP_WAVES > 0
+#define MSBG_SWEEP_LB(nt) __launch_bounds__(nt, MSBG_SWEEP_WAVES)
+#else
+#define MSBG_SWEEP_LB(nt) __launch_bounds__(nt)
+#endif
+__global__ MSBG_SWEEP_LB(Tile<BSX>::NT) void k_sweep(
     const int32_t *__restrict__ list, const int32_t *__restrict__ nbrTab,
     uint8_t *__restrict__ pagesRaw, uint8_t *__restrict__ sideRaw,
     const uint8_t *__restrict__ constsRaw, int64_t npages, float dtf, float lo01, float hi01,
@@ -216,14 +233,17 @@ __global__ __launch_bounds__((BSX / 2) * (BSX / 2)) void k_sweep(
   using S = Stor<ST>;
   using R = typename S::raw;
   using C = typename S::ct;
-  constexpr int NT = (BSX / 2) * (BSX / 2);
-  constexpr int P = BSX + 2;  // LDS row pitch (halo coords 0..BSX+1)
+  constexpr int PX = Tile<BSX>::PX, PY = Tile<BSX>::PY, NT = Tile<BSX>::NT;
+  constexpr int W = PX + 2, PN = (PX + 2) * (PY + 2);  // patch row stride / size
+  constexpr int P = BSX + 2;                            // LDS row pitch
   constexpr int PS = BSX * BSX;
   constexpr int64_t PE = (int64_t)BSX * BSX * BSX;
   constexpr int FS = 2 * PS, ES = 4 * BSX;
   constexpr int NR = 4 * BSX + 4;  // ring cells of one halo plane
   constexpr int RPT = (NR + NT - 1) / NT;
+  constexpr int LPT = PS / (4 * NT);  // 4-wide interior chunks per thread and plane
   constexpr int M = BSX - 1;
+  static_assert(LPT * 4 * NT == PS, "plane does not split into 4-wide chunks");
 
   __shared__ C buf[2][P * P];
   __shared__ int codes[27];
@@ -231,7 +251,7 @@ __global__ __launch_bounds__((BSX / 2) * (BSX / 2)) void k_sweep(
 
   const int tid = threadIdx.x;
   const int page = list[blockIdx.x];
-  // 27 codes; with 8^3 blocks the workgroup has only 16 threads
+  // 27 codes; small blocks have fewer than 27 threads
   for (int k = tid; k < 27; k += NT) codes[k] = nbrTab[(int64_t)page * 27 + k];
   __syncthreads();
 
@@ -244,10 +264,8 @@ __global__ __launch_bounds__((BSX / 2) * (BSX / 2)) void k_sweep(
   const R *consts = (const R *)constsRaw;  // [0..3] empty, [4..7] full
   const C dt = (C)dtf;
 
-  // load mapping: 4 consecutive x of one row
-  const int lx = 4 * (tid % (BSX / 4)), ly = tid / (BSX / 4);
-  // compute mapping: 2x2 column
-  const int x0 = 2 * (tid % (BSX / 2)), y0 = 2 * (tid / (BSX / 2));
+  // compute mapping: PX x PY column
+  const int x0 = PX * (tid % (BSX / PX)), y0 = PY * (tid / (BSX / PX));
 
   // interior of the planes below/above the block (uniform over the block)
   auto zsrc = [&](int dzi, bool top) -> const R * {
@@ -258,8 +276,8 @@ __global__ __launch_bounds__((BSX / 2) * (BSX / 2)) void k_sweep(
     return own + (top ? M : 0) * PS;  // Neumann: the block's own boundary plane
   };
   const R *zsrcM = zsrc(0, false), *zsrcP = zsrc(2, true);
-  const int moffM = (codes[4] == MSBG_BLOCK_EMPTY || codes[4] == MSBG_BLOCK_FULL) ? 0 : ly * BSX + lx;
-  const int moffP = (codes[22] == MSBG_BLOCK_EMPTY || codes[22] == MSBG_BLOCK_FULL) ? 0 : ly * BSX + lx;
+  const bool constM = codes[4] == MSBG_BLOCK_EMPTY || codes[4] == MSBG_BLOCK_FULL;
+  const bool constP = codes[22] == MSBG_BLOCK_EMPTY || codes[22] == MSBG_BLOCK_FULL;
 
   // ring cells of this thread and their sources
   Src<R> rs[RPT];
@@ -267,7 +285,7 @@ __global__ __launch_bounds__((BSX / 2) * (BSX / 2)) void k_sweep(
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
     int r = tid + q * NT;
-    if (r >= NR) r = NR - 1;  // duplicate of the last cell, never stored twice differently
+    if (r >= NR) r = NR - 1;  // loads a valid cell; only r < NR is stored
     int x, y;
     if (r < BSX) {
       x = -1;
@@ -289,7 +307,7 @@ __global__ __launch_bounds__((BSX / 2) * (BSX / 2)) void k_sweep(
     rlds[q] = (y + 1) * P + x + 1;
     const int dxi = x < 0 ? 0 : (x >= BSX ? 2 : 1), dyi = y < 0 ? 0 : (y >= BSX ? 2 : 1);
     const bool xside = dxi != 1, yside = dyi != 1;
-    // source pointer for neighbour code c at plane zz (zz = -1 / 0 / BSX)
+    // source pointer for the neighbour at (dxi, dyi, dzi) and plane zz
     auto one = [&](int dzi, int zz, int *stride) -> const R * {
       int c = codes[dzi * 9 + dyi * 3 + dxi];
       int zc = zz < 0 ? 0 : (zz > M ? M : zz);
@@ -335,97 +353,153 @@ __global__ __launch_bounds__((BSX / 2) * (BSX / 2)) void k_sweep(
     rs[q].stride = st0;
   }
 
-  C vin[4];
-  C vring[RPT];
+  // two register sets of staged plane values (prefetch distance 2 planes)
+  struct Staged {
+    C in[LPT * 4];
+    C ring[RPT];
+  };
   auto dec = [](R r) -> C { return (C)S::dec(r); };
 
-  auto load_plane = [&](int zz) {
-    const R *src = zz < 0 ? zsrcM + moffM : (zz >= BSX ? zsrcP + moffP : own + zz * PS + ly * BSX + lx);
-    R r[4];
-    load4(src, r);
-#pragma unroll
-    for (int k = 0; k < 4; k++) vin[k] = dec(r[k]);
-#pragma unroll
-    for (int q = 0; q < RPT; q++) vring[q] = dec(*rs[q].at(zz, BSX));
+  // chunk k of this thread: 4 consecutive x of one row
+  auto chunk_off = [&](int k) -> int {
+    int e = (tid + k * NT) * 4;
+    return e;  // = y * BSX + x with x multiple of 4
   };
 
-  auto put_plane = [&](C *b) {
-    C *row = b + (ly + 1) * P + lx + 1;
+  auto load_plane = [&](int zz, Staged &st) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) row[k] = vin[k];
+    for (int k = 0; k < LPT; k++) {
+      const int e = chunk_off(k);
+      const R *src = zz < 0 ? zsrcM + (constM ? 0 : e)
+                            : (zz >= BSX ? zsrcP + (constP ? 0 : e) : own + zz * PS + e);
+      R r[4];
+      load4(src, r);
+#pragma unroll
+      for (int u = 0; u < 4; u++) st.in[k * 4 + u] = dec(r[u]);
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; q++) st.ring[q] = dec(*rs[q].at(zz, BSX));
+  };
+
+  auto put_plane = [&](C *b, const Staged &st) {
+#pragma unroll
+    for (int k = 0; k < LPT; k++) {
+      const int e = chunk_off(k);
+      C *row = b + (e / BSX + 1) * P + (e % BSX) + 1;
+#pragma unroll
+      for (int u = 0; u < 4; u++) row[u] = st.in[k * 4 + u];
+    }
 #pragma unroll
     for (int q = 0; q < RPT; q++)
-      if (tid + q * NT < NR) b[rlds[q]] = vring[q];
+      if (tid + q * NT < NR) b[rlds[q]] = st.ring[q];
   };
 
-  auto get_patch = [&](const C *b, C (&p)[16]) {
+  auto get_patch = [&](const C *b, C (&p)[PN]) {
 #pragma unroll
-    for (int j = 0; j < 4; j++)
+    for (int j = 0; j < PY + 2; j++)
 #pragma unroll
-      for (int i = 0; i < 4; i++) p[j * 4 + i] = b[(y0 + j) * P + x0 + i];
+      for (int i = 0; i < PX + 2; i++) p[j * W + i] = b[(y0 + j) * P + x0 + i];
   };
 
   double acc = 0.0;
-  const bool wFace0 = x0 == 0, wFace1 = x0 == BSX - 2, wY0 = y0 == 0, wY1 = y0 == BSX - 2;
+  const bool wFace0 = x0 == 0, wFace1 = x0 == BSX - PX, wY0 = y0 == 0, wY1 = y0 == BSX - PY;
 
-  auto compute = [&](int z, const C (&Dp)[16], const C (&Mp)[16], const C (&Up)[16]) {
-    C o[4];
+  auto compute = [&](int z, const C (&Dp)[PN], const C (&Mp)[PN], const C (&Up)[PN]) {
+    C o[PX * PY];
     if constexpr (EXACT || sizeof(C) == 8) {
       const bool c01 = lo01 == 0.0f;
 #pragma unroll
-      for (int b = 0; b < 2; b++)
+      for (int b = 0; b < PY; b++)
 #pragma unroll
-        for (int a = 0; a < 2; a++)
-          o[b * 2 + a] = mcf_exact<C>(Dp, Mp, Up, (b + 1) * 4 + a + 1, dt, c01);
+        for (int a = 0; a < PX; a++)
+          o[b * PX + a] = mcf_exact<C, W>(Dp, Mp, Up, (b + 1) * W + a + 1, dt, c01);
     } else {
-      mcf_fast4(Dp, Mp, Up, dt, lo01, hi01, o);
+      mcf_fast<PX, PY>(Dp, Mp, Up, dt, lo01, hi01, o);
     }
-    R e[4];
+    R e[PX * PY];
 #pragma unroll
-    for (int k = 0; k < 4; k++) e[k] = S::enc(o[k]);
+    for (int k = 0; k < PX * PY; k++) e[k] = S::enc(o[k]);
 #pragma unroll
-    for (int b = 0; b < 2; b++) store2(own + z * PS + (y0 + b) * BSX + x0, e[b * 2], e[b * 2 + 1]);
+    for (int b = 0; b < PY; b++)
+#pragma unroll
+      for (int a = 0; a < PX; a += 2)
+        store2(own + z * PS + (y0 + b) * BSX + x0 + a, e[b * PX + a], e[b * PX + a + 1]);
     // keep the side-car faces and edges equal to the page
     if (wFace0) {
-      ownFaces[z * BSX + y0] = e[0];
-      ownFaces[z * BSX + y0 + 1] = e[2];
+#pragma unroll
+      for (int b = 0; b < PY; b++) ownFaces[z * BSX + y0 + b] = e[b * PX];
     }
     if (wFace1) {
-      ownFaces[PS + z * BSX + y0] = e[1];
-      ownFaces[PS + z * BSX + y0 + 1] = e[3];
+#pragma unroll
+      for (int b = 0; b < PY; b++) ownFaces[PS + z * BSX + y0 + b] = e[b * PX + PX - 1];
     }
     if (wFace0 && wY0) ownEdges[0 * BSX + z] = e[0];
-    if (wFace1 && wY0) ownEdges[1 * BSX + z] = e[1];
-    if (wFace0 && wY1) ownEdges[2 * BSX + z] = e[2];
-    if (wFace1 && wY1) ownEdges[3 * BSX + z] = e[3];
+    if (wFace1 && wY0) ownEdges[1 * BSX + z] = e[PX - 1];
+    if (wFace0 && wY1) ownEdges[2 * BSX + z] = e[(PY - 1) * PX];
+    if (wFace1 && wY1) ownEdges[3 * BSX + z] = e[PY * PX - 1];
     if (partial) {
 #pragma unroll
-      for (int b = 0; b < 2; b++)
+      for (int b = 0; b < PY; b++)
 #pragma unroll
-        for (int a = 0; a < 2; a++) {
-          double d = (double)S::dec(e[b * 2 + a]) - (double)Mp[(b + 1) * 4 + a + 1];
+        for (int a = 0; a < PX; a++) {
+          double d = (double)S::dec(e[b * PX + a]) - (double)Mp[(b + 1) * W + a + 1];
           acc += d * d;
         }
     }
   };
 
-  C pA[16], pB[16], pC[16];
+  C pA[PN], pB[PN], pC[PN];
+#if MSBG_SWEEP_PD == 2
+  Staged s0, s1;  // plane zz lives in s[zz & 1]
 
-  // prologue: planes -1 and 0 into pA, pB; plane 1 prefetched
-  load_plane(-1);
-  put_plane(buf[1]);
-  load_plane(0);
+  // prologue: planes -1 and 0 into pA, pB; planes 1 and 2 prefetched
+  load_plane(-1, s1);
+  load_plane(0, s0);
+  put_plane(buf[1], s1);
+  load_plane(1, s1);
   __syncthreads();
   get_patch(buf[1], pA);
-  put_plane(buf[0]);
-  load_plane(1);
+  put_plane(buf[0], s0);
+  load_plane(2, s0);
+  __syncthreads();
+  get_patch(buf[0], pB);
+
+  // step z: stage plane z+1, prefetch z+3 into its register set, compute z
+  auto step = [&](int z, Staged &st, C (&Dp)[PN], C (&Mp)[PN], C (&Up)[PN]) {
+    put_plane(buf[(z + 1) & 1], st);
+    if (z + 3 <= BSX) load_plane(z + 3, st);
+    __syncthreads();
+    get_patch(buf[(z + 1) & 1], Up);
+    compute(z, Dp, Mp, Up);
+  };
+
+  // unrolled by 6: patches rotate with period 3, register sets with period 2
+  for (int z = 0; z < BSX; z += 6) {
+    step(z, s1, pA, pB, pC);
+    if (z + 1 < BSX) step(z + 1, s0, pB, pC, pA);
+    if (z + 2 < BSX) step(z + 2, s1, pC, pA, pB);
+    if (z + 3 < BSX) step(z + 3, s0, pA, pB, pC);
+    if (z + 4 < BSX) step(z + 4, s1, pB, pC, pA);
+    if (z + 5 < BSX) step(z + 5, s0, pC, pA, pB);
+  }
+#else
+  Staged s0;
+
+  // prologue: planes -1 and 0 into pA, pB; plane 1 prefetched
+  load_plane(-1, s0);
+  put_plane(buf[1], s0);
+  load_plane(0, s0);
+  __syncthreads();
+  get_patch(buf[1], pA);
+  put_plane(buf[0], s0);
+  load_plane(1, s0);
   __syncthreads();
   get_patch(buf[0], pB);
 
   // step z: stage plane z+1 (prefetched), prefetch z+2, compute plane z
-  auto step = [&](int z, C (&Dp)[16], C (&Mp)[16], C (&Up)[16]) {
-    put_plane(buf[(z + 1) & 1]);
-    if (z + 2 <= BSX) load_plane(z + 2);
+  auto step = [&](int z, C (&Dp)[PN], C (&Mp)[PN], C (&Up)[PN]) {
+    put_plane(buf[(z + 1) & 1], s0);
+    if (z + 2 <= BSX) load_plane(z + 2, s0);
     __syncthreads();
     get_patch(buf[(z + 1) & 1], Up);
     compute(z, Dp, Mp, Up);
@@ -436,11 +510,12 @@ __global__ __launch_bounds__((BSX / 2) * (BSX / 2)) void k_sweep(
     if (z + 1 < BSX) step(z + 1, pB, pC, pA);
     if (z + 2 < BSX) step(z + 2, pC, pA, pB);
   }
+#endif
 
   if (partial) {
     // deterministic block reduction: wave shuffles, then waves in order
-    constexpr int W = NT < 64 ? NT : 64;
-    for (int off = W / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, W);
+    constexpr int WV = NT < 64 ? NT : 64;
+    for (int off = WV / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, WV);
     if ((tid & 63) == 0) red[tid >> 6] = acc;
     __syncthreads();
     if (tid == 0) {
@@ -476,7 +551,7 @@ int ensure_side(msbg_grid *g, ChanLevel &cl) {
 template <int BSX, int ST>
 int launch_t(msbg_grid *g, ChanLevel &cl, const int32_t *d_list, int64_t n, float dt, int c01,
              double *d_partial) {
-  constexpr int NT = (BSX / 2) * (BSX / 2);
+  constexpr int NT = Tile<BSX>::NT;
   MSBG_TRY((ensure_side<BSX, ST>(g, cl)));
   float lo = c01 ? 0.0f : -__builtin_inff(), hi = c01 ? 1.0f : __builtin_inff();
   for (int64_t off = 0; off < n; off += 0x7fffffff) {

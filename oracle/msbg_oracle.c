@@ -680,9 +680,11 @@ double or_mcf_voxel_f64(const double *h, int hsx, int hid, double dt,
   double H = (fy * fy + fz * fz) * fxx + (fx * fx + fz * fz) * fyy + (fx * fx + fy * fy) * fzz -
              2.0 * (fx * fy * fyx + fx * fz * fzx + fy * fz * fzy);
   double g = fx * fx + fy * fy + fz * fz;
-  H = g > 1e-7 ? H / g : 0.0;
+  /* the reference's threshold literal is the float 1e-7f (msbg4.cpp:604) */
+  H = g > (double)1e-7f ? H / g : 0.0;
   double D = dt * H;
-  D = vmind(vmaxd(D, -0.1), 0.1);
+  /* limiter with the reference's float literals -0.1f / 0.1f (msbg4.cpp:625) */
+  D = vmind(vmaxd(D, (double)-0.1f), (double)0.1f);
   double phi = f0 + D;
   if (doConstrZeroOne) phi = vmind(vmaxd(phi, 0.0), 1.0);
   return phi;
