@@ -47,6 +47,12 @@ CONFIGS = {
 
 STORAGES = {"u16": 1, "f32": 2, "f16": 3, "bf16": 4, "f64": 5}
 
+# HBM bytes per colour-pass launch / algorithmic bytes, measured with
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE on this workload, each
+# scaled by the same counter's reading for k_copy16 (a copy of known size:
+# FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); profiles/k_sweep_cfg2_f16.md
+TRAFFIC_RATIO = {("cfg2", "f16", "fast"): 1.163}
+
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
@@ -247,7 +253,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": (TRAFFIC_RATIO[(args.config, args.storage, args.arith)] * bytes_per_pass_avg
+                        if (args.config, args.storage, args.arith) in TRAFFIC_RATIO else None),
             "kernel": f"k_sweep<{bsx},{args.storage}> colour pass",
             "avg_launch_ms": avg_pass_ms,
             "launches": npasses,
