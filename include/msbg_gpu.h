@@ -209,6 +209,19 @@ int msbg_download_block(msbg_grid *g, int chan, int level, int bid,
 int msbg_upload_block(msbg_grid *g, int chan, int level, int bid,
                       const void *data);
 
+/* SparseGrid<T>::copyDataTo (sbg.h:550-557) with a storage change: chanDst
+ * gets chanSrc's block map at `level` and its values converted (decoded
+ * exactly, re-encoded with round-to-nearest-even) into dstStorage. */
+int msbg_channel_copy(msbg_grid *g, int chanSrc, int chanDst, int dstStorage,
+                      int level);
+
+/* Max |a-b| and RMS(a-b) over all value voxels of two channels with the
+ * same block map at `level`, accumulated in FP64 on the device. No
+ * counterpart in the reference; used to report storage-format error against
+ * an FP64 run. */
+int msbg_channel_diff(msbg_grid *g, int chanA, int chanB, int level,
+                      double *maxAbs, double *rms);
+
 /* Halo index table of (chan, level): for each value page p (in page order)
  * 27 entries k = (dz+1)*9 + (dy+1)*3 + (dx+1) holding the neighbour's page
  * or a MSBG_BLOCK_* code (OUT outside the domain). out: npages*27 ints. */

@@ -83,6 +83,8 @@ SIGNATURES = {
     "msbg_last_sweep_timing": (_I, [_P, _P, _P, _P]),
     "msbg_scene_spheres": (_I, [_U64, _I, _F, _F, _F, _F, _I64, _F, _F, _P]),
     "msbg_copy_bandwidth": (_I, [_I64, _I, _P]),
+    "msbg_channel_copy": (_I, [_P, _I, _I, _I, _I]),
+    "msbg_channel_diff": (_I, [_P, _I, _I, _I, _P, _P]),
 }
 
 _lib = None

@@ -572,6 +572,56 @@ int msbg_upload_block(msbg_grid *g, int chan, int level, int bid, const void *da
   return block_io(g, chan, level, bid, (void *)data, false);
 }
 
+int msbg_channel_copy(msbg_grid *g, int chanSrc, int chanDst, int dstStorage, int level) {
+  Channel *src;
+  ChanLevel *cs;
+  MSBG_TRY(get_level(g, chanSrc, level, &src, &cs));
+  MSBG_CHECK(chanDst > 0 && chanDst != chanSrc && storage_bytes(dstStorage) > 0, MSBG_EINVAL,
+             "msbg_channel_copy: bad destination %d / storage %d", chanDst, dstStorage);
+  Channel &dst = g->channels[chanDst];
+  src = find_channel(g, chanSrc);  // the map insert above may move nothing, re-find anyway
+  cs = &src->lv[level];
+  for (int l = 0; l < kMaxLevels; l++)
+    MSBG_CHECK(!dst.lv[l].defined || dst.storage == dstStorage || l == level, MSBG_EINVAL,
+               "channel %d already has storage %d", chanDst, dst.storage);
+  dst.id = chanDst;
+  dst.storage = dstStorage;
+  // same block map (kinds) as the source; const values converted
+  std::vector<uint8_t> kind((size_t)g->nblocks);
+  for (int64_t b = 0; b < g->nblocks; b++) {
+    int32_t c = cs->h_page[b];
+    kind[b] = c >= 0 ? MSBG_KIND_VALUE
+                     : (c == MSBG_BLOCK_EMPTY ? MSBG_KIND_EMPTY
+                                              : (c == MSBG_BLOCK_FULL ? MSBG_KIND_FULL : MSBG_KIND_NULL));
+  }
+  auto conv = [&](double raw) {
+    double v = src->storage == MSBG_STORAGE_U16 ? raw * (double)(1.0f / 65535.0f) : raw;
+    if (dstStorage == MSBG_STORAGE_U16) return std::nearbyint(v * 65535.0);
+    return v;
+  };
+  MSBG_TRY(define_level(g, dst, level, kind.data(), conv(cs->empty_raw), conv(cs->full_raw), 0.0));
+  MSBG_TRY(launch_convert(g, *cs, src->storage, dst.lv[level], dstStorage));
+  dst.lv[level].side_valid = false;
+  MSBG_HIP(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int msbg_channel_diff(msbg_grid *g, int chanA, int chanB, int level, double *maxAbs,
+                      double *rms) {
+  Channel *a, *b;
+  ChanLevel *ca, *cb;
+  MSBG_TRY(get_level(g, chanA, level, &a, &ca));
+  MSBG_TRY(get_level(g, chanB, level, &b, &cb));
+  MSBG_CHECK(ca->npages == cb->npages && ca->h_page == cb->h_page, MSBG_EINVAL,
+             "msbg_channel_diff: channels %d and %d have different block maps", chanA, chanB);
+  double mx = 0, ss = 0;
+  MSBG_TRY(launch_diff(g, *ca, a->storage, *cb, b->storage, &mx, &ss));
+  int64_t n = ca->npages * (int64_t)ca->bsx * ca->bsx * ca->bsx;
+  if (maxAbs) *maxAbs = mx;
+  if (rms) *rms = n ? std::sqrt(ss / (double)n) : 0.0;
+  return 0;
+}
+
 int msbg_get_halo_table(msbg_grid *g, int chan, int level, int32_t *out, int64_t cap) {
   Channel *ch;
   ChanLevel *cl;

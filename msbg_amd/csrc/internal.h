@@ -162,7 +162,9 @@ int launch_sweep_pass(msbg_grid *g, Channel &ch, const int32_t *d_list,
                       int64_t n, float dt, int doConstr01, double *d_partial);
 int launch_reduce_residual(msbg_grid *g, const double *d_partial, int64_t n,
                            double *d_out);
-int launch_domain_filter(msbg_grid *g, const float *d_in, int64_t n,
-                         float *d_out, int64_t *nkept);
+int launch_convert(msbg_grid *g, const ChanLevel &src, int sst, ChanLevel &dst,
+                   int dst_st);
+int launch_diff(msbg_grid *g, const ChanLevel &a, int ast, const ChanLevel &b, int bst,
+                double *maxAbs, double *sumSq);
 
 }  // namespace msbg

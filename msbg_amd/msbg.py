@@ -212,6 +212,17 @@ class MultiresSparseGrid:
         a = np.ascontiguousarray(pages, dtype=_NP_DTYPE[st])
         check(lib().msbg_channel_upload(self._h, chan, level, _ptr(a), a.nbytes))
 
+    def copyDataTo(self, chanSrc, chanDst, storage, level=0):
+        """SparseGrid::copyDataTo (sbg.h:550-557) into another storage format."""
+        check(lib().msbg_channel_copy(self._h, chanSrc, chanDst, storage, level))
+        self._storage[chanDst] = storage
+
+    def diff(self, chanA, chanB, level=0):
+        """(max |a-b|, RMS(a-b)) over the value voxels, FP64 on the device."""
+        mx, rms = C.c_double(), C.c_double()
+        check(lib().msbg_channel_diff(self._h, chanA, chanB, level, C.byref(mx), C.byref(rms)))
+        return mx.value, rms.value
+
     def halo_table(self, chan, level=0):
         _, np_ = self.pagemap(chan, level)
         out = np.empty((np_, 27), dtype=np.int32)

@@ -354,3 +354,42 @@ def test_cpp_facade_demo_matches_binding(tmp_path):
     xyz = M.scene_spheres(99, 6, 30, 98, 10, 25, 4000, 0.5, 0.3)
     gp = gpu_pipeline(S, bsx, xyz, numIter=5, dt=0.1, arith=M.ARITH_FAST)
     assert np.array_equal(got, gp["pages"].reshape(-1))
+
+
+# --------------------------------------------------------------------------
+# storage conversion, FP64 differences and the precision study
+# --------------------------------------------------------------------------
+def test_channel_copy_and_diff():
+    note("channel copy/diff")
+    S, bsx = 64, 16
+    rng = np.random.default_rng(12)
+    with M.MultiresSparseGrid.create("c", S, S, S, bsx, options=M.OPT_SINGLE_LEVEL, nLevels=2) as g:
+        kind = np.where(rng.random(g.nBlocks()) < 0.5, 3, 1).astype(np.uint8)
+        g.channel_define(1, M.F32, 0, kind, 0.0, 1.0, 0.0)
+        vals = rng.random(g.download(1).shape).astype(np.float32)
+        g.upload(1, 0, vals)
+        g.copyDataTo(1, 2, M.F16)
+        g.copyDataTo(1, 3, M.BF16)
+        g.copyDataTo(1, 4, M.F64)
+        g.copyDataTo(1, 5, M.U16)
+        assert np.array_equal(g.download(2).view(np.uint16), vals.astype(np.float16).view(np.uint16))
+        Lb = O.load()
+        bf = np.array([Lb.or_bf16_from_float(float(v)) for v in vals.ravel()], np.uint16)
+        assert np.array_equal(g.download(3).ravel(), bf)
+        assert np.array_equal(g.download(4), vals.astype(np.float64))
+        assert np.array_equal(g.download(5), np.rint(vals * np.float32(65535)).astype(np.uint16))
+        mx, rms = g.diff(2, 4)
+        d = vals.astype(np.float16).astype(np.float64) - vals.astype(np.float64)
+        assert abs(mx - np.abs(d).max()) < 1e-12 and abs(rms - np.sqrt((d ** 2).mean())) < 1e-12
+
+
+def test_precision_study_small():
+    from msbg_amd.studies import precision_study
+    note("precision study 256")
+    r = precision_study(S=256, nObj=24, rmin=10, rmax=40, nPts=6000, sweeps=8, every=4)
+    last = r["rows"][-1]
+    # FP32 storage tracks FP64 closely; 16-bit formats in order of their step
+    assert last["f32"]["max"] < 1e-4
+    assert last["f32"]["rms"] < last["u16"]["rms"] < last["f16"]["rms"] < last["bf16"]["rms"]
+    assert last["bf16"]["max"] < 0.1
+    assert all(np.isfinite(last[k]["residual"]) for k in ("f32", "f16", "bf16", "u16"))
