@@ -296,6 +296,37 @@ int msbg_last_sweep_timing(msbg_grid *g, double *total_ms,
 int msbg_copy_bandwidth(int64_t bytes, int reps, double *gbps);
 
 /* ======================================================================== */
+/* batched sweeps of independent grids                                      */
+/* ======================================================================== */
+
+/* A batch runs applyChannelPdeFast's 8-colour MCF sweep (msbg4.cpp:60-957,
+ * same per-grid semantics as msbg_apply_channel_pde_fast over every level-0
+ * value block) on many independent grids at once: one launch per colour pass
+ * and kernel group (block size x storage x arithmetic), the whole run
+ * captured into one HIP graph. No counterpart in the reference, which
+ * sweeps one grid per call. Member grids must outlive the batch and stay
+ * idle while it runs; a change to a member's blocks or pages is picked up
+ * by the next call. */
+typedef struct msbg_batch msbg_batch;
+int msbg_batch_create(msbg_batch **out);
+int msbg_batch_destroy(msbg_batch *b);
+int msbg_batch_add(msbg_batch *b, msbg_grid *g, int chan, float timeStep,
+                   int doConstrZeroOne);
+/* Time each voxel-column variant of each kernel group on the group's own
+ * blocks (`passes` launches of every colour; the pages are restored after)
+ * and keep the fastest; member grids adopt it for their own sweeps. */
+int msbg_batch_autotune(msbg_batch *b, int passes);
+/* Kernel groups of the batch and their variant (0 = 2x2, 1 = 4x2 columns). */
+int msbg_batch_get_tiles(msbg_batch *b, int *bsx, int *storage, int *tile,
+                         int cap, int *n);
+/* numIter rounded up to even sweeps of every member grid; useGraph != 0
+ * replays one captured HIP graph (captured on first use per sweep count).
+ * *ms receives the device time of the run. */
+int msbg_batch_run(msbg_batch *b, int numIter, int useGraph, double *ms);
+/* Voxel-column variant used by this grid's own sweeps (-1 = default). */
+int msbg_set_tile(msbg_grid *g, int tile);
+
+/* ======================================================================== */
 /* synthetic scenes (host)                                                  */
 /* ======================================================================== */
 

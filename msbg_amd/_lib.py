@@ -84,6 +84,13 @@ SIGNATURES = {
     "msbg_scene_spheres": (_I, [_U64, _I, _F, _F, _F, _F, _I64, _F, _F, _P]),
     "msbg_copy_bandwidth": (_I, [_I64, _I, _P]),
     "msbg_channel_copy": (_I, [_P, _I, _I, _I, _I]),
+    "msbg_batch_create": (_I, [C.POINTER(_P)]),
+    "msbg_batch_destroy": (_I, [_P]),
+    "msbg_batch_add": (_I, [_P, _P, _I, _F, _I]),
+    "msbg_batch_autotune": (_I, [_P, _I]),
+    "msbg_batch_get_tiles": (_I, [_P, _P, _P, _P, _I, _P]),
+    "msbg_batch_run": (_I, [_P, _I, _I, _P]),
+    "msbg_set_tile": (_I, [_P, _I]),
     "msbg_channel_diff": (_I, [_P, _I, _I, _I, _P, _P]),
 }
 

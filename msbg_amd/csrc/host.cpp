@@ -87,7 +87,7 @@ static Channel *find_channel(msbg_grid *g, int chan) {
 static int level_bsx(const msbg_grid *g, int level) { return g->bsx0 >> level; }
 
 // Make sure the halo index table of (chan, level) matches its block map.
-static int ensure_nbr(msbg_grid *g, ChanLevel &cl, int level) {
+int ensure_nbr(msbg_grid *g, ChanLevel &cl, int level) {
   if (cl.nbr_valid) return 0;
   if (cl.npages > 0) {
     MSBG_TRY(cl.d_nbr.ensure((size_t)cl.npages * 27));
@@ -799,15 +799,15 @@ int msbg_apply_channel_pde_fast(msbg_grid *g, int chPhi, int chTmp, int chTmp2,
       for (int64_t b = 0; b < g->nblocks; b++)
         if (cl->h_page[b] >= 0) MSBG_TRY(add(b));
     }
-    std::vector<int32_t> flat;
+    std::vector<int2> flat;
     off[0] = 0;
     for (int c = 0; c < 8; c++) {
-      flat.insert(flat.end(), per[c].begin(), per[c].end());
+      for (int32_t pg : per[c]) flat.push_back(make_int2(0, pg));
       off[c + 1] = (int64_t)flat.size();
     }
     if (off[8]) {
       MSBG_TRY(g->d_sweep_list.ensure((size_t)off[8]));
-      MSBG_HIP(hipMemcpyAsync(g->d_sweep_list.p, flat.data(), sizeof(int32_t) * off[8],
+      MSBG_HIP(hipMemcpyAsync(g->d_sweep_list.p, flat.data(), sizeof(int2) * off[8],
                               hipMemcpyHostToDevice, g->stream));
       MSBG_HIP(hipStreamSynchronize(g->stream));
     }
@@ -822,14 +822,24 @@ int msbg_apply_channel_pde_fast(msbg_grid *g, int chPhi, int chTmp, int chTmp2,
     MSBG_TRY(g->d_partial.ensure((size_t)std::max<int64_t>(ntot, 1)));
     MSBG_TRY(g->d_resid.ensure((size_t)nMaxIter));
   }
+  if (ntot) {
+    MSBG_TRY(sweep_prepare(g, *ch));
+    SweepGrid desc = sweep_desc(*ch, timeStep, doConstrZeroOne);
+    MSBG_TRY(g->d_desc.ensure(1));
+    MSBG_HIP(hipMemcpyAsync(g->d_desc.p, &desc, sizeof desc, hipMemcpyHostToDevice, g->stream));
+    MSBG_HIP(hipStreamSynchronize(g->stream));
+  }
+  const bool exact = g->arith == MSBG_ARITH_EXACT;
+  const int tile = g->tile[g->lg0];
   int64_t npasses = 0;
   MSBG_HIP(hipEventRecord(g->ev[0], g->stream));
   for (int it = 0; it < nMaxIter && ntot > 0; it++) {
     for (int pass = 0; pass < 8; pass++) {
       int64_t n = off[pass + 1] - off[pass];
       if (!n) continue;
-      MSBG_TRY(launch_sweep_pass(g, *ch, g->d_sweep_list.p + off[pass], n, timeStep,
-                                 doConstrZeroOne, residual ? g->d_partial.p + off[pass] : nullptr));
+      MSBG_TRY(launch_sweep_items(g->stream, g->bsx0, ch->storage, exact, tile, g->d_desc.p,
+                                  g->d_sweep_list.p + off[pass], n,
+                                  residual ? g->d_partial.p + off[pass] : nullptr));
       npasses++;
     }
     if (residual)

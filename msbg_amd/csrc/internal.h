@@ -72,6 +72,18 @@ struct DevBuf {
   ~DevBuf() { free(); }
 };
 
+// Everything the sweep kernel needs to know about one grid's channel at
+// level 0 (device-side descriptor; one per grid of a batched launch).
+struct SweepGrid {
+  uint8_t *pages;
+  uint8_t *side;
+  const uint8_t *consts;
+  const int32_t *nbr;
+  int64_t npages;
+  float dt, lo, hi;
+  int pad;
+};
+
 // One resolution level of one channel.
 struct ChanLevel {
   bool defined = false;
@@ -126,7 +138,9 @@ struct msbg_grid {
   std::map<int, msbg::Channel> channels;
 
   // sweep scratch
-  msbg::DevBuf<int32_t> d_sweep_list;   // colour-sorted page indices
+  msbg::DevBuf<int2> d_sweep_list;      // colour-sorted {grid 0, page} items
+  msbg::DevBuf<msbg::SweepGrid> d_desc; // this grid's sweep descriptor
+  int tile[8] = {-1, -1, -1, -1, -1, -1, -1, -1};  // voxel-column variant per bsx log2 (-1 = default)
   msbg::DevBuf<double> d_partial;       // per-block residual partials
   msbg::DevBuf<double> d_resid;         // per-sweep residuals
   uint64_t topo_version = 1;             // bumped when block maps or levels change
@@ -158,8 +172,15 @@ int launch_halo_fill(msbg_grid *g, Channel &ch, int level, int bid, int dmax,
                      unsigned opt, float *d_out);
 int launch_ghost(msbg_grid *g, Channel &ch, int level, const int32_t *d_bids,
                  int64_t n);
-int launch_sweep_pass(msbg_grid *g, Channel &ch, const int32_t *d_list,
-                      int64_t n, float dt, int doConstr01, double *d_partial);
+// sweep kernel of one colour pass over work items {grid, page}
+int sweep_prepare(msbg_grid *g, Channel &ch);  // side-car faces/edges + consts
+SweepGrid sweep_desc(Channel &ch, float dt, int doConstr01);
+int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, int tile,
+                       const SweepGrid *d_grids, const int2 *d_items, int64_t n,
+                       double *d_partial);
+int default_tile(int bsx);
+constexpr int kNumTiles = 2;  // voxel-column variants: 0 = 2x2, 1 = 4x2
+int ensure_nbr(msbg_grid *g, ChanLevel &cl, int level);
 int launch_reduce_residual(msbg_grid *g, const double *d_partial, int64_t n,
                            double *d_out);
 int launch_convert(msbg_grid *g, const ChanLevel &src, int sst, ChanLevel &dst,

@@ -35,12 +35,6 @@ namespace {
 #ifndef MSBG_SWEEP_PD
 #define MSBG_SWEEP_PD 1  // plane prefetch distance (1 or 2)
 #endif
-#ifndef MSBG_SWEEP_PX32
-#define MSBG_SWEEP_PX32 4  // voxel column width for 32^3 blocks
-#endif
-#ifndef MSBG_SWEEP_PY32
-#define MSBG_SWEEP_PY32 2  // voxel column height for 32^3 blocks
-#endif
 #ifndef MSBG_SWEEP_WAVES
 #define MSBG_SWEEP_WAVES 0  // minimum waves per SIMD requested (0 = none)
 #endif
@@ -209,31 +203,29 @@ struct Src {
   }
 };
 
-// Voxel column per thread: 4 x 2 for 32^3 blocks (128 threads), 2 x 2 for
-// smaller blocks. Measured on MI355X (FP16 pages): 4x2 with one plane of
-// prefetch ran 10% faster than 2x2; 2x4, 4x4, 8x2 (register pressure) and a
-// two-plane prefetch (lower occupancy) were slower.
-template <int BSX>
+// Voxel column per thread (PX x PY): the default is 4 x 2 for 32^3 blocks
+// (128 threads) and 2 x 2 for smaller blocks. Measured on MI355X (FP16
+// pages, 32^3 blocks): 4x2 with one plane of prefetch ran 10% faster than
+// 2x2; 2x4, 4x4, 8x2 (register pressure) and a two-plane prefetch (lower
+// occupancy) were slower. Batches pick per block size with an autotuner.
+template <int BSX, int PX, int PY>
 struct Tile {
-  static constexpr int PX = BSX >= 32 ? MSBG_SWEEP_PX32 : 2, PY = BSX >= 32 ? MSBG_SWEEP_PY32 : 2;
   static constexpr int NT = (BSX / PX) * (BSX / PY);
 };
 
-template <int BSX, int ST, bool EXACT>
 #if MSBG_SWEEP_WAVES > 0
 #define MSBG_SWEEP_LB(nt) __launch_bounds__(nt, MSBG_SWEEP_WAVES)
 #else
 #define MSBG_SWEEP_LB(nt) __launch_bounds__(nt)
 #endif
-__global__ MSBG_SWEEP_LB(Tile<BSX>::NT) void k_sweep(
-    const int32_t *__restrict__ list, const int32_t *__restrict__ nbrTab,
-    uint8_t *__restrict__ pagesRaw, uint8_t *__restrict__ sideRaw,
-    const uint8_t *__restrict__ constsRaw, int64_t npages, float dtf, float lo01, float hi01,
+template <int BSX, int ST, bool EXACT, int PX, int PY>
+__global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT)) void k_sweep(
+    const SweepGrid *__restrict__ grids, const int2 *__restrict__ items,
     double *__restrict__ partial) {
   using S = Stor<ST>;
   using R = typename S::raw;
   using C = typename S::ct;
-  constexpr int PX = Tile<BSX>::PX, PY = Tile<BSX>::PY, NT = Tile<BSX>::NT;
+  constexpr int NT = Tile<BSX, PX, PY>::NT;
   constexpr int W = PX + 2, PN = (PX + 2) * (PY + 2);  // patch row stride / size
   constexpr int P = BSX + 2;                            // LDS row pitch
   constexpr int PS = BSX * BSX;
@@ -245,12 +237,22 @@ __global__ MSBG_SWEEP_LB(Tile<BSX>::NT) void k_sweep(
   constexpr int M = BSX - 1;
   static_assert(LPT * 4 * NT == PS, "plane does not split into 4-wide chunks");
 
+  // work item: which grid, which page of it
+  const int2 item = items[blockIdx.x];
+  const SweepGrid G = grids[item.x];
+  const int32_t *__restrict__ nbrTab = G.nbr;
+  uint8_t *pagesRaw = G.pages;
+  uint8_t *sideRaw = G.side;
+  const uint8_t *constsRaw = G.consts;
+  const int64_t npages = G.npages;
+  const float dtf = G.dt, lo01 = G.lo, hi01 = G.hi;
+
   __shared__ C buf[2][P * P];
   __shared__ int codes[27];
   __shared__ double red[(NT + 63) / 64];
 
   const int tid = threadIdx.x;
-  const int page = list[blockIdx.x];
+  const int page = item.y;
   // 27 codes; small blocks have fewer than 27 threads
   for (int k = tid; k < 27; k += NT) codes[k] = nbrTab[(int64_t)page * 27 + k];
   __syncthreads();
@@ -549,38 +551,60 @@ int ensure_side(msbg_grid *g, ChanLevel &cl) {
 }
 
 template <int BSX, int ST>
-int launch_t(msbg_grid *g, ChanLevel &cl, const int32_t *d_list, int64_t n, float dt, int c01,
-             double *d_partial) {
-  constexpr int NT = Tile<BSX>::NT;
-  MSBG_TRY((ensure_side<BSX, ST>(g, cl)));
-  float lo = c01 ? 0.0f : -__builtin_inff(), hi = c01 ? 1.0f : __builtin_inff();
+int prepare_t(msbg_grid *g, ChanLevel &cl) {
+  return ensure_side<BSX, ST>(g, cl);
+}
+
+template <int BSX, int ST, bool EXACT, int PX, int PY>
+int launch_one(hipStream_t s, const SweepGrid *d_grids, const int2 *d_items, int64_t n,
+               double *d_partial) {
+  constexpr int NT = Tile<BSX, PX, PY>::NT;
   for (int64_t off = 0; off < n; off += 0x7fffffff) {
     unsigned cnt = (unsigned)(n - off < 0x7fffffff ? n - off : 0x7fffffff);
-    bool exact = g->arith == MSBG_ARITH_EXACT;
-    if constexpr (ST == MSBG_STORAGE_F64) exact = true;  // the FP64 run is always exact
-    if (exact)
-      hipLaunchKernelGGL((k_sweep<BSX, ST, true>), dim3(cnt), dim3(NT), 0, g->stream, d_list + off,
-                         cl.d_nbr.p, cl.d_pages.p, cl.d_side.p, cl.d_consts.p, cl.npages, dt, lo,
-                         hi, d_partial ? d_partial + off : nullptr);
-    else if constexpr (ST != MSBG_STORAGE_F64)
-      hipLaunchKernelGGL((k_sweep<BSX, ST, false>), dim3(cnt), dim3(NT), 0, g->stream, d_list + off,
-                         cl.d_nbr.p, cl.d_pages.p, cl.d_side.p, cl.d_consts.p, cl.npages, dt, lo,
-                         hi, d_partial ? d_partial + off : nullptr);
+    hipLaunchKernelGGL((k_sweep<BSX, ST, EXACT, PX, PY>), dim3(cnt), dim3(NT), 0, s, d_grids,
+                       d_items + off, d_partial ? d_partial + off : nullptr);
     MSBG_HIP(hipGetLastError());
   }
   return 0;
 }
 
+template <int BSX, int ST>
+int launch_st(hipStream_t s, bool exact, int tile, const SweepGrid *d_grids, const int2 *d_items,
+              int64_t n, double *d_partial) {
+  if constexpr (ST == MSBG_STORAGE_F64) exact = true;  // the FP64 run is always exact
+  if (exact)  // the exact path keeps the default column shape
+    return BSX >= 32 ? launch_one<BSX, ST, true, 4, 2>(s, d_grids, d_items, n, d_partial)
+                     : launch_one<BSX, ST, true, 2, 2>(s, d_grids, d_items, n, d_partial);
+  if constexpr (ST != MSBG_STORAGE_F64) {
+    if (tile == 1) return launch_one<BSX, ST, false, 4, 2>(s, d_grids, d_items, n, d_partial);
+    return launch_one<BSX, ST, false, 2, 2>(s, d_grids, d_items, n, d_partial);
+  }
+  return 0;
+}
+
 template <int BSX>
-int launch_bsx(msbg_grid *g, Channel &ch, const int32_t *d_list, int64_t n, float dt, int c01,
-               double *d_partial) {
+int launch_bsx(hipStream_t s, int storage, bool exact, int tile, const SweepGrid *d_grids,
+               const int2 *d_items, int64_t n, double *d_partial) {
+  switch (storage) {
+    case MSBG_STORAGE_U16: return launch_st<BSX, MSBG_STORAGE_U16>(s, exact, tile, d_grids, d_items, n, d_partial);
+    case MSBG_STORAGE_F32: return launch_st<BSX, MSBG_STORAGE_F32>(s, exact, tile, d_grids, d_items, n, d_partial);
+    case MSBG_STORAGE_F16: return launch_st<BSX, MSBG_STORAGE_F16>(s, exact, tile, d_grids, d_items, n, d_partial);
+    case MSBG_STORAGE_BF16: return launch_st<BSX, MSBG_STORAGE_BF16>(s, exact, tile, d_grids, d_items, n, d_partial);
+    case MSBG_STORAGE_F64: return launch_st<BSX, MSBG_STORAGE_F64>(s, exact, tile, d_grids, d_items, n, d_partial);
+  }
+  set_error("sweep: bad storage %d", storage);
+  return MSBG_EINVAL;
+}
+
+template <int BSX>
+int prepare_bsx(msbg_grid *g, Channel &ch) {
   ChanLevel &cl = ch.lv[0];
   switch (ch.storage) {
-    case MSBG_STORAGE_U16: return launch_t<BSX, MSBG_STORAGE_U16>(g, cl, d_list, n, dt, c01, d_partial);
-    case MSBG_STORAGE_F32: return launch_t<BSX, MSBG_STORAGE_F32>(g, cl, d_list, n, dt, c01, d_partial);
-    case MSBG_STORAGE_F16: return launch_t<BSX, MSBG_STORAGE_F16>(g, cl, d_list, n, dt, c01, d_partial);
-    case MSBG_STORAGE_BF16: return launch_t<BSX, MSBG_STORAGE_BF16>(g, cl, d_list, n, dt, c01, d_partial);
-    case MSBG_STORAGE_F64: return launch_t<BSX, MSBG_STORAGE_F64>(g, cl, d_list, n, dt, c01, d_partial);
+    case MSBG_STORAGE_U16: return prepare_t<BSX, MSBG_STORAGE_U16>(g, cl);
+    case MSBG_STORAGE_F32: return prepare_t<BSX, MSBG_STORAGE_F32>(g, cl);
+    case MSBG_STORAGE_F16: return prepare_t<BSX, MSBG_STORAGE_F16>(g, cl);
+    case MSBG_STORAGE_BF16: return prepare_t<BSX, MSBG_STORAGE_BF16>(g, cl);
+    case MSBG_STORAGE_F64: return prepare_t<BSX, MSBG_STORAGE_F64>(g, cl);
   }
   set_error("sweep: bad storage %d", ch.storage);
   return MSBG_EINVAL;
@@ -588,14 +612,42 @@ int launch_bsx(msbg_grid *g, Channel &ch, const int32_t *d_list, int64_t n, floa
 
 }  // namespace
 
-int launch_sweep_pass(msbg_grid *g, Channel &ch, const int32_t *d_list, int64_t n, float dt,
-                      int doConstr01, double *d_partial) {
+int default_tile(int bsx) { return bsx >= 32 ? 1 : 0; }
+
+int sweep_prepare(msbg_grid *g, Channel &ch) {
   switch (g->bsx0) {
-    case 8: return launch_bsx<8>(g, ch, d_list, n, dt, doConstr01, d_partial);
-    case 16: return launch_bsx<16>(g, ch, d_list, n, dt, doConstr01, d_partial);
-    case 32: return launch_bsx<32>(g, ch, d_list, n, dt, doConstr01, d_partial);
+    case 8: return prepare_bsx<8>(g, ch);
+    case 16: return prepare_bsx<16>(g, ch);
+    case 32: return prepare_bsx<32>(g, ch);
   }
   set_error("sweep: block size %d not supported by the sweep kernel (8, 16, 32)", g->bsx0);
+  return MSBG_EUNSUPP;
+}
+
+SweepGrid sweep_desc(Channel &ch, float dt, int doConstr01) {
+  ChanLevel &cl = ch.lv[0];
+  SweepGrid d{};
+  d.pages = cl.d_pages.p;
+  d.side = cl.d_side.p;
+  d.consts = cl.d_consts.p;
+  d.nbr = cl.d_nbr.p;
+  d.npages = cl.npages;
+  d.dt = dt;
+  d.lo = doConstr01 ? 0.0f : -__builtin_inff();
+  d.hi = doConstr01 ? 1.0f : __builtin_inff();
+  return d;
+}
+
+int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, int tile,
+                       const SweepGrid *d_grids, const int2 *d_items, int64_t n,
+                       double *d_partial) {
+  if (tile < 0) tile = default_tile(bsx);
+  switch (bsx) {
+    case 8: return launch_bsx<8>(s, storage, exact, tile, d_grids, d_items, n, d_partial);
+    case 16: return launch_bsx<16>(s, storage, exact, tile, d_grids, d_items, n, d_partial);
+    case 32: return launch_bsx<32>(s, storage, exact, tile, d_grids, d_items, n, d_partial);
+  }
+  set_error("sweep: block size %d not supported by the sweep kernel (8, 16, 32)", bsx);
   return MSBG_EUNSUPP;
 }
 

@@ -260,10 +260,59 @@ class MultiresSparseGrid:
             _ptr(res)))
         return res[:nsweeps] if residual else None
 
+    def set_tile(self, tile):
+        check(lib().msbg_set_tile(self._h, tile))
+
     def last_sweep_timing(self):
         t, a, n = C.c_double(), C.c_double(), C.c_int64()
         check(lib().msbg_last_sweep_timing(self._h, C.byref(t), C.byref(a), C.byref(n)))
         return t.value, a.value, n.value
+
+
+class Batch:
+    """Batched 8-colour MCF sweeps of independent grids (one graph launch)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().msbg_batch_create(C.byref(h)))
+        self._h = h
+        self._grids = []  # keep members alive
+
+    def add(self, grid, chan, timeStep=0.1, doConstrZeroOne=1):
+        check(lib().msbg_batch_add(self._h, grid._h, chan, timeStep, doConstrZeroOne))
+        self._grids.append(grid)
+
+    def autotune(self, passes=3):
+        check(lib().msbg_batch_autotune(self._h, passes))
+
+    def tiles(self):
+        n = C.c_int()
+        check(lib().msbg_batch_get_tiles(self._h, None, None, None, 0, C.byref(n)))
+        b, s, t = (np.zeros(n.value, np.int32) for _ in range(3))
+        check(lib().msbg_batch_get_tiles(self._h, _ptr(b), _ptr(s), _ptr(t), n.value, None))
+        return list(zip(b.tolist(), s.tolist(), t.tolist()))
+
+    def run(self, numIter, useGraph=True):
+        ms = C.c_double()
+        check(lib().msbg_batch_run(self._h, numIter, 1 if useGraph else 0, C.byref(ms)))
+        return ms.value
+
+    def destroy(self):
+        if self._h:
+            lib().msbg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.destroy()
 
 
 MCF_8COLOR = -(L.PDE_MEAN_CURVATURE + L.OPT_8_COLOR_SCHEME)

@@ -1,13 +1,19 @@
-"""Precision study of the MCF sweep over storage formats (FP32 / FP16 / BF16
-storage with FP32 arithmetic, UNORM16 as the reference format) against an
-FP64 run of the same sweep on the GPU.
+"""Studies of the MCF sweep on the GPU.
+
+precision_study: the sweep over storage formats (FP32 / FP16 / BF16 storage
+with FP32 arithmetic, UNORM16 as the reference format) against an FP64 run of
+the same sweep on the GPU.
 
 Every run starts from the same FP32 density field (splat + finalize into an
 FP32 channel, then converted on the device). Every `every` sweeps the study
 records, per storage: max and RMS difference to the FP64 field (accumulated
 in FP64 on the device) and the FP64 RMS of the last sweep's update.
 
-Run: python -m msbg_amd.studies [--size 1024 --sweeps 200 ...]; prints JSON.
+batched_study: many independent sparse grids of uneven size, level count and
+active fraction swept together (one graph launch per call), against the same
+grids swept one after another through applyChannelPdeFast.
+
+Run: python -m msbg_amd.studies [precision|batched] [options]; prints JSON.
 """
 import argparse
 import json
@@ -17,7 +23,7 @@ import time
 import numpy as np
 
 from . import _lib as L
-from .msbg import MCF_8COLOR, MultiresSparseGrid, scene_spheres
+from .msbg import MCF_8COLOR, Batch, MultiresSparseGrid, scene_spheres
 
 CH_INIT, CH_F64 = 40, 41
 STUDY = {"f32": (L.F32, 42), "f16": (L.F16, 43), "bf16": (L.BF16, 44), "u16": (L.U16, 45)}
@@ -77,18 +83,156 @@ def precision_study(S=1024, bsx=32, nObj=512, rmin=10.0, rmax=80.0, nPts=16000, 
         g.destroy()
 
 
+RSCAN_ACT = 3.0  # activation distance of a block to the nearest particle, voxels
+
+
+def _scene_grid(rng, S, bsx, nLevels, frac, nObj, chan, storage):
+    """One sparse grid: nObj seeded spheres sized so that the blocks within
+    RSCAN_ACT of their surfaces cover about `frac` of the domain; level-0
+    blocks hold pages, the rest of the channel is EMPTY."""
+    shell = 1.5 * bsx + 2 * RSCAN_ACT  # block-band thickness around a surface
+    r = float(np.sqrt(frac * S ** 3 / (nObj * 4 * np.pi * shell)))
+    r = min(max(r, 4.0), S / 4.0)
+    rmin, rmax = 0.7 * r, min(1.3 * r, S / 2.0 - 2.0)
+    nPts = int(min(max(np.pi * rmax ** 2, 400), 200000))  # ~2-voxel point spacing
+    xyz = scene_spheres(int(rng.integers(1 << 31)), nObj, rmax + 2, S - rmax - 2, rmin, rmax,
+                        nPts, 0.5, 0.5)
+    g = MultiresSparseGrid.create("batched", S, S, S, bsx, options=L.OPT_SINGLE_LEVEL,
+                                  nLevels=nLevels)
+    try:
+        g.points_upload(xyz)
+        if nLevels == 1:
+            act = g.activate(RSCAN_ACT) > 0
+            g.setRefinementMap(np.zeros(g.nBlocks(), np.int32))
+            kind = np.where(act, L.KIND_VALUE, L.KIND_EMPTY).astype(np.uint8)
+            g.channel_define(chan, storage, 0, kind, 0.0, None, 0.0)
+        else:
+            bands = [RSCAN_ACT] if nLevels == 2 else [RSCAN_ACT, RSCAN_ACT + 2 * bsx]
+            lv = g.levels_from_points(bands)
+            if nLevels > 2:
+                lv = g.regularizeRefinementMap(lv)
+            g.setRefinementMap(lv)
+            g.alloc_demo(chan, storage)
+        g.splat(chan, 2.0, 2.0)
+        g.finalize(chan, 2.0, 2.0)
+    except Exception:
+        g.destroy()
+        raise
+    return g
+
+
+def batched_study(nGrids=256, sizes=(128, 256, 512, 1024), blockSizes=(16, 32), levels=(1, 2, 3),
+                  fracRange=(0.01, 0.20), objRange=(8, 128), sweeps=50, dt=0.1, seed=2024,
+                  storage=L.F16, check=True, log=None):
+    """Timings (ms) of `sweeps` sweeps over all grids: one grid at a time,
+    batched eager launches, one batched graph launch, and the graph launch
+    after the autotuner. With check=True every batched result is compared
+    bitwise with the one-grid-at-a-time result on a copy of the channel."""
+    CH, CHB = 50, 51
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    grids, meta = [], []
+    try:
+        for k in range(nGrids):
+            S = int(rng.choice(sizes))
+            bsx = int(rng.choice(blockSizes))
+            nl = int(rng.choice(levels))
+            frac = float(rng.uniform(*fracRange))
+            nObj = int(rng.integers(objRange[0], objRange[1] + 1))
+            g = _scene_grid(rng, S, bsx, nl, frac, nObj, CH, storage)
+            grids.append(g)
+            _, npg = g.pagemap(CH)
+            meta.append(dict(S=S, bsx=bsx, nLevels=nl, frac_target=frac, nObj=nObj, pages=npg,
+                             frac=npg / g.nBlocks(), voxels=npg * bsx ** 3))
+            if log and (k + 1) % 32 == 0:
+                log(json.dumps({"built": k + 1, "seconds": time.perf_counter() - t0}))
+        vox = sum(m["voxels"] for m in meta)
+        out = {"grids": nGrids, "sweeps": sweeps, "dt": dt, "seed": seed, "storage": storage,
+               "voxels": vox, "setup_seconds": time.perf_counter() - t0,
+               "frac": [float(np.min([m["frac"] for m in meta])),
+                        float(np.mean([m["frac"] for m in meta])),
+                        float(np.max([m["frac"] for m in meta]))],
+               "by_size": {str(S): sum(m["S"] == S for m in meta) for S in sizes}}
+
+        def same(tag):
+            if not check:
+                return
+            for g in grids:
+                mx, _ = g.diff(CH, CHB)
+                assert mx == 0.0, (tag, mx)
+
+        def timed(fn):
+            for g in grids:
+                g.synchronize()
+            t = time.perf_counter()
+            fn()
+            for g in grids:
+                g.synchronize()
+            return (time.perf_counter() - t) * 1e3
+
+        def sequential(chan):
+            for g in grids:
+                g.applyChannelPdeFast(chan, 0, 0, None, MCF_8COLOR, 1, 0, 0, 0, sweeps, 1.0, dt)
+
+        def rate(ms):
+            return vox * sweeps / (ms * 1e-3)
+
+        for g in grids:
+            g.copyDataTo(CH, CHB, storage)
+        with Batch() as ba, Batch() as bb:
+            for g in grids:
+                ba.add(g, CH, dt, 1)
+                bb.add(g, CHB, dt, 1)
+            res = {}
+            res["sequential_ms"] = timed(lambda: sequential(CHB))
+            res["batch_eager_ms"] = timed(lambda: ba.run(sweeps, useGraph=False))
+            same("eager")
+            res["batch_graph_first_ms"] = timed(lambda: ba.run(sweeps, useGraph=True))
+            bb.run(sweeps, useGraph=False)
+            same("graph capture")
+            res["batch_graph_ms"] = timed(lambda: ba.run(sweeps, useGraph=True))
+            bb.run(sweeps, useGraph=False)
+            same("graph replay")
+            t = time.perf_counter()
+            ba.autotune(passes=4)
+            res["autotune_seconds"] = time.perf_counter() - t
+            res["tiles"] = ba.tiles()
+            res["batch_graph_tuned_ms"] = timed(lambda: ba.run(sweeps, useGraph=True))
+            res["sequential_tuned_ms"] = timed(lambda: sequential(CH))
+            for key in ("sequential_ms", "batch_eager_ms", "batch_graph_ms", "batch_graph_tuned_ms",
+                        "sequential_tuned_ms"):
+                res[key.replace("_ms", "_updates_per_s")] = rate(res[key])
+            res["speedup_graph_vs_sequential"] = res["sequential_ms"] / res["batch_graph_ms"]
+            res["speedup_tuned_vs_sequential"] = res["sequential_ms"] / res["batch_graph_tuned_ms"]
+            out.update(res)
+        out["seconds"] = time.perf_counter() - t0
+        out["grid_list"] = meta
+        return out
+    finally:
+        for g in grids:
+            g.destroy()
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
+    ap.add_argument("study", nargs="?", default="precision", choices=("precision", "batched"))
+    ap.add_argument("--grids", type=int, default=256)
+    ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--objects", type=int, default=512)
     ap.add_argument("--points", type=int, default=16000)
-    ap.add_argument("--sweeps", type=int, default=200)
+    ap.add_argument("--sweeps", type=int, default=None,
+                    help="default: 200 (precision), 50 (batched)")
     ap.add_argument("--every", type=int, default=20)
     ap.add_argument("--exact", action="store_true")
     a = ap.parse_args(argv)
-    res = precision_study(S=a.size, nObj=a.objects, nPts=a.points, sweeps=a.sweeps, every=a.every,
-                          arith=L.ARITH_EXACT if a.exact else L.ARITH_FAST,
-                          log=lambda s: print(s, file=sys.stderr, flush=True))
+    log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
+    if a.study == "batched":
+        print(json.dumps(batched_study(nGrids=a.grids, sweeps=a.sweeps or 50,
+                                       check=not a.no_check, log=log)))
+        return
+    res = precision_study(S=a.size, nObj=a.objects, nPts=a.points, sweeps=a.sweeps or 200,
+                          every=a.every, arith=L.ARITH_EXACT if a.exact else L.ARITH_FAST, log=log)
     print(json.dumps(res))
 
 

@@ -393,3 +393,93 @@ def test_precision_study_small():
     assert last["f32"]["rms"] < last["u16"]["rms"] < last["f16"]["rms"] < last["bf16"]["rms"]
     assert last["bf16"]["max"] < 0.1
     assert all(np.isfinite(last[k]["residual"]) for k in ("f32", "f16", "bf16", "u16"))
+
+
+# --------------------------------------------------------------------------
+# batched sweeps of independent grids (one graph launch)
+# --------------------------------------------------------------------------
+BCH = 30  # channel gpu_pipeline fills
+
+
+def _batch_grids(specs, storage=M.F16, seed=0):
+    grids = []
+    for k, (S, bsx) in enumerate(specs):
+        xyz = small_scene(S, nObj=4, nPts=2500, seed=seed + k, rmin=S / 12, rmax=S / 6)
+        gp = gpu_pipeline(S, bsx, xyz, storage=storage, do_sweep=False, arith=M.ARITH_FAST)
+        grids.append(gp["grid"])
+    return grids
+
+
+def test_batch_matches_single_grid_sweeps():
+    note("batch vs single")
+    specs = [(64, 16), (128, 32), (96, 16), (128, 16), (64, 32)]
+    a = _batch_grids(specs, seed=3)
+    b = _batch_grids(specs, seed=3)
+    for g in b:
+        g.applyChannelPdeFast(BCH, 0, 0, None, M.MCF_8COLOR, 1, 0, 0, 0, 4, 1.0, 0.1)
+    with M.Batch() as bt:
+        for g in a:
+            bt.add(g, BCH, 0.1, 1)
+        bt.run(4, useGraph=False)
+    for ga, gb in zip(a, b):
+        assert np.array_equal(ga.download(BCH).view(np.uint16),
+                              gb.download(BCH).view(np.uint16))
+
+
+def test_batch_graph_replays_match_eager():
+    """Each graph replay, with eager work and synchronisations between the
+    replays, equals the same sweeps run eagerly."""
+    note("batch graph replays")
+    specs = [(64, 16), (128, 32), (128, 16)]
+    a = _batch_grids(specs, seed=9)
+    b = _batch_grids(specs, seed=9)
+    with M.Batch() as ba, M.Batch() as bb:
+        for g in a:
+            ba.add(g, BCH, 0.1, 1)
+        for g in b:
+            bb.add(g, BCH, 0.1, 1)
+        for rep in range(3):
+            ba.run(2, useGraph=True)
+            bb.run(2, useGraph=False)
+            for g in a:  # eager work between the replays
+                g.synchronize()
+                g.download(BCH)
+            for ga, gb in zip(a, b):
+                assert np.array_equal(ga.download(BCH).view(np.uint16),
+                                      gb.download(BCH).view(np.uint16)), rep
+
+
+def test_batch_autotune_keeps_results():
+    note("batch autotune")
+    specs = [(128, 32), (128, 16)]
+    a = _batch_grids(specs, seed=21)
+    b = _batch_grids(specs, seed=21)
+    before = [g.download(BCH).copy() for g in a]
+    with M.Batch() as bt:
+        for g in a:
+            bt.add(g, BCH, 0.1, 1)
+        bt.autotune(passes=2)
+        tiles = bt.tiles()
+        assert len(tiles) == 2 and all(t in (0, 1) for _, _, t in tiles)
+        # tuning restores the pages it sweeps
+        for g, x in zip(a, before):
+            assert np.array_equal(g.download(BCH).view(np.uint16), x.view(np.uint16))
+        bt.run(2, useGraph=True)
+    for g in b:
+        g.applyChannelPdeFast(BCH, 0, 0, None, M.MCF_8COLOR, 1, 0, 0, 0, 2, 1.0, 0.1)
+    # the chosen variant computes the same update (up to its FP32 rounding order)
+    for ga, gb in zip(a, b):
+        d = np.abs(ga.download(BCH).astype(np.float32) -
+                   gb.download(BCH).astype(np.float32))
+        assert np.percentile(d, 99.9) <= 2e-3 and d.max() <= 0.2
+
+
+def test_batched_study_small():
+    """The batched-solve study at small sizes: every batched launch mode gives
+    the bitwise result of sweeping the grids one at a time."""
+    note("batched study small")
+    from msbg_amd.studies import batched_study
+    r = batched_study(nGrids=8, sizes=(64, 128), levels=(1, 2, 3), sweeps=4, seed=5)
+    assert r["voxels"] > 0 and r["grids"] == 8
+    assert sorted(int(k) for k in r["by_size"]) == [64, 128]
+    assert all(t in (0, 1) for _, _, t in r["tiles"])
