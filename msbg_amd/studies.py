@@ -87,22 +87,34 @@ RSCAN_ACT = 3.0  # activation distance of a block to the nearest particle, voxel
 
 
 def _scene_grid(rng, S, bsx, nLevels, frac, nObj, chan, storage):
-    """One sparse grid: nObj seeded spheres sized so that the blocks within
-    RSCAN_ACT of their surfaces cover about `frac` of the domain; level-0
-    blocks hold pages, the rest of the channel is EMPTY."""
+    """One sparse grid: nObj seeded spheres and ellipsoids sized so that the
+    blocks within RSCAN_ACT of their surfaces cover about `frac` of the
+    domain; level-0 blocks hold pages, the rest of the channel is EMPTY.
+    Returns (grid, objects used)."""
     shell = 1.5 * bsx + 2 * RSCAN_ACT  # block-band thickness around a surface
     r = float(np.sqrt(frac * S ** 3 / (nObj * 4 * np.pi * shell)))
-    r = min(max(r, 4.0), S / 4.0)
-    rmin, rmax = 0.7 * r, min(1.3 * r, S / 2.0 - 2.0)
-    nPts = int(min(max(np.pi * rmax ** 2, 400), 200000))  # ~2-voxel point spacing
-    xyz = scene_spheres(int(rng.integers(1 << 31)), nObj, rmax + 2, S - rmax - 2, rmin, rmax,
-                        nPts, 0.5, 0.5)
+    seed = int(rng.integers(1 << 31))
     g = MultiresSparseGrid.create("batched", S, S, S, bsx, options=L.OPT_SINGLE_LEVEL,
                                   nLevels=nLevels)
     try:
-        g.points_upload(xyz)
-        if nLevels == 1:
+        # the shell estimate misses block granularity; correct the radius (and,
+        # below the smallest radius, the object count) from the measured
+        # fraction a few times
+        for _ in range(5):
+            r = min(max(r, 3.0), S / 4.0)
+            rmin, rmax = 0.7 * r, min(1.3 * r, S / 2.0 - 2.0)
+            nPts = int(min(max(np.pi * rmax ** 2, 200), 200000))  # ~2-voxel point spacing
+            xyz = scene_spheres(seed, nObj, rmax + 2, S - rmax - 2, rmin, rmax, nPts, 0.5, 0.5)
+            g.points_upload(xyz)
             act = g.activate(RSCAN_ACT) > 0
+            got = max(float(act.mean()), 1e-9)
+            if abs(got / frac - 1.0) < 0.15:
+                break
+            if r <= 3.0 and got > frac and nObj > 1:
+                nObj = max(1, int(nObj * frac / got))
+            else:
+                r *= float(np.sqrt(frac / got))
+        if nLevels == 1:
             g.setRefinementMap(np.zeros(g.nBlocks(), np.int32))
             kind = np.where(act, L.KIND_VALUE, L.KIND_EMPTY).astype(np.uint8)
             g.channel_define(chan, storage, 0, kind, 0.0, None, 0.0)
@@ -118,7 +130,7 @@ def _scene_grid(rng, S, bsx, nLevels, frac, nObj, chan, storage):
     except Exception:
         g.destroy()
         raise
-    return g
+    return g, nObj
 
 
 def batched_study(nGrids=256, sizes=(128, 256, 512, 1024), blockSizes=(16, 32), levels=(1, 2, 3),
@@ -139,7 +151,7 @@ def batched_study(nGrids=256, sizes=(128, 256, 512, 1024), blockSizes=(16, 32), 
             nl = int(rng.choice(levels))
             frac = float(rng.uniform(*fracRange))
             nObj = int(rng.integers(objRange[0], objRange[1] + 1))
-            g = _scene_grid(rng, S, bsx, nl, frac, nObj, CH, storage)
+            g, nObj = _scene_grid(rng, S, bsx, nl, frac, nObj, CH, storage)
             grids.append(g)
             _, npg = g.pagemap(CH)
             meta.append(dict(S=S, bsx=bsx, nLevels=nl, frac_target=frac, nObj=nObj, pages=npg,
