@@ -338,6 +338,70 @@ int msbg_set_tile(msbg_grid *g, int tile);
 int msbg_scene_spheres(uint64_t seed, int nObj, float cmin, float cmax,
                        float rmin, float rmax, int64_t nPts, float sigma,
                        float ellipsoidFrac, float *xyz);
+/* The same scene with each surface displaced along the ray from its centre
+ * by radius * perlinAmp * noise(perlinFreq * point) (gradient noise, [-1,1];
+ * 0 <= perlinAmp < 1). perlinAmp = 0 gives msbg_scene_spheres' output. */
+int msbg_scene_objects(uint64_t seed, int nObj, float cmin, float cmax,
+                       float rmin, float rmax, int64_t nPts, float sigma,
+                       float ellipsoidFrac, float perlinAmp, float perlinFreq,
+                       float *xyz);
+
+/* ======================================================================== */
+/* learned warm start (no counterpart in the reference)                     */
+/* ======================================================================== */
+/* A 3-stage 3D U-Net (16/32/64 channels, 3^3 convolutions, BF16 weights and
+ * activations, FP32 accumulation on the matrix cores) that maps a 32^3 block
+ * with a 2-voxel halo (36^3 window) to the change of the block over K plain
+ * MCF sweeps.
+ * Tensors passed to msbg_unet_* are DEVICE pointers: windows x are BF16 bits
+ * [n][36][36][36], predictions and targets FP32 [n][32][32][32]. Training
+ * and backward batches must be 8 * 2^k. Parameters (FP32, host arrays): per
+ * convolution w[cout][cin][27] (tap = (dz*3+dy)*3+dx) then b[cout], ten
+ * convolutions, then the 1x1 head hw[16], hb. */
+typedef struct msbg_unet msbg_unet;
+/* He-normal weights from `seed`, zero biases, zero head (predicts 0). */
+int msbg_unet_create(uint64_t seed, msbg_unet **out);
+int msbg_unet_destroy(msbg_unet *u);
+int msbg_unet_num_params(msbg_unet *u, int64_t *n);
+int msbg_unet_get_params(msbg_unet *u, float *host, int64_t n);
+/* also resets the optimizer state */
+int msbg_unet_set_params(msbg_unet *u, const float *host, int64_t n);
+/* gradients of the last msbg_unet_backward / msbg_unet_train_step */
+int msbg_unet_get_grads(msbg_unet *u, float *host, int64_t n);
+int msbg_unet_forward(msbg_unet *u, const uint16_t *x, int n, float *y);
+/* forward, then the gradients of sum(dy * y) */
+int msbg_unet_backward(msbg_unet *u, const uint16_t *x, const float *dy,
+                       int n);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on mean((y - target)^2);
+ * *loss receives the loss before the step */
+int msbg_unet_train_step(msbg_unet *u, const uint16_t *x,
+                         const float *target, int n, float lr, double *loss);
+/* 36^3 windows of level-0 value blocks (host block ids) of a 32^3-block
+ * channel, under the sweep's halo rule, into x (device). */
+int msbg_unet_gather(msbg_grid *g, int chan, const int32_t *blocks, int n,
+                     uint16_t *x);
+/* the blocks' values as FP32 [n][32^3] into y (device) */
+int msbg_unet_extract(msbg_grid *g, int chan, const int32_t *blocks, int n,
+                      float *y);
+/* add dy (device, FP32) to the blocks' values, clamped to [0,1] */
+int msbg_unet_scatter(msbg_grid *g, int chan, const int32_t *blocks, int n,
+                      const float *y);
+/* Add the network's predicted change to every level-0 value block of `chan`
+ * (all windows are read before any block is written), `batch` blocks per
+ * forward pass; *ms receives the device time. */
+int msbg_warm_start(msbg_grid *g, int chan, msbg_unet *u, int batch,
+                    double *ms);
+/* One convolution and its two backward products on device tensors
+ * (channels-last BF16; w, b, dw, db FP32), for layer tests. Shapes must be
+ * ones the network uses. */
+int msbg_cnn_conv3d(const uint16_t *x, int n, int cin, int cout, int dout,
+                    int pad, const float *w, const float *b, int relu,
+                    uint16_t *y);
+int msbg_cnn_conv3d_bwd_data(const uint16_t *dy, int n, int cin, int cout,
+                             int din, int pad, const float *w, uint16_t *dx);
+int msbg_cnn_conv3d_bwd_weights(const uint16_t *x, const uint16_t *dy, int n,
+                                int cin, int cout, int dout, int pad,
+                                float *dw, float *db);
 
 #ifdef __cplusplus
 }

@@ -92,6 +92,23 @@ SIGNATURES = {
     "msbg_batch_run": (_I, [_P, _I, _I, _P]),
     "msbg_set_tile": (_I, [_P, _I]),
     "msbg_channel_diff": (_I, [_P, _I, _I, _I, _P, _P]),
+    "msbg_scene_objects": (_I, [_U64, _I, _F, _F, _F, _F, _I64, _F, _F, _F, _F, _P]),
+    "msbg_unet_create": (_I, [_U64, C.POINTER(_P)]),
+    "msbg_unet_destroy": (_I, [_P]),
+    "msbg_unet_num_params": (_I, [_P, _P]),
+    "msbg_unet_get_params": (_I, [_P, _P, _I64]),
+    "msbg_unet_set_params": (_I, [_P, _P, _I64]),
+    "msbg_unet_get_grads": (_I, [_P, _P, _I64]),
+    "msbg_unet_forward": (_I, [_P, _P, _I, _P]),
+    "msbg_unet_backward": (_I, [_P, _P, _P, _I]),
+    "msbg_unet_train_step": (_I, [_P, _P, _P, _I, _F, _P]),
+    "msbg_unet_gather": (_I, [_P, _I, _P, _I, _P]),
+    "msbg_unet_scatter": (_I, [_P, _I, _P, _I, _P]),
+    "msbg_unet_extract": (_I, [_P, _I, _P, _I, _P]),
+    "msbg_warm_start": (_I, [_P, _I, _P, _I, _P]),
+    "msbg_cnn_conv3d": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _I, _P]),
+    "msbg_cnn_conv3d_bwd_data": (_I, [_P, _I, _I, _I, _I, _I, _P, _P]),
+    "msbg_cnn_conv3d_bwd_weights": (_I, [_P, _P, _I, _I, _I, _I, _I, _P, _P]),
 }
 
 _lib = None

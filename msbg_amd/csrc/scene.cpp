@@ -1,7 +1,8 @@
 // Seeded synthetic particle scenes: jittered surface points of spheres and
-// randomly rotated ellipsoids. Each object has its own PRNG stream derived
-// from (seed, object index), so the output does not depend on the number of
-// host threads. Positions are computed in double and rounded once to float.
+// randomly rotated ellipsoids, optionally with Perlin-noise displaced
+// surfaces. Each object has its own PRNG stream derived from (seed, object
+// index), so the output does not depend on the number of host threads.
+// Positions are computed in double and rounded once to float.
 
 #include <cmath>
 #include <cstdint>
@@ -52,17 +53,68 @@ struct Xoshiro {
   }
 };
 
+// Gradient noise after Perlin, "Improving Noise" (SIGGRAPH 2002): quintic
+// fade, 12 edge gradients picked by a hashed lattice corner. The
+// permutation is a seeded shuffle, so each scene seed has its own field.
+struct Perlin {
+  int p[512];
+  explicit Perlin(uint64_t seed) {
+    Xoshiro r(seed);
+    for (int i = 0; i < 256; i++) p[i] = i;
+    for (int i = 255; i > 0; i--) {
+      int j = (int)(r.next() % (uint64_t)(i + 1));
+      int t = p[i];
+      p[i] = p[j];
+      p[j] = t;
+    }
+    for (int i = 0; i < 256; i++) p[256 + i] = p[i];
+  }
+  static double fade(double t) { return t * t * t * (t * (t * 6 - 15) + 10); }
+  static double lerp(double t, double a, double b) { return a + t * (b - a); }
+  static double grad(int h, double x, double y, double z) {
+    h &= 15;
+    double u = h < 8 ? x : y, v = h < 4 ? y : (h == 12 || h == 14 ? x : z);
+    return ((h & 1) ? -u : u) + ((h & 2) ? -v : v);
+  }
+  // in [-1, 1]
+  double operator()(double x, double y, double z) const {
+    const double fx = std::floor(x), fy = std::floor(y), fz = std::floor(z);
+    const int X = (int)fx & 255, Y = (int)fy & 255, Z = (int)fz & 255;
+    x -= fx;
+    y -= fy;
+    z -= fz;
+    const double u = fade(x), v = fade(y), w = fade(z);
+    const int A = p[X] + Y, AA = p[A] + Z, AB = p[A + 1] + Z, B = p[X + 1] + Y, BA = p[B] + Z,
+              BB = p[B + 1] + Z;
+    return lerp(w,
+                lerp(v, lerp(u, grad(p[AA], x, y, z), grad(p[BA], x - 1, y, z)),
+                     lerp(u, grad(p[AB], x, y - 1, z), grad(p[BB], x - 1, y - 1, z))),
+                lerp(v, lerp(u, grad(p[AA + 1], x, y, z - 1), grad(p[BA + 1], x - 1, y, z - 1)),
+                     lerp(u, grad(p[AB + 1], x, y - 1, z - 1),
+                          grad(p[BB + 1], x - 1, y - 1, z - 1))));
+  }
+};
+
 }  // namespace
 
 extern "C" int msbg_scene_spheres(uint64_t seed, int nObj, float cmin, float cmax, float rmin,
                                   float rmax, int64_t nPts, float sigma, float ellipsoidFrac,
                                   float *xyz) {
-  MSBG_CHECK(nObj >= 0 && nPts >= 0 && xyz && cmax >= cmin && rmax >= rmin && sigma >= 0,
-             MSBG_EINVAL, "msbg_scene_spheres: bad args");
+  return msbg_scene_objects(seed, nObj, cmin, cmax, rmin, rmax, nPts, sigma, ellipsoidFrac, 0.0f,
+                            0.0f, xyz);
+}
+
+extern "C" int msbg_scene_objects(uint64_t seed, int nObj, float cmin, float cmax, float rmin,
+                                  float rmax, int64_t nPts, float sigma, float ellipsoidFrac,
+                                  float perlinAmp, float perlinFreq, float *xyz) {
+  MSBG_CHECK(nObj >= 0 && nPts >= 0 && xyz && cmax >= cmin && rmax >= rmin && sigma >= 0 &&
+                 perlinAmp >= 0 && perlinAmp < 1 && perlinFreq >= 0,
+             MSBG_EINVAL, "msbg_scene_objects: bad args");
   struct Obj {
     double c[3], ax[3], R[9];
     uint64_t seed;
   };
+  const Perlin noise(seed ^ 0x6A09E667F3BCC909ull);
   Obj *objs = new Obj[nObj > 0 ? nObj : 1];
   {
     Xoshiro sr(seed);
@@ -94,6 +146,10 @@ extern "C" int msbg_scene_spheres(uint64_t seed, int nObj, float cmin, float cma
   for (int64_t o = o0; o < o1; o++) {
     const Obj &ob = objs[o];
     Xoshiro r(ob.seed);
+    // this object's window into the noise field (its own stream)
+    uint64_t ns = ob.seed ^ 0xA0761D6478BD642Full;
+    double off[3];
+    for (double &v : off) v = (double)(splitmix64(ns) >> 11) * 0x1.0p-53 * 256.0;
     float *out = xyz + (size_t)3 * nPts * o;
     for (int64_t i = 0; i < nPts; i++) {
       double n[3], nn = 0;
@@ -106,6 +162,12 @@ extern "C" int msbg_scene_spheres(uint64_t seed, int nObj, float cmin, float cma
       } while (nn < 1e-24);
       nn = std::sqrt(nn);
       double u[3] = {ob.ax[0] * n[0] / nn, ob.ax[1] * n[1] / nn, ob.ax[2] * n[2] / nn};
+      if (perlinAmp > 0) {  // displace along the ray: radius * (1 + amp * noise)
+        const double f = 1.0 + perlinAmp * noise(perlinFreq * u[0] + off[0],
+                                                  perlinFreq * u[1] + off[1],
+                                                  perlinFreq * u[2] + off[2]);
+        for (double &v : u) v *= f;
+      }
       for (int k = 0; k < 3; k++) {
         double p = ob.c[k] + ob.R[3 * k] * u[0] + ob.R[3 * k + 1] * u[1] + ob.R[3 * k + 2] * u[2];
         p += sigma * r.normal();

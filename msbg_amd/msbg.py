@@ -38,6 +38,116 @@ def scene_spheres(seed, nObj, cmin, cmax, rmin, rmax, nPts, sigma=0.5, ellipsoid
     return out
 
 
+def scene_objects(seed, nObj, cmin, cmax, rmin, rmax, nPts, sigma=0.5, ellipsoidFrac=0.0,
+                  perlinAmp=0.0, perlinFreq=0.0):
+    """scene_spheres with Perlin-noise displaced surfaces, (nObj*nPts, 3) float32."""
+    out = np.empty((nObj * nPts, 3), dtype=np.float32)
+    check(lib().msbg_scene_objects(seed, nObj, cmin, cmax, rmin, rmax, nPts, sigma,
+                                   ellipsoidFrac, perlinAmp, perlinFreq, _ptr(out)))
+    return out
+
+
+def _dptr(t):
+    """Device address of a tensor-like object with data_ptr() (e.g. a torch
+    CUDA tensor); the library only sees the address."""
+    return C.c_void_p(t.data_ptr())
+
+
+class UNet:
+    """Learned warm start: a 3-stage 3D U-Net mapping 36^3 block windows to
+    the block's change over K plain sweeps (no counterpart in the reference).
+    Tensor arguments are device tensors: windows BF16 [n,36,36,36],
+    predictions / targets / output gradients FP32 [n,32,32,32]."""
+
+    def __init__(self, seed=0):
+        h = C.c_void_p()
+        check(lib().msbg_unet_create(seed, C.byref(h)))
+        self._h = h
+        n = C.c_int64()
+        check(lib().msbg_unet_num_params(h, C.byref(n)))
+        self.nparam = n.value
+
+    def get_params(self):
+        out = np.empty(self.nparam, np.float32)
+        check(lib().msbg_unet_get_params(self._h, _ptr(out), out.size))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().msbg_unet_set_params(self._h, _ptr(p), p.size))
+
+    def get_grads(self):
+        out = np.empty(self.nparam, np.float32)
+        check(lib().msbg_unet_get_grads(self._h, _ptr(out), out.size))
+        return out
+
+    def forward(self, x, y):
+        check(lib().msbg_unet_forward(self._h, _dptr(x), x.shape[0], _dptr(y)))
+
+    def backward(self, x, dy):
+        check(lib().msbg_unet_backward(self._h, _dptr(x), _dptr(dy), x.shape[0]))
+
+    def train_step(self, x, target, lr):
+        loss = C.c_double()
+        check(lib().msbg_unet_train_step(self._h, _dptr(x), _dptr(target), x.shape[0], lr,
+                                         C.byref(loss)))
+        return loss.value
+
+    def warm_start(self, grid, chan, batch=64):
+        """Add the predicted change to every level-0 value block; device ms."""
+        ms = C.c_double()
+        check(lib().msbg_warm_start(grid._h, chan, self._h, batch, C.byref(ms)))
+        return ms.value
+
+    @staticmethod
+    def gather(grid, chan, blocks, x):
+        b = np.ascontiguousarray(blocks, dtype=np.int32)
+        check(lib().msbg_unet_gather(grid._h, chan, _ptr(b), b.size, _dptr(x)))
+
+    @staticmethod
+    def extract(grid, chan, blocks, y):
+        b = np.ascontiguousarray(blocks, dtype=np.int32)
+        check(lib().msbg_unet_extract(grid._h, chan, _ptr(b), b.size, _dptr(y)))
+
+    @staticmethod
+    def scatter(grid, chan, blocks, y):
+        b = np.ascontiguousarray(blocks, dtype=np.int32)
+        check(lib().msbg_unet_scatter(grid._h, chan, _ptr(b), b.size, _dptr(y)))
+
+    def destroy(self):
+        if self._h:
+            lib().msbg_unet_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.destroy()
+
+
+def cnn_conv3d(x, cin, cout, dout, pad, w, b, relu, y):
+    """One network convolution on device tensors (layer tests)."""
+    check(lib().msbg_cnn_conv3d(_dptr(x), x.shape[0], cin, cout, dout, pad, _dptr(w),
+                                None if b is None else _dptr(b), int(relu), _dptr(y)))
+
+
+def cnn_conv3d_bwd_data(dy, cin, cout, din, pad, w, dx):
+    check(lib().msbg_cnn_conv3d_bwd_data(_dptr(dy), dy.shape[0], cin, cout, din, pad, _dptr(w),
+                                         _dptr(dx)))
+
+
+def cnn_conv3d_bwd_weights(x, dy, cin, cout, dout, pad, dw, db):
+    check(lib().msbg_cnn_conv3d_bwd_weights(_dptr(x), _dptr(dy), x.shape[0], cin, cout, dout, pad,
+                                            _dptr(dw), None if db is None else _dptr(db)))
+
+
 def copy_bandwidth(nbytes=1 << 30, reps=5):
     """Measured read+write GB/s of a plain device copy of nbytes."""
     v = C.c_double()

@@ -1,0 +1,255 @@
+"""Learned warm start of the MCF solve: training data from the plain solver,
+training on the GPU, and time to tolerance of plain versus learned + plain.
+
+- Scenes: seeded spheres plus Perlin-displaced ellipsoids (radii [10, 150]),
+  splatted and finalized into an FP32 channel of 32^3 blocks, 3 levels by
+  distance to the particles (level 0 within 4 voxels).
+- Training pairs: a level-0 block's 36^3 window of the splatted field (BF16)
+  and the block's change over K plain sweeps (FP32), K = 64 by default.
+  Scenes are swept in groups through one batched launch.
+- Tolerance: the FP64 RMS of one sweep's update over the swept voxels (the
+  residual applyChannelPdeFast returns), checked after every sweep.
+- Learned + plain: msbg_warm_start adds the predicted change to every
+  level-0 block, then plain sweeps run to the tolerance. If the residual
+  rises between two checks, the field is restored from the splat and the
+  plain solve runs from there (the time spent counts).
+
+Run: python -m msbg_amd.warmstart [--scenes N ...]; prints JSON.
+Device tensors (windows, targets) are torch CUDA tensors; the library only
+sees their addresses.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .msbg import MCF_8COLOR, Batch, MultiresSparseGrid, UNet, scene_objects, scene_spheres
+
+CH_IN, CH_K, CH_RUN = 60, 61, 62
+RSCAN = 4.0
+LAYERS = [(1, 16, 36), (16, 16, 34), (16, 32, 16), (32, 32, 16), (32, 64, 8), (64, 64, 8),
+          (96, 32, 16), (32, 32, 16), (48, 16, 32), (16, 16, 32)]  # (cin, cout, din)
+PAD = [0, 0, 1, 1, 1, 1, 1, 1, 1, 1]
+
+
+def macs_per_block():
+    """Multiply-adds of one forward pass (convolutions and the head)."""
+    m = 0
+    for (cin, cout, din), pad in zip(LAYERS, PAD):
+        dout = din + 2 * pad - 2
+        m += dout ** 3 * 27 * cin * cout
+    return m + 32 ** 3 * 16
+
+
+def scene_points(seed, S, nObj, rmin=10.0, rmax=150.0, nPts=40000, perlinAmp=0.2,
+                 perlinFreq=1.0 / 48):
+    """Half spheres, half Perlin-displaced ellipsoids, centres inside the domain."""
+    ns = nObj // 2
+    m = rmax * (1 + perlinAmp) + 8
+    a = scene_spheres(seed, ns, m, S - m, rmin, rmax, nPts, 0.5, 0.0)
+    b = scene_objects(seed + 7919, nObj - ns, m, S - m, rmin, rmax, nPts, 0.5, 1.0, perlinAmp,
+                      perlinFreq)
+    return np.concatenate([a, b])
+
+
+def build_scene(seed, S, nObj, storage=L.F32, chan=CH_IN):
+    g = MultiresSparseGrid.create("warmstart", S, S, S, 32, options=L.OPT_SINGLE_LEVEL, nLevels=3)
+    try:
+        g.points_upload(scene_points(seed, S, nObj))
+        lv = g.regularizeRefinementMap(g.levels_from_points([RSCAN, RSCAN + 32]))
+        g.setRefinementMap(lv)
+        g.alloc_demo(chan, storage)
+        g.splat(chan, 2.0, 2.0)
+        g.finalize(chan, 2.0, 2.0)
+    except Exception:
+        g.destroy()
+        raise
+    return g
+
+
+def level0_blocks(g, chan):
+    pm, _ = g.pagemap(chan)
+    levels, _ = g.getBlockInfo()
+    return np.nonzero((pm >= 0) & (levels == 0))[0].astype(np.int32)
+
+
+def make_training_set(nScenes=10000, perScene=8, K=64, S=1024, seed=100, group=16,
+                      objRange=(8, 40), dt=0.1, log=None):
+    """(X, T, stats): BF16 windows [N,36,36,36] and FP32 changes [N,32,32,32]."""
+    import torch
+    N = nScenes * perScene
+    X = torch.empty((N, 36, 36, 36), dtype=torch.bfloat16, device="cuda")
+    T = torch.empty((N, 32, 32, 32), dtype=torch.float32, device="cuda")
+    base = torch.empty((perScene, 32, 32, 32), dtype=torch.float32, device="cuda")
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    n, blocks, sweep_ms = 0, 0, 0.0
+    for s0 in range(0, nScenes, group):
+        grids = []
+        try:
+            for s in range(s0, min(s0 + group, nScenes)):
+                grids.append(build_scene(seed * 1000003 + s, S, int(rng.integers(*objRange))))
+            for g in grids:
+                g.copyDataTo(CH_IN, CH_K, L.F32)
+            with Batch() as b:
+                for g in grids:
+                    b.add(g, CH_K, dt, 1)
+                sweep_ms += b.run(K, useGraph=False)
+            for g in grids:
+                cand = level0_blocks(g, CH_IN)
+                blocks += cand.size
+                pick = np.sort(rng.choice(cand, perScene, replace=cand.size < perScene))
+                UNet.gather(g, CH_IN, pick, X[n:n + perScene])
+                UNet.extract(g, CH_K, pick, T[n:n + perScene])
+                UNet.extract(g, CH_IN, pick, base)
+                T[n:n + perScene] -= base
+                n += perScene
+        finally:
+            for g in grids:
+                g.destroy()
+        if log and (s0 // group) % 16 == 0:
+            log(json.dumps({"scenes": s0 + len(grids), "seconds": time.perf_counter() - t0}))
+    torch.cuda.synchronize()
+    return X, T, {"scenes": nScenes, "samples": N, "K": K, "S": S,
+                  "mean_level0_blocks": blocks / nScenes, "sweep_ms": sweep_ms,
+                  "seconds": time.perf_counter() - t0}
+
+
+def train(net, X, T, epochs=4, batch=32, lr=1e-3, val_frac=0.05, seed=0, log=None):
+    import torch
+    N = X.shape[0]
+    nval = max(batch, int(N * val_frac))
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    perm = torch.randperm(N, generator=gen).to("cuda")
+    val, tr = perm[:nval], perm[nval:]
+    ntr = (tr.numel() // batch) * batch
+    steps = epochs * (ntr // batch)
+    hist, step = [], 0
+    t0 = time.perf_counter()
+
+    def val_loss():
+        y = torch.empty((batch, 32, 32, 32), dtype=torch.float32, device="cuda")
+        se, zero = 0.0, 0.0
+        for i in range(0, nval - batch + 1, batch):
+            idx = val[i:i + batch]
+            net.forward(X.index_select(0, idx).contiguous(), y)
+            t = T.index_select(0, idx)
+            se += float(((y - t) ** 2).sum())
+            zero += float((t ** 2).sum())
+        return se / max(zero, 1e-30)
+
+    rel0 = val_loss()
+    for ep in range(epochs):
+        order = tr[torch.randperm(tr.numel(), generator=gen).to("cuda")][:ntr]
+        losses = []
+        for i in range(0, ntr, batch):
+            idx = order[i:i + batch]
+            # cosine decay to lr/10
+            a = lr * (0.55 + 0.45 * np.cos(np.pi * step / max(steps, 1)))
+            losses.append(net.train_step(X.index_select(0, idx).contiguous(),
+                                         T.index_select(0, idx).contiguous(), float(a)))
+            step += 1
+        rel = val_loss()
+        hist.append({"epoch": ep + 1, "train_mse": float(np.mean(losses)), "val_rel_mse": rel,
+                     "seconds": time.perf_counter() - t0})
+        if log:
+            log(json.dumps(hist[-1]))
+    secs = time.perf_counter() - t0
+    return {"samples": N, "val": nval, "epochs": epochs, "batch": batch, "lr": lr, "steps": steps,
+            "val_rel_mse_initial": rel0, "history": hist, "seconds": secs,
+            "train_tflops": 3 * 2 * macs_per_block() * steps * batch / secs / 1e12}
+
+
+def solve_to_tol(g, chan, tol, dt, maxSweeps, chunk=20, watch_rise=False):
+    """Plain sweeps until the per-sweep residual drops below tol.
+    Returns (sweeps, device ms, residual history, rose)."""
+    sweeps, ms, hist = 0, 0.0, []
+    prev = None
+    while sweeps < maxSweeps:
+        r = g.applyChannelPdeFast(chan, 0, 0, None, MCF_8COLOR, 1, 0, 0, 0, chunk, 1.0, dt,
+                                  residual=True)
+        t, _, _ = g.last_sweep_timing()
+        hit = np.nonzero(r < tol)[0]
+        if hit.size:
+            k = int(hit[0]) + 1
+            return sweeps + k, ms + t * k / len(r), hist + [float(r[k - 1])], False
+        sweeps += len(r)
+        ms += t
+        hist.append(float(r[-1]))
+        m = float(np.mean(r))
+        if watch_rise and prev is not None and m > prev:
+            return sweeps, ms, hist, True
+        prev = m
+    return sweeps, ms, hist, False
+
+
+def time_to_tolerance(net, S=2048, nObj=128, seed=777, tol=1e-4, dt=0.1, maxSweeps=20000,
+                      batch=256, log=None):
+    g = build_scene(seed, S, nObj)
+    try:
+        nblk = int(level0_blocks(g, CH_IN).size)
+        g.set_arith(L.ARITH_FAST)
+        g.copyDataTo(CH_IN, CH_RUN, L.F32)
+        p_sweeps, p_ms, p_hist, _ = solve_to_tol(g, CH_RUN, tol, dt, maxSweeps)
+        if log:
+            log(json.dumps({"plain_sweeps": p_sweeps, "plain_ms": p_ms}))
+        g.copyDataTo(CH_IN, CH_RUN, L.F32)
+        ws_ms = net.warm_start(g, CH_RUN, batch=batch)
+        l_sweeps, l_ms, l_hist, rose = solve_to_tol(g, CH_RUN, tol, dt, maxSweeps, watch_rise=True)
+        fb = None
+        if rose:  # fallback: restore the splat and solve plainly
+            g.copyDataTo(CH_IN, CH_RUN, L.F32)
+            fb_sweeps, fb_ms, _, _ = solve_to_tol(g, CH_RUN, tol, dt, maxSweeps)
+            fb = {"sweeps": fb_sweeps, "ms": fb_ms}
+        learned_ms = ws_ms + l_ms + (fb["ms"] if fb else 0.0)
+        flops = 2.0 * macs_per_block() * nblk
+        out = {"S": S, "nObj": nObj, "seed": seed, "tol": tol, "level0_blocks": nblk,
+               "plain": {"sweeps": p_sweeps, "ms": p_ms, "residuals_every_20": p_hist[:50]},
+               "learned": {"warm_start_ms": ws_ms, "sweeps": l_sweeps, "sweep_ms": l_ms,
+                           "total_ms": learned_ms, "fallback": fb,
+                           "residuals_every_20": l_hist[:50]},
+               "speedup": p_ms / learned_ms,
+               "cnn": {"tflops": flops / (ws_ms * 1e-3) / 1e12,
+                       "frac_of_bf16_peak": flops / (ws_ms * 1e-3) / 2.5e15,
+                       "ms_per_block": ws_ms / nblk,
+                       "sweep_equivalents": ws_ms / (p_ms / p_sweeps)}}
+        if log:
+            log(json.dumps({k: out[k] for k in ("speedup", "cnn")}))
+        return out
+    finally:
+        g.destroy()
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scenes", type=int, default=10000)
+    ap.add_argument("--per-scene", type=int, default=8)
+    ap.add_argument("--K", type=int, default=64)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--eval-size", type=int, default=2048)
+    ap.add_argument("--eval-objects", type=int, default=128)
+    ap.add_argument("--eval-seeds", type=int, nargs="*", default=[777])
+    ap.add_argument("--seed", type=int, default=100)
+    a = ap.parse_args(argv)
+
+    def log(s):
+        print(s, file=sys.stderr, flush=True)
+
+    X, T, ds = make_training_set(a.scenes, a.per_scene, a.K, seed=a.seed, log=log)
+    log(json.dumps({"dataset": ds}))
+    with UNet(seed=a.seed) as net:
+        tr = train(net, X, T, a.epochs, a.batch, a.lr, seed=a.seed, log=log)
+        del X, T
+        evals = [time_to_tolerance(net, a.eval_size, a.eval_objects, s, log=log)
+                 for s in a.eval_seeds]
+    print(json.dumps({"dataset": ds, "training": tr, "eval": evals,
+                      "macs_per_block": macs_per_block()}))
+
+
+if __name__ == "__main__":
+    main()
