@@ -346,6 +346,16 @@ int msbg_scene_objects(uint64_t seed, int nObj, float cmin, float cmax,
                        float ellipsoidFrac, float perlinAmp, float perlinFreq,
                        float *xyz);
 
+/* The scene of msbg_scene_objects (same objects; the per-point random
+ * numbers come from a counter-based generator, so the points differ from
+ * the host generator's) generated on the device into the grid's particle
+ * buffer, filtered like msbg_points_upload. append != 0 keeps the points
+ * already held. *nkept receives the grid's particle count. */
+int msbg_points_generate(msbg_grid *g, uint64_t seed, int nObj, float cmin,
+                         float cmax, float rmin, float rmax, int64_t nPts,
+                         float sigma, float ellipsoidFrac, float perlinAmp,
+                         float perlinFreq, int append, int64_t *nkept);
+
 /* ======================================================================== */
 /* learned warm start (no counterpart in the reference)                     */
 /* ======================================================================== */

@@ -69,7 +69,7 @@ struct msbg_unet {
   DevBuf<float> y, dy, tgt;
   // backward
   DevBuf<uint16_t> ga[10], gc9, gc7, gp2, gp1, dzt, xtp;
-  DevBuf<float> wpart, hpart, hsum;
+  DevBuf<float> wpart, hpart, hsum, rsum;
   DevBuf<double> lpart, lsum;
 };
 
@@ -119,6 +119,7 @@ int ensure_cap(msbg_unet *u, int n, bool train) {
     MSBG_TRY(u->wpart.ensure(wp));
     MSBG_TRY(u->hpart.ensure((size_t)head_bwd_blocks(c) * kHead));
     MSBG_TRY(u->hsum.ensure(kHead));
+    MSBG_TRY(u->rsum.ensure((size_t)rowsum_scratch(96)));
     MSBG_TRY(u->lpart.ensure((size_t)mse_blocks((int64_t)c * kVout)));
     MSBG_TRY(u->lsum.ensure(1));
   }
@@ -170,7 +171,7 @@ int backward(msbg_unet *u, int n) {
     const int64_t vo = vol(L.dout);
     uint16_t *gout = u->ga[l].p;  // dL/d a[l], becomes dL/d z[l] in place
     MSBG_TRY(launch_relu_bwd_xpose(s, gout, u->a[l].p, n, vo, L.cout, true, u->dzt.p));
-    MSBG_TRY(launch_rowsum(s, u->dzt.p, L.cout, vo * n, u->g.p + u->offB[l]));
+    MSBG_TRY(launch_rowsum(s, u->dzt.p, L.cout, vo * n, u->rsum.p, u->g.p + u->offB[l]));
     MSBG_TRY(launch_xpose_pad(s, conv_input(u, l), n, L.din, cin_k(l), L.pad, cin16(l),
                               u->xtp.p));
     MSBG_TRY(launch_wgrad(s, u->dzt.p, u->xtp.p, n, cin16(l), L.cout, L.dout, u->wpart.p));
@@ -510,7 +511,9 @@ int msbg_cnn_conv3d_bwd_weights(const uint16_t *x, const uint16_t *dy, int n, in
   const int c16 = (cin + 15) / 16 * 16, din = dout + 2 - 2 * pad;
   const int chunks = wgrad_chunks(c16, cout, dout, n);
   DevBuf<uint16_t> dzt, xtp, dyc;
-  DevBuf<float> part;
+  DevBuf<float> part, rs;
+  MSBG_CHECK(cout <= 128, MSBG_EINVAL, "msbg_cnn_conv3d_bwd_weights: cout %d > 128", cout);
+  MSBG_TRY(rs.alloc((size_t)rowsum_scratch(cout)));
   MSBG_TRY(dyc.alloc((size_t)n * vol(dout) * cout));
   MSBG_TRY(dzt.alloc((size_t)n * vol(dout) * cout));
   MSBG_TRY(xtp.alloc((size_t)n * vol(dout + 2) * c16));
@@ -520,7 +523,7 @@ int msbg_cnn_conv3d_bwd_weights(const uint16_t *x, const uint16_t *dy, int n, in
   MSBG_TRY(launch_xpose_pad(nullptr, x, n, din, cin, pad, c16, xtp.p));
   MSBG_TRY(launch_wgrad(nullptr, dzt.p, xtp.p, n, c16, cout, dout, part.p));
   MSBG_TRY(launch_wgrad_reduce(nullptr, part.p, chunks, c16, cin, cout, dw));
-  if (db) MSBG_TRY(launch_rowsum(nullptr, dzt.p, cout, vol(dout) * n, db));
+  if (db) MSBG_TRY(launch_rowsum(nullptr, dzt.p, cout, vol(dout) * n, rs.p, db));
   MSBG_HIP(hipStreamSynchronize(nullptr));
   return 0;
 }

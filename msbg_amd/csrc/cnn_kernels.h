@@ -36,8 +36,11 @@ int launch_xpose_pad(hipStream_t s, const uint16_t *x, int n, int din, int c, in
 // dy := dy * (y > 0) when relu, in place; dz_t [c][v][n] = the result transposed
 int launch_relu_bwd_xpose(hipStream_t s, uint16_t *dy, const uint16_t *y, int n, int64_t v,
                           int c, bool relu, uint16_t *dz_t);
-// out[c] = sum of row c of a [rows][len] BF16 matrix (FP32, fixed order)
-int launch_rowsum(hipStream_t s, const uint16_t *a, int rows, int64_t len, float *out);
+// out[c] = sum of row c of a [rows][len] BF16 matrix (FP32, fixed order);
+// scratch holds rowsum_scratch(rows) floats (rows <= 128)
+int rowsum_scratch(int rows);
+int launch_rowsum(hipStream_t s, const uint16_t *a, int rows, int64_t len, float *scratch,
+                  float *out);
 int launch_pool2(hipStream_t s, const uint16_t *x, int n, int d, int c, uint16_t *y);
 int launch_pool2_bwd(hipStream_t s, const uint16_t *dy, int n, int d, int c, bool accumulate,
                      uint16_t *dx);

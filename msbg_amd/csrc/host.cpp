@@ -64,7 +64,7 @@ static int ilog2i(int v) {
 }
 
 // sbg.cpp:262-316: domain float bounds, beps doubling from 1e-4.
-static void domain_bounds(int sx, int sy, int sz, float dmin[3], float dmax[3]) {
+void domain_bounds(int sx, int sy, int sz, float dmin[3], float dmax[3]) {
   float s[3] = {(float)sx, (float)sy, (float)sz};
   float beps = 1e-4f;
   for (int iTry = 0; iTry < 10; iTry++) {

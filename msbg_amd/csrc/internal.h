@@ -43,6 +43,9 @@ void set_error(const char *fmt, ...);
 
 int storage_bytes(int storage);
 
+// float bounds of the domain of isInDomainRange (sbg.cpp:262-316)
+void domain_bounds(int sx, int sy, int sz, float dmin[3], float dmax[3]);
+
 // Run body(i) for i in [0, n) on up to hardware_concurrency() host threads,
 // in contiguous chunks.
 void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)> &body);

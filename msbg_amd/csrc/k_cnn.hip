@@ -314,20 +314,34 @@ __global__ void k_relu_bwd_xpose(uint16_t *__restrict__ dy, const uint16_t *__re
       make_uint4(w[0], w[1], w[2], w[3]);
 }
 
+constexpr int kRowChunks = 64;
+
+// partial[row][chunk]: chunk `blockIdx.y` of row `blockIdx.x`, fixed order
 __global__ __launch_bounds__(256) void k_rowsum(const uint16_t *__restrict__ a, int64_t len,
-                                                float *__restrict__ out) {
+                                                float *__restrict__ partial) {
   __shared__ float red[256];
-  const int row = blockIdx.x, tid = threadIdx.x;
+  const int row = blockIdx.x, ch = blockIdx.y, tid = threadIdx.x;
+  const int64_t per = (len + kRowChunks - 1) / kRowChunks;
+  const int64_t b = ch * per, e = b + per < len ? b + per : len;
   const uint16_t *p = a + (int64_t)row * len;
   float s = 0.f;
-  for (int64_t i = tid; i < len; i += 256) s += bf2f(p[i]);
+  for (int64_t i = b + tid; i < e; i += 256) s += bf2f(p[i]);
   red[tid] = s;
   __syncthreads();
   for (int k = 128; k > 0; k >>= 1) {
     if (tid < k) red[tid] += red[tid + k];
     __syncthreads();
   }
-  if (tid == 0) out[row] = red[0];
+  if (tid == 0) partial[(int64_t)row * kRowChunks + ch] = red[0];
+}
+
+__global__ void k_rowsum_final(const float *__restrict__ partial, int rows,
+                               float *__restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  float s = 0.f;
+  for (int c = 0; c < kRowChunks; c++) s += partial[(int64_t)r * kRowChunks + c];
+  out[r] = s;
 }
 
 __global__ void k_pool2(const uint16_t *__restrict__ x, int n, int d, int c,
@@ -494,13 +508,20 @@ __global__ __launch_bounds__(256) void k_head_bwd(const float *__restrict__ dy,
         ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
 }
 
-__global__ void k_sum_partials(const float *__restrict__ partial, int rows, int width,
-                               float *__restrict__ out) {
-  const int c = threadIdx.x;
-  if (c >= width) return;
+// out[c] = sum over rows of partial[row][c]: one block per column, fixed order
+__global__ __launch_bounds__(256) void k_sum_partials(const float *__restrict__ partial, int rows,
+                                                      int width, float *__restrict__ out) {
+  __shared__ float red[256];
+  const int c = blockIdx.x, tid = threadIdx.x;
   float s = 0.f;
-  for (int r = 0; r < rows; r++) s += partial[(int64_t)r * width + c];
-  out[c] = s;
+  for (int r = tid; r < rows; r += 256) s += partial[(int64_t)r * width + c];
+  red[tid] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (tid < k) red[tid] += red[tid + k];
+    __syncthreads();
+  }
+  if (tid == 0) out[c] = red[0];
 }
 
 __global__ __launch_bounds__(256) void k_mse(const float *__restrict__ y,
@@ -686,8 +707,13 @@ int launch_relu_bwd_xpose(hipStream_t s, uint16_t *dy, const uint16_t *y, int n,
   return 0;
 }
 
-int launch_rowsum(hipStream_t s, const uint16_t *a, int rows, int64_t len, float *out) {
-  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)rows), dim3(256), 0, s, a, len, out);
+int rowsum_scratch(int rows) { return rows * kRowChunks; }
+
+int launch_rowsum(hipStream_t s, const uint16_t *a, int rows, int64_t len, float *scratch,
+                  float *out) {
+  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)rows, kRowChunks), dim3(256), 0, s, a, len, scratch);
+  MSBG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_rowsum_final, dim3(1), dim3(128), 0, s, scratch, rows, out);
   MSBG_HIP(hipGetLastError());
   return 0;
 }
@@ -752,7 +778,8 @@ int launch_head_bwd(hipStream_t s, const float *dy, const uint16_t *d, int n, co
 }
 
 int launch_sum_partials(hipStream_t s, const float *partial, int rows, int width, float *out) {
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, partial, rows, width, out);
+  hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)width), dim3(256), 0, s, partial, rows, width,
+                     out);
   MSBG_HIP(hipGetLastError());
   return 0;
 }

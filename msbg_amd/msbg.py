@@ -230,6 +230,17 @@ class MultiresSparseGrid:
         self._npoints = nk.value
         return nk.value
 
+    def points_generate(self, seed, nObj, cmin, cmax, rmin, rmax, nPts, sigma=0.5,
+                        ellipsoidFrac=0.0, perlinAmp=0.0, perlinFreq=0.0, append=False):
+        """Scene points generated on the device (msbg_points_generate); returns
+        the grid's particle count."""
+        n = C.c_int64()
+        check(lib().msbg_points_generate(self._h, seed, nObj, cmin, cmax, rmin, rmax, nPts, sigma,
+                                         ellipsoidFrac, perlinAmp, perlinFreq, int(append),
+                                         C.byref(n)))
+        self._npoints = n.value
+        return n.value
+
     def points_download(self):
         out = np.empty((self._npoints, 3), dtype=np.float32)
         check(lib().msbg_points_download(self._h, _ptr(out), out.shape[0]))

@@ -26,7 +26,7 @@ import time
 import numpy as np
 
 from . import _lib as L
-from .msbg import MCF_8COLOR, Batch, MultiresSparseGrid, UNet, scene_objects, scene_spheres
+from .msbg import MCF_8COLOR, Batch, MultiresSparseGrid, UNet
 
 CH_IN, CH_K, CH_RUN = 60, 61, 62
 RSCAN = 4.0
@@ -44,21 +44,17 @@ def macs_per_block():
     return m + 32 ** 3 * 16
 
 
-def scene_points(seed, S, nObj, rmin=10.0, rmax=150.0, nPts=40000, perlinAmp=0.2,
-                 perlinFreq=1.0 / 48):
-    """Half spheres, half Perlin-displaced ellipsoids, centres inside the domain."""
-    ns = nObj // 2
-    m = rmax * (1 + perlinAmp) + 8
-    a = scene_spheres(seed, ns, m, S - m, rmin, rmax, nPts, 0.5, 0.0)
-    b = scene_objects(seed + 7919, nObj - ns, m, S - m, rmin, rmax, nPts, 0.5, 1.0, perlinAmp,
-                      perlinFreq)
-    return np.concatenate([a, b])
-
-
-def build_scene(seed, S, nObj, storage=L.F32, chan=CH_IN):
+def build_scene(seed, S, nObj, storage=L.F32, chan=CH_IN, rmin=10.0, rmax=150.0, nPts=40000,
+                perlinAmp=0.2, perlinFreq=1.0 / 48):
+    """Half spheres, half Perlin-displaced ellipsoids (points generated on
+    the device), splatted and finalized into `chan`."""
     g = MultiresSparseGrid.create("warmstart", S, S, S, 32, options=L.OPT_SINGLE_LEVEL, nLevels=3)
     try:
-        g.points_upload(scene_points(seed, S, nObj))
+        ns = nObj // 2
+        m = rmax * (1 + perlinAmp) + 8
+        g.points_generate(seed, ns, m, S - m, rmin, rmax, nPts, 0.5, 0.0)
+        g.points_generate(seed + 7919, nObj - ns, m, S - m, rmin, rmax, nPts, 0.5, 1.0, perlinAmp,
+                          perlinFreq, append=True)
         lv = g.regularizeRefinementMap(g.levels_from_points([RSCAN, RSCAN + 32]))
         g.setRefinementMap(lv)
         g.alloc_demo(chan, storage)

@@ -54,3 +54,19 @@ def test_create_without_device_fails_cleanly(have_gpu):
     with pytest.raises(M.MsbgError) as e:
         M.MultiresSparseGrid.create("x", 64, 64, 64, 16)
     assert "EDEVICE" in str(e.value)
+
+
+def test_scene_objects_perlin_displacement():
+    import numpy as np
+    from msbg_amd import scene_objects, scene_spheres
+    a = scene_spheres(99, 3, 100, 400, 30, 60, 2000, 0.5, 0.5)
+    b = scene_objects(99, 3, 100, 400, 30, 60, 2000, 0.5, 0.5, 0.0, 0.0)
+    assert np.array_equal(a, b)
+    # one sphere of radius 50, 30 % noise amplitude: radii stay inside the
+    # displaced band and vary far more than the jitter alone
+    p = scene_objects(5, 1, 256, 256, 50, 50, 20000, 0.0, 0.0, 0.3, 1.0 / 16)
+    r = np.linalg.norm(p.astype(np.float64) - 256.0, axis=1)
+    assert r.min() >= 50 * 0.7 - 1e-3 and r.max() <= 50 * 1.3 + 1e-3
+    assert r.std() > 2.0
+    # the same seed gives the same field
+    assert np.array_equal(p, scene_objects(5, 1, 256, 256, 50, 50, 20000, 0.0, 0.0, 0.3, 1.0 / 16))

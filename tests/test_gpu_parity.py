@@ -483,3 +483,27 @@ def test_batched_study_small():
     assert r["voxels"] > 0 and r["grids"] == 8
     assert sorted(int(k) for k in r["by_size"]) == [64, 128]
     assert all(t in (0, 1) for _, _, t in r["tiles"])
+
+
+def test_points_generate_on_device():
+    note("points generate")
+    S = 256
+    g = M.MultiresSparseGrid.create("pg", S, S, S, 16, options=M.OPT_SINGLE_LEVEL, nLevels=2)
+    with g:
+        n = g.points_generate(11, 1, 128, 128, 40, 40, 20000, 0.5, 0.0)
+        assert n == 20000
+        p = g.points_download().astype(np.float64)
+        r = np.linalg.norm(p - 128.0, axis=1)
+        assert abs(r.mean() - 40) < 0.05 and 0.4 < r.std() < 0.6
+        assert np.array_equal(g.points_download(), p.astype(np.float32))
+        # deterministic, and append keeps the earlier points in front
+        n2 = g.points_generate(11, 1, 128, 128, 40, 40, 20000, 0.5, 0.0)
+        assert n2 == 20000 and np.array_equal(g.points_download(), p.astype(np.float32))
+        n3 = g.points_generate(12, 2, 60, 200, 10, 20, 5000, 0.5, 1.0, 0.2, 0.05, append=True)
+        q = g.points_download()
+        assert n3 == 30000 and np.array_equal(q[:20000], p.astype(np.float32))
+        # a sphere across the domain corner: the filter of msbg_points_upload
+        n4 = g.points_generate(13, 1, 2, 2, 30, 30, 20000, 0.5, 0.0)
+        kept = g.points_download()
+        assert 1000 < n4 < 6000 and kept.shape == (n4, 3)
+        assert O.domain_filter(S, S, S, kept).all()
