@@ -40,9 +40,9 @@ constexpr LayerSpec kL[10] = {{1, 16, 0, 36, 34},  {16, 16, 0, 34, 32}, {16, 32,
 constexpr int kHead = 17;
 constexpr int64_t kVin = 36 * 36 * 36, kVout = 32 * 32 * 32;
 
-// input channels as the conv kernel sees them (layer 0's single channel is
-// padded to 8), and as the weight-gradient kernel sees them (16-multiples)
-inline int cin_k(int l) { return l == 0 ? 8 : kL[l].cin; }
+// input channels as the conv kernel sees them, and as the weight-gradient
+// kernel sees them (16-multiples: layer 0's single channel is padded)
+inline int cin_k(int l) { return kL[l].cin; }
 inline int cin16(int l) { return l == 0 ? 16 : kL[l].cin; }
 inline int64_t vol(int d) { return (int64_t)d * d * d; }
 
@@ -65,7 +65,7 @@ struct msbg_unet {
   bool swz_valid = false;
   int cap = 0;
   // forward activations (channels-last BF16)
-  DevBuf<uint16_t> x0, x8, a[10], p1, p2, c7, c9;
+  DevBuf<uint16_t> x0, a[10], p1, p2, c7, c9;
   DevBuf<float> y, dy, tgt;
   // backward
   DevBuf<uint16_t> ga[10], gc9, gc7, gp2, gp1, dzt, xtp;
@@ -77,7 +77,7 @@ namespace {
 
 const uint16_t *conv_input(msbg_unet *u, int l) {
   switch (l) {
-    case 0: return u->x8.p;
+    case 0: return u->x0.p;
     case 2: return u->p1.p;
     case 4: return u->p2.p;
     case 6: return u->c7.p;
@@ -91,7 +91,6 @@ int ensure_cap(msbg_unet *u, int n, bool train) {
   if (n <= u->cap && (!train || u->ga[0].n >= (size_t)n * vol(34) * 16)) return 0;
   const int c = n > u->cap ? n : u->cap;
   MSBG_TRY(u->x0.ensure((size_t)c * kVin));
-  MSBG_TRY(u->x8.ensure((size_t)c * kVin * 8));
   for (int l = 0; l < 10; l++) MSBG_TRY(u->a[l].ensure((size_t)c * vol(kL[l].dout) * kL[l].cout));
   MSBG_TRY(u->p1.ensure((size_t)c * vol(16) * 16));
   MSBG_TRY(u->p2.ensure((size_t)c * vol(8) * 32));
@@ -148,7 +147,6 @@ int swizzle_all(msbg_unet *u) {
 int forward(msbg_unet *u, int n) {
   hipStream_t s = u->stream;
   MSBG_TRY(swizzle_all(u));
-  MSBG_TRY(launch_expand8(s, u->x0.p, n, u->x8.p));
   for (int l = 0; l < 10; l++) {
     const LayerSpec &L = kL[l];
     MSBG_TRY(launch_conv(s, conv_input(u, l), n, cin_k(l), L.cout, L.dout, L.pad, u->wf[l].p,
@@ -481,7 +479,8 @@ int msbg_warm_start(msbg_grid *g, int chan, msbg_unet *u, int batch, double *ms)
 // and backward-weights products, on device tensors.
 int msbg_cnn_conv3d(const uint16_t *x, int n, int cin, int cout, int dout, int pad,
                     const float *w, const float *b, int relu, uint16_t *y) {
-  MSBG_CHECK(x && w && y && cin % 8 == 0, MSBG_EINVAL, "msbg_cnn_conv3d: bad args");
+  MSBG_CHECK(x && w && y && (cin == 1 || cin % 8 == 0), MSBG_EINVAL,
+             "msbg_cnn_conv3d: bad args");
   DevBuf<uint4> sw;
   MSBG_TRY(sw.alloc((size_t)swizzled_size(cin, cout)));
   MSBG_TRY(launch_swizzle(nullptr, w, cin, cin, cout, false, sw.p));
@@ -504,7 +503,7 @@ int msbg_cnn_conv3d_bwd_data(const uint16_t *dy, int n, int cin, int cout, int d
 
 int msbg_cnn_conv3d_bwd_weights(const uint16_t *x, const uint16_t *dy, int n, int cin, int cout,
                                 int dout, int pad, float *dw, float *db) {
-  MSBG_CHECK(x && dy && dw && cin % 8 == 0, MSBG_EINVAL,
+  MSBG_CHECK(x && dy && dw && (cin == 1 || cin % 8 == 0), MSBG_EINVAL,
              "msbg_cnn_conv3d_bwd_weights: bad args");
   MSBG_CHECK(n % 8 == 0 && (n & (n - 1)) == 0, MSBG_EINVAL,
              "msbg_cnn_conv3d_bwd_weights: batch %d must be 8 * 2^k", n);

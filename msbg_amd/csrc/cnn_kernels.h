@@ -19,7 +19,8 @@ int64_t swizzled_size(int cin, int cout);
 // weights of the backward-data conv (flipped taps, cin and cout swapped)
 int launch_swizzle(hipStream_t s, const float *w, int cin_true, int cin, int cout, bool flip,
                    uint4 *out);
-// y = relu?(conv(x) + bias); bias may be null
+// y = relu?(conv(x) + bias); bias may be null. cin 1 (the first layer) has
+// its own kernel; other cin are multiples of 8.
 int launch_conv(hipStream_t s, const uint16_t *x, int n, int cin, int cout, int dout, int pad,
                 const uint4 *wsw, const float *bias, bool relu, uint16_t *y);
 // dw partials for the conv x(din=dout+2-2pad) -> dz(dout): x_tp is x transposed and
@@ -48,8 +49,6 @@ int launch_upcat(hipStream_t s, const uint16_t *a, int ca, const uint16_t *b, in
                  uint16_t *out);
 int launch_upcat_bwd(hipStream_t s, const uint16_t *dout, int ca, int cb, int n, int d,
                      uint16_t *da, uint16_t *db);
-// x0 [n][36^3] -> [n][36^3][8] with the value in channel 0
-int launch_expand8(hipStream_t s, const uint16_t *x0, int n, uint16_t *x8);
 // y[n][32^3] = hb + sum_c hw[c] * d[n][v][c]; head = {hw[16], hb}
 int launch_head(hipStream_t s, const uint16_t *d, int n, const float *head, float *y);
 // dd = dy * hw (BF16); partials [blocks][17] of (sum dy*d_c, sum dy)

@@ -166,6 +166,74 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ x,
   }
 }
 
+// The single-channel first layer: k = tap (27 of 32 used), one channel, so
+// the slab holds one BF16 per voxel and each lane gathers its 8 taps.
+template <int COUT, int DOUT, int PAD>
+__global__ __launch_bounds__(256) void k_conv_c1(const uint16_t *__restrict__ x,
+                                                 const uint4 *__restrict__ wsw,
+                                                 const float *__restrict__ bias, int relu,
+                                                 uint16_t *__restrict__ y) {
+  using K = ConvShape<8, COUT, DOUT, PAD>;  // the tiling of the general kernel
+  constexpr int DIN = K::DIN, TZ = K::TZ, TY = K::TY, XP = K::XP, TM = K::TM, TN = K::TN,
+                SZ = K::SZ, SY = K::SY, SX = K::SX;
+  __shared__ uint16_t slab[SZ * SY * SX];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int q = lane >> 4, col = lane & 15;
+  const int n = blockIdx.y;
+  constexpr int NYT = DOUT / TY;
+  const int z0 = (int)(blockIdx.x / NYT) * TZ, y0 = (int)(blockIdx.x % NYT) * TY;
+  const uint16_t *xs = x + (int64_t)n * DIN * DIN * DIN;
+  for (int g = tid; g < SZ * SY * SX; g += 256) {
+    const int sx = g % SX, sy = (g / SX) % SY, sz = g / (SX * SY);
+    const int iz = z0 - PAD + sz, iy = y0 - PAD + sy, ix = sx - PAD;
+    uint16_t v = 0;
+    if (iz >= 0 && iz < DIN && iy >= 0 && iy < DIN && ix >= 0 && ix < DIN)
+      v = xs[((int64_t)iz * DIN + iy) * DIN + ix];
+    slab[g] = v;
+  }
+  bf16x8 a[TM];
+#pragma unroll
+  for (int m = 0; m < TM; m++) a[m] = __builtin_bit_cast(bf16x8, wsw[m * 64 + lane]);
+  __syncthreads();
+
+#pragma unroll
+  for (int t = 0; t < TN; t++) {
+    const int T = wave * TN + t;
+    const int tz = T / (TY * XP / 16), ty = (T / (XP / 16)) % TY, tx = (T % (XP / 16)) * 16 + col;
+    uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int tap = 8 * q + j;
+      uint32_t v = 0;
+      if (tap < 27)
+        v = slab[((tz + tap / 9) * SY + ty + (tap / 3) % 3) * SX + tx + tap % 3];
+      w[j >> 1] |= v << (16 * (j & 1));
+    }
+    const bf16x8 b = __builtin_bit_cast(bf16x8, make_uint4(w[0], w[1], w[2], w[3]));
+    f32x4 acc[TM];
+#pragma unroll
+    for (int m = 0; m < TM; m++)
+      acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0,
+                                                       0);
+    if (tx >= DOUT) continue;
+    uint16_t *o = y + ((int64_t)n * DOUT * DOUT * DOUT +
+                       ((int64_t)(z0 + tz) * DOUT + (y0 + ty)) * DOUT + tx) * COUT;
+#pragma unroll
+    for (int m = 0; m < TM; m++) {
+      const int co = m * 16 + 4 * q;
+      uint32_t h[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        float f = acc[m][r] + (bias ? bias[co + r] : 0.f);
+        if (relu) f = f > 0.f ? f : 0.f;
+        h[r] = f2bf(f);
+      }
+      *(uint2 *)(o + co) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+    }
+  }
+}
+
 __global__ void k_swizzle(const float *__restrict__ w, int cin_true, int cin, int cout, int flip,
                           uint4 *__restrict__ out, int64_t total) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -441,14 +509,7 @@ __global__ void k_upcat_bwd_b(const uint16_t *__restrict__ dout, int ca, int cb,
   *(uint4 *)(db + sv * cb + c8 * 8) = *(const uint4 *)(dout + sv * c + ca + c8 * 8);
 }
 
-__global__ void k_expand8(const uint16_t *__restrict__ x0, int64_t total,
-                          uint16_t *__restrict__ x8) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  *(uint4 *)(x8 + i * 8) = make_uint4((uint32_t)x0[i], 0, 0, 0);
-}
-
-constexpr int kHeadC = 16, kOut = 32, kIn = 36;
+constexpr int kHeadC = 16, kOut = 32;
 
 __global__ void k_head(const uint16_t *__restrict__ d, int n, const float *__restrict__ head,
                        float *__restrict__ y) {
@@ -586,6 +647,17 @@ int conv_t(hipStream_t s, const uint16_t *x, int n, const uint4 *wsw, const floa
   return 0;
 }
 
+template <int COUT, int DOUT, int PAD>
+int conv_c1_t(hipStream_t s, const uint16_t *x, int n, const uint4 *wsw, const float *bias,
+              bool relu, uint16_t *y) {
+  using K = ConvShape<8, COUT, DOUT, PAD>;
+  dim3 grid((DOUT / K::TZ) * (DOUT / K::TY), (unsigned)n);
+  hipLaunchKernelGGL((k_conv_c1<COUT, DOUT, PAD>), grid, dim3(256), 0, s, x, wsw, bias,
+                     relu ? 1 : 0, y);
+  MSBG_HIP(hipGetLastError());
+  return 0;
+}
+
 template <int CIN, int COUT, int DOUT>
 int wgrad_t(hipStream_t s, const uint16_t *dzt, const uint16_t *xtp, int n, float *partial) {
   int lg = 0;
@@ -614,7 +686,7 @@ int64_t swizzled_size(int cin, int cout) {
 
 int launch_swizzle(hipStream_t s, const float *w, int cin_true, int cin, int cout, bool flip,
                    uint4 *out) {
-  MSBG_CHECK(cin % 8 == 0 && cout % 16 == 0 && cin_true <= cin, MSBG_EINVAL,
+  MSBG_CHECK((cin == 1 || cin % 8 == 0) && cout % 16 == 0 && cin_true <= cin, MSBG_EINVAL,
              "swizzle: bad shape cin %d cout %d", cin, cout);
   const int64_t total = swizzled_size(cin, cout);
   hipLaunchKernelGGL(k_swizzle, dim3(blocks_for(total)), dim3(256), 0, s, w, cin_true, cin, cout,
@@ -629,7 +701,8 @@ int launch_conv(hipStream_t s, const uint16_t *x, int n, int cin, int cout, int 
 #define MSBG_CONV_CASE(CI, CO, D, P) \
   if (cin == CI && cout == CO && dout == D && pad == P) return conv_t<CI, CO, D, P>(s, x, n, wsw, bias, relu, y);
   // forward layers
-  MSBG_CONV_CASE(8, 16, 34, 0)
+  if (cin == 1 && cout == 16 && dout == 34 && pad == 0)
+    return conv_c1_t<16, 34, 0>(s, x, n, wsw, bias, relu, y);
   MSBG_CONV_CASE(16, 16, 32, 0)
   MSBG_CONV_CASE(16, 32, 16, 1)
   MSBG_CONV_CASE(32, 32, 16, 1)
@@ -749,13 +822,6 @@ int launch_upcat_bwd(hipStream_t s, const uint16_t *dout, int ca, int cb, int n,
   hipLaunchKernelGGL(k_upcat_bwd_a, dim3(blocks_for(ta)), dim3(256), 0, s, dout, ca, cb, n, d, da);
   MSBG_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_upcat_bwd_b, dim3(blocks_for(tb)), dim3(256), 0, s, dout, ca, cb, n, d, db);
-  MSBG_HIP(hipGetLastError());
-  return 0;
-}
-
-int launch_expand8(hipStream_t s, const uint16_t *x0, int n, uint16_t *x8) {
-  const int64_t total = (int64_t)n * kIn * kIn * kIn;
-  hipLaunchKernelGGL(k_expand8, dim3(blocks_for(total)), dim3(256), 0, s, x0, total, x8);
   MSBG_HIP(hipGetLastError());
   return 0;
 }

@@ -49,7 +49,12 @@ def scene_objects(seed, nObj, cmin, cmax, rmin, rmax, nPts, sigma=0.5, ellipsoid
 
 def _dptr(t):
     """Device address of a tensor-like object with data_ptr() (e.g. a torch
-    CUDA tensor); the library only sees the address."""
+    CUDA tensor); the library only sees the address. Work already queued on
+    the tensor's own stream (torch's current stream) is waited for first:
+    the library runs on its own streams."""
+    if getattr(t, "is_cuda", False):
+        import torch
+        torch.cuda.current_stream(t.device).synchronize()
     return C.c_void_p(t.data_ptr())
 
 

@@ -49,7 +49,7 @@ def assert_bf16_close(got, ref, what):
     assert ok.all(), (what, float(err.max()), int((~ok).sum()), scale)
 
 
-FWD = [(8, 16, 34, 0), (16, 16, 32, 0), (16, 32, 16, 1), (32, 32, 16, 1), (32, 64, 8, 1),
+FWD = [(1, 16, 34, 0), (16, 16, 32, 0), (16, 32, 16, 1), (32, 32, 16, 1), (32, 64, 8, 1),
        (64, 64, 8, 1), (96, 32, 16, 1), (48, 16, 32, 1), (16, 16, 32, 1)]
 
 
@@ -84,7 +84,7 @@ def test_conv_backward_data_matches_oracle(cin, cout, din, pad):
     assert_bf16_close(host(dx), O.conv3d_bwd_data(dy, w, pad), "bwd data")
 
 
-@pytest.mark.parametrize("cin,cout,dout,pad", [(8, 16, 34, 0), (16, 16, 32, 0), (16, 32, 16, 1),
+@pytest.mark.parametrize("cin,cout,dout,pad", [(1, 16, 34, 0), (16, 16, 32, 0), (16, 32, 16, 1),
                                                (32, 32, 16, 1), (32, 64, 8, 1), (64, 64, 8, 1),
                                                (96, 32, 16, 1), (48, 16, 32, 1)])
 def test_conv_backward_weights_matches_oracle(cin, cout, dout, pad):
