@@ -167,26 +167,30 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ x,
 }
 
 // The single-channel first layer: k = tap (27 of 32 used), one channel, so
-// the slab holds one BF16 per voxel and each lane gathers its 8 taps.
+// the slab holds one BF16 per voxel and each lane gathers its 8 taps. A
+// workgroup owns C1_TZ whole output planes; its waves stride over the
+// 16-voxel tiles.
+constexpr int C1_TZ = 2;
+
 template <int COUT, int DOUT, int PAD>
 __global__ __launch_bounds__(256) void k_conv_c1(const uint16_t *__restrict__ x,
                                                  const uint4 *__restrict__ wsw,
                                                  const float *__restrict__ bias, int relu,
                                                  uint16_t *__restrict__ y) {
-  using K = ConvShape<8, COUT, DOUT, PAD>;  // the tiling of the general kernel
-  constexpr int DIN = K::DIN, TZ = K::TZ, TY = K::TY, XP = K::XP, TM = K::TM, TN = K::TN,
-                SZ = K::SZ, SY = K::SY, SX = K::SX;
+  constexpr int DIN = DOUT + 2 - 2 * PAD, XP = (DOUT + 15) / 16 * 16, TM = COUT / 16;
+  constexpr int SZ = C1_TZ + 2, SY = DOUT + 2, SX = XP + 2;
+  constexpr int XT = XP / 16, NTILE = C1_TZ * DOUT * XT;
+  static_assert(DOUT % C1_TZ == 0, "z tiles");
   __shared__ uint16_t slab[SZ * SY * SX];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q = lane >> 4, col = lane & 15;
   const int n = blockIdx.y;
-  constexpr int NYT = DOUT / TY;
-  const int z0 = (int)(blockIdx.x / NYT) * TZ, y0 = (int)(blockIdx.x % NYT) * TY;
+  const int z0 = (int)blockIdx.x * C1_TZ;
   const uint16_t *xs = x + (int64_t)n * DIN * DIN * DIN;
   for (int g = tid; g < SZ * SY * SX; g += 256) {
     const int sx = g % SX, sy = (g / SX) % SY, sz = g / (SX * SY);
-    const int iz = z0 - PAD + sz, iy = y0 - PAD + sy, ix = sx - PAD;
+    const int iz = z0 - PAD + sz, iy = sy - PAD, ix = sx - PAD;
     uint16_t v = 0;
     if (iz >= 0 && iz < DIN && iy >= 0 && iy < DIN && ix >= 0 && ix < DIN)
       v = xs[((int64_t)iz * DIN + iy) * DIN + ix];
@@ -195,19 +199,22 @@ __global__ __launch_bounds__(256) void k_conv_c1(const uint16_t *__restrict__ x,
   bf16x8 a[TM];
 #pragma unroll
   for (int m = 0; m < TM; m++) a[m] = __builtin_bit_cast(bf16x8, wsw[m * 64 + lane]);
+  // this lane's 8 taps as slab offsets (taps 27..31 are zero)
+  int toff[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int tap = 8 * q + j;
+    toff[j] = tap < 27 ? ((tap / 9) * SY + (tap / 3) % 3) * SX + tap % 3 : -1;
+  }
   __syncthreads();
 
-#pragma unroll
-  for (int t = 0; t < TN; t++) {
-    const int T = wave * TN + t;
-    const int tz = T / (TY * XP / 16), ty = (T / (XP / 16)) % TY, tx = (T % (XP / 16)) * 16 + col;
+  for (int T = wave; T < NTILE; T += 4) {
+    const int tz = T / (DOUT * XT), ty = (T / XT) % DOUT, tx = (T % XT) * 16 + col;
+    const int base = (tz * SY + ty) * SX + tx;
     uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-      const int tap = 8 * q + j;
-      uint32_t v = 0;
-      if (tap < 27)
-        v = slab[((tz + tap / 9) * SY + ty + (tap / 3) % 3) * SX + tx + tap % 3];
+      const uint32_t v = toff[j] >= 0 ? slab[base + toff[j]] : 0u;
       w[j >> 1] |= v << (16 * (j & 1));
     }
     const bf16x8 b = __builtin_bit_cast(bf16x8, make_uint4(w[0], w[1], w[2], w[3]));
@@ -218,7 +225,7 @@ __global__ __launch_bounds__(256) void k_conv_c1(const uint16_t *__restrict__ x,
                                                        0);
     if (tx >= DOUT) continue;
     uint16_t *o = y + ((int64_t)n * DOUT * DOUT * DOUT +
-                       ((int64_t)(z0 + tz) * DOUT + (y0 + ty)) * DOUT + tx) * COUT;
+                       ((int64_t)(z0 + tz) * DOUT + ty) * DOUT + tx) * COUT;
 #pragma unroll
     for (int m = 0; m < TM; m++) {
       const int co = m * 16 + 4 * q;
@@ -650,8 +657,7 @@ int conv_t(hipStream_t s, const uint16_t *x, int n, const uint4 *wsw, const floa
 template <int COUT, int DOUT, int PAD>
 int conv_c1_t(hipStream_t s, const uint16_t *x, int n, const uint4 *wsw, const float *bias,
               bool relu, uint16_t *y) {
-  using K = ConvShape<8, COUT, DOUT, PAD>;
-  dim3 grid((DOUT / K::TZ) * (DOUT / K::TY), (unsigned)n);
+  dim3 grid(DOUT / C1_TZ, (unsigned)n);
   hipLaunchKernelGGL((k_conv_c1<COUT, DOUT, PAD>), grid, dim3(256), 0, s, x, wsw, bias,
                      relu ? 1 : 0, y);
   MSBG_HIP(hipGetLastError());

@@ -58,6 +58,9 @@ def load(kind="strict"):
             "or_halo_fill_f64": (None, [P, I, I, C.c_uint, P]),
             "or_mcf_sweeps": (I, [P, P, I, P, I, F, I, P, I]),
             "or_mcf_voxel": (F, [P, I, I, F, I]),
+            "or_pde_sweeps": (I, [P, P, I, P, I, P, P, I]),
+            "or_pde_voxel": (F, [I, P, I, I, F, F, F, I]),
+            "or_fast_exp": (F, [F]),
             "or_mcf_voxel_f64": (D, [P, I, I, D, I]),
             "or_ghost_downsample": (None, [P, I, P]),
             "or_prepare_ghost_blocks": (None, [P, I, P, P, P]),
@@ -176,6 +179,42 @@ def halo_fill(level, bid, dmax=1, opt=OPT_BC_NEUMANN, lo=None):
     load().or_halo_fill(C.byref(s), C.byref(so) if so is not None else None, bid, dmax, opt,
                         _p(out))
     return out
+
+
+MODE_COLOR8, MODE_REDBLACK, MODE_JACOBI = 0, 1, 2
+
+
+class PdeParams(C.Structure):
+    _fields_ = [("op", C.c_int), ("mode", C.c_int), ("dt", C.c_float),
+                ("doConstrZeroOne", C.c_int), ("densThr", C.c_float), ("densThrTol1", C.c_float),
+                ("densThrTol2", C.c_float), ("taubin", C.c_float), ("alpha", C.c_float)]
+
+
+def pde_sweeps(level, blocks, levels, numIter, op, mode=MODE_COLOR8, dt=0.1, doConstrZeroOne=1,
+               densThr=0.0, densThrTol1=0.0, densThrTol2=0.0, taubin=0.0, alpha=0.0,
+               residual=False, nthreads=0):
+    """or_pde_sweeps: applyChannelPdeFast's stencils 1/2/3/4/14/15/17 in
+    colour, red-black or Jacobi mode; pages of `level` are updated."""
+    b = np.ascontiguousarray(blocks, dtype=np.int32)
+    lv = None if levels is None else np.ascontiguousarray(levels, dtype=np.uint16)
+    n = numIter + (numIter % 2)
+    res = np.zeros(max(n, 1), np.float64) if residual else None
+    P = PdeParams(op, mode, dt, doConstrZeroOne, densThr, densThrTol1, densThrTol2, taubin, alpha)
+    s = level.struct()
+    rc = load().or_pde_sweeps(C.byref(s), _p(b), b.size, _p(lv), numIter, C.byref(P), _p(res),
+                              nthreads)
+    if rc < 0:
+        raise ValueError("or_pde_sweeps: unsupported combination")
+    return res[:n] if residual else None
+
+
+def pde_voxel(op, h, hid, dt, taubin=0.0, alpha=0.0, it=0):
+    h = np.ascontiguousarray(h, dtype=np.float32)
+    return load().or_pde_voxel(op, _p(h), h.shape[-1], hid, dt, taubin, alpha, it)
+
+
+def fast_exp(x):
+    return load().or_fast_exp(float(x))
 
 
 def mcf_sweeps(level, blocks, levels, numIter, dt, doConstrZeroOne=1, residual=False,

@@ -779,6 +779,267 @@ int or_mcf_sweeps(const or_level *L, const int32_t *blocks, int n,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Other stencils and driver modes of applyChannelPdeFast, msbg4.cpp:60-957 */
+/* ------------------------------------------------------------------------ */
+float or_fast_exp(float x0) {
+  /* fastmath.h:583-628: range reduction by ln 2 in two parts, a polynomial
+   * whose second coefficient is added twice, scaling by 2^n through the
+   * exponent bits, and the range selects */
+  const float MAXLOGF = 88.72283905206835f, MINLOGF = -88.f;
+  const float C1F = 0.693359375f, C2F = -2.12194440e-4f;
+  const float PX1 = 1.9875691500E-4f, PX2 = 1.3981999507E-3f, PX3 = 8.3334519073E-3f,
+              PX4 = 4.1665795894E-2f, PX5 = 1.6666665459E-1f, PX6 = 5.0000001201E-1f;
+  const float LOG2EF = 1.44269504088896341f;
+  float x = x0;
+  float z = floorf(LOG2EF * x + 0.5f);
+  x -= z * C1F;
+  x -= z * C2F;
+  const int n = (int)z;
+  const float x2 = x * x;
+  z = x * PX1;
+  z += PX2;
+  z *= x;
+  z += PX2;
+  z *= x;
+  z += PX3;
+  z *= x;
+  z += PX4;
+  z *= x;
+  z += PX5;
+  z *= x;
+  z += PX6;
+  z *= x2;
+  z += x + 1.0f;
+  union {
+    uint32_t u;
+    float f;
+  } sc;
+  sc.u = (uint32_t)(n + 0x7f) << 23;
+  z *= sc.f;
+  if (x0 < MINLOGF) z = 0.0f;
+  if (x0 > MAXLOGF) z = INFINITY;
+  return z;
+}
+
+/* G_ of msbg4.cpp:761-764 */
+static inline float pm_weight(float f, float alpha) {
+  if (alpha < 0) {
+    const float s = fabsf(f) * (1.0f / -alpha);
+    return or_fast_exp(-(s * s));
+  }
+  const float s = fabsf(f) / alpha;
+  return 1.0f / (1.0f + s * s);
+}
+
+float or_pde_voxel(int op, const float *h, int hsx, int hid, float dt, float taubin,
+                   float alpha, int iter) {
+  const int s1 = hsx, s2 = hsx * hsx;
+#define F_(dx, dy, dz) h[hid + (dx) + (dy)*s1 + (dz)*s2]
+  const float F0 = F_(0, 0, 0), F1 = F_(-1, 0, 0), F2 = F_(1, 0, 0), F3 = F_(0, -1, 0),
+              F4 = F_(0, 1, 0), F5 = F_(0, 0, -1), F6 = F_(0, 0, 1);
+  float phi = F0;
+  switch (op) {
+    case 1: { /* msbg4.cpp:682-712 */
+      float L = F1 + F2 + F3 + F4 + F5 + F6 - 6.0f * F0;
+      float GX = 0.5f * (F2 - F1), GY = 0.5f * (F4 - F3), GZ = 0.5f * (F6 - F5);
+      float G = sqrtf(GX * GX + GY * GY + GZ * GZ);
+      float D = dt * L * G;
+      if (taubin != 0.0f)
+        phi = (iter & 1) ? phi + D : phi - taubin * D;
+      else
+        phi += D;
+      break;
+    }
+    case 2: { /* msbg4.cpp:637-680, halo 2 */
+      float H = 42.f * F0 -
+                12.f * (F_(1, 0, 0) + F_(0, 1, 0) + F_(0, 0, 1) + F_(-1, 0, 0) + F_(0, -1, 0) +
+                        F_(0, 0, -1)) +
+                1.f * (F_(2, 0, 0) + F_(0, 2, 0) + F_(0, 0, 2) + F_(-2, 0, 0) + F_(0, -2, 0) +
+                       F_(0, 0, -2)) +
+                2.f * (F_(1, 0, 1) + F_(0, 1, 1) + F_(-1, 0, 1) + F_(0, -1, 1) + F_(1, 0, -1) +
+                       F_(0, 1, -1) + F_(-1, 0, -1) + F_(0, -1, -1) + F_(1, 1, 0) +
+                       F_(-1, 1, 0) + F_(-1, -1, 0) + F_(1, -1, 0));
+      float GX = 0.5f * (F2 - F1), GY = 0.5f * (F4 - F3), GZ = 0.5f * (F6 - F5);
+      float G = sqrtf(GX * GX + GY * GY + GZ * GZ);
+      float D = -dt * H * G;
+      D = vminf(vmaxf(D, -0.1f), 0.1f);
+      phi = F0 + D;
+      break;
+    }
+    case 3: { /* msbg4.cpp:713-735 */
+      float S = (1.0f / 6.0f) * (F1 + F2 + F3 + F4 + F5 + F6);
+      float D = dt * (S - phi);
+      phi += D;
+      break;
+    }
+    case 14:
+    case 15: { /* msbg4.cpp:736-790 */
+      float g1 = F1 - F0, g2 = F2 - F0, g3 = F3 - F0, g4 = F4 - F0, g5 = F5 - F0, g6 = F6 - F0;
+      float G1 = pm_weight(g1, alpha), G2 = pm_weight(g2, alpha), G3 = pm_weight(g3, alpha),
+            G4 = pm_weight(g4, alpha), G5 = pm_weight(g5, alpha), G6 = pm_weight(g6, alpha);
+      if (op == 15) {
+        float sum = G1 * F1 + G2 * F2 + G3 * F3 + G4 * F4 + G5 * F5 + G6 * F6;
+        float wsum = G1 + G2 + G3 + G4 + G5 + G6;
+        phi = (phi + dt * sum) * 1.0f / (1.0f + dt * wsum);
+      } else {
+        float sum = G1 * g1 + G2 * g2 + G3 * g3 + G4 * g4 + G5 * g5 + G6 * g6;
+        phi += dt * sum;
+      }
+      break;
+    }
+    case 17: /* msbg4.cpp:791-805 */
+      phi = (phi + dt * (F1 + F2 + F3 + F4 + F5 + F6)) * 1.0f / (1.0f + dt * 6.0f);
+      break;
+  }
+#undef F_
+  return phi;
+}
+
+static int or_pde_halo(int op) { return op == 2 ? 2 : 1; }
+
+int or_pde_sweeps(const or_level *L, const int32_t *blocks, int n, const uint16_t *levels,
+                  int numIter, const or_pde_params *P, double *residual, int nthreads) {
+  const int op = P->op;
+  if (!(op == 1 || op == 2 || op == 3 || op == 4 || op == 14 || op == 15 || op == 17)) return -1;
+  if (L->storage == OR_F64 && op != 4) return -1;
+  const int nMaxIter0 = numIter;
+  int nMaxIter = numIter;
+  if (nMaxIter % 2 == 1) nMaxIter++; /* msbg4.cpp:150-152 */
+  const int b = L->bsx, hw = or_pde_halo(op), hsx = b + 2 * hw, nh = hsx * hsx * hsx;
+  const int64_t nvb = (int64_t)b * b * b, nb = (int64_t)L->nbx * L->nby * L->nbz;
+  const int esz = L->storage == OR_F32 ? 4 : (L->storage == OR_F64 ? 8 : 2);
+  const int hard = P->densThr > 1e-12f || P->densThrTol1 > 1e-12f; /* msbg4.cpp:192-193 */
+  int64_t npages = 0;
+  for (int64_t i = 0; i < nb; i++)
+    if (L->page[i] >= 0 && L->page[i] + 1 > npages) npages = L->page[i] + 1;
+  /* swept blocks: listed, level 0, value blocks */
+  uint8_t *swept = (uint8_t *)calloc((size_t)nb, 1);
+  for (int i = 0; i < n; i++) {
+    const int bid = blocks[i];
+    if (L->page[bid] < 0) continue;
+    if (levels && levels[bid] != 0) continue;
+    swept[bid] = 1;
+  }
+  /* phi0: the field at entry (hard constraints), as floats per page */
+  float *phi0 = NULL;
+  if (hard) {
+    phi0 = (float *)malloc(sizeof(float) * (size_t)(npages * nvb + 1));
+    for (int64_t p = 0; p < npages; p++)
+      for (int64_t v = 0; v < nvb; v++)
+        phi0[p * nvb + v] = raw_to_float(L->storage, page_raw(L, p, (int)v));
+  }
+  /* Jacobi: a temporary level with the same page numbering; value blocks
+   * that are not swept are NULL in it (msbg4.cpp:285-297) */
+  or_level T = *L;
+  int32_t *tpage = NULL;
+  void *tpages = NULL;
+  if (P->mode == OR_MODE_JACOBI) {
+    tpage = (int32_t *)malloc(sizeof(int32_t) * (size_t)nb);
+    for (int64_t i = 0; i < nb; i++)
+      tpage[i] = L->page[i] >= 0 ? (swept[i] ? L->page[i] : OR_NULL) : L->page[i];
+    tpages = calloc((size_t)(npages * nvb + 1), (size_t)esz);
+    T.page = tpage;
+    T.pages = tpages;
+  }
+  const int npass = P->mode == OR_MODE_COLOR8 ? 8 : (P->mode == OR_MODE_REDBLACK ? 2 : 1);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  const or_level *src = L, *dst = L;
+  for (int it = 0; it < nMaxIter; it++) {
+    if (P->mode == OR_MODE_JACOBI) {
+      src = (it & 1) ? &T : L;
+      dst = (it & 1) ? L : &T;
+    }
+    const int copyOnly = P->mode == OR_MODE_JACOBI && it == nMaxIter - 1 && (nMaxIter0 % 2) == 1;
+    double rsum = 0.0;
+    int64_t ncnt = 0;
+    for (int pass = 0; pass < npass; pass++) {
+#ifdef _OPENMP
+#pragma omp parallel reduction(+ : rsum, ncnt)
+#endif
+      {
+        float *hf = (float *)malloc(sizeof(float) * (size_t)(nh + 16));
+        double *hd = (double *)malloc(sizeof(double) * (size_t)(nh + 16));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 4)
+#endif
+        for (int i = 0; i < n; i++) {
+          const int bid = blocks[i];
+          if (!swept[bid]) continue;
+          const int bx = bid % L->nbx, by = (bid / L->nbx) % L->nby, bz = bid / (L->nbx * L->nby);
+          if (P->mode == OR_MODE_COLOR8 && or_color8(bx, by, bz) != pass) continue;
+          if (P->mode == OR_MODE_REDBLACK && ((bx + by + bz) & 1) == pass) continue;
+          const int32_t pg = L->page[bid];
+          if (copyOnly) { /* msbg4.cpp:481-487 */
+            memcpy((char *)dst->pages + (size_t)pg * nvb * esz,
+                   (const char *)src->pages + (size_t)pg * nvb * esz, (size_t)(nvb * esz));
+            ncnt += nvb;
+            continue;
+          }
+          if (L->storage == OR_F64) {
+            or_halo_fill_f64(src, bid, 1, OR_OPT_BC_NEUMANN, hd);
+            for (int vz = 0, vid = 0; vz < b; vz++)
+              for (int vy = 0; vy < b; vy++)
+                for (int vx = 0; vx < b; vx++, vid++) {
+                  const int hid = (vx + 1) + (vy + 1) * hsx + (vz + 1) * hsx * hsx;
+                  double ph = or_mcf_voxel_f64(hd, hsx, hid, (double)P->dt,
+                                               hard ? 0 : P->doConstrZeroOne);
+                  if (hard) {
+                    const double p0 = phi0[(size_t)pg * nvb + vid];
+                    const double lo = vmaxd(0.0, p0 - (double)P->densThrTol1),
+                                 hi = vmind(1.0, p0 + (double)P->densThrTol2);
+                    ph = vmaxd(vmind(ph, hi), lo);
+                  }
+                  const double d = ph - hd[hid];
+                  rsum += d * d;
+                  ((double *)dst->pages)[(size_t)pg * nvb + vid] = ph;
+                }
+          } else {
+            or_halo_fill(src, NULL, bid, hw, OR_OPT_BC_NEUMANN, hf);
+            for (int vz = 0, vid = 0; vz < b; vz++)
+              for (int vy = 0; vy < b; vy++)
+                for (int vx = 0; vx < b; vx++, vid++) {
+                  const int hid = (vx + hw) + (vy + hw) * hsx + (vz + hw) * hsx * hsx;
+                  float ph;
+                  if (op == 4) {
+                    ph = or_mcf_voxel(hf, hsx, hid, P->dt, hard ? 0 : P->doConstrZeroOne);
+                  } else {
+                    ph = or_pde_voxel(op, hf, hsx, hid, P->dt, P->taubin, P->alpha, it);
+                    if (!hard && P->doConstrZeroOne) ph = vminf(vmaxf(ph, 0.0f), 1.0f);
+                  }
+                  if (hard) { /* msbg4.cpp:813-818 */
+                    const float p0 = phi0[(size_t)pg * nvb + vid];
+                    const float lo = vmaxf(0.0f, p0 - P->densThrTol1),
+                                hi = vminf(1.0f, p0 + P->densThrTol2);
+                    ph = vmaxf(vminf(ph, hi), lo);
+                  }
+                  store_float(dst, pg, vid, ph);
+                  if (residual) {
+                    const double nw = raw_to_float(L->storage, page_raw(dst, pg, vid));
+                    const double d = nw - (double)hf[hid];
+                    rsum += d * d;
+                  }
+                }
+          }
+          ncnt += nvb;
+        }
+        free(hf);
+        free(hd);
+      }
+    }
+    if (residual) residual[it] = ncnt ? sqrt(rsum / (double)ncnt) : 0.0;
+  }
+  free(swept);
+  free(phi0);
+  free(tpage);
+  free(tpages);
+  return nMaxIter;
+}
+
+/* ------------------------------------------------------------------------ */
 /* A14: ghost downsample, msbg.cpp:298-343 (weights), 2902-2974             */
 /* ------------------------------------------------------------------------ */
 static void ghost_weights(float w[64]) {

@@ -134,6 +134,44 @@ float or_mcf_voxel(const float *h, int hsx, int hid, float dt,
 double or_mcf_voxel_f64(const double *h, int hsx, int hid, double dt,
                         int doConstrZeroOne);
 
+/* ---- the other stencils and driver modes of applyChannelPdeFast
+ * (msbg4.cpp:60-957). op is laplTyp with the mode flags removed:
+ *   1  laplacian flow  D = dt*L*|grad|, Taubin on odd/even sweeps  (:682-712)
+ *   2  bi-laplacian    D = clamp(-dt*H*|grad|, +-0.1), halo 2      (:637-680)
+ *   3  diffusion       phi += dt*(mean6 - phi)                      (:713-735)
+ *   4  mean curvature  (the hot path above)
+ *   14 Perona-Malik    phi += dt*sum G(g_i) g_i                      (:736-790)
+ *   15 Perona-Malik, Jacobi form                                      (:736-790)
+ *   17 diffusion, Jacobi form                                         (:791-805)
+ * Ops 1, 3 and 17 read neighbours at fixed offsets 18 and 18*18 in the
+ * reference, the halo strides of 16^3 blocks; the restatement reads the true
+ * neighbours, which is the same thing at bsx 16 only.
+ * mode: colour (8 passes by parity bits, in place), red-black (2 passes;
+ * pass p updates blocks with (bx+by+bz)&1 != p, in place) or Jacobi (through
+ * a temporary channel whose value blocks that are not swept are NULL, as
+ * the reference's reset temporary is; an odd numIter makes the last sweep a
+ * copy). Hard constraints (densThr or densThrTol1 > 1e-12) clamp to
+ * [max(0, phi0 - tol1), min(1, phi0 + tol2)] around the field at entry,
+ * else doConstrZeroOne clamps to [0, 1]. FP32 arithmetic; FP64 channels
+ * support op 4 only. Returns the sweep count, or -1 for an unsupported
+ * combination. */
+enum { OR_MODE_COLOR8 = 0, OR_MODE_REDBLACK = 1, OR_MODE_JACOBI = 2 };
+typedef struct {
+  int op, mode;
+  float dt;
+  int doConstrZeroOne;
+  float densThr, densThrTol1, densThrTol2;
+  float taubin;  /* op 1: odd sweeps add D, even ones subtract taubin*D */
+  float alpha;   /* ops 14/15: edge scale; < 0 selects the exponential */
+} or_pde_params;
+int or_pde_sweeps(const or_level *L, const int32_t *blocks, int n, const uint16_t *levels,
+                  int numIter, const or_pde_params *P, double *residual, int nthreads);
+/* one voxel of op (1, 2, 3, 14, 15, 17) from a halo buffer of width hw */
+float or_pde_voxel(int op, const float *h, int hsx, int hid, float dt, float taubin,
+                   float alpha, int iter);
+/* FMA_FAST_EXP (fastmath.h:583-628), including its repeated PX2 term */
+float or_fast_exp(float x);
+
 /* ---- A14: 4x4x4 ghost downsample of one fine halo block into a coarse
  * block of bsxHi/2 edge (msbg.cpp:298-343, 2902-2974). */
 void or_ghost_downsample(const float *halo, int bsxHi, float *lo);
