@@ -266,14 +266,34 @@ int msbg_prepare_ghost_blocks(msbg_grid *g, int chan);
 
 /* MultiresSparseGrid::applyChannelPdeFast<T>(chPhi, chTmp, chTmp2,
  * blockList, laplTyp, doConstrZeroOne, densThr, densThrTol1, densThrTol2,
- * numIter, gridSpacing, timeStep, ...) (msbg.h:2150-2172,
- * msbg4.cpp:60-957) for laplTyp = -(PDE_MEAN_CURVATURE|OPT_8_COLOR_SCHEME):
- * numIter rounded up to even sweeps of 8 colour passes, in place, level-0
- * blocks of blockList only, Neumann halo. blockList == NULL sweeps every
- * level-0 value block. Thresholds must be 0 (no hard/soft constraints) and
- * gridSpacing 1 (msbg4.cpp:326-330). residual (optional, host
+ * numIter, gridSpacing, timeStep, ..., lambda, ..., taubinStrength, ...,
+ * anisDiffAlpha, ...) (msbg.h:2150-2172, msbg4.cpp:60-957).
+ * Mode from laplTyp: < 0 with OPT_8_COLOR_SCHEME = 8 colour passes in
+ * place; < 0 without it = red-black, 2 passes in place; >= 0 = Jacobi
+ * through the temporary channel chTmp (removed afterwards, as the reference
+ * resets it). Operators (|laplTyp| without the flags): 4 mean curvature,
+ * 1 laplacian flow (Taubin with taubinStrength), 3 diffusion, 14/15
+ * Perona-Malik (anisDiffAlpha != 0; < 0 selects the exponential), 17 Jacobi
+ * diffusion; 1, 3 and 17 need 16^3 blocks (they read fixed 16^3-halo
+ * offsets in the reference). numIter is rounded up to even; only level-0
+ * blocks of blockList are swept (all level-0 value blocks when NULL);
+ * Neumann halo. densThr or densThrTol1 > 1e-12 turn on hard constraints
+ * around the field at entry, snapshot into chTmp2 (removed afterwards);
+ * else doConstrZeroOne clamps to [0,1]. lambda is forced to 0 by the
+ * reference (no soft constraints). gridSpacing must be 1. The 8-colour
+ * MCF without constraints follows msbg_set_arith (fast or exact); every
+ * other combination runs the exact arithmetic. residual (optional, host
  * [sweeps]) receives the FP64 RMS of each sweep's update over the swept
- * voxels. Returns MSBG_EUNSUPP for other laplTyp values. */
+ * voxels. Unsupported values return MSBG_EUNSUPP (laplTyp 2, 10, 11,
+ * numIter < 0, timeStep < 0, FP64 channels with other operators). */
+int msbg_apply_channel_pde(msbg_grid *g, int chPhi, int chTmp, int chTmp2,
+                           const int32_t *blockList, int64_t nList, int laplTyp,
+                           int doConstrZeroOne, float densThr,
+                           float densThrTol1, float densThrTol2, int numIter,
+                           float gridSpacing, float timeStep, float lambda,
+                           float taubinStrength, float anisDiffAlpha,
+                           double *residual);
+/* msbg_apply_channel_pde with lambda, taubinStrength and anisDiffAlpha 0. */
 int msbg_apply_channel_pde_fast(msbg_grid *g, int chPhi, int chTmp,
                                 int chTmp2, const int32_t *blockList,
                                 int64_t nList, int laplTyp,

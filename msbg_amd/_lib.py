@@ -33,7 +33,9 @@ BLOCK_NULL, BLOCK_EMPTY, BLOCK_FULL, BLOCK_OUT = -1, -2, -3, -4
 KIND_NULL, KIND_EMPTY, KIND_FULL, KIND_VALUE = 0, 1, 2, 3
 ARITH_EXACT, ARITH_FAST = 0, 1
 
-ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ESTATE", -5: "EUNSUPP"}
+EINVAL, ENOMEM, EDEVICE, ESTATE, EUNSUPP = -1, -2, -3, -4, -5
+ERRORS = {EINVAL: "EINVAL", ENOMEM: "ENOMEM", EDEVICE: "EDEVICE", ESTATE: "ESTATE",
+          EUNSUPP: "EUNSUPP"}
 
 
 class MsbgError(RuntimeError):
@@ -80,6 +82,8 @@ SIGNATURES = {
     "msbg_prepare_ghost_blocks": (_I, [_P, _I]),
     "msbg_apply_channel_pde_fast": (_I, [_P, _I, _I, _I, _P, _I64, _I, _I, _F, _F, _F, _I, _F,
                                          _F, _P]),
+    "msbg_apply_channel_pde": (_I, [_P, _I, _I, _I, _P, _I64, _I, _I, _F, _F, _F, _I, _F, _F,
+                                    _F, _F, _F, _P]),
     "msbg_last_sweep_timing": (_I, [_P, _P, _P, _P]),
     "msbg_scene_spheres": (_I, [_U64, _I, _F, _F, _F, _F, _I64, _F, _F, _P]),
     "msbg_copy_bandwidth": (_I, [_I64, _I, _P]),

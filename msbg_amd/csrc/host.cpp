@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "storage.h"
 
 namespace msbg {
 
@@ -427,8 +428,14 @@ static int define_level(msbg_grid *g, Channel &ch, int level, const uint8_t *kin
   ChanLevel &cl = ch.lv[level];
   cl.defined = true;
   cl.bsx = level_bsx(g, level);
-  cl.empty_raw = empty_raw;
-  cl.full_raw = full_raw;
+  // the const values of half-width float channels are what their pages can hold
+  auto held = [&](double v) -> double {
+    if (ch.storage == MSBG_STORAGE_F16) return f16_bits_to_float(f16_bits_from_float((float)v));
+    if (ch.storage == MSBG_STORAGE_BF16) return bf16_bits_to_float(bf16_bits_from_float((float)v));
+    return v;
+  };
+  cl.empty_raw = held(empty_raw);
+  cl.full_raw = held(full_raw);
   cl.h_page.assign((size_t)g->nblocks, MSBG_BLOCK_NULL);
   int64_t np = 0;
   for (int64_t b = 0; b < g->nblocks; b++) {
@@ -744,25 +751,16 @@ int msbg_prepare_ghost_blocks(msbg_grid *g, int chan) {
   return 0;
 }
 
-int msbg_apply_channel_pde_fast(msbg_grid *g, int chPhi, int chTmp, int chTmp2,
-                                const int32_t *blockList, int64_t nList, int laplTyp,
-                                int doConstrZeroOne, float densThr, float densThrTol1,
-                                float densThrTol2, int numIter, float gridSpacing,
-                                float timeStep, double *residual) {
-  (void)chTmp;
-  (void)chTmp2;
-  MSBG_CHECK(g, MSBG_EINVAL, "applyChannelPdeFast: grid is NULL");
-  const int wanted = -(int)(MSBG_PDE_MEAN_CURVATURE | MSBG_OPT_8_COLOR_SCHEME);
-  MSBG_CHECK(laplTyp == wanted, MSBG_EUNSUPP,
-             "applyChannelPdeFast: only laplTyp=-(PDE_MEAN_CURVATURE+OPT_8_COLOR_SCHEME) is "
-             "implemented (got %d)", laplTyp);
-  MSBG_CHECK(densThr == 0 && densThrTol1 == 0 && densThrTol2 == 0, MSBG_EUNSUPP,
-             "applyChannelPdeFast: hard/soft constraints (densThr) are not implemented");
-  MSBG_CHECK(gridSpacing == 1.0f, MSBG_EINVAL, "gridSpacing must be 1 (msbg4.cpp:326-330)");
-  MSBG_CHECK(numIter >= 0, MSBG_EUNSUPP,
-             "numIter < 0 (epsComprRange) asserts in the reference (msbg4.cpp:240)");
-  MSBG_CHECK(timeStep >= 0 && std::isfinite(timeStep), MSBG_EUNSUPP,
-             "timeStep < 0 (2nd-order stepping) asserts in the reference (msbg4.cpp:180)");
+}  // extern "C"
+
+namespace {
+
+enum { MODE_COLOR8 = 0, MODE_REDBLACK = 1, MODE_JACOBI = 2 };
+
+// The 8-colour in-place MCF sweep without constraints: the hot path, with
+// cached colour lists and the fast or exact kernel (msbg4.cpp:60-957).
+int mcf_colour_inplace(msbg_grid *g, int chPhi, const int32_t *blockList, int64_t nList,
+                       int doConstrZeroOne, int numIter, float timeStep, double *residual) {
   Channel *ch;
   ChanLevel *cl;
   MSBG_TRY(get_level(g, chPhi, 0, &ch, &cl));
@@ -863,6 +861,221 @@ int msbg_apply_channel_pde_fast(msbg_grid *g, int chPhi, int chTmp, int chTmp2,
     }
   }
   return 0;
+}
+
+// Every other operator / mode / constraint combination of applyChannelPdeFast
+// (msbg4.cpp:60-957), exact FP32 arithmetic (FP64 for FP64 channels):
+//  - colour: 8 passes by parity bits, in place;
+//  - red-black: 2 passes, pass p updates blocks with (bx+by+bz)&1 != p, in
+//    place (msbg4.cpp:420-425);
+//  - Jacobi: through the temporary channel chTmp, which keeps chPhi's page
+//    numbering; value blocks that are not swept are NULL in it, as in the
+//    reference's reset temporary (msbg4.cpp:285-297); an odd numIter turns
+//    the last sweep into a copy (msbg4.cpp:481-487). chTmp is removed after.
+//  - hard constraints around the field at entry, snapshot in chTmp2
+//    (msbg4.cpp:262-267, 813-818), removed after.
+int pde_general(msbg_grid *g, int chPhi, int chTmp, int chTmp2, const int32_t *blockList,
+                int64_t nList, int op, int mode, int doConstrZeroOne, bool hard, float tol1,
+                float tol2, int numIter, float dt, float taubin, float alpha, double *residual) {
+  Channel *ch;
+  ChanLevel *cl;
+  MSBG_TRY(get_level(g, chPhi, 0, &ch, &cl));
+  const int storage = ch->storage;
+  const int npass = mode == MODE_COLOR8 ? 8 : (mode == MODE_REDBLACK ? 2 : 1);
+  std::vector<std::vector<int2>> per((size_t)npass);
+  std::vector<uint8_t> swept((size_t)g->nblocks, 0);
+  auto add = [&](int64_t bid) -> int {
+    MSBG_CHECK(bid >= 0 && bid < g->nblocks, MSBG_EINVAL, "block id %lld out of range",
+               (long long)bid);
+    const int32_t pg = cl->h_page[bid];
+    MSBG_CHECK(pg >= 0, MSBG_EINVAL,
+               "block %lld in blockList is not a value block (msbg4.cpp:444)", (long long)bid);
+    if (g->h_level[bid] != 0) return 0;  // msbg4.cpp:467-470
+    const int bx = (int)(bid % g->nbx), by = (int)((bid / g->nbx) % g->nby),
+              bz = (int)(bid / ((int64_t)g->nbx * g->nby));
+    int pass = 0;
+    if (mode == MODE_COLOR8) pass = (bx & 1) + ((by & 1) << 1) + ((bz & 1) << 2);
+    if (mode == MODE_REDBLACK) pass = ((bx + by + bz) & 1) ? 0 : 1;
+    per[(size_t)pass].push_back(make_int2(0, pg));
+    swept[(size_t)bid] = 1;
+    return 0;
+  };
+  if (blockList) {
+    for (int64_t i = 0; i < nList; i++) MSBG_TRY(add(blockList[i]));
+  } else {
+    for (int64_t b = 0; b < g->nblocks; b++)
+      if (cl->h_page[b] >= 0) MSBG_TRY(add(b));
+  }
+  std::vector<int2> flat;
+  std::vector<int64_t> off((size_t)npass + 1, 0);
+  for (int p = 0; p < npass; p++) {
+    off[(size_t)p] = (int64_t)flat.size();
+    flat.insert(flat.end(), per[(size_t)p].begin(), per[(size_t)p].end());
+  }
+  off[(size_t)npass] = (int64_t)flat.size();
+  const int64_t ntot = off[(size_t)npass];
+  int nMaxIter = numIter;
+  if (nMaxIter % 2 == 1) nMaxIter++;  // msbg4.cpp:150-152
+
+  MSBG_TRY(ensure_nbr(g, *cl, 0));
+  MSBG_TRY(sweep_prepare(g, *ch));
+  Channel *tmp = nullptr, *p0 = nullptr;
+  if (mode == MODE_JACOBI) {
+    MSBG_CHECK(chTmp > 0 && chTmp != chPhi && (!hard || chTmp != chTmp2), MSBG_EINVAL,
+               "applyChannelPdeFast: the Jacobi mode needs a temporary channel chTmp");
+    std::vector<uint8_t> kind((size_t)g->nblocks);
+    for (int64_t b = 0; b < g->nblocks; b++) {
+      const int32_t c = cl->h_page[b];
+      kind[(size_t)b] = c >= 0 ? MSBG_KIND_VALUE
+                               : (c == MSBG_BLOCK_EMPTY ? MSBG_KIND_EMPTY
+                                                        : (c == MSBG_BLOCK_FULL ? MSBG_KIND_FULL
+                                                                                : MSBG_KIND_NULL));
+    }
+    g->channels.erase(chTmp);  // the reference resets the temporary (msbg4.cpp:285-289)
+    Channel &t = g->channels[chTmp];
+    t.id = chTmp;
+    t.storage = storage;
+    MSBG_TRY(define_level(g, t, 0, kind.data(), cl->empty_raw, cl->full_raw, 0.0));
+    for (int64_t b = 0; b < g->nblocks; b++)
+      if (cl->h_page[b] >= 0 && !swept[(size_t)b]) t.lv[0].h_page[(size_t)b] = MSBG_BLOCK_NULL;
+    MSBG_TRY(upload_pagemap(g, t.lv[0]));
+    MSBG_TRY(ensure_nbr(g, t.lv[0], 0));
+    MSBG_TRY(sweep_prepare(g, t));
+    tmp = &t;
+  }
+  if (hard) {
+    MSBG_CHECK(chTmp2 > 0 && chTmp2 != chPhi, MSBG_EINVAL,
+               "applyChannelPdeFast: hard constraints need the snapshot channel chTmp2");
+    MSBG_TRY(msbg_channel_copy(g, chPhi, chTmp2, storage, 0));
+    p0 = find_channel(g, chTmp2);
+    MSBG_CHECK(p0 && p0->lv[0].h_page == cl->h_page, MSBG_EINVAL,
+               "applyChannelPdeFast: snapshot page numbering differs");
+  }
+  auto desc = [&](Channel &src, Channel &dst) {
+    SweepGrid d = sweep_desc(src, dt, doConstrZeroOne);
+    d.dst_pages = dst.lv[0].d_pages.p;
+    d.dst_side = dst.lv[0].d_side.p;
+    d.phi0 = p0 ? p0->lv[0].d_pages.p : nullptr;
+    d.tol1 = tol1;
+    d.tol2 = tol2;
+    d.taubin = op == 1 ? taubin : 0.0f;
+    d.alpha = alpha;
+    return d;
+  };
+  SweepGrid hd[2] = {tmp ? desc(*ch, *tmp) : desc(*ch, *ch), tmp ? desc(*tmp, *ch) : desc(*ch, *ch)};
+  MSBG_TRY(g->d_desc.ensure(2));
+  MSBG_HIP(hipMemcpyAsync(g->d_desc.p, hd, sizeof hd, hipMemcpyHostToDevice, g->stream));
+  DevBuf<int2> items;
+  if (ntot) {
+    MSBG_TRY(items.alloc((size_t)ntot));
+    MSBG_HIP(hipMemcpyAsync(items.p, flat.data(), sizeof(int2) * ntot, hipMemcpyHostToDevice,
+                            g->stream));
+  }
+  if (residual && nMaxIter > 0) {
+    MSBG_TRY(g->d_partial.ensure((size_t)std::max<int64_t>(ntot, 1)));
+    MSBG_TRY(g->d_resid.ensure((size_t)nMaxIter));
+  }
+  MSBG_HIP(hipStreamSynchronize(g->stream));
+  int64_t npasses = 0;
+  MSBG_HIP(hipEventRecord(g->ev[0], g->stream));
+  for (int it = 0; it < nMaxIter && ntot > 0; it++) {
+    const SweepGrid *dd = g->d_desc.p + (tmp ? (it & 1) : 0);
+    if (tmp && it == nMaxIter - 1 && (numIter % 2) == 1) {
+      MSBG_TRY(launch_copy_items(g->stream, g->bsx0, storage, dd, items.p, ntot));
+      if (residual) MSBG_HIP(hipMemsetAsync(g->d_resid.p + it, 0, sizeof(double), g->stream));
+      continue;
+    }
+    for (int p = 0; p < npass; p++) {
+      const int64_t n = off[(size_t)p + 1] - off[(size_t)p];
+      if (!n) continue;
+      MSBG_TRY(launch_pde_items(g->stream, g->bsx0, storage, op, hard, dd, items.p + off[(size_t)p],
+                                n, residual ? g->d_partial.p + off[(size_t)p] : nullptr, it));
+      npasses++;
+    }
+    if (residual) MSBG_TRY(launch_reduce_residual(g, g->d_partial.p, ntot, g->d_resid.p + it));
+  }
+  MSBG_HIP(hipEventRecord(g->ev[1], g->stream));
+  MSBG_HIP(hipEventSynchronize(g->ev[1]));
+  float ms = 0;
+  MSBG_HIP(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]));
+  g->last_total_ms = ms;
+  g->last_npasses = npasses;
+  g->last_avg_pass_ms = npasses ? ms / (double)npasses : 0.0;
+  if (residual && nMaxIter > 0) {
+    if (ntot) {
+      MSBG_HIP(hipMemcpy(residual, g->d_resid.p, sizeof(double) * nMaxIter,
+                         hipMemcpyDeviceToHost));
+      const int64_t nvox = ntot * (int64_t)cl->bsx * cl->bsx * cl->bsx;
+      for (int it = 0; it < nMaxIter; it++) residual[it] = std::sqrt(residual[it] / (double)nvox);
+    } else {
+      for (int it = 0; it < nMaxIter; it++) residual[it] = 0.0;
+    }
+  }
+  // the reference resets the temporaries (msbg4.cpp:949-953)
+  if (tmp) g->channels.erase(chTmp);
+  if (p0) g->channels.erase(chTmp2);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int msbg_apply_channel_pde(msbg_grid *g, int chPhi, int chTmp, int chTmp2,
+                           const int32_t *blockList, int64_t nList, int laplTyp,
+                           int doConstrZeroOne, float densThr, float densThrTol1,
+                           float densThrTol2, int numIter, float gridSpacing, float timeStep,
+                           float lambda, float taubinStrength, float anisDiffAlpha,
+                           double *residual) {
+  MSBG_CHECK(g, MSBG_EINVAL, "applyChannelPdeFast: grid is NULL");
+  (void)lambda;  // forced to 0 by the reference (msbg4.cpp:164): no soft constraints
+  // mode from the sign and the colour flag (msbg4.cpp:102-114)
+  int op = laplTyp, mode = MODE_JACOBI;
+  if (op < 0) {
+    op = -op;
+    if (op & MSBG_OPT_8_COLOR_SCHEME) {
+      op &= ~MSBG_OPT_8_COLOR_SCHEME;
+      mode = MODE_COLOR8;
+    } else {
+      mode = MODE_REDBLACK;
+    }
+  }
+  op &= ~1024;  // the reference's halo test flag (msbg4.cpp:119-123) changes nothing here
+  MSBG_CHECK(op == 1 || op == 3 || op == 4 || op == 14 || op == 15 || op == 17, MSBG_EUNSUPP,
+             "applyChannelPdeFast: laplTyp %d is not implemented (1, 3, 4, 14, 15, 17 are)", op);
+  MSBG_CHECK(!(op == 1 || op == 3 || op == 17) || g->bsx0 == 16, MSBG_EUNSUPP,
+             "applyChannelPdeFast: laplTyp %d reads neighbours at the fixed offsets of 16^3 "
+             "blocks (msbg4.cpp:688-693); block size is %d", op, g->bsx0);
+  MSBG_CHECK(!(op == 14 || op == 15) || anisDiffAlpha != 0.0f, MSBG_EINVAL,
+             "applyChannelPdeFast: Perona-Malik needs anisDiffAlpha != 0");
+  MSBG_CHECK(gridSpacing == 1.0f, MSBG_EINVAL, "gridSpacing must be 1 (msbg4.cpp:326-330)");
+  MSBG_CHECK(numIter >= 0, MSBG_EUNSUPP,
+             "numIter < 0 (epsComprRange) asserts in the reference (msbg4.cpp:240)");
+  MSBG_CHECK(timeStep >= 0 && std::isfinite(timeStep), MSBG_EUNSUPP,
+             "timeStep < 0 (2nd-order stepping) asserts in the reference (msbg4.cpp:180)");
+  const bool hard = densThr > 1e-12f || densThrTol1 > 1e-12f;  // msbg4.cpp:192-193
+  Channel *ch;
+  ChanLevel *cl;
+  MSBG_TRY(get_level(g, chPhi, 0, &ch, &cl));
+  MSBG_CHECK(ch->storage != MSBG_STORAGE_F64 || op == 4, MSBG_EUNSUPP,
+             "applyChannelPdeFast: FP64 channels run laplTyp 4 only");
+  if (op == 4 && mode == MODE_COLOR8 && !hard)
+    return mcf_colour_inplace(g, chPhi, blockList, nList, doConstrZeroOne, numIter, timeStep,
+                              residual);
+  return pde_general(g, chPhi, chTmp, chTmp2, blockList, nList, op, mode, doConstrZeroOne, hard,
+                     densThrTol1, densThrTol2, numIter, timeStep,
+                     std::fabs(taubinStrength) > 1e-20f ? taubinStrength : 0.0f, anisDiffAlpha,
+                     residual);
+}
+
+int msbg_apply_channel_pde_fast(msbg_grid *g, int chPhi, int chTmp, int chTmp2,
+                                const int32_t *blockList, int64_t nList, int laplTyp,
+                                int doConstrZeroOne, float densThr, float densThrTol1,
+                                float densThrTol2, int numIter, float gridSpacing,
+                                float timeStep, double *residual) {
+  return msbg_apply_channel_pde(g, chPhi, chTmp, chTmp2, blockList, nList, laplTyp,
+                                doConstrZeroOne, densThr, densThrTol1, densThrTol2, numIter,
+                                gridSpacing, timeStep, 0.0f, 0.0f, 0.0f, residual);
 }
 
 int msbg_last_sweep_timing(msbg_grid *g, double *total_ms, double *avg_pass_ms,

@@ -78,13 +78,21 @@ struct DevBuf {
 // Everything the sweep kernel needs to know about one grid's channel at
 // level 0 (device-side descriptor; one per grid of a batched launch).
 struct SweepGrid {
-  uint8_t *pages;
+  uint8_t *pages;  // pages and side-cars the halo and the old values come from
   uint8_t *side;
   const uint8_t *consts;
   const int32_t *nbr;
   int64_t npages;
   float dt, lo, hi;
   int pad;
+  // where the new values go: the same pages in place, another channel with
+  // the same page numbering in the Jacobi mode
+  uint8_t *dst_pages;
+  uint8_t *dst_side;
+  const uint8_t *phi0;  // hard constraints: the field at entry, same numbering
+  float tol1, tol2;     // ... clamp to [max(0, phi0-tol1), min(1, phi0+tol2)]
+  float taubin;         // laplacian flow: even sweeps subtract taubin * D
+  float alpha;          // Perona-Malik edge scale
 };
 
 // One resolution level of one channel.
@@ -181,6 +189,14 @@ SweepGrid sweep_desc(Channel &ch, float dt, int doConstr01);
 int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, int tile,
                        const SweepGrid *d_grids, const int2 *d_items, int64_t n,
                        double *d_partial);
+// the other applyChannelPdeFast operators (1, 3, 14, 15, 17) and MCF with
+// hard constraints, exact arithmetic, on the same item lists
+int launch_pde_items(hipStream_t s, int bsx, int storage, int op, bool hard,
+                     const SweepGrid *d_grids, const int2 *d_items, int64_t n, double *d_partial,
+                     int iter);
+// copy the listed pages and side-cars from pages to dst_pages of *d_grid
+int launch_copy_items(hipStream_t s, int bsx, int storage, const SweepGrid *d_grid,
+                      const int2 *d_items, int64_t n);
 int default_tile(int bsx);
 constexpr int kNumTiles = 2;  // voxel-column variants: 0 = 2x2, 1 = 4x2
 int ensure_nbr(msbg_grid *g, ChanLevel &cl, int level);

@@ -375,15 +375,17 @@ class MultiresSparseGrid:
 
     def applyChannelPdeFast(self, chPhi, chTmp, chTmp2, blockList, laplTyp, doConstrZeroOne,
                             densThr, densThrTol1, densThrTol2, numIter, gridSpacing=1.0,
-                            timeStep=0.025, residual=False):
-        """msbg4.cpp:60-957. Returns the per-sweep FP64 residuals if asked."""
+                            timeStep=0.025, residual=False, lambda_=0.0, taubinStrength=0.0,
+                            anisDiffAlpha=0.0):
+        """msbg4.cpp:60-957 (msbg_apply_channel_pde). Returns the per-sweep FP64
+        residuals if asked."""
         bl = None if blockList is None else np.ascontiguousarray(blockList, dtype=np.int32)
         nsweeps = numIter + (numIter % 2)
         res = np.zeros(max(nsweeps, 1), dtype=np.float64) if residual else None
-        check(lib().msbg_apply_channel_pde_fast(
+        check(lib().msbg_apply_channel_pde(
             self._h, chPhi, chTmp, chTmp2, _ptr(bl), 0 if bl is None else bl.size, laplTyp,
             doConstrZeroOne, densThr, densThrTol1, densThrTol2, numIter, gridSpacing, timeStep,
-            _ptr(res)))
+            lambda_, taubinStrength, anisDiffAlpha, _ptr(res)))
         return res[:nsweeps] if residual else None
 
     def set_tile(self, tile):

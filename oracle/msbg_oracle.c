@@ -120,7 +120,10 @@ static inline float raw_to_float(int st, double raw) {
   switch (st) {
     case OR_U16: return or_u16_to_float((uint16_t)raw);
     case OR_F64: return (float)raw;
-    default: return (float)raw; /* f32/f16/bf16 raw values are floats */
+    /* a const value of a half-width float channel is what its pages hold */
+    case OR_F16: return or_f16_to_float(or_f16_from_float((float)raw));
+    case OR_BF16: return or_bf16_to_float(or_bf16_from_float((float)raw));
+    default: return (float)raw;
   }
 }
 
