@@ -272,7 +272,8 @@ int msbg_prepare_ghost_blocks(msbg_grid *g, int chan);
  * place; < 0 without it = red-black, 2 passes in place; >= 0 = Jacobi
  * through the temporary channel chTmp (removed afterwards, as the reference
  * resets it). Operators (|laplTyp| without the flags): 4 mean curvature,
- * 1 laplacian flow (Taubin with taubinStrength), 3 diffusion, 14/15
+ * 1 laplacian flow (Taubin with taubinStrength), 2 bi-laplacian (halo 2),
+ * 3 diffusion, 14/15
  * Perona-Malik (anisDiffAlpha != 0; < 0 selects the exponential), 17 Jacobi
  * diffusion; 1, 3 and 17 need 16^3 blocks (they read fixed 16^3-halo
  * offsets in the reference). numIter is rounded up to even; only level-0
@@ -284,7 +285,7 @@ int msbg_prepare_ghost_blocks(msbg_grid *g, int chan);
  * MCF without constraints follows msbg_set_arith (fast or exact); every
  * other combination runs the exact arithmetic. residual (optional, host
  * [sweeps]) receives the FP64 RMS of each sweep's update over the swept
- * voxels. Unsupported values return MSBG_EUNSUPP (laplTyp 2, 10, 11,
+ * voxels. Unsupported values return MSBG_EUNSUPP (laplTyp 10, 11,
  * numIter < 0, timeStep < 0, FP64 channels with other operators). */
 int msbg_apply_channel_pde(msbg_grid *g, int chPhi, int chTmp, int chTmp2,
                            const int32_t *blockList, int64_t nList, int laplTyp,

@@ -975,6 +975,11 @@ int pde_general(msbg_grid *g, int chPhi, int chTmp, int chTmp2, const int32_t *b
     MSBG_TRY(g->d_partial.ensure((size_t)std::max<int64_t>(ntot, 1)));
     MSBG_TRY(g->d_resid.ensure((size_t)nMaxIter));
   }
+  constexpr int64_t kBilapChunk = 4096;
+  DevBuf<uint8_t> scratch;
+  if (op == 2 && ntot)
+    MSBG_TRY(scratch.alloc((size_t)std::min<int64_t>(ntot, kBilapChunk) * cl->bsx * cl->bsx *
+                           cl->bsx * storage_bytes(storage)));
   MSBG_HIP(hipStreamSynchronize(g->stream));
   int64_t npasses = 0;
   MSBG_HIP(hipEventRecord(g->ev[0], g->stream));
@@ -988,8 +993,19 @@ int pde_general(msbg_grid *g, int chPhi, int chTmp, int chTmp2, const int32_t *b
     for (int p = 0; p < npass; p++) {
       const int64_t n = off[(size_t)p + 1] - off[(size_t)p];
       if (!n) continue;
-      MSBG_TRY(launch_pde_items(g->stream, g->bsx0, storage, op, hard, dd, items.p + off[(size_t)p],
-                                n, residual ? g->d_partial.p + off[(size_t)p] : nullptr, it));
+      if (op == 2) {  // through scratch pages, in chunks
+        for (int64_t c0 = 0; c0 < n; c0 += kBilapChunk) {
+          const int64_t nc = std::min<int64_t>(kBilapChunk, n - c0);
+          MSBG_TRY(launch_bilap_items(g->stream, g->bsx0, storage, hard, dd,
+                                      items.p + off[(size_t)p] + c0, nc,
+                                      residual ? g->d_partial.p + off[(size_t)p] + c0 : nullptr,
+                                      scratch.p));
+        }
+      } else {
+        MSBG_TRY(launch_pde_items(g->stream, g->bsx0, storage, op, hard, dd,
+                                  items.p + off[(size_t)p], n,
+                                  residual ? g->d_partial.p + off[(size_t)p] : nullptr, it));
+      }
       npasses++;
     }
     if (residual) MSBG_TRY(launch_reduce_residual(g, g->d_partial.p, ntot, g->d_resid.p + it));
@@ -1041,8 +1057,9 @@ int msbg_apply_channel_pde(msbg_grid *g, int chPhi, int chTmp, int chTmp2,
     }
   }
   op &= ~1024;  // the reference's halo test flag (msbg4.cpp:119-123) changes nothing here
-  MSBG_CHECK(op == 1 || op == 3 || op == 4 || op == 14 || op == 15 || op == 17, MSBG_EUNSUPP,
-             "applyChannelPdeFast: laplTyp %d is not implemented (1, 3, 4, 14, 15, 17 are)", op);
+  MSBG_CHECK(op == 1 || op == 2 || op == 3 || op == 4 || op == 14 || op == 15 || op == 17,
+             MSBG_EUNSUPP,
+             "applyChannelPdeFast: laplTyp %d is not implemented (1, 2, 3, 4, 14, 15, 17 are)", op);
   MSBG_CHECK(!(op == 1 || op == 3 || op == 17) || g->bsx0 == 16, MSBG_EUNSUPP,
              "applyChannelPdeFast: laplTyp %d reads neighbours at the fixed offsets of 16^3 "
              "blocks (msbg4.cpp:688-693); block size is %d", op, g->bsx0);

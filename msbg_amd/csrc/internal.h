@@ -194,6 +194,10 @@ int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, int tile
 int launch_pde_items(hipStream_t s, int bsx, int storage, int op, bool hard,
                      const SweepGrid *d_grids, const int2 *d_items, int64_t n, double *d_partial,
                      int iter);
+// bi-laplacian (halo 2) of the listed blocks into `scratch` (n pages), then
+// committed to dst_pages and their side-cars
+int launch_bilap_items(hipStream_t s, int bsx, int storage, bool hard, const SweepGrid *d_grid,
+                       const int2 *d_items, int64_t n, double *d_partial, uint8_t *scratch);
 // copy the listed pages and side-cars from pages to dst_pages of *d_grid
 int launch_copy_items(hipStream_t s, int bsx, int storage, const SweepGrid *d_grid,
                       const int2 *d_items, int64_t n);

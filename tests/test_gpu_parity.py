@@ -239,9 +239,9 @@ def test_sweep_skips_non_level0_and_rejects_bad_lists():
     with pytest.raises(M.MsbgError):
         g.applyChannelPdeFast(M.CH_UINT16_1, 0, 0, np.array([empty_bid]), M.MCF_8COLOR, 1, 0, 0, 0,
                               2, 1.0, 0.1)
-    with pytest.raises(M.MsbgError):  # laplTyp 2 (bi-laplacian) is not built
-        g.applyChannelPdeFast(M.CH_UINT16_1, 0, 0, None, -(2 | M.OPT_8_COLOR_SCHEME), 1, 0, 0, 0, 2,
-                              1.0, 0.1)
+    with pytest.raises(M.MsbgError):  # laplTyp 10 (redistancing) is not built
+        g.applyChannelPdeFast(M.CH_UINT16_1, 0, 0, None, -(10 | M.OPT_8_COLOR_SCHEME), 1, 0, 0, 0,
+                              2, 1.0, 0.1)
     g.applyChannelPdeFast(M.CH_UINT16_1, 0, 0, np.zeros(0, np.int32), M.MCF_8COLOR, 1, 0, 0, 0, 2,
                           1.0, 0.1)
     assert np.array_equal(g.download(M.CH_UINT16_1), before)

@@ -80,7 +80,7 @@ def _run_both(g, pm, ost, empty, full, levels, S, bsx, laplTyp, numIter, blocks=
 
 CASES = [
     # (op, alpha, taubin)
-    (1, 0.0, 0.0), (1, 0.0, 0.63), (3, 0.0, 0.0), (14, 0.2, 0.0), (14, -0.15, 0.0),
+    (1, 0.0, 0.0), (1, 0.0, 0.63), (2, 0.0, 0.0), (3, 0.0, 0.0), (14, 0.2, 0.0), (14, -0.15, 0.0),
     (15, 0.3, 0.0), (17, 0.0, 0.0),
 ]
 
@@ -89,6 +89,9 @@ CASES = [
 @pytest.mark.parametrize("op,alpha,taubin", CASES)
 def test_operators_bitexact(op, alpha, taubin, mode):
     note("op", op, alpha, taubin, mode)
+    if op == 2 and mode == "redblack":
+        pytest.skip("blocks of one red-black pass share edges, which the bi-laplacian's stencil "
+                    "reads: the result depends on block order, in the reference as here")
     S, bsx = 64, 16
     lt = {"color8": -(op | FLAG8), "redblack": -op, "jacobi": op}[mode]
     for storage in (M.F32, M.U16):
@@ -130,14 +133,16 @@ def test_jacobi_block_list_and_odd_count():
 
 
 @pytest.mark.parametrize("bsx", [8, 32])
-def test_perona_malik_other_block_sizes(bsx):
-    note("pm bsx", bsx)
+def test_other_block_sizes(bsx):
+    note("pm / bilap bsx", bsx)
     S = 4 * bsx
-    g, pm, ost, empty, full, levels = _grid(S, bsx, M.F32, seed=13)
-    with g:
-        got, want, gr, wr = _run_both(g, pm, ost, empty, full, levels, S, bsx, -(14 | FLAG8), 2,
-                                      alpha=0.25)
-        assert np.array_equal(got, want)
+    for lt, kw in ((-(14 | FLAG8), dict(alpha=0.25)), (-(2 | FLAG8), {}),
+                   (2, dict(densThr=0.5, tol1=0.01, tol2=0.02))):
+        g, pm, ost, empty, full, levels = _grid(S, bsx, M.F32, seed=13)
+        with g:
+            got, want, gr, wr = _run_both(g, pm, ost, empty, full, levels, S, bsx, lt, 2, **kw)
+            assert np.array_equal(got, want), lt
+            np.testing.assert_allclose(gr, wr, rtol=1e-12, atol=1e-15)
 
 
 def test_mcf_red_black_runs():
@@ -158,8 +163,8 @@ def test_unsupported_and_bad_arguments():
             with pytest.raises(M.MsbgError) as e:
                 g.applyChannelPdeFast(CH, CH_T, 0, None, lt, 1, 0, 0, 0, 2, 1.0, 0.1)
             assert e.value.code == M._lib.EUNSUPP
-        with pytest.raises(M.MsbgError) as e:  # bi-laplacian (halo 2) is not built
-            g.applyChannelPdeFast(CH, CH_T, 0, None, -(2 | FLAG8), 1, 0, 0, 0, 2, 1.0, 0.1)
+        with pytest.raises(M.MsbgError) as e:  # redistancing (10) is not built
+            g.applyChannelPdeFast(CH, CH_T, 0, None, -(10 | FLAG8), 1, 0, 0, 0, 2, 1.0, 0.1)
         assert e.value.code == M._lib.EUNSUPP
         with pytest.raises(M.MsbgError) as e:  # Jacobi without a temporary
             g.applyChannelPdeFast(CH, 0, 0, None, 4, 1, 0, 0, 0, 2, 1.0, 0.1)
