@@ -28,33 +28,60 @@ namespace sweep {
 
 
 // ---- vector raw loads/stores -------------------------------------------
+// Page, side-car and const pointers come out of the grid descriptor as
+// generic pointers; casting them to the global address space at each access
+// makes the compiler emit global (not flat) loads and stores, which wait on
+// the vector-memory counter only.
+#ifndef MSBG_SWEEP_GLOBAL
+#define MSBG_SWEEP_GLOBAL 1
+#endif
+#if MSBG_SWEEP_GLOBAL
+#define MSBG_GLOBAL __attribute__((address_space(1)))
+#else
+#define MSBG_GLOBAL
+#endif
+template <typename T>
+__device__ inline T ldg(const T *p) {
+  return *(const MSBG_GLOBAL T *)p;
+}
+template <typename T>
+__device__ inline void stg(T *p, T v) {
+  *(MSBG_GLOBAL T *)p = v;
+}
+
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+
 __device__ inline void load4(const uint16_t *p, uint16_t v[4]) {
-  uint2 u = *(const uint2 *)p;
-  v[0] = (uint16_t)(u.x & 0xffffu);
-  v[1] = (uint16_t)(u.x >> 16);
-  v[2] = (uint16_t)(u.y & 0xffffu);
-  v[3] = (uint16_t)(u.y >> 16);
+  const uint64_t u = *(const MSBG_GLOBAL uint64_t *)p;
+  v[0] = (uint16_t)(u & 0xffffu);
+  v[1] = (uint16_t)((u >> 16) & 0xffffu);
+  v[2] = (uint16_t)((u >> 32) & 0xffffu);
+  v[3] = (uint16_t)(u >> 48);
 }
 __device__ inline void load4(const float *p, float v[4]) {
-  float4 u = *(const float4 *)p;
+  const f32x4_t u = *(const MSBG_GLOBAL f32x4_t *)p;
   v[0] = u.x;
   v[1] = u.y;
   v[2] = u.z;
   v[3] = u.w;
 }
 __device__ inline void load4(const double *p, double v[4]) {
-  double2 a = ((const double2 *)p)[0], b = ((const double2 *)p)[1];
+  const f64x2_t a = ((const MSBG_GLOBAL f64x2_t *)p)[0], b = ((const MSBG_GLOBAL f64x2_t *)p)[1];
   v[0] = a.x;
   v[1] = a.y;
   v[2] = b.x;
   v[3] = b.y;
 }
 __device__ inline void store2(uint16_t *p, uint16_t a, uint16_t b) {
-  *(uint32_t *)p = (uint32_t)a | ((uint32_t)b << 16);
+  *(MSBG_GLOBAL uint32_t *)p = (uint32_t)a | ((uint32_t)b << 16);
 }
-__device__ inline void store2(float *p, float a, float b) { *(float2 *)p = make_float2(a, b); }
+__device__ inline void store2(float *p, float a, float b) {
+  *(MSBG_GLOBAL f32x2_t *)p = f32x2_t{a, b};
+}
 __device__ inline void store2(double *p, double a, double b) {
-  *(double2 *)p = make_double2(a, b);
+  *(MSBG_GLOBAL f64x2_t *)p = f64x2_t{a, b};
 }
 
 // ---- the MCF update --------------------------------------------------------
@@ -332,7 +359,7 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT)) void k_sweep(
   const int tid = threadIdx.x;
   const int page = item.y;
   // 27 codes; small blocks have fewer than 27 threads
-  for (int k = tid; k < 27; k += NT) codes[k] = nbrTab[(int64_t)page * 27 + k];
+  for (int k = tid; k < 27; k += NT) codes[k] = ldg(nbrTab + (int64_t)page * 27 + k);
   __syncthreads();
 
   R *pages = (R *)pagesRaw;
@@ -462,7 +489,7 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT)) void k_sweep(
       for (int u = 0; u < 4; u++) st.in[k * 4 + u] = dec(r[u]);
     }
 #pragma unroll
-    for (int q = 0; q < RPT; q++) st.ring[q] = dec(*rs[q].at(zz, BSX));
+    for (int q = 0; q < RPT; q++) st.ring[q] = dec(ldg(rs[q].at(zz, BSX)));
   };
 
   auto put_plane = [&](C *b, const Staged &st) {
@@ -522,7 +549,7 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT)) void k_sweep(
       for (int b = 0; b < PY; b++)
 #pragma unroll
         for (int a = 0; a < PX; a++) {
-          const C f0 = (C)S::dec(p0[(y0 + b) * BSX + x0 + a]);
+          const C f0 = (C)S::dec(ldg(p0 + (y0 + b) * BSX + x0 + a));
           C lo = f0 - (C)G.tol1, hi = f0 + (C)G.tol2;
           lo = (C)0 > lo ? (C)0 : lo;  // Vec8f max(0, phi0 - tol1)
           hi = (C)1 < hi ? (C)1 : hi;  // Vec8f min(1, phi0 + tol2)
@@ -543,16 +570,16 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT)) void k_sweep(
     // keep the side-car faces and edges equal to the page
     if (wFace0) {
 #pragma unroll
-      for (int b = 0; b < PY; b++) dFaces[z * BSX + y0 + b] = e[b * PX];
+      for (int b = 0; b < PY; b++) stg(dFaces + z * BSX + y0 + b, e[b * PX]);
     }
     if (wFace1) {
 #pragma unroll
-      for (int b = 0; b < PY; b++) dFaces[PS + z * BSX + y0 + b] = e[b * PX + PX - 1];
+      for (int b = 0; b < PY; b++) stg(dFaces + PS + z * BSX + y0 + b, e[b * PX + PX - 1]);
     }
-    if (wFace0 && wY0) dEdges[0 * BSX + z] = e[0];
-    if (wFace1 && wY0) dEdges[1 * BSX + z] = e[PX - 1];
-    if (wFace0 && wY1) dEdges[2 * BSX + z] = e[(PY - 1) * PX];
-    if (wFace1 && wY1) dEdges[3 * BSX + z] = e[PY * PX - 1];
+    if (wFace0 && wY0) stg(dEdges + 0 * BSX + z, e[0]);
+    if (wFace1 && wY0) stg(dEdges + 1 * BSX + z, e[PX - 1]);
+    if (wFace0 && wY1) stg(dEdges + 2 * BSX + z, e[(PY - 1) * PX]);
+    if (wFace1 && wY1) stg(dEdges + 3 * BSX + z, e[PY * PX - 1]);
     if (partial) {
 #pragma unroll
       for (int b = 0; b < PY; b++)
