@@ -317,6 +317,48 @@ int msbg_last_sweep_timing(msbg_grid *g, double *total_ms,
 int msbg_copy_bandwidth(int64_t bytes, int reps, double *gbps);
 
 /* ======================================================================== */
+/* field consumers: interpolation and the ray-march renderer                */
+/* ======================================================================== */
+
+enum {
+  MSBG_IP_LINEAR = 0,   /* cell-centred trilinear (IP_LINEAR) */
+  MSBG_IP_BSPLINE3 = 1  /* cubic B-spline with gradient (IP_BSPLINE_CUBIC) */
+};
+
+/* Interpolate (chan, level) at n positions xyz[3*i..] in that level's voxel
+ * coordinates (cell centres at i + 0.5), options 0:
+ *  MSBG_IP_LINEAR: SparseGrid<uint16_t>::interpolateUint16ToFloatFast /
+ *    SparseGrid<float>::interpolateFloatFast (sbg.h:4920-5031, 4623-4800),
+ *    with interpolateGen0<IP_LINEAR>'s clamped border branch
+ *    (sbg.h:3170-3245) where the stencil leaves the domain;
+ *  MSBG_IP_BSPLINE3: interpolateWithDerivs0(IP_BSPLINE_CUBIC)
+ *    (sbg.h:3903-4135): value and gradient (grad[3*i+k], may be NULL).
+ * Values are in the interpolant's units: raw UNORM16 integers (0..65535)
+ * for UNORM16 channels, as the reference returns them before
+ * RENDER_DENS_TO_FLOAT_UI16; decoded values for the other storages. NULL
+ * blocks read as the empty value (the reference's fast paths would
+ * dereference NULL). xyz, val, grad: host arrays. The reference leaves FMA
+ * contraction to its compiler (-ffast-math); this library fixes it as its
+ * kernel source states, so results agree with the reference to rounding. */
+int msbg_interpolate(msbg_grid *g, int chan, int level, int kind,
+                     const float *xyz, int64_t n, float *val, float *grad);
+
+/* RDR::RaymarchRenderer(width, height, level-0 grid of chan) with
+ * setCamera(camPos, lookAt, zoom), setSunLight(sunPos),
+ * setSurfaceColor(color), then render() (render.h:58-125,
+ * render.cpp:19-100; the demo's render_scene, msbg_demo.cpp:129-185):
+ * iso-0.5 ray march in steps of one voxel (1/max(sx,sy,sz) of the unit
+ * cube), linear hit refinement, cubic B-spline gradient normal, Lambert
+ * shading. rgb[y*width + x] = BMP_MKRGB(r,g,b) = 0x02000000 | b<<16 | g<<8
+ * | r (bitmap.h:545-546), row 0 at the top. UNORM16 densities are scaled
+ * by 1/65535; the other storages are used as they are. rgb is a host
+ * array; *ms (may be NULL) receives the device time of the render kernel. */
+int msbg_render(msbg_grid *g, int chan, int width, int height,
+                const float camPos[3], const float lookAt[3], float zoom,
+                const float sunPos[3], const float color[3], uint32_t *rgb,
+                double *ms);
+
+/* ======================================================================== */
 /* batched sweeps of independent grids                                      */
 /* ======================================================================== */
 

@@ -21,6 +21,7 @@ OPT_SINGLE_LEVEL = 1 << 14
 OPT_BC_COARSE_LEVEL = 1 << 26
 OPT_8_COLOR_SCHEME = 1 << 29
 PDE_MEAN_CURVATURE = 4
+IP_LINEAR, IP_BSPLINE3 = 0, 1  # msbg_interpolate kinds
 CH_NULL, CH_FLOAT_1, CH_UINT16_1 = 0, 1, 30
 
 BLK_DOM_BORDER = 1 << 1
@@ -96,6 +97,8 @@ SIGNATURES = {
     "msbg_batch_run": (_I, [_P, _I, _I, _P]),
     "msbg_set_tile": (_I, [_P, _I]),
     "msbg_channel_diff": (_I, [_P, _I, _I, _I, _P, _P]),
+    "msbg_interpolate": (_I, [_P, _I, _I, _I, _P, _I64, _P, _P]),
+    "msbg_render": (_I, [_P, _I, _I, _I, _P, _P, _F, _P, _P, _P, _P]),
     "msbg_scene_objects": (_I, [_U64, _I, _F, _F, _F, _F, _I64, _F, _F, _F, _F, _P]),
     "msbg_points_generate": (_I, [_P, _U64, _I, _F, _F, _F, _F, _I64, _F, _F, _F, _F, _I, _P]),
     "msbg_unet_create": (_I, [_U64, C.POINTER(_P)]),

@@ -95,6 +95,24 @@ struct SweepGrid {
   float alpha;          // Perona-Malik edge scale
 };
 
+// One channel level as the interpolators and the renderer read it.
+struct RenderVol {
+  const int32_t *page;  // block map codes
+  const void *pages;    // page pool
+  int sx, sy, sz;       // voxel extents of the level
+  int lb;               // log2 of the level's block edge
+  int nbx, nby;
+  float empty, full;    // interpolant inputs of the const blocks
+  float dmin[3], dmax[3];  // float domain bounds (sbg.cpp:262-316)
+};
+
+// RaymarchRenderer's camera and material, widened to double (render.h:66-77)
+struct RenderCam {
+  double pos[3], look[3];
+  double focal;
+  double sun[3], color[3];
+};
+
 // One resolution level of one channel.
 struct ChanLevel {
   bool defined = false;
@@ -194,6 +212,11 @@ int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, int tile
 int launch_pde_items(hipStream_t s, int bsx, int storage, int op, bool hard,
                      const SweepGrid *d_grids, const int2 *d_items, int64_t n, double *d_partial,
                      int iter);
+// batched interpolation of a channel level (kind MSBG_IP_*) and the renderer
+int launch_interp_points(hipStream_t s, const RenderVol &V, int storage, int kind,
+                         const float *d_pts, int64_t n, float *d_val, float *d_grad);
+int launch_render(hipStream_t s, const RenderVol &V, int storage, const RenderCam &C, int W,
+                  int H, uint32_t *d_rgb);
 // bi-laplacian (halo 2) of the listed blocks into `scratch` (n pages), then
 // committed to dst_pages and their side-cars
 int launch_bilap_items(hipStream_t s, int bsx, int storage, bool hard, const SweepGrid *d_grid,

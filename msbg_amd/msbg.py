@@ -391,6 +391,32 @@ class MultiresSparseGrid:
     def set_tile(self, tile):
         check(lib().msbg_set_tile(self._h, tile))
 
+    # -- field consumers ---------------------------------------------------------
+    def interpolate(self, chan, xyz, kind=0, level=0):
+        """Trilinear (kind 0, interpolateUint16ToFloatFast / interpolateFloatFast,
+        sbg.h:4623-5031) or cubic B-spline with gradient (kind 1,
+        interpolateWithDerivs0, sbg.h:3903-4135) at voxel coordinates of the
+        level. Returns values (raw units for UNORM16), and gradients for kind 1."""
+        p = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        val = np.empty(p.shape[0], dtype=np.float32)
+        grad = np.empty((p.shape[0], 3), dtype=np.float32) if kind == 1 else None
+        check(lib().msbg_interpolate(self._h, chan, level, kind, _ptr(p), p.shape[0], _ptr(val),
+                                     _ptr(grad)))
+        return val if kind == 0 else (val, grad)
+
+    def render(self, chan, width=3840, height=2160, camPos=(.5, .8, .7), lookAt=(.5, .5, .5),
+               zoom=1.0, sunPos=(.5, 5, 5), color=(0.8, 0.6, 0.4), timing=False):
+        """RDR::RaymarchRenderer::render of the level-0 grid (render.cpp:19-100)
+        with the demo's camera (msbg_demo.cpp:122-166): [height, width] uint32
+        BMP_MKRGB words; with timing=True also the kernel's device ms."""
+        f3 = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+        out = np.empty((height, width), dtype=np.uint32)
+        ms = C.c_double()
+        check(lib().msbg_render(self._h, chan, width, height, _ptr(f3(camPos)), _ptr(f3(lookAt)),
+                                float(zoom), _ptr(f3(sunPos)), _ptr(f3(color)), _ptr(out),
+                                C.byref(ms)))
+        return (out, ms.value) if timing else out
+
     def last_sweep_timing(self):
         t, a, n = C.c_double(), C.c_double(), C.c_int64()
         check(lib().msbg_last_sweep_timing(self._h, C.byref(t), C.byref(a), C.byref(n)))

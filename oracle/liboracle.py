@@ -64,6 +64,11 @@ def load(kind="strict"):
             "or_mcf_voxel_f64": (D, [P, I, I, D, I]),
             "or_ghost_downsample": (None, [P, I, P]),
             "or_prepare_ghost_blocks": (None, [P, I, P, P, P]),
+            "or_interp_linear": (F, [P, P]),
+            "or_bspline3_weights": (None, [F, P, P]),
+            "or_interp_points": (None, [P, I, P, I64, P, P]),
+            "or_camera_basis": (None, [P, P, P, P, P]),
+            "or_render": (None, [P, P, I, I, I, I, P, I]),
             "or_version": (C.c_uint64, []),
         }
         for n, (r, a) in sig.items():
@@ -249,3 +254,61 @@ def prepare_ghost_blocks(levels, fc_lists, flags):
     off[1:] = np.cumsum([len(x) for x in fc_lists])
     fl = np.ascontiguousarray(flags, dtype=np.uint16)
     load().or_prepare_ghost_blocks(arr, len(levels), _p(fc), _p(off), _p(fl))
+
+
+# ---- field consumers: interpolation and the ray-march renderer ----
+
+def interp_linear(level, pts, kind="strict"):
+    """or_interp_points kind 0: trilinear interpolation at voxel coordinates
+    (raw UNORM16 units for UNORM16 levels)."""
+    pts = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+    val = np.zeros(pts.shape[0], np.float32)
+    s = level.struct()
+    load(kind).or_interp_points(C.byref(s), 0, _p(pts), pts.shape[0], _p(val), None)
+    return val
+
+
+def interp_bspline3(level, pts, kind="strict"):
+    """or_interp_points kind 1: cubic B-spline value and gradient."""
+    pts = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+    val = np.zeros(pts.shape[0], np.float32)
+    grad = np.zeros((pts.shape[0], 3), np.float32)
+    s = level.struct()
+    load(kind).or_interp_points(C.byref(s), 1, _p(pts), pts.shape[0], _p(val), _p(grad))
+    return val, grad
+
+
+def bspline3_weights(t):
+    w, d = np.zeros(4, np.float32), np.zeros(4, np.float32)
+    load().or_bspline3_weights(float(t), _p(w), _p(d))
+    return w, d
+
+
+class OrCamera(C.Structure):
+    _fields_ = [("pos", C.c_double * 3), ("look", C.c_double * 3), ("focal", C.c_double),
+                ("sun", C.c_double * 3), ("color", C.c_double * 3)]
+
+
+def camera(pos, look, focal, sun, color):
+    """Camera as the demo builds it: every input is a float widened to double."""
+    f = lambda v: [float(np.float32(x)) for x in v]
+    return OrCamera((C.c_double * 3)(*f(pos)), (C.c_double * 3)(*f(look)),
+                    float(np.float32(focal)), (C.c_double * 3)(*f(sun)),
+                    (C.c_double * 3)(*f(color)))
+
+
+def camera_basis(pos, look):
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (pos, look)]
+    out = [np.zeros(3, np.float64) for _ in range(3)]
+    load().or_camera_basis(_p(a[0]), _p(a[1]), *[_p(o) for o in out])
+    return out
+
+
+def render(level, cam, W, H, y0=0, y1=None, nthreads=0, kind="strict"):
+    """or_render: rows y0..y1-1 of the W x H image, uint32 BMP_MKRGB words
+    (nthreads 0: OpenMP's default, OMP_NUM_THREADS)."""
+    y1 = H if y1 is None else y1
+    out = np.zeros((y1 - y0, W), np.uint32)
+    s = level.struct()
+    load(kind).or_render(C.byref(s), C.byref(cam), W, H, y0, y1, _p(out), nthreads)
+    return out

@@ -1113,3 +1113,278 @@ void or_prepare_ghost_blocks(or_level *lvl, int nLevels, const int32_t *fc,
   }
   (void)storage_size;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Field consumers: interpolation of a level and the demo's ray-march        */
+/* renderer (msbg.h:469-484, sbg.h:3109-3245, 3903-4135, 4920-5031,          */
+/* render.h:58-125, render.cpp:19-100).                                      */
+/*                                                                          */
+/* The reference compiles these with -O3 -march=native -ffast-math, so the  */
+/* compiler contracts multiply-adds into FMAs where it sees fit. This       */
+/* restatement fixes one choice, stated at each expression: vectorclass's    */
+/* mul_add is an FMA (vectorf128.h:1133-1141 with __FMA__), "a*b + c" and    */
+/* "a += b*c" contract into fma(a, b, c) / fma(b, c, a) with the first       */
+/* product fused, everything else is rounded per operation.                 */
+/* ------------------------------------------------------------------------ */
+
+/* interpolant input at voxel (x,y,z): the raw UNORM16 integer for UNORM16
+ * levels (SparseGrid<uint16_t> interpolates raw values and scales after,
+ * msbg.h:478-484), the decoded value otherwise. NULL blocks read as the
+ * empty value (getValuePtr returns NULL and the caller substitutes
+ * _emptyValue, sbg.h:3211-3222; the fast paths would dereference NULL). */
+static inline float ip_input(const or_level *L, int x, int y, int z) {
+  int lb = ilog2i(L->bsx), m = L->bsx - 1;
+  int32_t c = L->page[((int64_t)(z >> lb) * L->nby + (y >> lb)) * L->nbx + (x >> lb)];
+  double raw;
+  if (c >= 0)
+    raw = page_raw(L, c, ((z & m) * L->bsx + (y & m)) * L->bsx + (x & m));
+  else
+    raw = c == OR_FULL ? L->full_raw : L->empty_raw;
+  return L->storage == OR_U16 ? (float)raw : raw_to_float(L->storage, raw);
+}
+
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* interpolateGen0<float,float,IP_LINEAR> border branch (sbg.h:3170-3245):
+ * indices clamped to the domain; Interpolate1D (sbg.h:2934-2939)
+ * p1*t + p0*(1-t) as fma(p1, t, p0*(1-t)). */
+float or_interp_linear_border(const or_level *L, const float p[3]) {
+  float q[3], t[3];
+  int i0[3], s[3] = {L->sx, L->sy, L->sz};
+  for (int k = 0; k < 3; k++) {
+    q[k] = p[k] - 0.5f;
+    i0[k] = (int)floorf(q[k]);
+    t[k] = q[k] - (float)i0[k];
+  }
+  float bz[2];
+  for (int k = 0; k < 2; k++) {
+    int zz = clampi(i0[2] + k, 0, s[2] - 1);
+    float by[2];
+    for (int j = 0; j < 2; j++) {
+      int yy = clampi(i0[1] + j, 0, s[1] - 1);
+      float bx[2];
+      for (int i = 0; i < 2; i++) bx[i] = ip_input(L, clampi(i0[0] + i, 0, s[0] - 1), yy, zz);
+      by[j] = fmaf(bx[1], t[0], bx[0] * (1.0f - t[0]));
+    }
+    bz[k] = fmaf(by[1], t[1], by[0] * (1.0f - t[1]));
+  }
+  return fmaf(bz[1], t[2], bz[0] * (1.0f - t[2]));
+}
+
+/* interpolateUint16ToFloatFast / interpolateFloatFast (sbg.h:4920-5031,
+ * 4623-4800): cell-centred trilinear interpolation; the lerps are
+ * vlerp_ps = mul_add(b - a, t, a) in x, then y, then z; a stencil that
+ * leaves [0, s-1] falls back to the border branch. Not scaled. */
+float or_interp_linear(const or_level *L, const float p[3]) {
+  float q[3], t[3];
+  int i0[3], s[3] = {L->sx, L->sy, L->sz};
+  for (int k = 0; k < 3; k++) {
+    q[k] = p[k] - 0.5f;
+    float f = floorf(q[k]);
+    i0[k] = (int)f;
+    t[k] = q[k] - f;
+  }
+  for (int k = 0; k < 3; k++)
+    if (i0[k] < 0 || i0[k] >= s[k] - 1) return or_interp_linear_border(L, p);
+  float c[8];
+  for (int n = 0; n < 8; n++)
+    c[n] = ip_input(L, i0[0] + (n & 1), i0[1] + ((n >> 1) & 1), i0[2] + (n >> 2));
+  float x0 = fmaf(c[1] - c[0], t[0], c[0]), x1 = fmaf(c[3] - c[2], t[0], c[2]);
+  float x2 = fmaf(c[5] - c[4], t[0], c[4]), x3 = fmaf(c[7] - c[6], t[0], c[6]);
+  float y0 = fmaf(x1 - x0, t[1], x0), y1 = fmaf(x3 - x2, t[1], x2);
+  return fmaf(y1 - y0, t[2], y0);
+}
+
+/* MtCubicBSplineWeights1D (mtool.h:344-363), products left to right */
+void or_bspline3_weights(float t, float w[4], float d[4]) {
+  w[0] = (1.f / 6.f) * (1.f - t) * (1.f - t) * (1.f - t);
+  w[1] = (2.f / 3.f) - 0.5f * t * t * (2.f - t);
+  w[2] = (2.f / 3.f) - 0.5f * (1.f - t) * (1.f - t) * (1.f + t);
+  w[3] = (1.f / 6.f) * t * t * t;
+  d[0] = -(1.f - t) * (1.f - t) / 2.f;
+  d[1] = 0.5f * t * t - 1.0f * (2.f - t) * t;
+  d[2] = 1.f * (1.f - t) * (t + 1.f) - 0.5f * (1.f - t) * (1.f - t);
+  d[3] = t * t / 2.f;
+}
+
+/* interpolateWithDerivs0 with IP_BSPLINE_CUBIC, options 0
+ * (sbg.h:3903-4135): 4^3 window from floor(p - 0.5) - 1, clamped to the
+ * domain; value += w*f and grad += w'*f as FMAs in z, y, x order. */
+void or_interp_bspline3(const or_level *L, const float p[3], float *val, float grad[3]) {
+  float t[3], wv[3][4], wd[3][4];
+  int i0[3], s[3] = {L->sx, L->sy, L->sz};
+  for (int k = 0; k < 3; k++) {
+    float q = p[k] - 0.5f, f = floorf(q);
+    t[k] = q - f;
+    i0[k] = (int)f - 1;
+    or_bspline3_weights(t[k], wv[k], wd[k]);
+  }
+  float v = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  for (int dz = 0; dz < 4; dz++) {
+    int zz = clampi(i0[2] + dz, 0, s[2] - 1);
+    for (int dy = 0; dy < 4; dy++) {
+      int yy = clampi(i0[1] + dy, 0, s[1] - 1);
+      float wgy = wv[2][dz] * wv[1][dy];
+      float ax = wv[2][dz] * wv[1][dy], ay = wv[2][dz] * wd[1][dy], az = wd[2][dz] * wv[1][dy];
+      for (int dx = 0; dx < 4; dx++) {
+        float f = ip_input(L, clampi(i0[0] + dx, 0, s[0] - 1), yy, zz);
+        float w = wgy * wv[0][dx];
+        v = fmaf(w, f, v);
+        gx = fmaf(ax * wd[0][dx], f, gx);
+        gy = fmaf(ay * wv[0][dx], f, gy);
+        gz = fmaf(az * wv[0][dx], f, gz);
+      }
+    }
+  }
+  *val = v;
+  grad[0] = gx;
+  grad[1] = gy;
+  grad[2] = gz;
+}
+
+/* ---- RaymarchRenderer (render.h, render.cpp), in double ---- */
+static inline double v3dot(const double a[3], const double b[3]) {
+  /* (a.x*b.x + a.y*b.y) + a.z*b.z, both additions contracted */
+  return fma(a[2], b[2], fma(a[0], b[0], a[1] * b[1]));
+}
+static inline void v3normalize(const double v[3], double o[3]) {
+  double len = sqrt(v3dot(v, v));
+  if (len > (double)1e-20f) {
+    o[0] = v[0] / len;
+    o[1] = v[1] / len;
+    o[2] = v[2] / len;
+  } else {
+    o[0] = 0.0;
+    o[1] = 1.0;
+    o[2] = 0.0;
+  }
+}
+static inline void v3cross(const double a[3], const double b[3], double o[3]) {
+  o[0] = fma(a[1], b[2], -(a[2] * b[1]));
+  o[1] = fma(a[2], b[0], -(a[0] * b[2]));
+  o[2] = fma(a[0], b[1], -(a[1] * b[0]));
+}
+
+/* setCamera (render.h:66-77): forward, right, up */
+void or_camera_basis(const double pos[3], const double look[3], double fwd[3], double right[3],
+                     double up[3]) {
+  const double worldUp[3] = {0.0, 1.0, 0.0};
+  double d[3] = {look[0] - pos[0], look[1] - pos[1], look[2] - pos[2]}, r[3];
+  v3normalize(d, fwd);
+  v3cross(fwd, worldUp, r);
+  v3normalize(r, right);
+  v3cross(right, fwd, up);
+}
+
+/* sampleDensity (render.h:82-110) */
+static double rd_sample(const or_level *L, int res, float scale, const double p[3]) {
+  float pos[3];
+  int ip[3], s[3] = {L->sx, L->sy, L->sz};
+  for (int k = 0; k < 3; k++) {
+    pos[k] = (float)res * (float)p[k];
+    ip[k] = (int)floorf(pos[k]);
+    if (ip[k] < 0 || ip[k] > s[k] - 1) return 0.0;
+  }
+  int lb = ilog2i(L->bsx);
+  int32_t c = L->page[((int64_t)(ip[2] >> lb) * L->nby + (ip[1] >> lb)) * L->nbx + (ip[0] >> lb)];
+  if (c == OR_EMPTY || c == OR_NULL) return 0.0; /* isEmptyBlock (sbg.h:1990-2000) */
+  float f = or_interp_linear(L, pos);
+  return (double)(f * scale);
+}
+
+/* sampleDensityGradient (render.h:112-125) */
+static void rd_gradient(const or_level *L, int res, float scale, const float dmin[3],
+                        const float dmax[3], const double p[3], double g[3]) {
+  float pos[3], val, gr[3] = {0.0f, 0.0f, 0.0f};
+  int in = 1;
+  for (int k = 0; k < 3; k++) {
+    pos[k] = (float)res * (float)p[k];
+    if (pos[k] < dmin[k] || pos[k] > dmax[k]) in = 0;
+  }
+  if (in) {
+    or_interp_bspline3(L, pos, &val, gr);
+    for (int k = 0; k < 3; k++) gr[k] *= scale;
+  }
+  for (int k = 0; k < 3; k++) g[k] = -1.0 * (double)gr[k];
+}
+
+/* render one pixel (render.cpp:37-94) */
+static uint32_t rd_pixel(const or_level *L, const or_camera *C, const double fwd[3],
+                         const double right[3], const double up[3], const float dmin[3],
+                         const float dmax[3], int W, int H, int x, int y) {
+  const int res = L->sx > L->sy ? (L->sx > L->sz ? L->sx : L->sz) : (L->sy > L->sz ? L->sy : L->sz);
+  const float scale = L->storage == OR_U16 ? 1.0f / 65535.0f : 1.0f;
+  const double h = 1.0 / (double)res;
+  const double ISO = 0.5;
+  const int maxSteps = (int)(2.0 / h);
+  const double invW = 1.0 / (double)W, invH = 1.0 / (double)H, aspect = (double)W / (double)H;
+  /* ((x + 0.5f)*invW - 0.5f)*2*aspect and (0.5f - (y + 0.5f)*invH)*2, the
+   * pixel centre in float, the rest in double */
+  double ndcX = fma((double)((float)x + 0.5f), invW, -0.5) * 2.0 * aspect;
+  double ndcY = fma(-(double)((float)y + 0.5f), invH, 0.5) * 2.0;
+  double dir0[3], dir[3];
+  for (int k = 0; k < 3; k++) dir0[k] = fma(up[k], ndcY, fma(fwd[k], C->focal, right[k] * ndcX));
+  v3normalize(dir0, dir);
+  double pos[3] = {C->pos[0], C->pos[1], C->pos[2]}, prevPos[3] = {pos[0], pos[1], pos[2]};
+  double prevD = rd_sample(L, res, scale, pos), hitPos[3] = {0, 0, 0};
+  int hit = 0;
+  for (int step = 0; step < maxSteps; ++step) {
+    for (int k = 0; k < 3; k++) pos[k] = fma(dir[k], h, pos[k]);
+    double d = rd_sample(L, res, scale, pos);
+    if (prevD < ISO && d >= ISO) {
+      double t = (ISO - prevD) / (d - prevD);
+      for (int k = 0; k < 3; k++) hitPos[k] = fma(pos[k] - prevPos[k], t, prevPos[k]);
+      hit = 1;
+      break;
+    }
+    prevD = d;
+    for (int k = 0; k < 3; k++) prevPos[k] = pos[k];
+  }
+  uint32_t out = 0x02000000u;
+  if (hit) {
+    double g[3], n[3], l0[3], l[3];
+    rd_gradient(L, res, scale, dmin, dmax, hitPos, g);
+    v3normalize(g, n);
+    for (int k = 0; k < 3; k++) l0[k] = C->sun[k] - hitPos[k];
+    v3normalize(l0, l);
+    double diff = v3dot(n, l);
+    diff = diff < 0.0 ? 0.0 : diff; /* std::max(dot, 0.0) */
+    uint32_t b8[3];
+    for (int k = 0; k < 3; k++) {
+      double c = C->color[k] * diff;
+      c = c < 0.0 ? 0.0 : (1.0 < c ? 1.0 : c); /* std::clamp */
+      b8[k] = (uint32_t)(uint8_t)fma(c, 255.0, 0.5);
+    }
+    out |= (b8[2] << 16) | (b8[1] << 8) | b8[0];
+  }
+  return out;
+}
+
+void or_render(const or_level *L, const or_camera *C, int W, int H, int y0, int y1,
+               uint32_t *rgb, int nthreads) {
+  double fwd[3], right[3], up[3];
+  float dmin[3], dmax[3];
+  or_camera_basis(C->pos, C->look, fwd, right, up);
+  or_domain_bounds(L->sx, L->sy, L->sz, dmin, dmax);
+  if (nthreads <= 0) nthreads = omp_get_max_threads(); /* OMP_NUM_THREADS */
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+  for (int y = y0; y < y1; y++)
+    for (int x = 0; x < W; x++)
+      rgb[(size_t)(y - y0) * W + x] = rd_pixel(L, C, fwd, right, up, dmin, dmax, W, H, x, y);
+}
+
+/* batch form of the interpolators: kind 0 = trilinear (val only),
+ * 1 = cubic B-spline (val and grad[3*i..]) */
+void or_interp_points(const or_level *L, int kind, const float *pts, int64_t n, float *val,
+                      float *grad) {
+  for (int64_t i = 0; i < n; i++) {
+    if (kind == 0) {
+      val[i] = or_interp_linear(L, pts + 3 * i);
+    } else {
+      float g[3];
+      or_interp_bspline3(L, pts + 3 * i, &val[i], g);
+      if (grad)
+        for (int k = 0; k < 3; k++) grad[3 * i + k] = g[k];
+    }
+  }
+}

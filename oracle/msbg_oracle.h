@@ -183,6 +183,38 @@ void or_ghost_downsample(const float *halo, int bsxHi, float *lo);
 void or_prepare_ghost_blocks(or_level *lvl, int nLevels, const int32_t *fc,
                              const int32_t *fc_off, const uint16_t *flags);
 
+/* ---- field consumers of a level (msbg.h:469-484, render.h, render.cpp).
+ * Positions are voxel coordinates of the level (cell centres at i + 0.5).
+ * Results are in the interpolant's units: raw UNORM16 integers for UNORM16
+ * levels, decoded values otherwise (the caller scales by 1/65535). The
+ * contraction of multiply-adds into FMAs, which the reference leaves to
+ * its compiler (-ffast-math), is fixed as stated in the source. */
+/* interpolateUint16ToFloatFast / interpolateFloatFast (sbg.h:4623-4800,
+ * 4920-5031), with the border fallback of interpolateGen0 (sbg.h:3170-3245) */
+float or_interp_linear(const or_level *L, const float p[3]);
+float or_interp_linear_border(const or_level *L, const float p[3]);
+/* MtCubicBSplineWeights1D (mtool.h:344-363) */
+void or_bspline3_weights(float t, float w[4], float d[4]);
+/* interpolateWithDerivs0, IP_BSPLINE_CUBIC, options 0 (sbg.h:3903-4135) */
+void or_interp_bspline3(const or_level *L, const float p[3], float *val, float grad[3]);
+/* RaymarchRenderer (render.h:23-160, render.cpp:19-100): camera, sun and
+ * surface colour as the demo passes them (msbg_demo.cpp:122-166), widened
+ * to double. rgb[(y-y0)*W + x] = BMP_MKRGB(r,g,b) (bitmap.h:545-546) for
+ * rows y0..y1-1 of a W x H image. */
+typedef struct {
+  double pos[3], look[3];
+  double focal;
+  double sun[3], color[3];
+} or_camera;
+void or_camera_basis(const double pos[3], const double look[3], double fwd[3],
+                     double right[3], double up[3]);
+void or_render(const or_level *L, const or_camera *C, int W, int H, int y0, int y1,
+               uint32_t *rgb, int nthreads);
+/* batch form: kind 0 = or_interp_linear (val), 1 = or_interp_bspline3
+ * (val, grad[3*i+k]; grad may be NULL) */
+void or_interp_points(const or_level *L, int kind, const float *pts, int64_t n, float *val,
+                      float *grad);
+
 /* ---- misc ---- */
 uint64_t or_version(void);
 
