@@ -352,11 +352,24 @@ int msbg_interpolate(msbg_grid *g, int chan, int level, int kind,
  * shading. rgb[y*width + x] = BMP_MKRGB(r,g,b) = 0x02000000 | b<<16 | g<<8
  * | r (bitmap.h:545-546), row 0 at the top. UNORM16 densities are scaled
  * by 1/65535; the other storages are used as they are. rgb is a host
- * array; *ms (may be NULL) receives the device time of the render kernel. */
+ * array; *ms (may be NULL) receives the device time of the render kernel.
+ * The kernel advances through EMPTY blocks without sampling every step and
+ * stops rays that have left the domain for good; the pixels are those of
+ * the step-by-step march. */
 int msbg_render(msbg_grid *g, int chan, int width, int height,
                 const float camPos[3], const float lookAt[3], float zoom,
                 const float sunPos[3], const float color[3], uint32_t *rgb,
                 double *ms);
+/* msbg_render through a counting instance of the same kernel: also
+ * returns counts = {march positions advanced, density samples taken,
+ * samples that interpolated (value or FULL blocks), rays that hit}. The
+ * kernel advances through EMPTY blocks without sampling and stops rays that
+ * have left the domain for good; neither changes a pixel. No counterpart in
+ * the reference: it gives the per-pixel work behind the render rate. */
+int msbg_render_counts(msbg_grid *g, int chan, int width, int height,
+                       const float camPos[3], const float lookAt[3], float zoom,
+                       const float sunPos[3], const float color[3],
+                       uint32_t *rgb, int64_t counts[4]);
 
 /* ======================================================================== */
 /* batched sweeps of independent grids                                      */

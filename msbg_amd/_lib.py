@@ -99,6 +99,7 @@ SIGNATURES = {
     "msbg_channel_diff": (_I, [_P, _I, _I, _I, _P, _P]),
     "msbg_interpolate": (_I, [_P, _I, _I, _I, _P, _I64, _P, _P]),
     "msbg_render": (_I, [_P, _I, _I, _I, _P, _P, _F, _P, _P, _P, _P]),
+    "msbg_render_counts": (_I, [_P, _I, _I, _I, _P, _P, _F, _P, _P, _P, _P]),
     "msbg_scene_objects": (_I, [_U64, _I, _F, _F, _F, _F, _I64, _F, _F, _F, _F, _P]),
     "msbg_points_generate": (_I, [_P, _U64, _I, _F, _F, _F, _F, _I64, _F, _F, _F, _F, _I, _P]),
     "msbg_unet_create": (_I, [_U64, C.POINTER(_P)]),

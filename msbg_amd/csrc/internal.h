@@ -216,7 +216,7 @@ int launch_pde_items(hipStream_t s, int bsx, int storage, int op, bool hard,
 int launch_interp_points(hipStream_t s, const RenderVol &V, int storage, int kind,
                          const float *d_pts, int64_t n, float *d_val, float *d_grad);
 int launch_render(hipStream_t s, const RenderVol &V, int storage, const RenderCam &C, int W,
-                  int H, uint32_t *d_rgb);
+                  int H, uint32_t *d_rgb, unsigned long long *d_counts = nullptr);
 // bi-laplacian (halo 2) of the listed blocks into `scratch` (n pages), then
 // committed to dst_pages and their side-cars
 int launch_bilap_items(hipStream_t s, int bsx, int storage, bool hard, const SweepGrid *d_grid,

@@ -12,7 +12,8 @@ using namespace msbg;
 
 namespace {
 
-int level_vol(msbg_grid *g, int chan, int level, RenderVol *V, int *storage) {
+int level_vol(msbg_grid *g, int chan, int level, RenderVol *V, int *storage,
+              int64_t *npages = nullptr) {
   MSBG_CHECK(g && level >= 0 && level < g->nlevels, MSBG_EINVAL, "bad grid/level");
   auto it = g->channels.find(chan);
   MSBG_CHECK(it != g->channels.end() && it->second.lv[level].defined, MSBG_ESTATE,
@@ -41,7 +42,63 @@ int level_vol(msbg_grid *g, int chan, int level, RenderVol *V, int *storage) {
   domain_bounds(v.sx, v.sy, v.sz, v.dmin, v.dmax);
   *V = v;
   *storage = ch.storage;
+  if (npages) *npages = cl.npages;
   return 0;
+}
+
+
+int render_impl(msbg_grid *g, int chan, int width, int height, const float camPos[3],
+                const float lookAt[3], float zoom, const float sunPos[3], const float color[3],
+                uint32_t *rgb, double *ms, int64_t *counts) {
+  MSBG_CHECK(g && camPos && lookAt && sunPos && color && rgb, MSBG_EINVAL,
+             "msbg_render: NULL argument");
+  MSBG_CHECK(width > 0 && height > 0 && (int64_t)width * height <= (1LL << 31), MSBG_EINVAL,
+             "msbg_render: image %d x %d", width, height);
+  RenderVol V;
+  int storage;
+  MSBG_TRY(level_vol(g, chan, 0, &V, &storage));
+  RenderCam C{};
+  for (int k = 0; k < 3; k++) {
+    C.pos[k] = camPos[k];
+    C.look[k] = lookAt[k];
+    C.sun[k] = sunPos[k];
+    C.color[k] = color[k];
+  }
+  C.focal = zoom;
+  const size_t npix = (size_t)width * height;
+  DevBuf<uint32_t> d_rgb;
+  DevBuf<unsigned long long> d_cnt;
+  MSBG_TRY(d_rgb.alloc(npix));
+  if (counts) {
+    MSBG_TRY(d_cnt.alloc(4));
+    MSBG_HIP(hipMemsetAsync(d_cnt.p, 0, 4 * sizeof(unsigned long long), g->stream));
+  }
+  hipEvent_t e0, e1;
+  MSBG_HIP(hipEventCreate(&e0));
+  MSBG_HIP(hipEventCreate(&e1));
+  int rc = 0;
+  if (hipEventRecord(e0, g->stream) != hipSuccess) rc = MSBG_EDEVICE;
+  if (!rc) rc = launch_render(g->stream, V, storage, C, width, height, d_rgb.p, d_cnt.p);
+  if (!rc && hipEventRecord(e1, g->stream) != hipSuccess) rc = MSBG_EDEVICE;
+  if (!rc && hipMemcpyAsync(rgb, d_rgb.p, sizeof(uint32_t) * npix, hipMemcpyDeviceToHost,
+                            g->stream) != hipSuccess)
+    rc = MSBG_EDEVICE;
+  unsigned long long hc[4] = {0, 0, 0, 0};
+  if (!rc && counts &&
+      hipMemcpyAsync(hc, d_cnt.p, sizeof hc, hipMemcpyDeviceToHost, g->stream) != hipSuccess)
+    rc = MSBG_EDEVICE;
+  if (!rc && hipStreamSynchronize(g->stream) != hipSuccess) rc = MSBG_EDEVICE;
+  float t = 0.0f;
+  if (!rc && ms) {
+    if (hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = MSBG_EDEVICE;
+    *ms = t;
+  }
+  if (!rc && counts)
+    for (int k = 0; k < 4; k++) counts[k] = (int64_t)hc[k];
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc == MSBG_EDEVICE) set_error("msbg_render: device error");
+  return rc;
 }
 
 }  // namespace
@@ -78,44 +135,16 @@ int msbg_interpolate(msbg_grid *g, int chan, int level, int kind, const float *x
 int msbg_render(msbg_grid *g, int chan, int width, int height, const float camPos[3],
                 const float lookAt[3], float zoom, const float sunPos[3], const float color[3],
                 uint32_t *rgb, double *ms) {
-  MSBG_CHECK(g && camPos && lookAt && sunPos && color && rgb, MSBG_EINVAL,
-             "msbg_render: NULL argument");
-  MSBG_CHECK(width > 0 && height > 0 && (int64_t)width * height <= (1LL << 31), MSBG_EINVAL,
-             "msbg_render: image %d x %d", width, height);
-  RenderVol V;
-  int storage;
-  MSBG_TRY(level_vol(g, chan, 0, &V, &storage));
-  RenderCam C{};
-  for (int k = 0; k < 3; k++) {
-    C.pos[k] = camPos[k];
-    C.look[k] = lookAt[k];
-    C.sun[k] = sunPos[k];
-    C.color[k] = color[k];
-  }
-  C.focal = zoom;
-  const size_t npix = (size_t)width * height;
-  DevBuf<uint32_t> d_rgb;
-  MSBG_TRY(d_rgb.alloc(npix));
-  hipEvent_t e0, e1;
-  MSBG_HIP(hipEventCreate(&e0));
-  MSBG_HIP(hipEventCreate(&e1));
-  int rc = 0;
-  if (hipEventRecord(e0, g->stream) != hipSuccess) rc = MSBG_EDEVICE;
-  if (!rc) rc = launch_render(g->stream, V, storage, C, width, height, d_rgb.p);
-  if (!rc && hipEventRecord(e1, g->stream) != hipSuccess) rc = MSBG_EDEVICE;
-  if (!rc && hipMemcpyAsync(rgb, d_rgb.p, sizeof(uint32_t) * npix, hipMemcpyDeviceToHost,
-                            g->stream) != hipSuccess)
-    rc = MSBG_EDEVICE;
-  if (!rc && hipStreamSynchronize(g->stream) != hipSuccess) rc = MSBG_EDEVICE;
-  float t = 0.0f;
-  if (!rc && ms) {
-    if (hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = MSBG_EDEVICE;
-    *ms = t;
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (rc == MSBG_EDEVICE) set_error("msbg_render: device error");
-  return rc;
+  return render_impl(g, chan, width, height, camPos, lookAt, zoom, sunPos, color, rgb, ms,
+                     nullptr);
+}
+
+int msbg_render_counts(msbg_grid *g, int chan, int width, int height, const float camPos[3],
+                       const float lookAt[3], float zoom, const float sunPos[3],
+                       const float color[3], uint32_t *rgb, int64_t counts[4]) {
+  MSBG_CHECK(counts, MSBG_EINVAL, "msbg_render_counts: counts is NULL");
+  return render_impl(g, chan, width, height, camPos, lookAt, zoom, sunPos, color, rgb, nullptr,
+                     counts);
 }
 
 }  // extern "C"

@@ -417,6 +417,18 @@ class MultiresSparseGrid:
                                 C.byref(ms)))
         return (out, ms.value) if timing else out
 
+    def render_counts(self, chan, width=3840, height=2160, camPos=(.5, .8, .7),
+                      lookAt=(.5, .5, .5), zoom=1.0, sunPos=(.5, 5, 5), color=(0.8, 0.6, 0.4)):
+        """render() through the counting kernel: (image, {advanced, samples, interp, hits})."""
+        f3 = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+        out = np.empty((height, width), dtype=np.uint32)
+        cnt = np.zeros(4, dtype=np.int64)
+        check(lib().msbg_render_counts(self._h, chan, width, height, _ptr(f3(camPos)),
+                                       _ptr(f3(lookAt)), float(zoom), _ptr(f3(sunPos)),
+                                       _ptr(f3(color)), _ptr(out), _ptr(cnt)))
+        return out, dict(advanced=int(cnt[0]), samples=int(cnt[1]), interp=int(cnt[2]),
+                         hits=int(cnt[3]))
+
     def last_sweep_timing(self):
         t, a, n = C.c_double(), C.c_double(), C.c_int64()
         check(lib().msbg_last_sweep_timing(self._h, C.byref(t), C.byref(a), C.byref(n)))

@@ -97,10 +97,17 @@ def test_interpolate_coarse_level():
                               O.interp_linear(L, pts))
 
 
-@pytest.mark.parametrize("storage", [M.F32, M.U16])
-def test_render_sphere_bitexact(storage):
-    note("render sphere", storage)
-    S, bsx = 64, 16
+OUTSIDE_CAM = dict(camPos=(1.5, 0.2, -0.4), lookAt=(.62, .41, .47), zoom=2.5, sunPos=(4, 6, -2),
+                   color=(0.9, 0.9, 0.9))
+
+
+@pytest.mark.parametrize("storage,bsx", [(M.F32, 16), (M.U16, 16), (M.U16, 8), (M.F32, 32)])
+def test_render_sphere_bitexact(storage, bsx):
+    """Includes a camera outside the domain (rays enter through the faces)
+    and three block sizes (the kernel advances through EMPTY blocks
+    without sampling)."""
+    note("render sphere", storage, bsx)
+    S = 64
     ost = O.F32 if storage == M.F32 else O.U16
     L = sphere_level(S, bsx, storage=ost)
     with M.MultiresSparseGrid.create("rs", S, S, S, bsx, nLevels=2) as g:
@@ -112,13 +119,18 @@ def test_render_sphere_bitexact(storage):
         pm, _ = g.pagemap(CH)
         assert np.array_equal(pm, L.page)
         g.upload(CH, 0, L.pages)
-        for W, H, cam in ((160, 90, DEMO_CAM),
+        for W, H, cam in ((160, 90, DEMO_CAM), (120, 80, OUTSIDE_CAM),
                           (97, 61, dict(camPos=(.2, .3, -.4), lookAt=(.6, .4, .5), zoom=1.7,
                                         sunPos=(-3, 2, -1), color=(1.0, 0.9, 0.2)))):
             img = g.render(CH, W, H, **cam)
             want = O.render(L, ocam(**cam), W, H)
             assert np.array_equal(img, want), int((img != want).sum())
             assert ((img & 0xffffff) != 0).sum() > 200
+            img2, cnt = g.render_counts(CH, W, H, **cam)
+            assert np.array_equal(img2, img)
+            # a hit facing away from the sun shades to black
+            assert cnt["hits"] >= int(((img & 0xffffff) != 0).sum())
+            assert cnt["interp"] <= cnt["samples"] <= cnt["advanced"] + W * H
 
 
 def test_render_demo_scene_bitexact():
