@@ -11,6 +11,8 @@
 #pragma once
 
 #include <cstdint>
+#include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -62,6 +64,46 @@ struct StorageOf<double> {
 static inline int getBlockColor8(int bx, int by, int bz) {
   return (bx & 1) + ((by & 1) << 1) + ((bz & 1) << 2);
 }
+
+class MultiresSparseGrid;
+
+}  // namespace MSBG
+
+namespace SBG {
+
+// One channel level of a MultiresSparseGrid, in the role of the reference's
+// SBG::SparseGrid<T> (sbg.h) as the renderer and interpolators use it. Owned
+// by its MultiresSparseGrid.
+template <typename Data_T>
+class SparseGrid {
+ public:
+  SparseGrid(msbg_grid *g, int chan, int level, int sx, int sy, int sz)
+      : g_(g), chan_(chan), level_(level), sx_(sx), sy_(sy), sz_(sz) {}
+  int sx() const { return sx_; }
+  int sy() const { return sy_; }
+  int sz() const { return sz_; }
+  int64_t sxyzMax() const { return sx_ > sy_ ? (sx_ > sz_ ? sx_ : sz_) : (sy_ > sz_ ? sy_ : sz_); }
+  // interpolateUint16ToFloatFast / interpolateFloatFast (sbg.h:4623-5031) at
+  // n positions xyz[3*i..] (voxel coordinates), batched: one launch
+  void interpolateFast(const float *xyz, int64_t n, float *out) const {
+    MSBG::check(msbg_interpolate(g_, chan_, level_, MSBG_IP_LINEAR, xyz, n, out, nullptr));
+  }
+  // interpolateWithDerivs<0>(IP_BSPLINE_CUBIC, ...) (sbg.h:3903-4135), batched
+  void interpolateBSplineWithDerivs(const float *xyz, int64_t n, float *val, float *grad) const {
+    MSBG::check(msbg_interpolate(g_, chan_, level_, MSBG_IP_BSPLINE3, xyz, n, val, grad));
+  }
+  msbg_grid *grid() const { return g_; }
+  int channel() const { return chan_; }
+  int level() const { return level_; }
+
+ private:
+  msbg_grid *g_;
+  int chan_, level_, sx_, sy_, sz_;
+};
+
+}  // namespace SBG
+
+namespace MSBG {
 
 class MultiresSparseGrid {
  public:
@@ -157,6 +199,13 @@ class MultiresSparseGrid {
   void getBlockData(int chan, int bid, Data_T *out, int level = 0) {
     check(msbg_download_block(g_, chan, level, bid, out));
   }
+  // getUint16Channel / getFloatChannel (msbg.h): the channel's level grid
+  SBG::SparseGrid<uint16_t> *getUint16Channel(int chan, int level) {
+    return channelGrid<uint16_t>(u16_, chan, level);
+  }
+  SBG::SparseGrid<float> *getFloatChannel(int chan, int level) {
+    return channelGrid<float>(f32_, chan, level);
+  }
 
  private:
   explicit MultiresSparseGrid(msbg_grid *g) : g_(g) {
@@ -169,10 +218,73 @@ class MultiresSparseGrid {
     check(msbg_get_block_info(g_, l.data(), f.data()));
     for (size_t i = 0; i < info_.size(); i++) info_[i] = BlockInfo{l[i], f[i]};
   }
+  template <typename T>
+  SBG::SparseGrid<T> *channelGrid(std::map<int64_t, std::unique_ptr<SBG::SparseGrid<T>>> &m,
+                                  int chan, int level) {
+    auto &p = m[(int64_t)chan * 16 + level];
+    if (!p)
+      p.reset(new SBG::SparseGrid<T>(g_, chan, level, dims_[0] >> level, dims_[1] >> level,
+                                     dims_[2] >> level));
+    return p.get();
+  }
   msbg_grid *g_;
   int dims_[8] = {0};
   int64_t nblocks_ = 0;
   std::vector<BlockInfo> info_;
+  std::map<int64_t, std::unique_ptr<SBG::SparseGrid<uint16_t>>> u16_;
+  std::map<int64_t, std::unique_ptr<SBG::SparseGrid<float>>> f32_;
 };
 
 }  // namespace MSBG
+
+namespace RDR {
+
+typedef double RealType;
+
+struct Vec3 {  // render.h:23-40
+  RealType x = 0, y = 0, z = 0;
+  constexpr Vec3() = default;
+  constexpr Vec3(RealType xx, RealType yy, RealType zz) : x(xx), y(yy), z(zz) {}
+};
+
+// RDR::RaymarchRenderer (render.h:55-160, render.cpp:19-100) over the
+// library's renderer. render() returns the image as BMP_MKRGB words, row 0
+// at the top, where the reference returns a BmpBitmap holding the same
+// words. Camera, sun and colour are passed on as float, the precision the
+// demo gives them (msbg_demo.cpp:122-166).
+class RaymarchRenderer {
+ public:
+  RaymarchRenderer(int w, int h, SBG::SparseGrid<uint16_t> *sgDensity)
+      : width_(w), height_(h), sg_(sgDensity) {
+    setCamera({0, 0, -5}, {0, 0, 0}, 1.0f);
+    setSunLight({10, 10, -10});
+    setSurfaceColor({1, 1, 1});
+  }
+  void setCamera(const Vec3 &position, const Vec3 &lookAt, float focalLength) {
+    set3(cam_, position);
+    set3(look_, lookAt);
+    focal_ = focalLength;
+  }
+  void setSunLight(const Vec3 &sunPos) { set3(sun_, sunPos); }
+  void setSurfaceColor(const Vec3 &color) { set3(color_, color); }
+  std::vector<uint32_t> render() const {
+    std::vector<uint32_t> rgb((size_t)width_ * height_);
+    MSBG::check(msbg_render(sg_->grid(), sg_->channel(), width_, height_, cam_, look_, focal_,
+                            sun_, color_, rgb.data(), nullptr));
+    return rgb;
+  }
+  int width() const { return width_; }
+  int height() const { return height_; }
+
+ private:
+  static void set3(float *d, const Vec3 &v) {
+    d[0] = (float)v.x;
+    d[1] = (float)v.y;
+    d[2] = (float)v.z;
+  }
+  int width_, height_;
+  SBG::SparseGrid<uint16_t> *sg_;
+  float cam_[3], look_[3], sun_[3], color_[3], focal_ = 1.0f;
+};
+
+}  // namespace RDR

@@ -1,8 +1,10 @@
 // The demo's hot path (msbg_demo.cpp:339-727) written against the C++
 // facade, the way a reference caller would: create, activate, refinement
-// map, allocate, splat, finalize, applyChannelPdeFast. Writes the final
-// UNORM16 pages of the active blocks to argv[1] (raw, block order of the
-// active list) for comparison with the Python binding.
+// map, allocate, splat, finalize, applyChannelPdeFast, then render_scene's
+// ray-march render (msbg_demo.cpp:129-185). Writes the final UNORM16 pages
+// of the active blocks to argv[1] (raw, block order of the active list)
+// and, given argv[2], a 320x180 frame of BMP_MKRGB words to it, for
+// comparison with the Python binding.
 #include <cstdio>
 #include <vector>
 
@@ -41,6 +43,19 @@ int main(int argc, char **argv) {
       std::fwrite(blk.data(), 2, blk.size(), f);
     }
     std::fclose(f);
+    if (argc > 2) {
+      float camPos[3] = {.5f, .8f, .7f}, camLookAt[3] = {.5, .5, .5}, camLight[3] = {.5, 5, 5};
+      SBG::SparseGrid<uint16_t> *sg = msbg->getUint16Channel(MSBG::CH_UINT16_1, 0);
+      RDR::RaymarchRenderer renderer(320, 180, sg);
+      renderer.setCamera({camPos[0], camPos[1], camPos[2]},
+                         {camLookAt[0], camLookAt[1], camLookAt[2]}, 1.0f);
+      renderer.setSunLight({camLight[0], camLight[1], camLight[2]});
+      renderer.setSurfaceColor({0.8f, 0.6f, 0.4f});
+      std::vector<uint32_t> img = renderer.render();
+      FILE *fi = std::fopen(argv[2], "wb");
+      std::fwrite(img.data(), 4, img.size(), fi);
+      std::fclose(fi);
+    }
     std::printf("facade_demo: %zu active blocks\n", activeBlocks.size());
     MSBG::MultiresSparseGrid::destroy(msbg);
   } catch (const MSBG::Error &e) {

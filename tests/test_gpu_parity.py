@@ -347,14 +347,20 @@ def test_cpp_facade_demo_matches_binding(tmp_path):
     exe = os.path.join(os.path.dirname(__file__), "cpp", "facade_demo")
     if not os.path.exists(exe):
         pytest.fail("tests/cpp/facade_demo not built (run the build step)")
-    out = tmp_path / "pages.bin"
-    r = subprocess.run([exe, str(out)], capture_output=True, text=True, timeout=120)
+    out, frame = tmp_path / "pages.bin", tmp_path / "frame.bin"
+    r = subprocess.run([exe, str(out), str(frame)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     got = np.fromfile(out, dtype=np.uint16)
     S, bsx = 128, 16
     xyz = M.scene_spheres(99, 6, 30, 98, 10, 25, 4000, 0.5, 0.3)
     gp = gpu_pipeline(S, bsx, xyz, numIter=5, dt=0.1, arith=M.ARITH_FAST)
     assert np.array_equal(got, gp["pages"].reshape(-1))
+    # RDR::RaymarchRenderer through the facade = msbg_render through ctypes
+    img = np.fromfile(frame, dtype=np.uint32).reshape(180, 320)
+    want = gp["grid"].render(30, 320, 180, camPos=(.5, .8, .7), lookAt=(.5, .5, .5), zoom=1.0,
+                             sunPos=(.5, 5, 5), color=(0.8, 0.6, 0.4))
+    assert np.array_equal(img, want)
+    assert ((img & 0xffffff) != 0).sum() > 1000
 
 
 # --------------------------------------------------------------------------
