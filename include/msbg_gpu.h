@@ -303,6 +303,28 @@ int msbg_apply_channel_pde_fast(msbg_grid *g, int chPhi, int chTmp,
                                 int numIter, float gridSpacing,
                                 float timeStep, double *residual);
 
+/* Coarse-level sweeps. No counterpart in the reference, which sweeps level
+ * 0 only (msbg4.cpp:467-470): the 8-colour MCF sweep of
+ * msbg_apply_channel_pde_fast (laplTyp -(4 + OPT_8_COLOR_SCHEME)) applied to
+ * the blocks of resolution level `level` on that level's own pages (block
+ * edge bsx0 >> level, which must be 8, 16 or 32). blockList as there (NULL:
+ * every value block of the level; blocks of other levels are skipped).
+ * haloOpt is the halo rule of fillHaloBlock_ (halo.h:267-355):
+ * MSBG_OPT_BC_NEUMANN (NULL neighbours and the outside of the domain give
+ * the block's own clamped voxel, the level-0 rule), or
+ * MSBG_OPT_BC_COARSE_LEVEL [| MSBG_OPT_INTERPOLATE] (NULL neighbours read
+ * level+1, which must be defined: its voxel at x/2, or interpolateGhost at
+ * (x+.5)/2; halo voxels outside the domain are clipped into it first).
+ * Finer neighbours are read from the level's ghost pages
+ * (msbg_prepare_ghost_blocks) and are not swept. numIter is rounded up to
+ * even; residual (optional, host [sweeps]) gets the FP64 RMS update per
+ * sweep. FP64 channels support MSBG_OPT_BC_NEUMANN only. Arithmetic follows
+ * msbg_set_arith. */
+int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level,
+                                 unsigned haloOpt, const int32_t *blockList,
+                                 int64_t nList, int numIter, float timeStep,
+                                 int doConstrZeroOne, double *residual);
+
 /* Device-time of the last msbg_apply_channel_pde_fast call (HIP events on
  * the grid's stream around the sweep kernels): total ms and per-launch
  * average ms of the colour-pass kernel. */

@@ -86,6 +86,7 @@ SIGNATURES = {
     "msbg_apply_channel_pde": (_I, [_P, _I, _I, _I, _P, _I64, _I, _I, _F, _F, _F, _I, _F, _F,
                                     _F, _F, _F, _P]),
     "msbg_last_sweep_timing": (_I, [_P, _P, _P, _P]),
+    "msbg_apply_channel_pde_level": (_I, [_P, _I, _I, C.c_uint, _P, _I64, _I, _F, _I, _P]),
     "msbg_scene_spheres": (_I, [_U64, _I, _F, _F, _F, _F, _I64, _F, _F, _P]),
     "msbg_copy_bandwidth": (_I, [_I64, _I, _P]),
     "msbg_channel_copy": (_I, [_P, _I, _I, _I, _I]),

@@ -84,7 +84,9 @@ struct SweepGrid {
   const int32_t *nbr;
   int64_t npages;
   float dt, lo, hi;
-  int pad;
+  // outside-the-domain halo cells: 0 = the block's own clamped voxel
+  // (OPT_BC_NEUMANN), 1 = clipped to the domain first (OPT_BC_COARSE_LEVEL)
+  int outClip;
   // where the new values go: the same pages in place, another channel with
   // the same page numbering in the Jacobi mode
   uint8_t *dst_pages;
@@ -203,6 +205,11 @@ int launch_ghost(msbg_grid *g, Channel &ch, int level, const int32_t *d_bids,
                  int64_t n);
 // sweep kernel of one colour pass over work items {grid, page}
 int sweep_prepare(msbg_grid *g, Channel &ch);  // side-car faces/edges + consts
+// the same for any channel level (its own block size) or a working copy
+int sweep_prepare_level(msbg_grid *g, ChanLevel &cl, int storage);
+// ghost pages of a coarse-level sweep (level `level`, values from level+1)
+int launch_fill_ghost_pages(msbg_grid *g, Channel &ch, int level, unsigned opt,
+                            const int32_t *d_bids, int64_t n, uint8_t *d_pages, int64_t firstPage);
 SweepGrid sweep_desc(Channel &ch, float dt, int doConstr01);
 int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, int tile,
                        const SweepGrid *d_grids, const int2 *d_items, int64_t n,

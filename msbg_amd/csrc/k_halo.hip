@@ -249,6 +249,41 @@ GhostWeights ghost_weights() {
   return W;
 }
 
+// Ghost pages of a coarse-level sweep: page gp (block bids[i]) of level L
+// gets, in every voxel, the value getOutOfBlockValue gives an in-domain
+// voxel of a NULL block with OPT_BC_COARSE_LEVEL [| OPT_INTERPOLATE]
+// (halo.h:288-352): interpolateGhost of level Lo at ((x+.5)/2, ...) or its
+// voxel (x/2, y/2, z/2). Stored raw in the channel's storage, so the sweep
+// reads back what fillHaloBlock_ would have produced.
+template <int ST>
+__global__ void k_fill_ghost_pages(LevelView L, LevelView Lo, unsigned opt,
+                                   const int32_t *__restrict__ bids, int64_t firstPage,
+                                   uint8_t *__restrict__ pages) {
+  using R = typename Stor<ST>::raw;
+  const int nv = 1 << (3 * L.lg);
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const int64_t i = blockIdx.y;
+  const int bid = bids[i];
+  const int bx = bid % L.nbx, by = (bid / L.nbx) % L.nby, bz = bid / (L.nbx * L.nby);
+  const int m = L.bsx - 1;
+  const int x = bx * L.bsx + (v & m), y = by * L.bsx + ((v >> L.lg) & m),
+            z = bz * L.bsx + (v >> (2 * L.lg));
+  double raw;
+  if (opt & MSBG_OPT_INTERPOLATE) {
+    raw = interpolate_ghost_raw(Lo, ((float)x + 0.5f) / 2, ((float)y + 0.5f) / 2,
+                                ((float)z + 0.5f) / 2);
+  } else {
+    int xl = x / 2, yl = y / 2, zl = z / 2;
+    clip3(Lo, xl, yl, zl);
+    raw = get_value_raw(Lo, xl, yl, zl);
+  }
+  R out;
+  if constexpr (ST == MSBG_STORAGE_U16) out = (R)(uint16_t)raw;
+  else out = Stor<ST>::enc((float)raw);
+  ((R *)pages)[(firstPage + i) * nv + v] = out;
+}
+
 }  // namespace
 
 int launch_halo_fill(msbg_grid *g, Channel &ch, int level, int bid, int dmax, unsigned opt,
@@ -260,6 +295,39 @@ int launch_halo_fill(msbg_grid *g, Channel &ch, int level, int bid, int dmax, un
   hipLaunchKernelGGL(k_halo_fill, dim3((n + 255) / 256), dim3(256), 0, g->stream, L, Lo, bid, dmax,
                      opt, d_out);
   MSBG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_fill_ghost_pages(msbg_grid *g, Channel &ch, int level, unsigned opt,
+                            const int32_t *d_bids, int64_t n, uint8_t *d_pages,
+                            int64_t firstPage) {
+  LevelView L = make_view(g, ch, level);
+  LevelView Lo = make_view(g, ch, level + 1);
+  const int nv = L.bsx * L.bsx * L.bsx;
+  for (int64_t off = 0; off < n; off += 65535) {
+    const unsigned cnt = (unsigned)(n - off < 65535 ? n - off : 65535);
+    const dim3 grid((unsigned)((nv + 255) / 256), cnt);
+    switch (ch.storage) {
+      case MSBG_STORAGE_U16:
+        hipLaunchKernelGGL(k_fill_ghost_pages<MSBG_STORAGE_U16>, grid, dim3(256), 0, g->stream, L,
+                           Lo, opt, d_bids + off, firstPage + off, d_pages);
+        break;
+      case MSBG_STORAGE_F32:
+        hipLaunchKernelGGL(k_fill_ghost_pages<MSBG_STORAGE_F32>, grid, dim3(256), 0, g->stream, L,
+                           Lo, opt, d_bids + off, firstPage + off, d_pages);
+        break;
+      case MSBG_STORAGE_F16:
+        hipLaunchKernelGGL(k_fill_ghost_pages<MSBG_STORAGE_F16>, grid, dim3(256), 0, g->stream, L,
+                           Lo, opt, d_bids + off, firstPage + off, d_pages);
+        break;
+      case MSBG_STORAGE_BF16:
+        hipLaunchKernelGGL(k_fill_ghost_pages<MSBG_STORAGE_BF16>, grid, dim3(256), 0, g->stream,
+                           L, Lo, opt, d_bids + off, firstPage + off, d_pages);
+        break;
+      default: set_error("ghost pages: unsupported storage %d", ch.storage); return MSBG_EUNSUPP;
+    }
+    MSBG_HIP(hipGetLastError());
+  }
   return 0;
 }
 

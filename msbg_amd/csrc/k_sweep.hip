@@ -21,6 +21,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "internal.h"
 #include "storage.h"
 #include "sweep_core.h"
@@ -116,6 +118,28 @@ int prepare_bsx(msbg_grid *g, Channel &ch) {
 }  // namespace
 
 int default_tile(int bsx) { return bsx >= 32 ? 1 : 0; }
+
+int sweep_prepare_level(msbg_grid *g, ChanLevel &cl, int storage) {
+  auto st = [&](auto bsxc) -> int {
+    constexpr int B = decltype(bsxc)::value;
+    switch (storage) {
+      case MSBG_STORAGE_U16: return prepare_t<B, MSBG_STORAGE_U16>(g, cl);
+      case MSBG_STORAGE_F32: return prepare_t<B, MSBG_STORAGE_F32>(g, cl);
+      case MSBG_STORAGE_F16: return prepare_t<B, MSBG_STORAGE_F16>(g, cl);
+      case MSBG_STORAGE_BF16: return prepare_t<B, MSBG_STORAGE_BF16>(g, cl);
+      case MSBG_STORAGE_F64: return prepare_t<B, MSBG_STORAGE_F64>(g, cl);
+    }
+    set_error("sweep: bad storage %d", storage);
+    return MSBG_EINVAL;
+  };
+  switch (cl.bsx) {
+    case 8: return st(std::integral_constant<int, 8>{});
+    case 16: return st(std::integral_constant<int, 16>{});
+    case 32: return st(std::integral_constant<int, 32>{});
+  }
+  set_error("sweep: block size %d not supported by the sweep kernel (8, 16, 32)", cl.bsx);
+  return MSBG_EUNSUPP;
+}
 
 int sweep_prepare(msbg_grid *g, Channel &ch) {
   switch (g->bsx0) {

@@ -418,10 +418,9 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT)) void k_sweep(
     rlds[q] = (y + 1) * P + x + 1;
     const int dxi = x < 0 ? 0 : (x >= BSX ? 2 : 1), dyi = y < 0 ? 0 : (y >= BSX ? 2 : 1);
     const bool xside = dxi != 1, yside = dyi != 1;
-    // source pointer for the neighbour at (dxi, dyi, dzi) and plane zz
-    auto one = [&](int dzi, int zz, int *stride) -> const R * {
-      int c = codes[dzi * 9 + dyi * 3 + dxi];
-      int zc = zz < 0 ? 0 : (zz > M ? M : zz);
+    // source pointer of the cell (xx, yy) of plane zz (each in [-1, BSX]) in
+    // the block with code c, or nullptr for NULL and outside-the-domain codes
+    auto resolve = [&](int c, int xx, int yy, int zz, int *stride) -> const R * {
       if (c == MSBG_BLOCK_EMPTY) {
         *stride = 0;
         return consts;
@@ -430,21 +429,49 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT)) void k_sweep(
         *stride = 0;
         return consts + 4;
       }
-      if (c >= 0) {
-        int zn = zz & M;  // plane inside the neighbour
-        if (xside && yside) {  // corner: the neighbour's opposite z-edge
-          int k = (x < 0 ? 1 : 0) + (y < 0 ? 2 : 0);
-          *stride = 1;
-          return edges + (int64_t)c * ES + k * BSX + zn;
-        }
-        if (xside) {  // x-face: the neighbour's opposite face, [z][y]
-          *stride = BSX;
-          return faces + (int64_t)c * FS + (x < 0 ? 1 : 0) * PS + zn * BSX + y;
-        }
-        *stride = PS;  // y-face: a contiguous row of the neighbour page
-        return pages + (int64_t)c * PE + zn * PS + (y < 0 ? M : 0) * BSX + x;
+      if (c < 0) return nullptr;
+      const bool xs = xx < 0 || xx > M, ys = yy < 0 || yy > M;
+      const int zn = zz & M;  // plane inside that block
+      if (xs && ys) {  // corner: the neighbour's opposite z-edge
+        const int k = (xx < 0 ? 1 : 0) + (yy < 0 ? 2 : 0);
+        *stride = 1;
+        return edges + (int64_t)c * ES + k * BSX + zn;
       }
+      if (xs) {  // x-face: the neighbour's opposite face, [z][y]
+        *stride = BSX;
+        return faces + (int64_t)c * FS + (xx < 0 ? 1 : 0) * PS + zn * BSX + yy;
+      }
+      *stride = PS;  // a row of that block's page
+      return pages + (int64_t)c * PE + zn * PS + (yy & M) * BSX + (xx & M);
+    };
+    // source pointer for the neighbour at (dxi, dyi, dzi) and plane zz
+    auto one = [&](int dzi, int zz, int *stride) -> const R * {
+      int c = codes[dzi * 9 + dyi * 3 + dxi];
+      const R *src;
+      if (c == MSBG_BLOCK_OUT && G.outClip) {
+        // getOutOfBlockValue with OPT_BC_COARSE_LEVEL (halo.h:288-300):
+        // coordinates outside the domain are clipped to it first, which
+        // lands in the block itself or an in-domain neighbour
+        int xx = x, yy = y, z2 = zz, dx2 = dxi, dy2 = dyi, dz2 = dzi;
+        if (dxi != 1 && codes[12 + dxi] == MSBG_BLOCK_OUT) {
+          dx2 = 1;
+          xx = x < 0 ? 0 : M;
+        }
+        if (dyi != 1 && codes[10 + 3 * dyi] == MSBG_BLOCK_OUT) {
+          dy2 = 1;
+          yy = y < 0 ? 0 : M;
+        }
+        if (dzi != 1 && codes[dzi * 9 + 4] == MSBG_BLOCK_OUT) {
+          dz2 = 1;
+          z2 = zz < 0 ? 0 : M;
+        }
+        src = resolve(codes[dz2 * 9 + dy2 * 3 + dx2], xx, yy, z2, stride);
+      } else {
+        src = resolve(c, x, y, zz, stride);
+      }
+      if (src) return src;
       // NULL block or outside the domain: the block's own clamped voxel
+      const int zc = zz < 0 ? 0 : (zz > M ? M : zz);
       if (xside && yside) {
         int k = (x < 0 ? 0 : 1) + (y < 0 ? 0 : 2);
         *stride = 1;

@@ -388,6 +388,19 @@ class MultiresSparseGrid:
             lambda_, taubinStrength, anisDiffAlpha, _ptr(res)))
         return res[:nsweeps] if residual else None
 
+    def applyChannelPdeLevel(self, chan, level, haloOpt, blockList=None, numIter=2,
+                             timeStep=0.1, doConstrZeroOne=1, residual=False):
+        """8-colour MCF sweeps of the level-`level` blocks on that level's grid
+        (msbg_apply_channel_pde_level; the reference sweeps level 0 only).
+        haloOpt: OPT_BC_NEUMANN or OPT_BC_COARSE_LEVEL [| OPT_INTERPOLATE]."""
+        bl = None if blockList is None else np.ascontiguousarray(blockList, dtype=np.int32)
+        nsweeps = numIter + (numIter % 2)
+        res = np.zeros(max(nsweeps, 1), dtype=np.float64) if residual else None
+        check(lib().msbg_apply_channel_pde_level(
+            self._h, chan, level, haloOpt, _ptr(bl), 0 if bl is None else bl.size, numIter,
+            timeStep, doConstrZeroOne, _ptr(res)))
+        return res[:nsweeps] if residual else None
+
     def set_tile(self, tile):
         check(lib().msbg_set_tile(self._h, tile))
 

@@ -57,6 +57,7 @@ def load(kind="strict"):
             "or_halo_fill": (None, [P, P, I, I, C.c_uint, P]),
             "or_halo_fill_f64": (None, [P, I, I, C.c_uint, P]),
             "or_mcf_sweeps": (I, [P, P, I, P, I, F, I, P, I]),
+            "or_mcf_sweeps_level": (I, [P, P, P, I, P, I, C.c_uint, I, F, I, P, I]),
             "or_mcf_voxel": (F, [P, I, I, F, I]),
             "or_pde_sweeps": (I, [P, P, I, P, I, P, P, I]),
             "or_pde_voxel": (F, [I, P, I, I, F, F, F, I]),
@@ -231,6 +232,25 @@ def mcf_sweeps(level, blocks, levels, numIter, dt, doConstrZeroOne=1, residual=F
     s = level.struct()
     load(kind).or_mcf_sweeps(C.byref(s), _p(b), b.size, _p(lv), numIter, dt, doConstrZeroOne,
                              _p(res), nthreads)
+    return res[:n] if residual else None
+
+
+def mcf_sweeps_level(level, lo, blocks, levels, sweepLevel, opt, numIter, dt,
+                     doConstrZeroOne=1, residual=False, nthreads=0):
+    """or_mcf_sweeps_level: 8-colour MCF sweeps of the blocks at sweepLevel on
+    that level's pages, halo rule `opt` (NEUMANN, or COARSE_LEVEL [| INTERPOLATE]
+    reading `lo`, the next coarser level)."""
+    b = np.ascontiguousarray(blocks, dtype=np.int32)
+    lv = np.ascontiguousarray(levels, dtype=np.uint16)
+    n = numIter + (numIter % 2)
+    res = np.zeros(max(n, 1), np.float64) if residual else None
+    s = level.struct()
+    so = lo.struct() if lo is not None else None
+    rc = load().or_mcf_sweeps_level(C.byref(s), C.byref(so) if so is not None else None, _p(b),
+                                    b.size, _p(lv), sweepLevel, opt, numIter, dt,
+                                    doConstrZeroOne, _p(res), nthreads)
+    if rc < 0:
+        raise ValueError("or_mcf_sweeps_level: unsupported combination")
     return res[:n] if residual else None
 
 

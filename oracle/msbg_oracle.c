@@ -140,11 +140,19 @@ static inline double page_raw(const or_level *L, int64_t page, int vid) {
   return 0;
 }
 
+/* a const value as the channel holds it: half-width float channels store
+ * their empty/full values rounded to the storage precision */
+static inline double const_raw(int st, double r) {
+  if (st == OR_F16) return or_f16_to_float(or_f16_from_float((float)r));
+  if (st == OR_BF16) return or_bf16_to_float(or_bf16_from_float((float)r));
+  return r;
+}
+
 /* value of block code c (non-NULL) at voxel vid, as raw storage value */
 static inline double block_raw(const or_level *L, int32_t c, int vid) {
   if (c >= 0) return page_raw(L, c, vid);
-  if (c == OR_EMPTY) return L->empty_raw;
-  return L->full_raw;
+  if (c == OR_EMPTY) return const_raw(L->storage, L->empty_raw);
+  return const_raw(L->storage, L->full_raw);
 }
 
 static inline int ilog2i(int v) {
@@ -466,7 +474,7 @@ static inline void clip3(const or_level *L, int *x, int *y, int *z) {
 /* SparseGrid::getValue(int) (sbg.h:2622-2648): emptyValue for NULL */
 static double get_value_raw(const or_level *L, int x, int y, int z) {
   int32_t c = code_at(L, x, y, z);
-  if (c == OR_NULL) return L->empty_raw;
+  if (c == OR_NULL) return const_raw(L->storage, L->empty_raw);
   return block_raw(L, c, vid_at(L, x, y, z));
 }
 
@@ -713,6 +721,15 @@ static inline void store_float(const or_level *L, int64_t page, int vid, float v
 int or_mcf_sweeps(const or_level *L, const int32_t *blocks, int n,
                   const uint16_t *levels, int numIter, float dt,
                   int doConstrZeroOne, double *residual, int nthreads) {
+  return or_mcf_sweeps_level(L, NULL, blocks, n, levels, 0, OR_OPT_BC_NEUMANN, numIter, dt,
+                             doConstrZeroOne, residual, nthreads);
+}
+
+int or_mcf_sweeps_level(const or_level *L, const or_level *Lo, const int32_t *blocks, int n,
+                        const uint16_t *levels, int sweepLevel, unsigned opt, int numIter,
+                        float dt, int doConstrZeroOne, double *residual, int nthreads) {
+  if (L->storage == OR_F64 && opt != OR_OPT_BC_NEUMANN) return -1;
+  if ((opt & OR_OPT_BC_COARSE_LEVEL) && !Lo) return -1;
   int nMaxIter = numIter;
   if (nMaxIter % 2 == 1) nMaxIter++;
   const int b = L->bsx, hsx = b + 2, nh = hsx * hsx * hsx;
@@ -740,7 +757,7 @@ int or_mcf_sweeps(const or_level *L, const int32_t *blocks, int n,
           int bid = blocks[i];
           int bx = bid % L->nbx, by = (bid / L->nbx) % L->nby, bz = bid / (L->nbx * L->nby);
           if (or_color8(bx, by, bz) != pass) continue;
-          if (levels && levels[bid] != 0) continue;
+          if (levels && levels[bid] != sweepLevel) continue;
           int32_t pg = L->page[bid];
           if (pg < 0) continue; /* UT_ASSERT0(isValueBlock) in the reference */
           if (isF64) {
@@ -756,7 +773,7 @@ int or_mcf_sweeps(const or_level *L, const int32_t *blocks, int n,
                   ((double *)L->pages)[(size_t)pg * nvb + vid] = ph;
                 }
           } else {
-            or_halo_fill(L, NULL, bid, 1, OR_OPT_BC_NEUMANN, hf);
+            or_halo_fill(L, Lo, bid, 1, opt, hf);
             for (int vz = 0, vid = 0; vz < b; vz++)
               for (int vy = 0; vy < b; vy++)
                 for (int vx = 0; vx < b; vx++, vid++) {

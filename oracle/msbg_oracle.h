@@ -128,6 +128,19 @@ void or_halo_fill_f64(const or_level *L, int bid, int dmax, unsigned opt,
 int or_mcf_sweeps(const or_level *L, const int32_t *blocks, int n,
                   const uint16_t *levels, int numIter, float dt,
                   int doConstrZeroOne, double *residual, int nthreads);
+/* Coarse-level sweeps: an extension of the sweep above that the reference
+ * does not run (it updates level 0 only, msbg4.cpp:467-470). L is level
+ * sweepLevel of the channel (bsx_l blocks, its pages including the ghost
+ * pages prepareGhostBlocks writes for finer neighbours); the blocks of
+ * `blocks` whose level is sweepLevel are swept in 8 colour passes, each
+ * from the halo fillHaloBlock_ builds with `opt`: OR_OPT_BC_NEUMANN (the
+ * level-0 rule), or OR_OPT_BC_COARSE_LEVEL [| OR_OPT_INTERPOLATE], where
+ * NULL neighbours read level sweepLevel+1 (Lo) and out-of-domain voxels are
+ * clipped first (halo.h:280-352). FP64 supports NEUMANN only. Returns the
+ * sweep count, -1 for an unsupported combination. */
+int or_mcf_sweeps_level(const or_level *L, const or_level *Lo, const int32_t *blocks, int n,
+                        const uint16_t *levels, int sweepLevel, unsigned opt, int numIter,
+                        float dt, int doConstrZeroOne, double *residual, int nthreads);
 /* The per-voxel MCF update from a halo buffer (msbg4.cpp:559-637,813-825). */
 float or_mcf_voxel(const float *h, int hsx, int hid, float dt,
                    int doConstrZeroOne);

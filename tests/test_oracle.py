@@ -465,3 +465,92 @@ def test_ghost_downsample_closed_forms():
     # window of halo indices 2v..2v+3 is centred at halo coordinate 2v+1.5
     want = a * (2 * cx + 1.5) + b * (2 * cy + 1.5) + c * (2 * cz + 1.5) + d
     assert np.abs(lo - want).max() < 1e-6
+
+
+# --------------------------------------------------------------------------
+# coarse-level sweeps (an extension: the reference sweeps level 0 only)
+# vs the dense restatement with the COARSE_LEVEL halo rule
+# --------------------------------------------------------------------------
+def _coarse_halo(pm, pages, S, bsx, bid, lo_dense, empty=0.0, full=1.0):
+    """getOutOfBlockValue with OPT_BC_COARSE_LEVEL (halo.h:288-352), no
+    interpolation: in-domain non-NULL blocks give their value; NULL blocks
+    read the coarser level at (x/2, y/2, z/2); out-of-domain voxels are
+    clipped to the domain first and read like in-domain ones."""
+    nb = S // bsx
+    bz, rem = divmod(bid, nb * nb)
+    by, bx = divmod(rem, nb)
+    o = np.array([bx, by, bz]) * bsx
+    h = np.zeros((bsx + 2,) * 3, np.float32)
+    for hz in range(bsx + 2):
+        for hy in range(bsx + 2):
+            for hx in range(bsx + 2):
+                g = np.clip(o + np.array([hx, hy, hz]) - 1, 0, S - 1)
+                b = g // bsx
+                code = pm[(b[2] * nb + b[1]) * nb + b[0]]
+                if code >= 0:
+                    v = g % bsx
+                    h[hz, hy, hx] = pages[code, v[2], v[1], v[0]]
+                elif code == O.EMPTY:
+                    h[hz, hy, hx] = empty
+                elif code == O.FULL:
+                    h[hz, hy, hx] = full
+                else:
+                    gl = g // 2
+                    h[hz, hy, hx] = lo_dense[gl[2], gl[1], gl[0]]
+    return h
+
+
+def test_coarse_level_sweep_matches_dense_restatement():
+    S0, bsx0 = 64, 16
+    S1, b1 = S0 // 2, bsx0 // 2  # level 1: 32^3 voxels in 8^3 blocks
+    nb = S0 // bsx0
+    rng = np.random.default_rng(21)
+    levels = rng.choice([0, 1, 2], nb ** 3, p=[0.3, 0.45, 0.25]).astype(np.uint16)
+    # level 1 holds its own blocks and (as prepareGhostBlocks leaves it)
+    # ghost copies at some finer blocks; the rest is NULL
+    pm1 = np.full(nb ** 3, O.NULL, np.int32)
+    own = np.nonzero(levels == 1)[0]
+    ghost = np.nonzero((levels == 0) & (rng.random(nb ** 3) < 0.6))[0]
+    vals = np.sort(np.concatenate([own, ghost])).astype(np.int32)
+    pm1[vals] = np.arange(vals.size)
+    pm1[np.nonzero((levels == 0) & (pm1 == O.NULL))[0][:1]] = O.FULL
+    z, y, x = np.meshgrid(*[np.arange(S1)] * 3, indexing="ij")
+    f1 = (0.5 + 0.4 * np.sin(x / 4.0) * np.cos(y / 5.0 + z / 6.0)).astype(np.float32)
+    pages1 = np.stack([f1[(b // (nb * nb)) * b1:(b // (nb * nb) + 1) * b1,
+                          ((b // nb) % nb) * b1:((b // nb) % nb + 1) * b1,
+                          (b % nb) * b1:(b % nb + 1) * b1] for b in vals])
+    # level 2: 16^3 voxels in 4^3 blocks; its value blocks everywhere but one
+    S2, b2 = S0 // 4, bsx0 // 4
+    pm2 = np.arange(nb ** 3, dtype=np.int32)
+    pm2[5] = O.NULL
+    f2 = rng.random((S2, S2, S2)).astype(np.float32)
+    pages2 = np.stack([f2[(b // (nb * nb)) * b2:(b // (nb * nb) + 1) * b2,
+                          ((b // nb) % nb) * b2:((b // nb) % nb + 1) * b2,
+                          (b % nb) * b2:(b % nb + 1) * b2] for b in range(nb ** 3)])
+    lo_dense = f2.copy()
+    bz, rem = divmod(5, nb * nb)
+    by, bx = divmod(rem, nb)
+    lo_dense[bz * b2:(bz + 1) * b2, by * b2:(by + 1) * b2, bx * b2:(bx + 1) * b2] = 0.0
+    L1 = O.Level(S1, S1, S1, b1, pm1, pages1.copy(), O.F32)
+    L2 = O.Level(S2, S2, S2, b2, pm2, pages2, O.F32)
+    res = O.mcf_sweeps_level(L1, L2, vals, levels, 1, O.OPT_BC_COARSE_LEVEL, 2, 0.1, 1,
+                             residual=True)
+    assert res.size == 2 and np.all(res > 0)
+    ref = pages1.copy()
+    for _ in range(2):
+        for c in range(8):
+            for bid in own:
+                bz, rem = divmod(int(bid), nb * nb)
+                by, bx = divmod(rem, nb)
+                if (bx & 1) + 2 * (by & 1) + 4 * (bz & 1) != c:
+                    continue
+                h = _coarse_halo(pm1, ref, S1, b1, int(bid), lo_dense)
+                ref[pm1[bid]] = _np_mcf(h, 0.1)
+    assert np.array_equal(L1.pages, ref)
+    # ghost pages of finer blocks are read, never written
+    gi = pm1[ghost]
+    assert np.array_equal(L1.pages[gi], pages1[gi])
+    # the level-0 rule at level 1 is a different result
+    L1n = O.Level(S1, S1, S1, b1, pm1, pages1.copy(), O.F32)
+    O.mcf_sweeps_level(L1n, None, vals, levels, 1, O.OPT_BC_NEUMANN, 2, 0.1, 1)
+    assert not np.array_equal(L1n.pages, L1.pages)
