@@ -287,14 +287,146 @@ def bench_render(args):
     print(json.dumps(res))
 
 
+def levels_cpu_baseline(g, chan, cfg, levels, level, opt, budget_s):
+    """The fast oracle's or_mcf_sweeps_level on a sample of the same level's
+    blocks (the level and the next coarser one are downloaded whole)."""
+    from oracle import liboracle as O
+    S, bsx = cfg["S"], cfg["bsx"]
+    ost = {1: O.U16, 2: O.F32, 3: O.F16, 4: O.BF16}[g._storage[chan]]
+    L = []
+    for lv in (level, level + 1):
+        if lv >= cfg["nLevels"]:
+            L.append(None)
+            continue
+        pm, _ = g.pagemap(chan, lv)
+        pages = g.download(chan, lv)
+        if ost == O.F16:
+            pages = pages.view(np.uint16)
+        s = S >> lv
+        L.append(O.Level(s, s, s, bsx >> lv, pm, pages, ost, 0.0, None))
+    blocks = np.nonzero(levels == level)[0].astype(np.int32)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    nv = (bsx >> level) ** 3
+    take = blocks[::max(1, blocks.size // 256)]
+    t0 = time.perf_counter()
+    O.mcf_sweeps_level(L[0], L[1], take, levels, level, opt, 2, 0.1, 1, nthreads=threads,
+                       kind="fast")
+    t = time.perf_counter() - t0
+    n = int(min(blocks.size, max(take.size, take.size * budget_s / max(t, 1e-3))))
+    take = blocks[::max(1, blocks.size // n)]
+    t0 = time.perf_counter()
+    O.mcf_sweeps_level(L[0], L[1], take, levels, level, opt, 2, 0.1, 1, nthreads=threads,
+                       kind="fast")
+    t = time.perf_counter() - t0
+    return {"value": take.size * nv * 2 / t, "unit": "voxel-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"level {level}: {take.size} of its {blocks.size} blocks ({bsx >> level}^3), "
+                      f"2 sweeps with the same halo rule, {t:.1f} s on {threads} threads "
+                      f"(oracle built -O3 -ffast-math -march=x86-64-v3 -fopenmp)"}
+
+
+def bench_levels(args):
+    """Coarse-level sweeps of the multiresolution scene: after the splat and
+    prepareGhostBlocks, levels 1..n-1 are swept on their own grids with the
+    COARSE_LEVEL|INTERPOLATE halo (the coarsest level with NEUMANN)."""
+    import msbg_amd as M
+    cfg = CONFIGS[args.config or "cfg2"]
+    storage = STORAGES[args.storage]
+    S, bsx, nL = cfg["S"], cfg["bsx"], cfg["nLevels"]
+    xyz = build_scene(cfg, args.seed)
+    chan = 30
+    g = M.MultiresSparseGrid.create("levels", S, S, S, bsx, options=0, nLevels=nL)
+    g.points_upload(xyz)
+    lv = g.regularizeRefinementMap(g.levels_from_points(cfg["bands"]))
+    g.setRefinementMap(lv)
+    levels, _ = g.getBlockInfo()
+    for level in range(nL):
+        kind = np.where(levels == level, 3, 0).astype(np.uint8)
+        g.channel_define(chan, storage, level, kind, 0.0, None, 0.0)
+    g.set_exists(None)
+    g.splat(chan, 2.0, 2.0)
+    g.finalize(chan, 2.0, 2.0)
+    g.prepareGhostBlocks(chan)
+    g.set_arith(M.ARITH_FAST if args.arith == "fast" else M.ARITH_EXACT)
+    esz = {1: 2, 2: 4, 3: 2, 4: 2}[storage]
+    per, tot_vox, tot_ms, tot_wall = {}, 0, 0.0, 0.0
+    for level in range(1, nL):
+        opt = (M.OPT_BC_COARSE_LEVEL | M.OPT_INTERPOLATE) if level + 1 < nL else M.OPT_BC_NEUMANN
+        nb = int((levels == level).sum())
+        nvox = nb * (bsx >> level) ** 3
+        for _ in range(args.warmup):
+            g.applyChannelPdeLevel(chan, level, opt, None, 2, 0.1, 1)
+        g.synchronize()
+        kern, passes = 0.0, 0
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.applyChannelPdeLevel(chan, level, opt, None, 2, 0.1, 1)
+            t, a, n = g.last_sweep_timing()
+            kern += t
+            passes += n
+        g.synchronize()
+        wall = time.perf_counter() - t0
+        per[level] = {"blocks": nb, "block_edge": bsx >> level, "voxels": nvox,
+                      "halo": "COARSE_LEVEL|INTERPOLATE" if level + 1 < nL else "NEUMANN",
+                      "sweep_rate": nvox * 2 * args.steps / (kern * 1e-3),
+                      "call_rate_incl_setup": nvox * 2 * args.steps / wall,
+                      "avg_pass_ms": kern / max(passes, 1),
+                      "hbm_frac": nvox * 2 * esz / 8 / (kern / max(passes, 1) * 1e-3) / 1e9 /
+                                  HBM_PEAK_GBS}
+        log(f"level {level}: {per[level]}")
+        tot_vox += nvox * 2 * args.steps
+        tot_ms += kern
+        tot_wall += wall
+    value = tot_vox / (tot_ms * 1e-3)
+    lvl1 = per[1]
+    achieved = lvl1["voxels"] * 2 * esz / 8 / (lvl1["avg_pass_ms"] * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = levels_cpu_baseline(g, chan, cfg, levels, 1,
+                                      M.OPT_BC_COARSE_LEVEL | M.OPT_INTERPOLATE,
+                                      args.cpu_seconds)
+        except OSError as e:
+            log(f"cpu baseline unavailable: {e}")
+    res = {
+        "metric": "voxel-updates/s/sweep",
+        "value": value,
+        "unit": "voxel-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tot_wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.config or 'cfg2'}: {cfg['desc']}; coarse-level sweeps of levels "
+                        f"1..{nL - 1} after splat + prepareGhostBlocks; step = 2 sweeps per level",
+            "storage": args.storage, "arith": args.arith,
+            "levels": {str(k): v for k, v in per.items()},
+            "rate_incl_call_setup": tot_vox / tot_wall,
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_sweep<{bsx >> 1},{args.storage}> level-1 colour pass",
+                     "avg_launch_ms": lvl1["avg_pass_ms"]},
+        "cpu_baseline": cpu,
+    }
+    g.destroy()
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--path", default="sweep", choices=["sweep", "render"],
+    ap.add_argument("--path", default="sweep", choices=["sweep", "render", "levels"],
                     help="sweep: the 8-colour MCF sweep (default); render: the demo's 4K "
-                         "ray-march render of the smoothed field")
+                         "ray-march render of the smoothed field; levels: coarse-level sweeps "
+                         "of the multiresolution scene")
     ap.add_argument("--config", default=None, choices=sorted(CONFIGS),
                     help="default cfg2 (sweep), cfg1 (render)")
     ap.add_argument("--storage", default="f16", choices=sorted(STORAGES))
@@ -312,6 +444,9 @@ def main():
 
     if args.path == "render":
         bench_render(args)
+        return
+    if args.path == "levels":
+        bench_levels(args)
         return
     args.config = args.config or "cfg2"
     import msbg_amd as M

@@ -325,9 +325,10 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level,
                                  int64_t nList, int numIter, float timeStep,
                                  int doConstrZeroOne, double *residual);
 
-/* Device-time of the last msbg_apply_channel_pde_fast call (HIP events on
- * the grid's stream around the sweep kernels): total ms and per-launch
- * average ms of the colour-pass kernel. */
+/* Device-time of the last msbg_apply_channel_pde_fast or
+ * msbg_apply_channel_pde_level call (HIP events on the grid's stream around
+ * the sweep kernels): total ms and per-launch average ms of the colour-pass
+ * kernel. */
 int msbg_last_sweep_timing(msbg_grid *g, double *total_ms,
                            double *avg_pass_ms, int64_t *npasses);
 

@@ -171,6 +171,8 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
     MSBG_TRY(d_resid.alloc((size_t)nMaxIter));
   }
   const bool exact = g->arith == MSBG_ARITH_EXACT;
+  int64_t npasses = 0;
+  MSBG_HIP(hipEventRecord(g->ev[0], g->stream));
   for (int iter = 0; iter < nMaxIter; iter++) {
     for (int pass = 0; pass < 8; pass++) {
       const int64_t n = off[pass + 1] - off[pass];
@@ -178,9 +180,11 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
       MSBG_TRY(launch_sweep_items(g->stream, cl.bsx, ch.storage, exact, -1, d_desc.p,
                                   d_items.p + off[pass], n,
                                   residual ? d_partial.p + off[pass] : nullptr));
+      npasses++;
     }
     if (residual) MSBG_TRY(launch_reduce_residual(g, d_partial.p, ntot, d_resid.p + iter));
   }
+  MSBG_HIP(hipEventRecord(g->ev[1], g->stream));
   // the level's own pages come back; its side-cars are stale now
   MSBG_HIP(hipMemcpyAsync(cl.d_pages.p, wl.d_pages.p, (size_t)cl.npages * pageBytes,
                           hipMemcpyDeviceToDevice, g->stream));
@@ -190,6 +194,11 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
                             hipMemcpyDeviceToHost, g->stream));
   }
   MSBG_HIP(hipStreamSynchronize(g->stream));
+  float ms = 0.0f;
+  MSBG_HIP(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]));
+  g->last_total_ms = ms;  // the sweep passes only, as msbg_last_sweep_timing reports
+  g->last_npasses = npasses;
+  g->last_avg_pass_ms = npasses ? ms / (double)npasses : 0.0;
   if (residual) {
     const double nvox = (double)ntot * cl.bsx * cl.bsx * cl.bsx;
     for (int i = 0; i < nMaxIter; i++) residual[i] = std::sqrt(residual[i] / nvox);

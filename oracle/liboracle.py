@@ -236,7 +236,7 @@ def mcf_sweeps(level, blocks, levels, numIter, dt, doConstrZeroOne=1, residual=F
 
 
 def mcf_sweeps_level(level, lo, blocks, levels, sweepLevel, opt, numIter, dt,
-                     doConstrZeroOne=1, residual=False, nthreads=0):
+                     doConstrZeroOne=1, residual=False, nthreads=0, kind="strict"):
     """or_mcf_sweeps_level: 8-colour MCF sweeps of the blocks at sweepLevel on
     that level's pages, halo rule `opt` (NEUMANN, or COARSE_LEVEL [| INTERPOLATE]
     reading `lo`, the next coarser level)."""
@@ -246,7 +246,7 @@ def mcf_sweeps_level(level, lo, blocks, levels, sweepLevel, opt, numIter, dt,
     res = np.zeros(max(n, 1), np.float64) if residual else None
     s = level.struct()
     so = lo.struct() if lo is not None else None
-    rc = load().or_mcf_sweeps_level(C.byref(s), C.byref(so) if so is not None else None, _p(b),
+    rc = load(kind).or_mcf_sweeps_level(C.byref(s), C.byref(so) if so is not None else None, _p(b),
                                     b.size, _p(lv), sweepLevel, opt, numIter, dt,
                                     doConstrZeroOne, _p(res), nthreads)
     if rc < 0:
