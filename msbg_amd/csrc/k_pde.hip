@@ -27,10 +27,12 @@ int launch_op(hipStream_t s, const SweepGrid *d_grids, const int2 *d_items, int6
               double *d_partial, int iter) {
   constexpr int PX = BSX >= 32 ? 4 : 2, PY = 2;
   constexpr int NT = Tile<BSX, PX, PY>::NT;
-  for (int64_t off = 0; off < n; off += 0x7fffffff) {
-    const unsigned cnt = (unsigned)(n - off < 0x7fffffff ? n - off : 0x7fffffff);
-    hipLaunchKernelGGL((k_sweep<BSX, ST, true, PX, PY, OP, HARD>), dim3(cnt), dim3(NT), 0, s,
-                       d_grids, d_items + off, d_partial ? d_partial + off : nullptr, iter);
+  constexpr int B = Pack<NT>::BPW;
+  for (int64_t off = 0; off < n; off += 0x40000000) {
+    const int cnt = (int)(n - off < 0x40000000 ? n - off : 0x40000000);
+    hipLaunchKernelGGL((k_sweep<BSX, ST, true, PX, PY, OP, HARD, B>),
+                       dim3((unsigned)((cnt + B - 1) / B)), dim3(NT * B), 0, s, d_grids,
+                       d_items + off, d_partial ? d_partial + off : nullptr, iter, cnt);
     MSBG_HIP(hipGetLastError());
   }
   return 0;

@@ -64,10 +64,12 @@ template <int BSX, int ST, bool EXACT, int PX, int PY>
 int launch_one(hipStream_t s, const SweepGrid *d_grids, const int2 *d_items, int64_t n,
                double *d_partial) {
   constexpr int NT = Tile<BSX, PX, PY>::NT;
-  for (int64_t off = 0; off < n; off += 0x7fffffff) {
-    unsigned cnt = (unsigned)(n - off < 0x7fffffff ? n - off : 0x7fffffff);
-    hipLaunchKernelGGL((k_sweep<BSX, ST, EXACT, PX, PY>), dim3(cnt), dim3(NT), 0, s, d_grids,
-                       d_items + off, d_partial ? d_partial + off : nullptr, 0);
+  constexpr int B = Pack<NT>::BPW;
+  for (int64_t off = 0; off < n; off += 0x40000000) {
+    const int cnt = (int)(n - off < 0x40000000 ? n - off : 0x40000000);
+    hipLaunchKernelGGL((k_sweep<BSX, ST, EXACT, PX, PY, OP_MCF, false, B>),
+                       dim3((unsigned)((cnt + B - 1) / B)), dim3(NT * B), 0, s, d_grids,
+                       d_items + off, d_partial ? d_partial + off : nullptr, 0, cnt);
     MSBG_HIP(hipGetLastError());
   }
   return 0;

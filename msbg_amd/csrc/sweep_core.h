@@ -320,12 +320,23 @@ struct Tile {
 #else
 #define MSBG_SWEEP_LB(nt) __launch_bounds__(nt)
 #endif
+// Blocks per workgroup: a block with fewer columns than a wave (8^3 blocks
+// have 16 with 2x2 columns) shares its workgroup with others, each group of
+// NT threads working on its own item with its own LDS planes.
+template <int NT>
+struct Pack {
+  static constexpr int BPW = NT >= 64 ? 1 : 64 / NT;
+};
+
 // OP: the stencil (OP_MCF is the hot path); HARD: hard constraints around
-// the field at entry (G.phi0); iter: the sweep index (Taubin's parity).
-template <int BSX, int ST, bool EXACT, int PX, int PY, int OP = OP_MCF, bool HARD = false>
-__global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT)) void k_sweep(
+// the field at entry (G.phi0); iter: the sweep index (Taubin's parity);
+// nItems: the number of items of this launch (the last workgroup may hold
+// fewer than BPW).
+template <int BSX, int ST, bool EXACT, int PX, int PY, int OP = OP_MCF, bool HARD = false,
+          int BPW = 1>
+__global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
     const SweepGrid *__restrict__ grids, const int2 *__restrict__ items,
-    double *__restrict__ partial, int iter) {
+    double *__restrict__ partial, int iter, int nItems) {
   using S = Stor<ST>;
   using R = typename S::raw;
   using C = typename S::ct;
@@ -341,8 +352,15 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT)) void k_sweep(
   constexpr int M = BSX - 1;
   static_assert(LPT * 4 * NT == PS, "plane does not split into 4-wide chunks");
 
-  // work item: which grid, which page of it
-  const int2 item = items[blockIdx.x];
+  static_assert(BPW == 1 || BPW * NT <= 64, "packed blocks must share one wave");
+  // work item: which grid, which page of it. A group past the end of the
+  // list works on the last item without storing anything, so that it still
+  // reaches every barrier.
+  const int sub = BPW > 1 ? (int)threadIdx.x / NT : 0;
+  const int tid = BPW > 1 ? (int)threadIdx.x % NT : (int)threadIdx.x;
+  const int itemIdx = (int)blockIdx.x * BPW + sub;
+  const bool active = itemIdx < nItems;
+  const int2 item = items[active ? itemIdx : nItems - 1];
   const SweepGrid G = grids[item.x];
   const int32_t *__restrict__ nbrTab = G.nbr;
   uint8_t *pagesRaw = G.pages;
@@ -352,11 +370,12 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT)) void k_sweep(
   const float dtf = G.dt, lo01 = G.lo, hi01 = G.hi;
   (void)iter;
 
-  __shared__ C buf[2][P * P];
-  __shared__ int codes[27];
+  __shared__ C bufs[BPW][2][P * P];
+  __shared__ int codesAll[BPW][27];
   __shared__ double red[(NT + 63) / 64];
+  C(&buf)[2][P * P] = bufs[sub];
+  int *codes = codesAll[sub];
 
-  const int tid = threadIdx.x;
   const int page = item.y;
   // 27 codes; small blocks have fewer than 27 threads
   for (int k = tid; k < 27; k += NT) codes[k] = ldg(nbrTab + (int64_t)page * 27 + k);
@@ -543,6 +562,7 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT)) void k_sweep(
   const bool wFace0 = x0 == 0, wFace1 = x0 == BSX - PX, wY0 = y0 == 0, wY1 = y0 == BSX - PY;
 
   auto compute = [&](int z, const C (&Dp)[PN], const C (&Mp)[PN], const C (&Up)[PN]) {
+    if (BPW > 1 && !active) return;
     C o[PX * PY];
     if constexpr (OP == OP_MCF) {
       if constexpr (EXACT || HARD || sizeof(C) == 8) {
@@ -686,12 +706,16 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT)) void k_sweep(
     // deterministic block reduction: wave shuffles, then waves in order
     constexpr int WV = NT < 64 ? NT : 64;
     for (int off = WV / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, WV);
-    if ((tid & 63) == 0) red[tid >> 6] = acc;
-    __syncthreads();
-    if (tid == 0) {
-      double sum = 0.0;
-      for (int w = 0; w < (NT + 63) / 64; w++) sum += red[w];
-      partial[blockIdx.x] = sum;
+    if constexpr (BPW > 1) {
+      if (tid == 0 && active) partial[itemIdx] = acc;  // one group per item
+    } else {
+      if ((tid & 63) == 0) red[tid >> 6] = acc;
+      __syncthreads();
+      if (tid == 0) {
+        double sum = 0.0;
+        for (int w = 0; w < (NT + 63) / 64; w++) sum += red[w];
+        partial[itemIdx] = sum;
+      }
     }
   }
 }
