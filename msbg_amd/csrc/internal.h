@@ -55,6 +55,23 @@ template <typename T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;  // owns its allocation: moved, never copied
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      free();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
   int alloc(size_t count) {
     free();
     if (count == 0) return 0;
@@ -142,6 +159,20 @@ struct Channel {
   ChanLevel lv[kMaxLevels];
 };
 
+// Cached set-up of a coarse-level sweep of one (channel, level, halo rule):
+// the working pool (the level's pages followed by ghost pages), its map and
+// halo table, the ghost list and the colour-sorted items. Valid while the
+// grid's topology version and the block list are those of `key`.
+struct LevelWork {
+  uint64_t key = 0;
+  ChanLevel wl;
+  DevBuf<int32_t> d_ghost;
+  int64_t nghost = 0;
+  DevBuf<int2> d_items;
+  int64_t off[9] = {0};
+  DevBuf<SweepGrid> d_desc;
+};
+
 }  // namespace msbg
 
 struct msbg_grid {
@@ -172,6 +203,7 @@ struct msbg_grid {
   msbg::DevBuf<int2> d_sweep_list;      // colour-sorted {grid 0, page} items
   msbg::DevBuf<msbg::SweepGrid> d_desc; // this grid's sweep descriptor
   int tile[8] = {-1, -1, -1, -1, -1, -1, -1, -1};  // voxel-column variant per bsx log2 (-1 = default)
+  std::map<uint64_t, msbg::LevelWork> level_work;  // coarse-level sweep set-ups
   msbg::DevBuf<double> d_partial;       // per-block residual partials
   msbg::DevBuf<double> d_resid;         // per-sweep residuals
   uint64_t topo_version = 1;             // bumped when block maps or levels change

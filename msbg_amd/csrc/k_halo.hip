@@ -250,7 +250,7 @@ GhostWeights ghost_weights() {
 }
 
 // Ghost pages of a coarse-level sweep: page gp (block bids[i]) of level L
-// gets, in every voxel, the value getOutOfBlockValue gives an in-domain
+// gets, in every voxel of its outer shell, the value getOutOfBlockValue gives an in-domain
 // voxel of a NULL block with OPT_BC_COARSE_LEVEL [| OPT_INTERPOLATE]
 // (halo.h:288-352): interpolateGhost of level Lo at ((x+.5)/2, ...) or its
 // voxel (x/2, y/2, z/2). Stored raw in the channel's storage, so the sweep
@@ -267,8 +267,12 @@ __global__ void k_fill_ghost_pages(LevelView L, LevelView Lo, unsigned opt,
   const int bid = bids[i];
   const int bx = bid % L.nbx, by = (bid / L.nbx) % L.nby, bz = bid / (L.nbx * L.nby);
   const int m = L.bsx - 1;
-  const int x = bx * L.bsx + (v & m), y = by * L.bsx + ((v >> L.lg) & m),
-            z = bz * L.bsx + (v >> (2 * L.lg));
+  const int vx = v & m, vy = (v >> L.lg) & m, vz = v >> (2 * L.lg);
+  // only the page's one-voxel shell is ever read: neighbour rows and planes,
+  // the side-car faces and edges built from it, and out-of-domain cells
+  // clipped next to the swept block
+  if (vx > 0 && vx < m && vy > 0 && vy < m && vz > 0 && vz < m) return;
+  const int x = bx * L.bsx + vx, y = by * L.bsx + vy, z = bz * L.bsx + vz;
   double raw;
   if (opt & MSBG_OPT_INTERPOLATE) {
     raw = interpolate_ghost_raw(Lo, ((float)x + 0.5f) / 2, ((float)y + 0.5f) / 2,

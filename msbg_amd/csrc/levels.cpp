@@ -58,36 +58,105 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
                "OPT_BC_COARSE_LEVEL: FP64 channels sweep with OPT_BC_NEUMANN only");
   }
   const int nMaxIter = numIter + (numIter & 1);  // msbg4.cpp:150-152
+  const size_t pageBytes = (size_t)cl.bsx * cl.bsx * cl.bsx * storage_bytes(ch.storage);
 
-  // swept blocks: the list's (or every) value block at this level, by colour
-  std::vector<int32_t> per[8];
-  std::vector<int32_t> swept;
-  auto add = [&](int64_t bid) -> int {
-    MSBG_CHECK(bid >= 0 && bid < g->nblocks, MSBG_EINVAL, "block id %lld out of range",
-               (long long)bid);
-    if (g->h_level[bid] != level) return 0;
-    int32_t pg = cl.h_page[bid];
-    MSBG_CHECK(pg >= 0, MSBG_EINVAL, "block %lld is not a value block at level %d",
-               (long long)bid, level);
-    int bx = (int)(bid % g->nbx), by = (int)((bid / g->nbx) % g->nby),
-        bz = (int)(bid / ((int64_t)g->nbx * g->nby));
-    per[(bx & 1) + ((by & 1) << 1) + ((bz & 1) << 2)].push_back(pg);
-    swept.push_back((int32_t)bid);
-    return 0;
-  };
-  if (blockList) {
-    for (int64_t i = 0; i < nList; i++) MSBG_TRY(add(blockList[i]));
-  } else {
-    for (int64_t b = 0; b < g->nblocks; b++)
-      if (g->h_level[b] == level && cl.h_page[b] >= 0) MSBG_TRY(add(b));
+  // the set-up depends on the topology and the block list only: cached
+  uint64_t key = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
+  mix(g->topo_version);
+  mix(blockList ? (uint64_t)nList : ~0ull);
+  if (blockList)
+    for (int64_t i = 0; i < nList; i++) mix((uint32_t)blockList[i]);
+  const uint64_t slot = ((uint64_t)(uint32_t)chan << 8) | ((uint64_t)level << 4) |
+                        (coarse ? ((haloOpt & MSBG_OPT_INTERPOLATE) ? 2u : 1u) : 0u);
+  {
+    // a stale set-up is destroyed in place (its device buffers freed)
+    auto itw = g->level_work.find(slot);
+    if (itw != g->level_work.end() && itw->second.key != key) g->level_work.erase(itw);
   }
-  std::vector<int2> items;
-  int64_t off[9];
-  off[0] = 0;
-  for (int c = 0; c < 8; c++) {
-    for (int32_t pg : per[c]) items.push_back(make_int2(0, pg));
-    off[c + 1] = (int64_t)items.size();
+  LevelWork &W = g->level_work[slot];
+  ChanLevel &wl = W.wl;
+  if (W.key != key) {
+    // swept blocks: the list's (or every) value block at this level, by colour
+    std::vector<int32_t> per[8];
+    std::vector<int32_t> swept;
+    auto add = [&](int64_t bid) -> int {
+      MSBG_CHECK(bid >= 0 && bid < g->nblocks, MSBG_EINVAL, "block id %lld out of range",
+                 (long long)bid);
+      if (g->h_level[bid] != level) return 0;
+      int32_t pg = cl.h_page[bid];
+      MSBG_CHECK(pg >= 0, MSBG_EINVAL, "block %lld is not a value block at level %d",
+                 (long long)bid, level);
+      int bx = (int)(bid % g->nbx), by = (int)((bid / g->nbx) % g->nby),
+          bz = (int)(bid / ((int64_t)g->nbx * g->nby));
+      per[(bx & 1) + ((by & 1) << 1) + ((bz & 1) << 2)].push_back(pg);
+      swept.push_back((int32_t)bid);
+      return 0;
+    };
+    if (blockList) {
+      for (int64_t i = 0; i < nList; i++) MSBG_TRY(add(blockList[i]));
+    } else {
+      for (int64_t b = 0; b < g->nblocks; b++)
+        if (g->h_level[b] == level && cl.h_page[b] >= 0) MSBG_TRY(add(b));
+    }
+    std::vector<int2> items;
+    W.off[0] = 0;
+    for (int c = 0; c < 8; c++) {
+      for (int32_t pg : per[c]) items.push_back(make_int2(0, pg));
+      W.off[c + 1] = (int64_t)items.size();
+    }
+    // working map: NULL blocks next to a swept block become ghost pages
+    wl.defined = true;
+    wl.bsx = cl.bsx;
+    wl.empty_raw = cl.empty_raw;
+    wl.full_raw = cl.full_raw;
+    wl.h_page = cl.h_page;
+    std::vector<int32_t> ghostBids;
+    if (coarse) {
+      for (int32_t bid : swept) {
+        const int bx = bid % g->nbx, by = (bid / g->nbx) % g->nby,
+                  bz = (int)(bid / ((int64_t)g->nbx * g->nby));
+        for (int dz = -1; dz <= 1; dz++)
+          for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+              const int x = bx + dx, y = by + dy, z = bz + dz;
+              if (x < 0 || y < 0 || z < 0 || x >= g->nbx || y >= g->nby || z >= g->nbz)
+                continue;
+              const int64_t nb = ((int64_t)z * g->nby + y) * g->nbx + x;
+              if (wl.h_page[nb] != MSBG_BLOCK_NULL) continue;
+              wl.h_page[nb] = (int32_t)(cl.npages + (int64_t)ghostBids.size());
+              ghostBids.push_back((int32_t)nb);
+            }
+      }
+    }
+    W.nghost = (int64_t)ghostBids.size();
+    wl.npages = cl.npages + W.nghost;
+    if (W.off[8]) {
+      MSBG_TRY(wl.d_pages.alloc((size_t)wl.npages * pageBytes));
+      if (W.nghost) {
+        MSBG_TRY(W.d_ghost.alloc((size_t)W.nghost));
+        MSBG_HIP(hipMemcpyAsync(W.d_ghost.p, ghostBids.data(), sizeof(int32_t) * W.nghost,
+                                hipMemcpyHostToDevice, g->stream));
+      }
+      wl.h_page_bid.assign((size_t)wl.npages, -1);
+      for (int64_t b = 0; b < g->nblocks; b++)
+        if (wl.h_page[b] >= 0) wl.h_page_bid[wl.h_page[b]] = (int32_t)b;
+      MSBG_TRY(wl.d_page.alloc((size_t)g->nblocks));
+      MSBG_TRY(wl.d_page_bid.alloc((size_t)wl.npages));
+      MSBG_HIP(hipMemcpyAsync(wl.d_page.p, wl.h_page.data(), sizeof(int32_t) * g->nblocks,
+                              hipMemcpyHostToDevice, g->stream));
+      MSBG_HIP(hipMemcpyAsync(wl.d_page_bid.p, wl.h_page_bid.data(),
+                              sizeof(int32_t) * wl.npages, hipMemcpyHostToDevice, g->stream));
+      MSBG_TRY(wl.d_nbr.alloc((size_t)wl.npages * 27));
+      MSBG_TRY(launch_build_nbr(g, wl, level));
+      MSBG_TRY(W.d_items.alloc((size_t)W.off[8]));
+      MSBG_HIP(hipMemcpyAsync(W.d_items.p, items.data(), sizeof(int2) * W.off[8],
+                              hipMemcpyHostToDevice, g->stream));
+      MSBG_HIP(hipStreamSynchronize(g->stream));  // the host vectors go out of scope
+    }
+    W.key = key;
   }
+  const int64_t *off = W.off;
   const int64_t ntot = off[8];
   if (ntot == 0 || nMaxIter == 0) {
     if (residual)
@@ -95,56 +164,15 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
     return 0;
   }
 
-  // working map: NULL blocks next to a swept block become ghost pages
-  ChanLevel wl;
-  wl.defined = true;
-  wl.bsx = cl.bsx;
-  wl.empty_raw = cl.empty_raw;
-  wl.full_raw = cl.full_raw;
-  wl.h_page = cl.h_page;
-  std::vector<int32_t> ghostBids;
-  if (coarse) {
-    for (int32_t bid : swept) {
-      const int bx = bid % g->nbx, by = (bid / g->nbx) % g->nby,
-                bz = (int)(bid / ((int64_t)g->nbx * g->nby));
-      for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-          for (int dx = -1; dx <= 1; dx++) {
-            const int x = bx + dx, y = by + dy, z = bz + dz;
-            if (x < 0 || y < 0 || z < 0 || x >= g->nbx || y >= g->nby || z >= g->nbz) continue;
-            const int64_t nb = ((int64_t)z * g->nby + y) * g->nbx + x;
-            if (wl.h_page[nb] != MSBG_BLOCK_NULL) continue;
-            wl.h_page[nb] = (int32_t)(cl.npages + (int64_t)ghostBids.size());
-            ghostBids.push_back((int32_t)nb);
-          }
-    }
-  }
-  const int64_t nghost = (int64_t)ghostBids.size();
-  wl.npages = cl.npages + nghost;
-  const size_t pageBytes = (size_t)cl.bsx * cl.bsx * cl.bsx * storage_bytes(ch.storage);
-  MSBG_TRY(wl.d_pages.alloc((size_t)wl.npages * pageBytes));
+  // per call: the level's current pages, ghosts from the current level l+1,
+  // side-cars of both
   MSBG_HIP(hipMemcpyAsync(wl.d_pages.p, cl.d_pages.p, (size_t)cl.npages * pageBytes,
                           hipMemcpyDeviceToDevice, g->stream));
-  DevBuf<int32_t> d_ghost;
-  if (nghost) {
-    MSBG_TRY(d_ghost.alloc((size_t)nghost));
-    MSBG_HIP(hipMemcpyAsync(d_ghost.p, ghostBids.data(), sizeof(int32_t) * nghost,
-                            hipMemcpyHostToDevice, g->stream));
-    MSBG_TRY(launch_fill_ghost_pages(g, ch, level, haloOpt, d_ghost.p, nghost, wl.d_pages.p,
-                                     cl.npages));
-  }
-  wl.h_page_bid.assign((size_t)wl.npages, -1);
-  for (int64_t b = 0; b < g->nblocks; b++)
-    if (wl.h_page[b] >= 0) wl.h_page_bid[wl.h_page[b]] = (int32_t)b;
-  MSBG_TRY(wl.d_page.alloc((size_t)g->nblocks));
-  MSBG_TRY(wl.d_page_bid.alloc((size_t)wl.npages));
-  MSBG_HIP(hipMemcpyAsync(wl.d_page.p, wl.h_page.data(), sizeof(int32_t) * g->nblocks,
-                          hipMemcpyHostToDevice, g->stream));
-  MSBG_HIP(hipMemcpyAsync(wl.d_page_bid.p, wl.h_page_bid.data(), sizeof(int32_t) * wl.npages,
-                          hipMemcpyHostToDevice, g->stream));
-  MSBG_TRY(wl.d_nbr.alloc((size_t)wl.npages * 27));
-  MSBG_TRY(launch_build_nbr(g, wl, level));
-  MSBG_TRY(sweep_prepare_level(g, wl, ch.storage));  // side-cars of every page, incl. ghosts
+  if (W.nghost)
+    MSBG_TRY(launch_fill_ghost_pages(g, ch, level, haloOpt, W.d_ghost.p, W.nghost,
+                                     wl.d_pages.p, cl.npages));
+  wl.side_valid = false;
+  MSBG_TRY(sweep_prepare_level(g, wl, ch.storage));
 
   SweepGrid desc{};
   desc.pages = wl.d_pages.p;
@@ -158,14 +186,9 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
   desc.outClip = coarse ? 1 : 0;
   desc.dst_pages = desc.pages;
   desc.dst_side = desc.side;
-  DevBuf<SweepGrid> d_desc;
-  DevBuf<int2> d_items;
   DevBuf<double> d_partial, d_resid;
-  MSBG_TRY(d_desc.alloc(1));
-  MSBG_TRY(d_items.alloc((size_t)ntot));
-  MSBG_HIP(hipMemcpyAsync(d_desc.p, &desc, sizeof desc, hipMemcpyHostToDevice, g->stream));
-  MSBG_HIP(hipMemcpyAsync(d_items.p, items.data(), sizeof(int2) * ntot, hipMemcpyHostToDevice,
-                          g->stream));
+  MSBG_TRY(W.d_desc.ensure(1));
+  MSBG_HIP(hipMemcpyAsync(W.d_desc.p, &desc, sizeof desc, hipMemcpyHostToDevice, g->stream));
   if (residual) {
     MSBG_TRY(d_partial.alloc((size_t)ntot));
     MSBG_TRY(d_resid.alloc((size_t)nMaxIter));
@@ -177,8 +200,8 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
     for (int pass = 0; pass < 8; pass++) {
       const int64_t n = off[pass + 1] - off[pass];
       if (!n) continue;
-      MSBG_TRY(launch_sweep_items(g->stream, cl.bsx, ch.storage, exact, -1, d_desc.p,
-                                  d_items.p + off[pass], n,
+      MSBG_TRY(launch_sweep_items(g->stream, cl.bsx, ch.storage, exact, -1, W.d_desc.p,
+                                  W.d_items.p + off[pass], n,
                                   residual ? d_partial.p + off[pass] : nullptr));
       npasses++;
     }

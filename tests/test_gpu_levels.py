@@ -146,3 +146,33 @@ def test_level_sweep_block_list_and_errors():
         with pytest.raises(M.MsbgError) as e:
             g.applyChannelPdeLevel(CH, 3, M.OPT_BC_NEUMANN, None, 2, 0.1, 1)
         assert e.value.code == M._lib.EINVAL
+
+
+def test_level_sweep_cached_setup_follows_changes():
+    """The set-up (ghost list, halo table) is cached across calls; ghost
+    values and pages are taken afresh each call, and a topology change
+    rebuilds it."""
+    S, bsx = 128, 32
+    note("levels cache")
+    g, levels, empty, full = _grid(S, bsx, M.F32, seed=12, ghosts=False)
+    with g:
+        g.set_arith(M.ARITH_EXACT)
+        opt = M.OPT_BC_COARSE_LEVEL | M.OPT_INTERPOLATE
+        b1 = np.nonzero(levels == 1)[0]
+        L = _olevels(g, S, bsx, M.F32, empty, full)
+        O.mcf_sweeps_level(L[1], L[2], b1, levels, 1, opt, 2, 0.1, 1)
+        g.applyChannelPdeLevel(CH, 1, opt, None, 2, 0.1, 1)
+        assert np.array_equal(g.download(CH, 1), L[1].pages)
+        # new coarse values: the second call must read them
+        new2 = np.random.default_rng(1).random(L[2].pages.shape).astype(np.float32)
+        g.upload(CH, 2, new2)
+        L[2].pages[...] = new2
+        O.mcf_sweeps_level(L[1], L[2], b1, levels, 1, opt, 2, 0.1, 1)
+        g.applyChannelPdeLevel(CH, 1, opt, None, 2, 0.1, 1)
+        assert np.array_equal(g.download(CH, 1), L[1].pages)
+        # a topology change (ghost pages of finer blocks appear) rebuilds it
+        g.prepareGhostBlocks(CH)
+        L = _olevels(g, S, bsx, M.F32, empty, full)
+        O.mcf_sweeps_level(L[1], L[2], b1, levels, 1, opt, 2, 0.1, 1)
+        g.applyChannelPdeLevel(CH, 1, opt, None, 2, 0.1, 1)
+        assert np.array_equal(g.download(CH, 1), L[1].pages)
