@@ -73,11 +73,10 @@ int render_impl(msbg_grid *g, int chan, int width, int height, const float camPo
     MSBG_TRY(d_cnt.alloc(4));
     MSBG_HIP(hipMemsetAsync(d_cnt.p, 0, 4 * sizeof(unsigned long long), g->stream));
   }
-  hipEvent_t e0, e1;
-  MSBG_HIP(hipEventCreate(&e0));
-  MSBG_HIP(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = 0;
-  if (hipEventRecord(e0, g->stream) != hipSuccess) rc = MSBG_EDEVICE;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = MSBG_EDEVICE;
+  if (!rc && hipEventRecord(e0, g->stream) != hipSuccess) rc = MSBG_EDEVICE;
   if (!rc) rc = launch_render(g->stream, V, storage, C, width, height, d_rgb.p, d_cnt.p);
   if (!rc && hipEventRecord(e1, g->stream) != hipSuccess) rc = MSBG_EDEVICE;
   if (!rc && hipMemcpyAsync(rgb, d_rgb.p, sizeof(uint32_t) * npix, hipMemcpyDeviceToHost,
@@ -95,8 +94,8 @@ int render_impl(msbg_grid *g, int chan, int width, int height, const float camPo
   }
   if (!rc && counts)
     for (int k = 0; k < 4; k++) counts[k] = (int64_t)hc[k];
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   if (rc == MSBG_EDEVICE) set_error("msbg_render: device error");
   return rc;
 }
