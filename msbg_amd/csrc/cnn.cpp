@@ -149,12 +149,17 @@ int forward(msbg_unet *u, int n) {
   MSBG_TRY(swizzle_all(u));
   for (int l = 0; l < 10; l++) {
     const LayerSpec &L = kL[l];
-    MSBG_TRY(launch_conv(s, conv_input(u, l), n, cin_k(l), L.cout, L.dout, L.pad, u->wf[l].p,
-                         u->p.p + u->offB[l], true, u->a[l].p));
+    if (l == 6)  // decoder 2 reads [upsample(a5), a3] in place
+      MSBG_TRY(launch_conv_cat(s, u->a[5].p, 64, u->a[3].p, 32, n, L.cout, L.dout, u->wf[l].p,
+                               u->p.p + u->offB[l], true, u->a[l].p));
+    else if (l == 8)  // decoder 1 reads [upsample(a7), a1] in place
+      MSBG_TRY(launch_conv_cat(s, u->a[7].p, 32, u->a[1].p, 16, n, L.cout, L.dout, u->wf[l].p,
+                               u->p.p + u->offB[l], true, u->a[l].p));
+    else
+      MSBG_TRY(launch_conv(s, conv_input(u, l), n, cin_k(l), L.cout, L.dout, L.pad, u->wf[l].p,
+                           u->p.p + u->offB[l], true, u->a[l].p));
     if (l == 1) MSBG_TRY(launch_pool2(s, u->a[1].p, n, 32, 16, u->p1.p));
     if (l == 3) MSBG_TRY(launch_pool2(s, u->a[3].p, n, 16, 32, u->p2.p));
-    if (l == 5) MSBG_TRY(launch_upcat(s, u->a[5].p, 64, u->a[3].p, 32, n, 16, u->c7.p));
-    if (l == 7) MSBG_TRY(launch_upcat(s, u->a[7].p, 32, u->a[1].p, 16, n, 32, u->c9.p));
   }
   return launch_head(s, u->a[9].p, n, u->p.p + u->offH, u->y.p);
 }
@@ -170,6 +175,10 @@ int backward(msbg_unet *u, int n) {
     uint16_t *gout = u->ga[l].p;  // dL/d a[l], becomes dL/d z[l] in place
     MSBG_TRY(launch_relu_bwd_xpose(s, gout, u->a[l].p, n, vo, L.cout, true, u->dzt.p));
     MSBG_TRY(launch_rowsum(s, u->dzt.p, L.cout, vo * n, u->rsum.p, u->g.p + u->offB[l]));
+    // the weight gradient of a decoder conv needs its concatenated input:
+    // built here, where only training needs it
+    if (l == 8) MSBG_TRY(launch_upcat(s, u->a[7].p, 32, u->a[1].p, 16, n, 32, u->c9.p));
+    if (l == 6) MSBG_TRY(launch_upcat(s, u->a[5].p, 64, u->a[3].p, 32, n, 16, u->c7.p));
     MSBG_TRY(launch_xpose_pad(s, conv_input(u, l), n, L.din, cin_k(l), L.pad, cin16(l),
                               u->xtp.p));
     MSBG_TRY(launch_wgrad(s, u->dzt.p, u->xtp.p, n, cin16(l), L.cout, L.dout, u->wpart.p));

@@ -23,6 +23,10 @@ int launch_swizzle(hipStream_t s, const float *w, int cin_true, int cin, int cou
 // its own kernel; other cin are multiples of 8.
 int launch_conv(hipStream_t s, const uint16_t *x, int n, int cin, int cout, int dout, int pad,
                 const uint4 *wsw, const float *bias, bool relu, uint16_t *y);
+// the decoder's conv over [upsample(a), b] without materialising the concat
+int launch_conv_cat(hipStream_t s, const uint16_t *a, int ca, const uint16_t *b, int cb, int n,
+                    int cout, int dout, const uint4 *wsw, const float *bias, bool relu,
+                    uint16_t *y);
 // dw partials for the conv x(din=dout+2-2pad) -> dz(dout): x_tp is x transposed and
 // zero padded to [cin16][dout+2]^3[n], dz_t is [cout][dout^3][n]; n % 8 == 0
 int wgrad_chunks(int cin, int cout, int dout, int n);

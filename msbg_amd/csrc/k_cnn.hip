@@ -80,12 +80,18 @@ struct ConvShape {
   static_assert(NTILE % 4 == 0, "voxel tiles per wave");
 };
 
-template <int CIN, int COUT, int DOUT, int PAD>
+// CA > 0: the input is the decoder's concatenation, read in place: its
+// first CA channels are the coarser activation x (edge DIN/2, nearest
+// upsampling), the other CIN - CA are the skip activation xb (edge DIN). A
+// channel chunk never straddles the two.
+template <int CIN, int COUT, int DOUT, int PAD, int CA = 0>
 __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ x,
+                                              const uint16_t *__restrict__ xb,
                                               const uint4 *__restrict__ wsw,
                                               const float *__restrict__ bias, int relu,
                                               uint16_t *__restrict__ y) {
   using K = ConvShape<CIN, COUT, DOUT, PAD>;
+  static_assert(CA % K::CC == 0 && CA < CIN, "concat boundary inside a channel chunk");
   constexpr int DIN = K::DIN, TZ = K::TZ, TY = K::TY, XP = K::XP, CC = K::CC, TM = K::TM,
                 TN = K::TN, SY = K::SY, SX = K::SX, VS8 = K::VS8;
   __shared__ uint4 slab[K::SLAB];
@@ -119,8 +125,21 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ x,
       const int sx = sv % SX, sy = (sv / SX) % SY, sz = sv / (SX * SY);
       const int iz = z0 - PAD + sz, iy = y0 - PAD + sy, ix = sx - PAD;
       uint4 v = make_uint4(0, 0, 0, 0);
-      if (iz >= 0 && iz < DIN && iy >= 0 && iy < DIN && ix >= 0 && ix < DIN)
-        v = *(const uint4 *)(xs + (((int64_t)iz * DIN + iy) * DIN + ix) * CIN + ch * CC + c8 * 8);
+      if (iz >= 0 && iz < DIN && iy >= 0 && iy < DIN && ix >= 0 && ix < DIN) {
+        if constexpr (CA == 0) {
+          v = *(const uint4 *)(xs + (((int64_t)iz * DIN + iy) * DIN + ix) * CIN + ch * CC +
+                               c8 * 8);
+        } else if (ch * CC < CA) {  // upsampled coarse channels
+          constexpr int DH = DIN / 2;
+          const uint16_t *xa = x + (int64_t)n * DH * DH * DH * CA;
+          v = *(const uint4 *)(xa + (((int64_t)(iz >> 1) * DH + (iy >> 1)) * DH + (ix >> 1)) * CA +
+                               ch * CC + c8 * 8);
+        } else {  // skip channels
+          const uint16_t *xk = xb + (int64_t)n * DIN * DIN * DIN * (CIN - CA);
+          v = *(const uint4 *)(xk + (((int64_t)iz * DIN + iy) * DIN + ix) * (CIN - CA) +
+                               ch * CC - CA + c8 * 8);
+        }
+      }
       slab[sv * VS8 + c8] = v;
     }
     __syncthreads();
@@ -643,12 +662,12 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
 
 inline unsigned blocks_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
-template <int CIN, int COUT, int DOUT, int PAD>
+template <int CIN, int COUT, int DOUT, int PAD, int CA = 0>
 int conv_t(hipStream_t s, const uint16_t *x, int n, const uint4 *wsw, const float *bias, bool relu,
-           uint16_t *y) {
+           uint16_t *y, const uint16_t *xb = nullptr) {
   using K = ConvShape<CIN, COUT, DOUT, PAD>;
   dim3 grid((DOUT / K::TZ) * (DOUT / K::TY), (unsigned)n);
-  hipLaunchKernelGGL((k_conv<CIN, COUT, DOUT, PAD>), grid, dim3(256), 0, s, x, wsw, bias,
+  hipLaunchKernelGGL((k_conv<CIN, COUT, DOUT, PAD, CA>), grid, dim3(256), 0, s, x, xb, wsw, bias,
                      relu ? 1 : 0, y);
   MSBG_HIP(hipGetLastError());
   return 0;
@@ -725,6 +744,18 @@ int launch_conv(hipStream_t s, const uint16_t *x, int n, int cin, int cout, int 
   MSBG_CONV_CASE(16, 16, 34, 2)
 #undef MSBG_CONV_CASE
   set_error("conv: no kernel for cin %d cout %d dout %d pad %d", cin, cout, dout, pad);
+  return MSBG_EUNSUPP;
+}
+
+int launch_conv_cat(hipStream_t s, const uint16_t *a, int ca, const uint16_t *b, int cb, int n,
+                    int cout, int dout, const uint4 *wsw, const float *bias, bool relu,
+                    uint16_t *y) {
+  MSBG_CHECK(n >= 1 && n <= 65535, MSBG_EINVAL, "conv: batch %d outside [1, 65535]", n);
+  if (ca == 64 && cb == 32 && cout == 32 && dout == 16)
+    return conv_t<96, 32, 16, 1, 64>(s, a, n, wsw, bias, relu, y, b);
+  if (ca == 32 && cb == 16 && cout == 16 && dout == 32)
+    return conv_t<48, 16, 32, 1, 32>(s, a, n, wsw, bias, relu, y, b);
+  set_error("conv: no concatenating kernel for %d+%d -> %d at %d^3", ca, cb, cout, dout);
   return MSBG_EUNSUPP;
 }
 
