@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "internal.h"
 #include "storage.h"
@@ -211,6 +212,132 @@ __global__ void k_ghost(LevelView L, LevelView Lo, GhostWeights W, const int32_t
     ((uint16_t *)loPages)[idx] = Stor<ST>::enc(val);
 }
 
+// The same downsample with the halo staged once in LDS: one workgroup per
+// fine-coarse block. Each fine voxel feeds eight coarse voxels, so the
+// per-tap version above evaluates the halo rule 8x per voxel; here the
+// (bsx+2)^3 halo is evaluated once, in z-chunks of 2*CZ+2 planes (the
+// windows of CZ coarse planes), as float. The taps and their summation order
+// are those of k_ghost, so the results are identical.
+//
+// Staging resolves halo_value(..., MSBG_OPT_BC_COARSE_LEVEL) from two
+// 27-entry tables of the block's neighbour codes at this level and the
+// coarser one (the levels share one block grid), so every voxel becomes a
+// constant or one page load; the loads of kGhostBatch voxels are issued
+// together.
+constexpr int kGhostThreads = 512, kGhostBatch = 12;
+
+template <int ST, int BSX>
+__global__ __launch_bounds__(kGhostThreads) void k_ghost_tile(LevelView L, LevelView Lo,
+                                                              GhostWeights W,
+                                                              const int32_t *__restrict__ bids,
+                                                              uint8_t *__restrict__ loPages) {
+#pragma clang fp contract(off)
+  using R = typename Stor<ST>::raw;
+  constexpr int H = BSX + 2, PL = H * H, BL = BSX / 2, CZ = BL < 4 ? BL : 4, NP = 2 * CZ + 2;
+  constexpr int M = BSX - 1, LG = BSX == 4 ? 2 : (BSX == 8 ? 3 : (BSX == 16 ? 4 : 5));
+  constexpr int ML = BL - 1, LGL = LG - 1;
+  __shared__ float tile[NP * PL];
+  __shared__ int32_t nbL[27], nbLo[27];
+  const int bid = bids[blockIdx.x];
+  const int bx = bid % L.nbx, by = (bid / L.nbx) % L.nby, bz = bid / (L.nbx * L.nby);
+  if (threadIdx.x < 27) {
+    const int k = threadIdx.x, x2 = bx + k % 3 - 1, y2 = by + (k / 3) % 3 - 1, z2 = bz + k / 9 - 1;
+    const bool in = x2 >= 0 && x2 < L.nbx && y2 >= 0 && y2 < L.nby && z2 >= 0 && z2 < L.nbz;
+    const int64_t b2 = ((int64_t)z2 * L.nby + y2) * L.nbx + x2;
+    nbL[k] = in ? L.page[b2] : MSBG_BLOCK_OUT;
+    nbLo[k] = in ? Lo.page[b2] : MSBG_BLOCK_OUT;
+  }
+  const int pg = Lo.page[bid];
+  const R *Lp = (const R *)L.pages, *Lop = (const R *)Lo.pages;
+  const float lEmpty = (float)L.empty_raw, lFull = (float)L.full_raw;
+  const float loEmpty = (float)Lo.empty_raw, loFull = (float)Lo.full_raw;
+  // the block-local voxel (x, y, z), each in [-1, BSX]: a page element or a
+  // constant, as halo_value resolves it
+  auto source = [&](int x, int y, int z, float &cv) -> const R * {
+    int dx = x < 0 ? -1 : (x > M ? 1 : 0), dy = y < 0 ? -1 : (y > M ? 1 : 0),
+        dz = z < 0 ? -1 : (z > M ? 1 : 0);
+    int c = nbL[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1];
+    if (c == MSBG_BLOCK_OUT) {
+      // outside the domain: clip to it (the axes that leave it land on the
+      // block's own face)
+      if (dx && nbL[13 + dx] == MSBG_BLOCK_OUT) { x = x < 0 ? 0 : M; dx = 0; }
+      if (dy && nbL[13 + 3 * dy] == MSBG_BLOCK_OUT) { y = y < 0 ? 0 : M; dy = 0; }
+      if (dz && nbL[13 + 9 * dz] == MSBG_BLOCK_OUT) { z = z < 0 ? 0 : M; dz = 0; }
+      c = nbL[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1];
+    }
+    const int k = (dz + 1) * 9 + (dy + 1) * 3 + dx + 1;
+    if (c >= 0)
+      return Lp + (int64_t)c * (BSX * BSX * BSX) + ((((z & M) << LG) + (y & M)) << LG) + (x & M);
+    if (c != MSBG_BLOCK_NULL) {
+      cv = c == MSBG_BLOCK_EMPTY ? lEmpty : lFull;
+      return nullptr;
+    }
+    // NULL block: the coarser level's voxel (x/2, y/2, z/2) of the same
+    // block (getValue: a NULL coarse block reads its empty value)
+    const int c2 = nbLo[k];
+    const int gx = bx * BSX + x, gy = by * BSX + y, gz = bz * BSX + z;
+    const int xl = gx / 2, yl = gy / 2, zl = gz / 2;
+    if (c2 >= 0)
+      return Lop + (int64_t)c2 * (BL * BL * BL) + ((((zl & ML) << LGL) + (yl & ML)) << LGL) +
+             (xl & ML);
+    cv = c2 == MSBG_BLOCK_FULL ? loFull : loEmpty;
+    return nullptr;
+  };
+  __syncthreads();
+  for (int vz0 = 0; vz0 < BL; vz0 += CZ) {
+    for (int e0 = threadIdx.x; e0 < NP * PL; e0 += kGhostBatch * kGhostThreads) {
+      const R *src[kGhostBatch];
+      float val[kGhostBatch];
+#pragma unroll
+      for (int u = 0; u < kGhostBatch; u++) {
+        const int e = e0 + u * kGhostThreads;
+        src[u] = nullptr;
+        val[u] = 0.0f;
+        if (e < NP * PL) {
+          const int hzl = e / PL, r = e - hzl * PL, hy = r / H, hx = r - hy * H;
+          src[u] = source(hx - 1, hy - 1, 2 * vz0 + hzl - 1, val[u]);
+        }
+      }
+      R raw[kGhostBatch];
+#pragma unroll
+      for (int u = 0; u < kGhostBatch; u++)
+        if (src[u]) raw[u] = *src[u];
+#pragma unroll
+      for (int u = 0; u < kGhostBatch; u++) {
+        const int e = e0 + u * kGhostThreads;
+        if (e < NP * PL) tile[e] = src[u] ? (float)Stor<ST>::dec(raw[u]) : val[u];
+      }
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < CZ * BL * BL; o += kGhostThreads) {
+      const int vx = o % BL, vy = (o / BL) % BL, vzl = o / (BL * BL);
+      const float *t0 = tile + (2 * vzl) * PL + (2 * vy) * H + 2 * vx;
+      float S[4][4];
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int l = 0; l < 4; l++) S[j][l] = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+          for (int l = 0; l < 4; l++)
+            S[j][l] = S[j][l] + W.w[16 * k + 4 * j + l] * t0[k * PL + j * H + l];
+      float T[4];
+#pragma unroll
+      for (int l = 0; l < 4; l++) T[l] = ((S[0][l] + S[1][l]) + S[2][l]) + S[3][l];
+      const float v = (T[0] + T[2]) + (T[1] + T[3]);
+      const int64_t idx = (int64_t)pg * (BL * BL * BL) + ((vz0 + vzl) * BL + vy) * BL + vx;
+      if constexpr (ST == MSBG_STORAGE_F32)
+        ((float *)loPages)[idx] = v;
+      else
+        ((uint16_t *)loPages)[idx] = Stor<ST>::enc(v);
+    }
+    __syncthreads();
+  }
+}
+
 LevelView make_view(msbg_grid *g, Channel &ch, int level) {
   LevelView v{};
   if (level >= g->nlevels || !ch.lv[level].defined) return v;
@@ -339,6 +466,32 @@ int launch_ghost(msbg_grid *g, Channel &ch, int level, const int32_t *d_bids, in
   LevelView L = make_view(g, ch, level);
   LevelView Lo = make_view(g, ch, level + 1);
   GhostWeights W = ghost_weights();
+  if (n <= 0) return 0;
+  if (L.bsx >= 4 && L.bsx <= 32) {
+    uint8_t *lp = ch.lv[level + 1].d_pages.p;
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3((unsigned)n), dim3(kGhostThreads), 0, g->stream, L, Lo, W, d_bids,
+                         lp);
+    };
+    auto by_size = [&](auto st) {
+      constexpr int ST = decltype(st)::value;
+      switch (L.bsx) {
+        case 4: go(k_ghost_tile<ST, 4>); break;
+        case 8: go(k_ghost_tile<ST, 8>); break;
+        case 16: go(k_ghost_tile<ST, 16>); break;
+        default: go(k_ghost_tile<ST, 32>); break;
+      }
+    };
+    switch (ch.storage) {
+      case MSBG_STORAGE_F32: by_size(std::integral_constant<int, MSBG_STORAGE_F32>{}); break;
+      case MSBG_STORAGE_F16: by_size(std::integral_constant<int, MSBG_STORAGE_F16>{}); break;
+      case MSBG_STORAGE_BF16: by_size(std::integral_constant<int, MSBG_STORAGE_BF16>{}); break;
+      default: set_error("ghost: unsupported storage %d", ch.storage); return MSBG_EUNSUPP;
+    }
+    MSBG_HIP(hipGetLastError());
+    return 0;
+  }
+  // 2^3 and 64^3 blocks: the per-tap kernel
   int bl = L.bsx / 2;
   int nv = bl * bl * bl;
   for (int64_t off = 0; off < n; off += 65535) {

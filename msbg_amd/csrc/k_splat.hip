@@ -1,14 +1,17 @@
 // Particle splat and finalize kernels (msbg_demo.cpp:541-680).
 //
 // The reference splats colour by colour with one task per home block so
-// that plain read-modify-writes never race (msbg_demo.cpp:541-636). Here one
-// thread owns one particle and the min goes through atomics, which makes the
-// colour order unnecessary: min is order-independent, and the block
-// value-ness that decides the early abandon does not change while splatting.
+// that plain read-modify-writes never race (msbg_demo.cpp:541-636). Here the
+// particles are binned into cells and each cell's footprints are min'ed into
+// an LDS tile, merged into the pages colour by colour (k_splat_tile). Wider
+// footprints take k_splat: one thread per particle with atomics. Both give
+// the reference's result: min is order-independent, and the block value-ness
+// that decides the early abandon does not change while splatting.
 //
 // Reference paths are relative to the reference's src/ directory.
 
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include "internal.h"
 #include "storage.h"
@@ -88,6 +91,187 @@ __global__ void k_splat(const float *__restrict__ xyz, int64_t n, float rScan, f
   }
 }
 
+// ---- binned splat: LDS tiles, scattered colour by colour ------------------
+// Particles are binned into cells of kCell^3 voxels (counting sort). A
+// workgroup takes one cell: every footprint of its particles lies in the
+// cell grown by kHalo voxels per side (a kTile^3 tile of u32 codes in LDS),
+// so the particles min their codes into LDS with ds_min_u32, one wave per
+// particle and one lane per footprint voxel. The tile is then merged into
+// the pages with plain loads and stores: the cells of one colour (cell
+// coordinate parities) are 2 kCell apart and kCell >= 2 kHalo, so their
+// tiles are disjoint, and the 8 colours run as 8 launches in stream order.
+// The particle's walk in z,y,x order with its early abandon
+// (msbg_demo.cpp:615-632) becomes a ballot over the footprint in the same
+// order: lanes before the first in-range voxel of a non-value block apply,
+// and the particle ends there.
+constexpr int kCell = 16, kHalo = 5, kTile = kCell + 2 * kHalo, kTileVox = kTile * kTile * kTile;
+constexpr int kTileBlk = 125;  // blocks a tile can touch: (kTile/8 + 1)^3 at 8^3 blocks
+constexpr uint32_t kNoCode = 0xffffu;
+
+struct CellGrid {
+  int ncx, ncy, ncz, sx, sy, sz;
+};
+
+__device__ inline int cell_of(const CellGrid &G, float x, float y, float z) {
+  auto ax = [](float p, int s, int n) {
+    float f = __builtin_floorf(p);
+    f = __builtin_fminf(__builtin_fmaxf(f, 0.0f), (float)(s - 1));  // NaN -> 0
+    int c = (int)f / kCell;
+    return c < n ? c : n - 1;
+  };
+  return (ax(z, G.sz, G.ncz) * G.ncy + ax(y, G.sy, G.ncy)) * G.ncx + ax(x, G.sx, G.ncx);
+}
+
+__global__ void k_cell_count(const float *__restrict__ xyz, int64_t n, CellGrid G,
+                             uint32_t *__restrict__ cnt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  atomicAdd(&cnt[cell_of(G, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2])], 1u);
+}
+
+__global__ void k_cell_scatter(const float *__restrict__ xyz, int64_t n, CellGrid G,
+                               const uint32_t *__restrict__ off, uint32_t *__restrict__ cursor,
+                               float *__restrict__ sorted) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+  const int c = cell_of(G, x, y, z);
+  const uint32_t at = off[c] + atomicAdd(&cursor[c], 1u);
+  sorted[3 * (int64_t)at] = x;
+  sorted[3 * (int64_t)at + 1] = y;
+  sorted[3 * (int64_t)at + 2] = z;
+}
+
+// selects the non-empty cells of one colour (cell coordinate parities)
+struct CellOfColour {
+  const uint32_t *cnt;
+  CellGrid G;
+  int col;
+  __device__ bool operator()(int c) const {
+    const int cx = c % G.ncx, cy = (c / G.ncx) % G.ncy, cz = c / (G.ncx * G.ncy);
+    return ((cx & 1) | ((cy & 1) << 1) | ((cz & 1) << 2)) == col && cnt[c] != 0;
+  }
+};
+
+constexpr int kSplatThreads = 512, kMergeBatch = 12;
+
+template <int ESZ>
+__global__ __launch_bounds__(kSplatThreads) void k_splat_tile(
+    const float *__restrict__ sorted, const uint32_t *__restrict__ off,
+    const uint32_t *__restrict__ cnt, const int32_t *__restrict__ list, CellGrid G, float rScan,
+    float distSqMax, float distSqMaxInv, int lg, int nbx, int nby,
+    const int32_t *__restrict__ pagemap, uint8_t *__restrict__ pages) {
+#pragma clang fp contract(off)
+  __shared__ uint32_t tile[kTileVox];
+  __shared__ int32_t bcode[kTileBlk];
+  const int cell = list[blockIdx.x];
+  const int cx = cell % G.ncx, cy = (cell / G.ncx) % G.ncy, cz = cell / (G.ncx * G.ncy);
+  const int tx0 = cx * kCell - kHalo, ty0 = cy * kCell - kHalo, tz0 = cz * kCell - kHalo;
+  // blocks under the in-domain part of the tile
+  const int bxa = (tx0 > 0 ? tx0 : 0) >> lg, bya = (ty0 > 0 ? ty0 : 0) >> lg,
+            bza = (tz0 > 0 ? tz0 : 0) >> lg;
+  const int bxb = (tx0 + kTile - 1 < G.sx - 1 ? tx0 + kTile - 1 : G.sx - 1) >> lg;
+  const int byb = (ty0 + kTile - 1 < G.sy - 1 ? ty0 + kTile - 1 : G.sy - 1) >> lg;
+  const int bzb = (tz0 + kTile - 1 < G.sz - 1 ? tz0 + kTile - 1 : G.sz - 1) >> lg;
+  const int nbtx = bxb - bxa + 1, nbty = byb - bya + 1, nbtz = bzb - bza + 1;
+  for (int e = threadIdx.x; e < kTileVox; e += blockDim.x) tile[e] = kNoCode;
+  for (int e = threadIdx.x; e < nbtx * nbty * nbtz; e += blockDim.x) {
+    const int ex = e % nbtx, ey = (e / nbtx) % nbty, ez = e / (nbtx * nbty);
+    bcode[e] = pagemap[((int64_t)(bza + ez) * nby + (bya + ey)) * nbx + (bxa + ex)];
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const uint32_t p0 = off[cell], np = cnt[cell];
+  const int smax[3] = {G.sx - 1, G.sy - 1, G.sz - 1};
+  for (uint32_t q = wave; q < np; q += nwaves) {
+    const float *pp = sorted + 3 * (int64_t)(p0 + q);
+    const float p[3] = {pp[0], pp[1], pp[2]};
+    int lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      int a = (int)__builtin_ceilf(p[k] - rScan - 0.5f);
+      int b = (int)__builtin_floorf(p[k] + rScan - 0.5f);
+      lo[k] = a > 0 ? a : 0;
+      hi[k] = b < smax[k] ? b : smax[k];
+    }
+    const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+    if (nx <= 0 || ny <= 0 || nz <= 0) continue;
+    const int nxy = nx * ny, nbox = nxy * nz;
+    const float x0 = p[0] - 0.5f, y0 = p[1] - 0.5f, z0 = p[2] - 0.5f;
+    // t -> (t / nxy, t % nxy) through (t + 0.5) / n in float: exact, as the
+    // box has at most kTile^3 < 2^15 voxels and n <= kTile^2
+    const float rxy = 1.0f / (float)nxy, rx = 1.0f / (float)nx;
+    // the footprint box in z,y,x order, 64 voxels per step
+    for (int t0 = 0; t0 < nbox; t0 += 64) {
+      const int t = t0 + lane;
+      const int qz = (int)(((float)t + 0.5f) * rxy), r = t - qz * nxy;
+      const int qy = (int)(((float)r + 0.5f) * rx);
+      const int iz = lo[2] + qz, iy = lo[1] + qy, ix = lo[0] + (r - qy * nx);
+      bool inR = false;
+      int c = 0;
+      float d2 = 0.0f;
+      if (t < nbox) {
+        const float dz = (float)iz - z0, dy = (float)iy - y0, dx = (float)ix - x0;
+        const float dzy = dz * dz + dy * dy;
+        d2 = dzy + dx * dx;
+        inR = !(d2 > distSqMax);
+        if (inR)
+          c = bcode[(((iz >> lg) - bza) * nbty + ((iy >> lg) - bya)) * nbtx + ((ix >> lg) - bxa)];
+      }
+      const uint64_t bad = __ballot(inR && c < 0);
+      const int lim = bad ? __builtin_ctzll(bad) : 64;
+      if (inR && lane < lim) {
+        const float f = d2 * distSqMaxInv;
+        const uint32_t v = (uint32_t)(uint16_t)(int)__builtin_roundf(f * 65535.0f);
+        atomicMin(&tile[((iz - tz0) * kTile + (iy - ty0)) * kTile + (ix - tx0)], v);
+      }
+      if (bad) break;  // !isValueBlock -> goto loop_exit
+    }
+  }
+  __syncthreads();
+
+  // merge the touched voxels (only value blocks were touched), the loads of
+  // kMergeBatch voxels in flight together
+  const int64_t pageElems = (int64_t)1 << (3 * lg);
+  const int msk = (1 << lg) - 1;
+  for (int e0 = threadIdx.x; e0 < kTileVox; e0 += kMergeBatch * kSplatThreads) {
+    uint32_t v[kMergeBatch], cur[kMergeBatch];
+    int64_t el[kMergeBatch];
+#pragma unroll
+    for (int u = 0; u < kMergeBatch; u++) {
+      const int e = e0 + u * kSplatThreads;
+      v[u] = e < kTileVox ? tile[e] : kNoCode;
+      el[u] = 0;
+      if (v[u] != kNoCode) {
+        const int tz = e / (kTile * kTile), r = e - tz * (kTile * kTile), ty = r / kTile;
+        const int ix = tx0 + (r - ty * kTile), iy = ty0 + ty, iz = tz0 + tz;
+        const int c =
+            bcode[(((iz >> lg) - bza) * nbty + ((iy >> lg) - bya)) * nbtx + ((ix >> lg) - bxa)];
+        el[u] = c * pageElems + ((((iz & msk) << lg) + (iy & msk)) << lg) + (ix & msk);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kMergeBatch; u++) {
+      cur[u] = kNoCode;
+      if (v[u] != kNoCode) {
+        if constexpr (ESZ == 2)
+          cur[u] = ((const uint16_t *)pages)[el[u]];
+        else  // FP32 / FP64 pages hold the code in the low 32-bit word of the element
+          cur[u] = *(const uint32_t *)(pages + el[u] * ESZ);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kMergeBatch; u++) {
+      if (v[u] == kNoCode || v[u] >= cur[u]) continue;
+      if constexpr (ESZ == 2)
+        ((uint16_t *)pages)[el[u]] = (uint16_t)v[u];
+      else
+        *(uint32_t *)(pages + el[u] * ESZ) = v[u];
+    }
+  }
+}
+
 // distance codes start at 65535 (UNORM 1.0, msbg_demo.cpp:494-496) in the
 // pages of the non-UNORM16 storages
 template <typename W>
@@ -150,6 +334,73 @@ inline unsigned grid_cap(int64_t n) {
   return (unsigned)(b < 1 ? 1 : b);
 }
 
+int splat_binned(msbg_grid *g, ChanLevel &cl, int esz, float rScan, float distSqMax,
+                 float distSqMaxInv) {
+  hipStream_t s = g->stream;
+  const int64_t n = g->npoints;
+  const CellGrid G{(g->sx + kCell - 1) / kCell, (g->sy + kCell - 1) / kCell,
+                   (g->sz + kCell - 1) / kCell, g->sx, g->sy, g->sz};
+  const int64_t nc = (int64_t)G.ncx * G.ncy * G.ncz;
+  MSBG_CHECK(nc < ((int64_t)1 << 31), MSBG_EINVAL, "splat: %lld cells exceed the scan's range",
+             (long long)nc);
+  // the even cells (colour 0) are the most numerous colour
+  const int64_t cap = (int64_t)((G.ncx + 1) / 2) * ((G.ncy + 1) / 2) * ((G.ncz + 1) / 2);
+  DevBuf<uint32_t> cnt, off, cursor, ncol;
+  DevBuf<float> sorted;
+  DevBuf<int32_t> lists;
+  MSBG_TRY(cnt.alloc(nc));
+  MSBG_TRY(off.alloc(nc));
+  MSBG_TRY(cursor.alloc(nc));
+  MSBG_TRY(ncol.alloc(8));
+  MSBG_TRY(sorted.alloc(3 * n));
+  MSBG_TRY(lists.alloc(8 * cap));
+  MSBG_HIP(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t) * nc, s));
+  MSBG_HIP(hipMemsetAsync(cursor.p, 0, sizeof(uint32_t) * nc, s));
+  MSBG_HIP(hipMemsetAsync(ncol.p, 0, sizeof(uint32_t) * 8, s));
+  const unsigned gp = (unsigned)((n + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(k_cell_count, dim3(gp), dim3(kThreads), 0, s, g->d_xyz.p, n, G, cnt.p);
+  MSBG_HIP(hipGetLastError());
+  size_t bytes = 0;
+  MSBG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt.p, off.p, (int)nc, s));
+  DevBuf<uint8_t> work;
+  MSBG_TRY(work.alloc(bytes > 0 ? bytes : 1));
+  MSBG_HIP(hipcub::DeviceScan::ExclusiveSum(work.p, bytes, cnt.p, off.p, (int)nc, s));
+  hipLaunchKernelGGL(k_cell_scatter, dim3(gp), dim3(kThreads), 0, s, g->d_xyz.p, n, G, off.p,
+                     cursor.p, sorted.p);
+  MSBG_HIP(hipGetLastError());
+  // one list of non-empty cells per colour
+  hipcub::CountingInputIterator<int> cells(0);
+  size_t sbytes = 0;
+  MSBG_HIP(hipcub::DeviceSelect::If(nullptr, sbytes, cells, lists.p, ncol.p, (int)nc,
+                                    CellOfColour{cnt.p, G, 0}, s));
+  DevBuf<uint8_t> swork;
+  MSBG_TRY(swork.alloc(sbytes > 0 ? sbytes : 1));
+  for (int col = 0; col < 8; col++)
+    MSBG_HIP(hipcub::DeviceSelect::If(swork.p, sbytes, cells, lists.p + col * cap, ncol.p + col,
+                                      (int)nc, CellOfColour{cnt.p, G, col}, s));
+  uint32_t h[8];
+  MSBG_HIP(hipMemcpyAsync(h, ncol.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  MSBG_HIP(hipStreamSynchronize(s));
+  for (int col = 0; col < 8; col++) {
+    if (!h[col]) continue;
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(h[col]), dim3(kSplatThreads), 0, s, sorted.p, off.p, cnt.p,
+                         lists.p + col * cap, G, rScan, distSqMax, distSqMaxInv, g->lg0, g->nbx,
+                         g->nby, cl.d_page.p, cl.d_pages.p);
+    };
+    if (esz == 2)
+      go(k_splat_tile<2>);
+    else if (esz == 4)
+      go(k_splat_tile<4>);
+    else
+      go(k_splat_tile<8>);
+    MSBG_HIP(hipGetLastError());
+  }
+  // the work buffers are freed on return
+  MSBG_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 }  // namespace
 
 int launch_splat(msbg_grid *g, Channel &ch, float rScan) {
@@ -176,8 +427,12 @@ int launch_splat(msbg_grid *g, Channel &ch, float rScan) {
   }
   if (!g->npoints) return 0;
   const float distSqMax = rScan * rScan, distSqMaxInv = 1.0f / distSqMax;
-  int64_t nb = (g->npoints + kThreads - 1) / kThreads;
   int esz = storage_bytes(ch.storage);
+  // the binned splat needs every footprint inside its cell's tile, with a
+  // quarter voxel to spare for the float rounding of the footprint bounds
+  if (rScan + 0.5f <= (float)kHalo - 0.25f && g->npoints < ((int64_t)1 << 31) && esz != 1)
+    return splat_binned(g, cl, esz, rScan, distSqMax, distSqMaxInv);
+  int64_t nb = (g->npoints + kThreads - 1) / kThreads;
   auto args = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(kThreads), 0, g->stream, g->d_xyz.p,
                        g->npoints, rScan, distSqMax, distSqMaxInv, g->sx, g->sy, g->sz, g->lg0,

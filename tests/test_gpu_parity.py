@@ -125,6 +125,46 @@ def test_splat_abandon_quirk_gpu():
         assert np.array_equal(got, L.pages)
 
 
+# The binned LDS splat (rScan + 0.5 <= 4.75) and the per-particle atomic one
+# (wider footprints) against the oracle's colour-ordered splat: particles up
+# to the domain's faces and integer positions, value/EMPTY/FULL/NULL blocks
+# mixed (the early abandon), the code in every storage (the low 32 bits of
+# FP32/FP64).
+@pytest.mark.parametrize("S,bsx,rscan,storage", [
+    (64, 16, 4.0, M.U16), (48, 8, 1.5, M.U16), (64, 32, 4.25, M.F32), (64, 16, 6.0, M.U16),
+    (96, 32, 3.0, M.F64), (64, 16, 2.0, M.BF16), (64, 32, 4.5, M.F16)])
+def test_splat_paths_bitexact(S, bsx, rscan, storage):
+    note("splat paths", S, bsx, rscan, storage)
+    nb = S // bsx
+    rng = np.random.default_rng(int(S + 10 * rscan + storage))
+    kind = rng.choice([3, 3, 3, 3, 1, 2, 0], nb ** 3).astype(np.uint8)
+    c = rng.uniform(0, S, (12, 3))
+    pts = np.concatenate([
+        c[rng.integers(0, 12, 30000)] + rng.normal(0, 3.0, (30000, 3)),
+        rng.uniform(-7, S + 7, (5000, 3)),
+        np.floor(rng.uniform(-2, S + 2, (2000, 3)))]).astype(np.float32)
+    with M.MultiresSparseGrid.create("s", S, S, S, bsx, options=M.OPT_SINGLE_LEVEL, nLevels=2) as g:
+        g.channel_define(M.CH_UINT16_1, storage, 0, kind, 0.0, None, 65535.0 if storage == M.U16
+                         else 0.0)
+        pm, npg = g.pagemap(M.CH_UINT16_1, 0)
+        assert npg > 0
+        nk = g.points_upload(pts)
+        g.splat(M.CH_UINT16_1, rscan / 2, rscan / 2)
+        got = g.download(M.CH_UINT16_1)
+    # the upload keeps the points of the domain range (msbg_demo.cpp:356-360)
+    pts = np.ascontiguousarray(pts[O.domain_filter(S, S, S, pts) == 1])
+    assert nk == pts.shape[0] and nk > 20000
+    if storage in (M.F32, M.F64):
+        got = got.view(np.uint32).reshape(-1, 2 if storage == M.F64 else 1)[:, 0]
+    else:
+        got = got.view(np.uint16)
+    L = O.Level(S, S, S, bsx, pm, np.full((npg, bsx, bsx, bsx), 65535, np.uint16), O.U16)
+    O.splat_u16(L, pts, rscan)
+    want = L.pages.reshape(-1)
+    assert np.array_equal(got.reshape(-1), want)
+    assert (want < 65535).sum() > 1000
+
+
 @pytest.mark.parametrize("storage", [M.U16, M.F32, M.F16, M.BF16, M.F64])
 def test_finalize_exhaustive(storage):
     note("finalize exhaustive", storage)
@@ -284,7 +324,11 @@ def _multilevel_grid(S=64, bsx=16, seed=1, storage=M.F32):
         pm, npg = g.pagemap(M.CH_FLOAT_1, level)
         b = bsx >> level
         data = rng.random((npg, b, b, b)).astype(np.float32)
-        g.upload(M.CH_FLOAT_1, level, data)
+        if storage == M.F16:  # raw binary16 bits
+            data = data.astype(np.float16).view(np.uint16)
+        elif storage == M.BF16:  # raw bfloat16 bits (truncated values)
+            data = (data.view(np.uint32) >> 16).astype(np.uint16)
+        g.upload(M.CH_FLOAT_1, level, data.view(np.float16) if storage == M.F16 else data)
         pages.append(data)
         pms.append(pm)
     return g, levels, flags, pms, pages
@@ -315,21 +359,31 @@ def test_halo_fill_hook_bitexact(opt):
                     assert np.array_equal(got, dense)
 
 
-def test_prepare_ghost_blocks_bitexact():
-    note("ghost blocks")
-    S, bsx = 64, 16
-    g, levels, flags, pms, pages = _multilevel_grid(S, bsx, seed=4)
+_GHOST_OST = {M.F32: O.F32, M.F16: O.F16, M.BF16: O.BF16}
+
+
+# block sizes 8..32 take the LDS-staged kernel, level 1 of 8^3 blocks (4^3)
+# too; 64^3 blocks the per-tap one
+@pytest.mark.parametrize("bsx,storage,seed", [
+    (16, M.F32, 4), (32, M.F16, 5), (8, M.BF16, 6), (32, M.F32, 7), (16, M.F16, 8),
+    (64, M.F32, 9)])
+def test_prepare_ghost_blocks_bitexact(bsx, storage, seed):
+    note("ghost blocks", bsx, storage)
+    S = 4 * bsx
+    g, levels, flags, pms, pages = _multilevel_grid(S, bsx, seed=seed, storage=storage)
     with g:
         g.prepareGhostBlocks(M.CH_FLOAT_1)
         _, fl = g.getBlockInfo()
         post_pm = [g.pagemap(M.CH_FLOAT_1, l)[0] for l in range(3)]
         post = [g.download(M.CH_FLOAT_1, l) for l in range(3)]
+        if storage == M.F16:
+            post = [p.view(np.uint16) for p in post]
         olv = []
         for l in range(3):
             b = bsx >> l
-            pg = np.zeros((int((post_pm[l] >= 0).sum()), b, b, b), np.float32)
+            pg = np.zeros((int((post_pm[l] >= 0).sum()), b, b, b), pages[l].dtype)
             pg[:pages[l].shape[0]] = pages[l]
-            olv.append(_olevel(S, bsx, l, post_pm[l], pg))
+            olv.append(_olevel(S, bsx, l, post_pm[l], pg, _GHOST_OST[storage]))
         fcs = [g.blocksFineCoarse(l) for l in range(3)]
         O.prepare_ghost_blocks(olv, fcs, fl)
         for l in range(3):
