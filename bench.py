@@ -110,6 +110,8 @@ def setup_grid(cfg, xyz, storage, chan=30):
         g.prepareGhostBlocks(chan)
         g.synchronize()
         t["ghost_s"] = time.perf_counter() - t0
+        # fine-coarse blocks whose ghost pages the pass computed
+        t["ghost_blocks"] = [len(g.blocksFineCoarse(level)) for level in range(nL - 1)]
     n0 = int((levels == 0).sum())
     return g, nk, n0, t
 
@@ -512,8 +514,9 @@ def main():
             "active_voxels": int(nvox),
             "sweeps_timed": sweeps,
             "hbm_fraction": value * 2 * esz / (HBM_PEAK_GBS * 1e9),
-            "setup_s": {k: round(v, 4) for k, v in tsetup.items()},
+            "setup_s": {k: round(v, 4) for k, v in tsetup.items() if k.endswith("_s")},
             "splat_particles_per_s": nk / tsetup["splat_s"] if tsetup.get("splat_s") else None,
+            "ghost_blocks": tsetup.get("ghost_blocks"),
         },
         "roofline": {
             "bound": "hbm",

@@ -219,12 +219,12 @@ __global__ void k_ghost(LevelView L, LevelView Lo, GhostWeights W, const int32_t
 // windows of CZ coarse planes), as float. The taps and their summation order
 // are those of k_ghost, so the results are identical.
 //
-// Staging resolves halo_value(..., MSBG_OPT_BC_COARSE_LEVEL) from two
+// Staging: the block's own voxels come from its page in 16-byte loads; the
+// one-voxel shell resolves halo_value(..., MSBG_OPT_BC_COARSE_LEVEL) from two
 // 27-entry tables of the block's neighbour codes at this level and the
-// coarser one (the levels share one block grid), so every voxel becomes a
-// constant or one page load; the loads of kGhostBatch voxels are issued
-// together.
-constexpr int kGhostThreads = 512, kGhostBatch = 12;
+// coarser one (the levels share one block grid), so every shell voxel becomes
+// a constant or one page load, kGhostBatch of them issued together.
+constexpr int kGhostThreads = 512, kGhostBatch = 8, kGhostVec = 4;
 
 template <int ST, int BSX>
 __global__ __launch_bounds__(kGhostThreads) void k_ghost_tile(LevelView L, LevelView Lo,
@@ -284,18 +284,84 @@ __global__ __launch_bounds__(kGhostThreads) void k_ghost_tile(LevelView L, Level
     return nullptr;
   };
   __syncthreads();
+  // interior rows come from the block's own page in vector loads of VB bytes
+  constexpr int VB = BSX * (int)sizeof(R) < 16 ? BSX * (int)sizeof(R) : 16;
+  constexpr int VEC = VB / (int)sizeof(R), SEGS = BSX / VEC, PER = 4 * H - 4;
+  using V = typename std::conditional<VB == 16, uint4, uint2>::type;
+  static_assert(VB == 16 || VB == 8, "vector width");
+  const R *ownp = nbL[13] >= 0 ? Lp + (int64_t)nbL[13] * (BSX * BSX * BSX) : nullptr;
   for (int vz0 = 0; vz0 < BL; vz0 += CZ) {
-    for (int e0 = threadIdx.x; e0 < NP * PL; e0 += kGhostBatch * kGhostThreads) {
+    const int zlo = 2 * vz0 - 1;  // block-local z of tile plane 0
+    // tile planes [pa, pb) lie inside the block
+    const int pa = zlo < 0 ? 1 : 0, pb = zlo + NP - 1 > M ? NP - 1 : NP;
+    int nsrc = NP * PL;  // elements resolved through source()
+    if (ownp) {
+      const int nseg = (pb - pa) * BSX * SEGS;
+      for (int s0 = threadIdx.x; s0 < nseg; s0 += kGhostVec * kGhostThreads) {
+        V raw[kGhostVec];
+        int dst[kGhostVec];
+#pragma unroll
+        for (int u = 0; u < kGhostVec; u++) {
+          const int sg = s0 + u * kGhostThreads;
+          dst[u] = -1;
+          if (sg < nseg) {
+            const int hzl = pa + sg / (BSX * SEGS), r = sg % (BSX * SEGS), y = r / SEGS;
+            const int xs = (r % SEGS) * VEC;
+            raw[u] = *(const V *)(ownp + ((zlo + hzl) * BSX + y) * BSX + xs);
+            dst[u] = hzl * PL + (y + 1) * H + xs + 1;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kGhostVec; u++) {
+          if (dst[u] < 0) continue;
+          R v[VEC];
+          __builtin_memcpy(v, &raw[u], sizeof(V));
+#pragma unroll
+          for (int i = 0; i < VEC; i++) tile[dst[u] + i] = (float)Stor<ST>::dec(v[i]);
+        }
+      }
+      // the rest: whole planes outside the block, the perimeter of the others
+      nsrc = pa * PL + (pb - pa) * PER + (NP - pb) * PL;
+    }
+    for (int e0 = threadIdx.x; e0 < nsrc; e0 += kGhostBatch * kGhostThreads) {
       const R *src[kGhostBatch];
       float val[kGhostBatch];
+      int dst[kGhostBatch];
 #pragma unroll
       for (int u = 0; u < kGhostBatch; u++) {
-        const int e = e0 + u * kGhostThreads;
+        int e = e0 + u * kGhostThreads;
         src[u] = nullptr;
         val[u] = 0.0f;
-        if (e < NP * PL) {
-          const int hzl = e / PL, r = e - hzl * PL, hy = r / H, hx = r - hy * H;
-          src[u] = source(hx - 1, hy - 1, 2 * vz0 + hzl - 1, val[u]);
+        dst[u] = -1;
+        if (e < nsrc) {
+          int hzl, hy, hx;
+          if (!ownp) {
+            hzl = e / PL;
+            hy = (e - hzl * PL) / H;
+            hx = e - hzl * PL - hy * H;
+          } else if (e < pa * PL) {
+            hzl = 0;
+            hy = e / H;
+            hx = e - hy * H;
+          } else if ((e -= pa * PL) < (pb - pa) * PER) {
+            hzl = pa + e / PER;
+            int q = e % PER;
+            if (q < 2 * H) {
+              hy = q < H ? 0 : H - 1;
+              hx = q < H ? q : q - H;
+            } else {
+              q -= 2 * H;
+              hy = 1 + q / 2;
+              hx = (q & 1) ? H - 1 : 0;
+            }
+          } else {
+            e -= (pb - pa) * PER;
+            hzl = NP - 1;
+            hy = e / H;
+            hx = e - hy * H;
+          }
+          dst[u] = hzl * PL + hy * H + hx;
+          src[u] = source(hx - 1, hy - 1, zlo + hzl, val[u]);
         }
       }
       R raw[kGhostBatch];
@@ -303,10 +369,8 @@ __global__ __launch_bounds__(kGhostThreads) void k_ghost_tile(LevelView L, Level
       for (int u = 0; u < kGhostBatch; u++)
         if (src[u]) raw[u] = *src[u];
 #pragma unroll
-      for (int u = 0; u < kGhostBatch; u++) {
-        const int e = e0 + u * kGhostThreads;
-        if (e < NP * PL) tile[e] = src[u] ? (float)Stor<ST>::dec(raw[u]) : val[u];
-      }
+      for (int u = 0; u < kGhostBatch; u++)
+        if (dst[u] >= 0) tile[dst[u]] = src[u] ? (float)Stor<ST>::dec(raw[u]) : val[u];
     }
     __syncthreads();
     for (int o = threadIdx.x; o < CZ * BL * BL; o += kGhostThreads) {

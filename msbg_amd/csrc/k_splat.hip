@@ -184,49 +184,66 @@ __global__ __launch_bounds__(kSplatThreads) void k_splat_tile(
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   const uint32_t p0 = off[cell], np = cnt[cell];
   const int smax[3] = {G.sx - 1, G.sy - 1, G.sz - 1};
-  for (uint32_t q = wave; q < np; q += nwaves) {
-    const float *pp = sorted + 3 * (int64_t)(p0 + q);
-    const float p[3] = {pp[0], pp[1], pp[2]};
-    int lo[3], hi[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      int a = (int)__builtin_ceilf(p[k] - rScan - 0.5f);
-      int b = (int)__builtin_floorf(p[k] + rScan - 0.5f);
-      lo[k] = a > 0 ? a : 0;
-      hi[k] = b < smax[k] ? b : smax[k];
+  // the wave's particles are wave, wave + nwaves, ...: each lane loads one
+  // of the next 64, then the wave takes them one by one from its lanes
+  auto lane_f = [](float v, int k) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), k));
+  };
+  for (uint32_t qb = wave; qb < np; qb += 64u * nwaves) {
+    const uint32_t ql = qb + (uint32_t)lane * nwaves;
+    float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+    if (ql < np) {
+      const float *pp = sorted + 3 * (int64_t)(p0 + ql);
+      lx = pp[0];
+      ly = pp[1];
+      lz = pp[2];
     }
-    const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
-    if (nx <= 0 || ny <= 0 || nz <= 0) continue;
-    const int nxy = nx * ny, nbox = nxy * nz;
-    const float x0 = p[0] - 0.5f, y0 = p[1] - 0.5f, z0 = p[2] - 0.5f;
-    // t -> (t / nxy, t % nxy) through (t + 0.5) / n in float: exact, as the
-    // box has at most kTile^3 < 2^15 voxels and n <= kTile^2
-    const float rxy = 1.0f / (float)nxy, rx = 1.0f / (float)nx;
-    // the footprint box in z,y,x order, 64 voxels per step
-    for (int t0 = 0; t0 < nbox; t0 += 64) {
-      const int t = t0 + lane;
-      const int qz = (int)(((float)t + 0.5f) * rxy), r = t - qz * nxy;
-      const int qy = (int)(((float)r + 0.5f) * rx);
-      const int iz = lo[2] + qz, iy = lo[1] + qy, ix = lo[0] + (r - qy * nx);
-      bool inR = false;
-      int c = 0;
-      float d2 = 0.0f;
-      if (t < nbox) {
-        const float dz = (float)iz - z0, dy = (float)iy - y0, dx = (float)ix - x0;
-        const float dzy = dz * dz + dy * dy;
-        d2 = dzy + dx * dx;
-        inR = !(d2 > distSqMax);
-        if (inR)
-          c = bcode[(((iz >> lg) - bza) * nbty + ((iy >> lg) - bya)) * nbtx + ((ix >> lg) - bxa)];
+    const uint32_t left = (np - qb + nwaves - 1) / nwaves;
+    const int nq = left < 64u ? (int)left : 64;
+    for (int kq = 0; kq < nq; kq++) {
+      const float p[3] = {lane_f(lx, kq), lane_f(ly, kq), lane_f(lz, kq)};
+      int lo[3], hi[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        int a = (int)__builtin_ceilf(p[k] - rScan - 0.5f);
+        int b = (int)__builtin_floorf(p[k] + rScan - 0.5f);
+        lo[k] = a > 0 ? a : 0;
+        hi[k] = b < smax[k] ? b : smax[k];
       }
-      const uint64_t bad = __ballot(inR && c < 0);
-      const int lim = bad ? __builtin_ctzll(bad) : 64;
-      if (inR && lane < lim) {
-        const float f = d2 * distSqMaxInv;
-        const uint32_t v = (uint32_t)(uint16_t)(int)__builtin_roundf(f * 65535.0f);
-        atomicMin(&tile[((iz - tz0) * kTile + (iy - ty0)) * kTile + (ix - tx0)], v);
+      const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+      if (nx <= 0 || ny <= 0 || nz <= 0) continue;
+      const int nxy = nx * ny, nbox = nxy * nz;
+      const float x0 = p[0] - 0.5f, y0 = p[1] - 0.5f, z0 = p[2] - 0.5f;
+      // t -> (t / nxy, t % nxy) through (t + 0.5) / n in float: exact, as the
+      // box has at most kTile^3 < 2^15 voxels and n <= kTile^2
+      const float rxy = 1.0f / (float)nxy, rx = 1.0f / (float)nx;
+      // the footprint box in z,y,x order, 64 voxels per step
+      for (int t0 = 0; t0 < nbox; t0 += 64) {
+        const int t = t0 + lane;
+        const int qz = (int)(((float)t + 0.5f) * rxy), r = t - qz * nxy;
+        const int qy = (int)(((float)r + 0.5f) * rx);
+        const int iz = lo[2] + qz, iy = lo[1] + qy, ix = lo[0] + (r - qy * nx);
+        bool inR = false;
+        int c = 0;
+        float d2 = 0.0f;
+        if (t < nbox) {
+          const float dz = (float)iz - z0, dy = (float)iy - y0, dx = (float)ix - x0;
+          const float dzy = dz * dz + dy * dy;
+          d2 = dzy + dx * dx;
+          inR = !(d2 > distSqMax);
+          if (inR)
+            c = bcode[(((iz >> lg) - bza) * nbty + ((iy >> lg) - bya)) * nbtx +
+                      ((ix >> lg) - bxa)];
+        }
+        const uint64_t bad = __ballot(inR && c < 0);
+        const int lim = bad ? __builtin_ctzll(bad) : 64;
+        if (inR && lane < lim) {
+          const float f = d2 * distSqMaxInv;
+          const uint32_t v = (uint32_t)(uint16_t)(int)__builtin_roundf(f * 65535.0f);
+          atomicMin(&tile[((iz - tz0) * kTile + (iy - ty0)) * kTile + (ix - tx0)], v);
+        }
+        if (bad) break;  // !isValueBlock -> goto loop_exit
       }
-      if (bad) break;  // !isValueBlock -> goto loop_exit
     }
   }
   __syncthreads();
