@@ -277,6 +277,7 @@ int msbg_points_upload(msbg_grid *g, const float *xyz, int64_t n, int64_t *nkept
   MSBG_TRY(g->d_xyz.ensure((size_t)(3 * std::max<int64_t>(m, 1))));
   if (m) MSBG_HIP(hipMemcpy(g->d_xyz.p, src, sizeof(float) * 3 * m, hipMemcpyHostToDevice));
   g->npoints = m;
+  g->bin_valid = false;
   if (nkept) *nkept = m;
   return 0;
 }

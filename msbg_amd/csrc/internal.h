@@ -194,6 +194,11 @@ struct msbg_grid {
   // particles
   msbg::DevBuf<float> d_xyz;
   int64_t npoints = 0;
+  // the particles counting-sorted into 16^3 cells (splat, activation);
+  // rebuilt after the point set changes
+  msbg::DevBuf<float> bin_xyz;
+  msbg::DevBuf<uint32_t> bin_off, bin_cnt;
+  bool bin_valid = false;
   msbg::DevBuf<uint32_t> d_count;
   msbg::DevBuf<int32_t> d_tmp_levels;
 
@@ -219,6 +224,9 @@ namespace msbg {
 // ---- launchers implemented in the .hip files ----
 // activation: counts[bid] += 1 over each particle's trunc box
 int launch_activate(msbg_grid *g, float rScan, uint32_t *d_count);
+// builds g->bin_* from the current particles unless they are up to date;
+// returns MSBG_EUNSUPP when the point set is too large to bin
+int ensure_binned(msbg_grid *g);
 // level bands: level = first band whose count is nonzero, else nLevels-1
 int launch_level_band(msbg_grid *g, const uint32_t *d_count, int band,
                       int nbands, int32_t *d_levels);

@@ -130,6 +130,7 @@ extern "C" int msbg_points_generate(msbg_grid *g, uint64_t seed, int nObj, float
   std::swap(g->d_xyz.p, xyz.p);
   std::swap(g->d_xyz.n, xyz.n);
   g->npoints = old + kept;
+  g->bin_valid = false;
   if (nkept) *nkept = g->npoints;
   return 0;
 }

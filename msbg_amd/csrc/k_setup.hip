@@ -41,6 +41,66 @@ __global__ void k_activate(const float *__restrict__ xyz, int64_t n, float scale
         atomicAdd(&count[((int64_t)bz * nby + by) * nbx + bx], 1u);
 }
 
+// The same counts over the particles in cell order (ensure_binned):
+// neighbouring lanes share most of their blocks, so each wave adds every
+// block of the union of its lanes' boxes once, with the number of lanes
+// whose box holds it. Waves whose union exceeds 64 blocks add per lane.
+__device__ inline int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ inline int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+__global__ void k_activate_binned(const float *__restrict__ xyz, int64_t n, float scale,
+                                  float rScanBsx, int nbx, int nby, int nbz,
+                                  uint32_t *__restrict__ count) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
+  if (i < n) {
+    const int bmax[3] = {nbx - 1, nby - 1, nbz - 1};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      float bp = xyz[3 * i + k] * scale;
+      int a = (int)(bp - rScanBsx), b = (int)(bp + rScanBsx);  // truncate_to_int
+      lo[k] = a > 0 ? a : 0;
+      hi[k] = b < bmax[k] ? b : bmax[k];
+    }
+  }
+  const bool has = lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2];
+  int ulo[3], uhi[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    ulo[k] = wave_min(has ? lo[k] : 0x7fffffff);
+    uhi[k] = wave_max(has ? hi[k] : -1);
+  }
+  const int ex = uhi[0] - ulo[0] + 1, ey = uhi[1] - ulo[1] + 1, ez = uhi[2] - ulo[2] + 1;
+  if (ex <= 0 || ey <= 0 || ez <= 0) return;  // no lane of the wave has a block
+  if (ex * ey * ez <= 64) {
+    const int lane = threadIdx.x & 63;
+    for (int bz = ulo[2]; bz <= uhi[2]; bz++)
+      for (int by = ulo[1]; by <= uhi[1]; by++)
+        for (int bx = ulo[0]; bx <= uhi[0]; bx++) {
+          const bool in = has && bx >= lo[0] && bx <= hi[0] && by >= lo[1] && by <= hi[1] &&
+                          bz >= lo[2] && bz <= hi[2];
+          const uint64_t m = __ballot(in);
+          if (m && lane == __builtin_ctzll(m))
+            atomicAdd(&count[((int64_t)bz * nby + by) * nbx + bx], (uint32_t)__builtin_popcountll(m));
+        }
+    return;
+  }
+  if (!has) return;
+  for (int bz = lo[2]; bz <= hi[2]; bz++)
+    for (int by = lo[1]; by <= hi[1]; by++)
+      for (int bx = lo[0]; bx <= hi[0]; bx++)
+        atomicAdd(&count[((int64_t)bz * nby + by) * nbx + bx], 1u);
+}
+
 // level bands: the widest band initialises (count ? band : nLevels-1),
 // narrower bands overwrite where their count is nonzero.
 __global__ void k_level_band(const uint32_t *__restrict__ count, int64_t nb, int band, int init,
@@ -155,8 +215,14 @@ int launch_activate(msbg_grid *g, float rScan, uint32_t *d_count) {
   if (g->npoints == 0) return 0;
   const float scale = 1.0f / (float)g->bsx0;  // scale2DestBlockGrid
   const float rScanBsx = rScan / (float)g->bsx0;
-  hipLaunchKernelGGL(k_activate, dim3(grid_for(g->npoints)), dim3(kThreads), 0, g->stream,
-                     g->d_xyz.p, g->npoints, scale, rScanBsx, g->nbx, g->nby, g->nbz, d_count);
+  if (ensure_binned(g) == 0) {
+    hipLaunchKernelGGL(k_activate_binned, dim3((unsigned)((g->npoints + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, g->stream, g->bin_xyz.p, g->npoints, scale, rScanBsx,
+                       g->nbx, g->nby, g->nbz, d_count);
+  } else {  // too many particles to bin with 32-bit offsets
+    hipLaunchKernelGGL(k_activate, dim3(grid_for(g->npoints)), dim3(kThreads), 0, g->stream,
+                       g->d_xyz.p, g->npoints, scale, rScanBsx, g->nbx, g->nby, g->nbz, d_count);
+  }
   MSBG_HIP(hipGetLastError());
   return 0;
 }

@@ -13,6 +13,9 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "internal.h"
 #include "storage.h"
 
@@ -317,32 +320,49 @@ __device__ inline float finalize_value(uint32_t u, float rParticle, float nbDist
   return 1.0f - ls;
 }
 
+// one element: the code in its storage word -> the density in the same word
+template <int ST, typename Wd>
+__device__ inline Wd finalize_word(Wd w, float rParticle, float nbDist, float distSqMax) {
+#pragma clang fp contract(off)
+  bool z;
+  if constexpr (ST == MSBG_STORAGE_U16) {
+    float f = finalize_value(w, rParticle, nbDist, distSqMax, &z);
+    return z ? (Wd)0 : (Wd)(uint16_t)(int)__builtin_roundf(f * 65535.0f);
+  } else if constexpr (ST == MSBG_STORAGE_F16 || ST == MSBG_STORAGE_BF16) {
+    float f = finalize_value(w, rParticle, nbDist, distSqMax, &z);
+    return Stor<ST>::enc(z ? 0.0f : f);
+  } else if constexpr (ST == MSBG_STORAGE_F32) {
+    float f = finalize_value(w, rParticle, nbDist, distSqMax, &z);
+    return __builtin_bit_cast(uint32_t, z ? 0.0f : f);
+  } else {
+    float f = finalize_value((uint32_t)w, rParticle, nbDist, distSqMax, &z);
+    return __builtin_bit_cast(uint64_t, z ? 0.0 : (double)f);
+  }
+}
+
+// 16 bytes of elements per thread and step; n is a whole number of pages
 template <int ST>
 __global__ void k_finalize(uint8_t *__restrict__ pages, int64_t n, float rParticle, float nbDist,
                            float distSqMax) {
-#pragma clang fp contract(off)
+  using Wd = typename std::conditional<
+      ST == MSBG_STORAGE_F32, uint32_t,
+      typename std::conditional<ST == MSBG_STORAGE_F64, uint64_t, uint16_t>::type>::type;
+  constexpr int K = 16 / (int)sizeof(Wd);
+  const int64_t nv = n / K;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {
-    bool z;
-    if constexpr (ST == MSBG_STORAGE_U16) {
-      uint16_t *p = (uint16_t *)pages;
-      float f = finalize_value(p[i], rParticle, nbDist, distSqMax, &z);
-      p[i] = z ? (uint16_t)0 : (uint16_t)(int)__builtin_roundf(f * 65535.0f);
-    } else if constexpr (ST == MSBG_STORAGE_F16 || ST == MSBG_STORAGE_BF16) {
-      uint16_t *p = (uint16_t *)pages;
-      float f = finalize_value(p[i], rParticle, nbDist, distSqMax, &z);
-      p[i] = Stor<ST>::enc(z ? 0.0f : f);
-    } else if constexpr (ST == MSBG_STORAGE_F32) {
-      uint32_t *p = (uint32_t *)pages;
-      float f = finalize_value(p[i], rParticle, nbDist, distSqMax, &z);
-      ((float *)pages)[i] = z ? 0.0f : f;
-    } else {
-      uint64_t *p = (uint64_t *)pages;
-      float f = finalize_value((uint32_t)p[i], rParticle, nbDist, distSqMax, &z);
-      ((double *)pages)[i] = z ? 0.0 : (double)f;
-    }
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = i; v < nv; v += stride) {
+    const uint4 q = ((const uint4 *)pages)[v];
+    Wd e[K];
+    __builtin_memcpy(e, &q, 16);
+#pragma unroll
+    for (int k = 0; k < K; k++) e[k] = finalize_word<ST>(e[k], rParticle, nbDist, distSqMax);
+    uint4 o;
+    __builtin_memcpy(&o, e, 16);
+    ((uint4 *)pages)[v] = o;
   }
+  for (int64_t t = nv * K + i; t < n; t += stride)
+    ((Wd *)pages)[t] = finalize_word<ST>(((Wd *)pages)[t], rParticle, nbDist, distSqMax);
 }
 
 inline unsigned grid_cap(int64_t n) {
@@ -351,57 +371,43 @@ inline unsigned grid_cap(int64_t n) {
   return (unsigned)(b < 1 ? 1 : b);
 }
 
+CellGrid cell_grid(const msbg_grid *g) {
+  return CellGrid{(g->sx + kCell - 1) / kCell, (g->sy + kCell - 1) / kCell,
+                  (g->sz + kCell - 1) / kCell, g->sx, g->sy, g->sz};
+}
+
 int splat_binned(msbg_grid *g, ChanLevel &cl, int esz, float rScan, float distSqMax,
                  float distSqMaxInv) {
   hipStream_t s = g->stream;
-  const int64_t n = g->npoints;
-  const CellGrid G{(g->sx + kCell - 1) / kCell, (g->sy + kCell - 1) / kCell,
-                   (g->sz + kCell - 1) / kCell, g->sx, g->sy, g->sz};
+  MSBG_TRY(ensure_binned(g));
+  const CellGrid G = cell_grid(g);
   const int64_t nc = (int64_t)G.ncx * G.ncy * G.ncz;
-  MSBG_CHECK(nc < ((int64_t)1 << 31), MSBG_EINVAL, "splat: %lld cells exceed the scan's range",
-             (long long)nc);
   // the even cells (colour 0) are the most numerous colour
   const int64_t cap = (int64_t)((G.ncx + 1) / 2) * ((G.ncy + 1) / 2) * ((G.ncz + 1) / 2);
-  DevBuf<uint32_t> cnt, off, cursor, ncol;
-  DevBuf<float> sorted;
+  DevBuf<uint32_t> ncol;
   DevBuf<int32_t> lists;
-  MSBG_TRY(cnt.alloc(nc));
-  MSBG_TRY(off.alloc(nc));
-  MSBG_TRY(cursor.alloc(nc));
   MSBG_TRY(ncol.alloc(8));
-  MSBG_TRY(sorted.alloc(3 * n));
   MSBG_TRY(lists.alloc(8 * cap));
-  MSBG_HIP(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t) * nc, s));
-  MSBG_HIP(hipMemsetAsync(cursor.p, 0, sizeof(uint32_t) * nc, s));
   MSBG_HIP(hipMemsetAsync(ncol.p, 0, sizeof(uint32_t) * 8, s));
-  const unsigned gp = (unsigned)((n + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(k_cell_count, dim3(gp), dim3(kThreads), 0, s, g->d_xyz.p, n, G, cnt.p);
-  MSBG_HIP(hipGetLastError());
-  size_t bytes = 0;
-  MSBG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt.p, off.p, (int)nc, s));
-  DevBuf<uint8_t> work;
-  MSBG_TRY(work.alloc(bytes > 0 ? bytes : 1));
-  MSBG_HIP(hipcub::DeviceScan::ExclusiveSum(work.p, bytes, cnt.p, off.p, (int)nc, s));
-  hipLaunchKernelGGL(k_cell_scatter, dim3(gp), dim3(kThreads), 0, s, g->d_xyz.p, n, G, off.p,
-                     cursor.p, sorted.p);
-  MSBG_HIP(hipGetLastError());
+  const uint32_t *cnt = g->bin_cnt.p, *off = g->bin_off.p;
+  const float *sorted = g->bin_xyz.p;
   // one list of non-empty cells per colour
   hipcub::CountingInputIterator<int> cells(0);
   size_t sbytes = 0;
   MSBG_HIP(hipcub::DeviceSelect::If(nullptr, sbytes, cells, lists.p, ncol.p, (int)nc,
-                                    CellOfColour{cnt.p, G, 0}, s));
+                                    CellOfColour{cnt, G, 0}, s));
   DevBuf<uint8_t> swork;
   MSBG_TRY(swork.alloc(sbytes > 0 ? sbytes : 1));
   for (int col = 0; col < 8; col++)
     MSBG_HIP(hipcub::DeviceSelect::If(swork.p, sbytes, cells, lists.p + col * cap, ncol.p + col,
-                                      (int)nc, CellOfColour{cnt.p, G, col}, s));
+                                      (int)nc, CellOfColour{cnt, G, col}, s));
   uint32_t h[8];
   MSBG_HIP(hipMemcpyAsync(h, ncol.p, sizeof(h), hipMemcpyDeviceToHost, s));
   MSBG_HIP(hipStreamSynchronize(s));
   for (int col = 0; col < 8; col++) {
     if (!h[col]) continue;
     auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(h[col]), dim3(kSplatThreads), 0, s, sorted.p, off.p, cnt.p,
+      hipLaunchKernelGGL(kern, dim3(h[col]), dim3(kSplatThreads), 0, s, sorted, off, cnt,
                          lists.p + col * cap, G, rScan, distSqMax, distSqMaxInv, g->lg0, g->nbx,
                          g->nby, cl.d_page.p, cl.d_pages.p);
     };
@@ -419,6 +425,49 @@ int splat_binned(msbg_grid *g, ChanLevel &cl, int esz, float rScan, float distSq
 }
 
 }  // namespace
+
+int ensure_binned(msbg_grid *g) {
+  if (g->bin_valid) return 0;
+  hipStream_t s = g->stream;
+  const int64_t n = g->npoints;
+  const CellGrid G = cell_grid(g);
+  const int64_t nc = (int64_t)G.ncx * G.ncy * G.ncz;
+  if (n >= ((int64_t)1 << 31) || nc >= ((int64_t)1 << 31)) {
+    set_error("binning: %lld particles / %lld cells exceed the 32-bit offsets", (long long)n,
+              (long long)nc);
+    return MSBG_EUNSUPP;
+  }
+  DevBuf<uint32_t> cursor;
+  MSBG_TRY(g->bin_cnt.ensure(nc));
+  MSBG_TRY(g->bin_off.ensure(nc));
+  MSBG_TRY(cursor.alloc(nc));
+  MSBG_TRY(g->bin_xyz.ensure(3 * std::max<int64_t>(n, 1)));
+  MSBG_HIP(hipMemsetAsync(g->bin_cnt.p, 0, sizeof(uint32_t) * nc, s));
+  MSBG_HIP(hipMemsetAsync(cursor.p, 0, sizeof(uint32_t) * nc, s));
+  if (n) {
+    const unsigned gp = (unsigned)((n + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_cell_count, dim3(gp), dim3(kThreads), 0, s, g->d_xyz.p, n, G,
+                       g->bin_cnt.p);
+    MSBG_HIP(hipGetLastError());
+  }
+  size_t bytes = 0;
+  MSBG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, g->bin_cnt.p, g->bin_off.p, (int)nc,
+                                            s));
+  DevBuf<uint8_t> work;
+  MSBG_TRY(work.alloc(bytes > 0 ? bytes : 1));
+  MSBG_HIP(hipcub::DeviceScan::ExclusiveSum(work.p, bytes, g->bin_cnt.p, g->bin_off.p, (int)nc,
+                                            s));
+  if (n) {
+    const unsigned gp = (unsigned)((n + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_cell_scatter, dim3(gp), dim3(kThreads), 0, s, g->d_xyz.p, n, G,
+                       g->bin_off.p, cursor.p, g->bin_xyz.p);
+    MSBG_HIP(hipGetLastError());
+  }
+  // the work buffers are freed on return
+  MSBG_HIP(hipStreamSynchronize(s));
+  g->bin_valid = true;
+  return 0;
+}
 
 int launch_splat(msbg_grid *g, Channel &ch, float rScan) {
   ChanLevel &cl = ch.lv[0];
