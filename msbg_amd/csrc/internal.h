@@ -167,6 +167,7 @@ struct LevelWork {
   uint64_t key = 0;
   ChanLevel wl;
   DevBuf<int32_t> d_ghost;
+  DevBuf<uint32_t> d_gmask;  // per ghost page: bit (dz+1)*9+(dy+1)*3+dx+1 = swept block at +d
   int64_t nghost = 0;
   DevBuf<int2> d_items;
   int64_t off[9] = {0};
@@ -249,7 +250,8 @@ int sweep_prepare(msbg_grid *g, Channel &ch);  // side-car faces/edges + consts
 int sweep_prepare_level(msbg_grid *g, ChanLevel &cl, int storage);
 // ghost pages of a coarse-level sweep (level `level`, values from level+1)
 int launch_fill_ghost_pages(msbg_grid *g, Channel &ch, int level, unsigned opt,
-                            const int32_t *d_bids, int64_t n, uint8_t *d_pages, int64_t firstPage);
+                            const int32_t *d_bids, const uint32_t *d_mask, int64_t n,
+                            uint8_t *d_pages, int64_t firstPage);
 SweepGrid sweep_desc(Channel &ch, float dt, int doConstr01);
 int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, int tile,
                        const SweepGrid *d_grids, const int2 *d_items, int64_t n,

@@ -441,28 +441,63 @@ GhostWeights ghost_weights() {
 }
 
 // Ghost pages of a coarse-level sweep: page gp (block bids[i]) of level L
-// gets, in every voxel of its outer shell, the value getOutOfBlockValue gives an in-domain
-// voxel of a NULL block with OPT_BC_COARSE_LEVEL [| OPT_INTERPOLATE]
-// (halo.h:288-352): interpolateGhost of level Lo at ((x+.5)/2, ...) or its
-// voxel (x/2, y/2, z/2). Stored raw in the channel's storage, so the sweep
-// reads back what fillHaloBlock_ would have produced.
+// gets, in the voxels a swept neighbour reads, the value getOutOfBlockValue
+// gives an in-domain voxel of a NULL block with OPT_BC_COARSE_LEVEL
+// [| OPT_INTERPOLATE] (halo.h:288-352): interpolateGhost of level Lo at
+// ((x+.5)/2, ...) or its voxel (x/2, y/2, z/2). Stored raw in the channel's
+// storage, so the sweep reads back what fillHaloBlock_ would have produced.
+//
+// A swept block at +d (d in {-1,0,1}^3) reads the ghost's one-voxel layer
+// facing it: voxels with v_k = bsx-1 where d_k = 1 and v_k = 0 where
+// d_k = -1 (rows and planes, the side-car faces and edges built from them,
+// and out-of-domain cells clipped next to the swept block all lie there).
+// masks[i] has bit (d_z+1)*9 + (d_y+1)*3 + d_x+1 set for each swept
+// neighbour; only those layers are filled (the whole shell when masks is
+// null). Threads walk the shell: the planes z = 0 and z = bsx-1, then the
+// ring of every plane between them.
 template <int ST>
 __global__ void k_fill_ghost_pages(LevelView L, LevelView Lo, unsigned opt,
-                                   const int32_t *__restrict__ bids, int64_t firstPage,
+                                   const int32_t *__restrict__ bids,
+                                   const uint32_t *__restrict__ masks, int64_t firstPage,
                                    uint8_t *__restrict__ pages) {
   using R = typename Stor<ST>::raw;
-  const int nv = 1 << (3 * L.lg);
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nv) return;
+  const int B = L.bsx, m = B - 1, per = 4 * B - 4;
+  const int nshell = 2 * B * B + (B - 2) * per;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nshell) return;
+  int vx, vy, vz;
+  if (s < 2 * B * B) {
+    vz = s < B * B ? 0 : m;
+    const int r = s - (vz ? B * B : 0);
+    vy = r / B;
+    vx = r - vy * B;
+  } else {
+    const int t = s - 2 * B * B, q = t % per;
+    vz = 1 + t / per;
+    if (q < 2 * B) {
+      vy = q < B ? 0 : m;
+      vx = q < B ? q : q - B;
+    } else {
+      vy = 1 + (q - 2 * B) / 2;
+      vx = ((q - 2 * B) & 1) ? m : 0;
+    }
+  }
   const int64_t i = blockIdx.y;
+  if (masks) {
+    const uint32_t mk = masks[i];
+    const int cx = vx == 0 ? -1 : (vx == m ? 1 : 0), cy = vy == 0 ? -1 : (vy == m ? 1 : 0),
+              cz = vz == 0 ? -1 : (vz == m ? 1 : 0);
+    bool need = false;
+    for (int k = 1; k < 8 && !need; k++) {  // the directions whose layer holds the voxel
+      if (((k & 1) && !cx) || ((k & 2) && !cy) || ((k & 4) && !cz)) continue;
+      const int dx = (k & 1) ? cx : 0, dy = (k & 2) ? cy : 0, dz = (k & 4) ? cz : 0;
+      need = (mk >> ((dz + 1) * 9 + (dy + 1) * 3 + dx + 1)) & 1u;
+    }
+    if (!need) return;
+  }
+  const int v = (((vz << L.lg) + vy) << L.lg) + vx;
   const int bid = bids[i];
   const int bx = bid % L.nbx, by = (bid / L.nbx) % L.nby, bz = bid / (L.nbx * L.nby);
-  const int m = L.bsx - 1;
-  const int vx = v & m, vy = (v >> L.lg) & m, vz = v >> (2 * L.lg);
-  // only the page's one-voxel shell is ever read: neighbour rows and planes,
-  // the side-car faces and edges built from it, and out-of-domain cells
-  // clipped next to the swept block
-  if (vx > 0 && vx < m && vy > 0 && vy < m && vz > 0 && vz < m) return;
   const int x = bx * L.bsx + vx, y = by * L.bsx + vy, z = bz * L.bsx + vz;
   double raw;
   if (opt & MSBG_OPT_INTERPOLATE) {
@@ -476,7 +511,7 @@ __global__ void k_fill_ghost_pages(LevelView L, LevelView Lo, unsigned opt,
   R out;
   if constexpr (ST == MSBG_STORAGE_U16) out = (R)(uint16_t)raw;
   else out = Stor<ST>::enc((float)raw);
-  ((R *)pages)[(firstPage + i) * nv + v] = out;
+  ((R *)pages)[(firstPage + i) * ((int64_t)B * B * B) + v] = out;
 }
 
 }  // namespace
@@ -494,30 +529,31 @@ int launch_halo_fill(msbg_grid *g, Channel &ch, int level, int bid, int dmax, un
 }
 
 int launch_fill_ghost_pages(msbg_grid *g, Channel &ch, int level, unsigned opt,
-                            const int32_t *d_bids, int64_t n, uint8_t *d_pages,
-                            int64_t firstPage) {
+                            const int32_t *d_bids, const uint32_t *d_mask, int64_t n,
+                            uint8_t *d_pages, int64_t firstPage) {
   LevelView L = make_view(g, ch, level);
   LevelView Lo = make_view(g, ch, level + 1);
-  const int nv = L.bsx * L.bsx * L.bsx;
+  const int B = L.bsx, nshell = 2 * B * B + (B - 2) * (4 * B - 4);
   for (int64_t off = 0; off < n; off += 65535) {
     const unsigned cnt = (unsigned)(n - off < 65535 ? n - off : 65535);
-    const dim3 grid((unsigned)((nv + 255) / 256), cnt);
+    const dim3 grid((unsigned)((nshell + 255) / 256), cnt);
+    const uint32_t *mk = d_mask ? d_mask + off : nullptr;
     switch (ch.storage) {
       case MSBG_STORAGE_U16:
         hipLaunchKernelGGL(k_fill_ghost_pages<MSBG_STORAGE_U16>, grid, dim3(256), 0, g->stream, L,
-                           Lo, opt, d_bids + off, firstPage + off, d_pages);
+                           Lo, opt, d_bids + off, mk, firstPage + off, d_pages);
         break;
       case MSBG_STORAGE_F32:
         hipLaunchKernelGGL(k_fill_ghost_pages<MSBG_STORAGE_F32>, grid, dim3(256), 0, g->stream, L,
-                           Lo, opt, d_bids + off, firstPage + off, d_pages);
+                           Lo, opt, d_bids + off, mk, firstPage + off, d_pages);
         break;
       case MSBG_STORAGE_F16:
         hipLaunchKernelGGL(k_fill_ghost_pages<MSBG_STORAGE_F16>, grid, dim3(256), 0, g->stream, L,
-                           Lo, opt, d_bids + off, firstPage + off, d_pages);
+                           Lo, opt, d_bids + off, mk, firstPage + off, d_pages);
         break;
       case MSBG_STORAGE_BF16:
         hipLaunchKernelGGL(k_fill_ghost_pages<MSBG_STORAGE_BF16>, grid, dim3(256), 0, g->stream,
-                           L, Lo, opt, d_bids + off, firstPage + off, d_pages);
+                           L, Lo, opt, d_bids + off, mk, firstPage + off, d_pages);
         break;
       default: set_error("ghost pages: unsupported storage %d", ch.storage); return MSBG_EUNSUPP;
     }

@@ -13,13 +13,14 @@
 // Finer neighbours are read from the ghost pages prepareGhostBlocks keeps at
 // level l; they are value pages there and are never swept.
 //
-// GPU design: the level's pages are copied into a working pool followed by
-// "ghost pages" for every NULL block next to a swept block, each filled
-// once with the coarse-level values of its voxels (level l+1 does not change
-// during the call). The halo table of the working pool points at them, so
-// the unchanged sweep kernel reads coarse-level halos like neighbour pages;
-// out-of-domain cells are clipped in the kernel's ring setup. The swept
-// pages are copied back at the end.
+// GPU design: the level's page pool is followed by "ghost pages" for every
+// NULL block next to a swept block, each filled once per call with the
+// coarse-level values of its voxels (level l+1 does not change during the
+// call). The pool gets the room for them once, at set-up; a call then lends
+// the pool to the working level (its own block map and halo table) and
+// sweeps it in place. The halo table points at the ghost pages, so the
+// unchanged sweep kernel reads coarse-level halos like neighbour pages;
+// out-of-domain cells are clipped in the kernel's ring setup.
 
 #include <hip/hip_runtime.h>
 
@@ -130,12 +131,35 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
       }
     }
     W.nghost = (int64_t)ghostBids.size();
+    // per ghost page, the directions of its swept neighbours: only the
+    // layers facing them are ever read
+    std::vector<uint32_t> gmask((size_t)W.nghost, 0u);
+    if (W.nghost) {
+      std::vector<uint8_t> isSwept((size_t)g->nblocks, 0);
+      for (int32_t bid : swept) isSwept[bid] = 1;
+      for (int64_t i = 0; i < W.nghost; i++) {
+        const int32_t bid = ghostBids[i];
+        const int bx = bid % g->nbx, by = (bid / g->nbx) % g->nby,
+                  bz = (int)(bid / ((int64_t)g->nbx * g->nby));
+        for (int dz = -1; dz <= 1; dz++)
+          for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+              const int x = bx + dx, y = by + dy, z = bz + dz;
+              if (x < 0 || y < 0 || z < 0 || x >= g->nbx || y >= g->nby || z >= g->nbz)
+                continue;
+              if (isSwept[((int64_t)z * g->nby + y) * g->nbx + x])
+                gmask[i] |= 1u << ((dz + 1) * 9 + (dy + 1) * 3 + dx + 1);
+            }
+      }
+    }
     wl.npages = cl.npages + W.nghost;
     if (W.off[8]) {
-      MSBG_TRY(wl.d_pages.alloc((size_t)wl.npages * pageBytes));
       if (W.nghost) {
         MSBG_TRY(W.d_ghost.alloc((size_t)W.nghost));
+        MSBG_TRY(W.d_gmask.alloc((size_t)W.nghost));
         MSBG_HIP(hipMemcpyAsync(W.d_ghost.p, ghostBids.data(), sizeof(int32_t) * W.nghost,
+                                hipMemcpyHostToDevice, g->stream));
+        MSBG_HIP(hipMemcpyAsync(W.d_gmask.p, gmask.data(), sizeof(uint32_t) * W.nghost,
                                 hipMemcpyHostToDevice, g->stream));
       }
       wl.h_page_bid.assign((size_t)wl.npages, -1);
@@ -164,13 +188,34 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
     return 0;
   }
 
-  // per call: the level's current pages, ghosts from the current level l+1,
+  // room for the ghost pages behind the level's pages, kept with the pool
+  // (checked per call: another operation may have replaced the pool)
+  const size_t need = (size_t)wl.npages * pageBytes;
+  if (cl.d_pages.n < need) {
+    DevBuf<uint8_t> np;
+    MSBG_TRY(np.alloc(need));
+    if (cl.npages)
+      MSBG_HIP(hipMemcpyAsync(np.p, cl.d_pages.p, (size_t)cl.npages * pageBytes,
+                              hipMemcpyDeviceToDevice, g->stream));
+    MSBG_HIP(hipStreamSynchronize(g->stream));
+    std::swap(cl.d_pages.p, np.p);
+    std::swap(cl.d_pages.n, np.n);
+  }
+  // per call: the level's pool is lent to the working level for the call
+  // (given back on every return path), ghosts from the current level l+1,
   // side-cars of both
-  MSBG_HIP(hipMemcpyAsync(wl.d_pages.p, cl.d_pages.p, (size_t)cl.npages * pageBytes,
-                          hipMemcpyDeviceToDevice, g->stream));
+  struct Lend {
+    ChanLevel &a, &b;
+    Lend(ChanLevel &x, ChanLevel &y) : a(x), b(y) { swap(); }
+    ~Lend() { swap(); }
+    void swap() {
+      std::swap(a.d_pages.p, b.d_pages.p);
+      std::swap(a.d_pages.n, b.d_pages.n);
+    }
+  } lend(cl, wl);
   if (W.nghost)
-    MSBG_TRY(launch_fill_ghost_pages(g, ch, level, haloOpt, W.d_ghost.p, W.nghost,
-                                     wl.d_pages.p, cl.npages));
+    MSBG_TRY(launch_fill_ghost_pages(g, ch, level, haloOpt, W.d_ghost.p, W.d_gmask.p,
+                                     W.nghost, wl.d_pages.p, cl.npages));
   wl.side_valid = false;
   MSBG_TRY(sweep_prepare_level(g, wl, ch.storage));
 
@@ -208,9 +253,7 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
     if (residual) MSBG_TRY(launch_reduce_residual(g, d_partial.p, ntot, d_resid.p + iter));
   }
   MSBG_HIP(hipEventRecord(g->ev[1], g->stream));
-  // the level's own pages come back; its side-cars are stale now
-  MSBG_HIP(hipMemcpyAsync(cl.d_pages.p, wl.d_pages.p, (size_t)cl.npages * pageBytes,
-                          hipMemcpyDeviceToDevice, g->stream));
+  // the level's pages changed in place; its side-cars are stale now
   cl.side_valid = false;
   if (residual) {
     MSBG_HIP(hipMemcpyAsync(residual, d_resid.p, sizeof(double) * nMaxIter,
