@@ -157,6 +157,10 @@ struct CellOfColour {
 };
 
 constexpr int kSplatThreads = 512, kMergeBatch = 12;
+// footprint boxes of the tile path have at most 9 voxels a side
+// (2 rScan + 1 with rScan + 0.5 <= kHalo - 0.25): 81 <= 2 * 64 per face
+constexpr int kFaceSets = 2;
+static_assert((2 * kHalo - 1) * (2 * kHalo - 1) <= kFaceSets * 64, "face sets");
 
 template <int ESZ>
 __global__ __launch_bounds__(kSplatThreads) void k_splat_tile(
@@ -215,37 +219,60 @@ __global__ __launch_bounds__(kSplatThreads) void k_splat_tile(
       }
       const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
       if (nx <= 0 || ny <= 0 || nz <= 0) continue;
-      const int nxy = nx * ny, nbox = nxy * nz;
+      const int nxy = nx * ny;
       const float x0 = p[0] - 0.5f, y0 = p[1] - 0.5f, z0 = p[2] - 0.5f;
-      // t -> (t / nxy, t % nxy) through (t + 0.5) / n in float: exact, as the
-      // box has at most kTile^3 < 2^15 voxels and n <= kTile^2
-      const float rxy = 1.0f / (float)nxy, rx = 1.0f / (float)nx;
-      // the footprint box in z,y,x order, 64 voxels per step
-      for (int t0 = 0; t0 < nbox; t0 += 64) {
-        const int t = t0 + lane;
-        const int qz = (int)(((float)t + 0.5f) * rxy), r = t - qz * nxy;
-        const int qy = (int)(((float)r + 0.5f) * rx);
-        const int iz = lo[2] + qz, iy = lo[1] + qy, ix = lo[0] + (r - qy * nx);
-        bool inR = false;
-        int c = 0;
-        float d2 = 0.0f;
-        if (t < nbox) {
-          const float dz = (float)iz - z0, dy = (float)iy - y0, dx = (float)ix - x0;
-          const float dzy = dz * dz + dy * dy;
-          d2 = dzy + dx * dx;
-          inR = !(d2 > distSqMax);
-          if (inR)
-            c = bcode[(((iz >> lg) - bza) * nbty + ((iy >> lg) - bya)) * nbtx +
-                      ((ix >> lg) - bxa)];
+      // lanes over the box's (y, x) face in y,x order (at most kFaceSets
+      // sets of 64), the planes in z order: z,y,x order is lane order
+      // within a plane. Per lane and set, what does not depend on z.
+      // t -> (t / nx, t % nx) through (t + 0.5) / nx in float: exact for
+      // t < 2^15 and nx <= kTile.
+      const float rx = 1.0f / (float)nx;
+      bool fv[kFaceSets];
+      float fdy2[kFaceSets], fdx2[kFaceSets];
+      int fblk[kFaceSets], ftile[kFaceSets];
+#pragma unroll
+      for (int u = 0; u < kFaceSets; u++) {
+        const int t = u * 64 + lane;
+        const int qy = (int)(((float)t + 0.5f) * rx);
+        const int iy = lo[1] + qy, ix = lo[0] + (t - qy * nx);
+        fv[u] = t < nxy;
+        const float dy = (float)iy - y0, dx = (float)ix - x0;
+        fdy2[u] = dy * dy;
+        fdx2[u] = dx * dx;
+        fblk[u] = ((iy >> lg) - bya) * nbtx + ((ix >> lg) - bxa);
+        ftile[u] = (iy - ty0) * kTile + (ix - tx0);
+      }
+      const int nsets = (nxy + 63) >> 6;
+      // the box spans at most two blocks in z: both codes per lane, read once
+      const int bzlo = lo[2] >> lg;
+      const int zb0 = (bzlo - bza) * (nbty * nbtx), zb1 = ((hi[2] >> lg) - bza) * (nbty * nbtx);
+      int fc0[kFaceSets], fc1[kFaceSets];
+#pragma unroll
+      for (int u = 0; u < kFaceSets; u++) {
+        fc0[u] = fv[u] ? bcode[zb0 + fblk[u]] : 0;
+        fc1[u] = fv[u] ? bcode[zb1 + fblk[u]] : 0;
+      }
+      bool done = false;
+      for (int iz = lo[2]; iz <= hi[2] && !done; iz++) {
+        const float dz = (float)iz - z0, dz2 = dz * dz;
+        const bool zlo = (iz >> lg) == bzlo;
+        const int tz = (iz - tz0) * (kTile * kTile);
+#pragma unroll
+        for (int u = 0; u < kFaceSets; u++) {
+          if (u >= nsets || done) break;
+          const float dzy = dz2 + fdy2[u];  // dz*dz + dy*dy, then + dx*dx
+          const float d2 = dzy + fdx2[u];
+          const bool inR = fv[u] && !(d2 > distSqMax);
+          const int c = zlo ? fc0[u] : fc1[u];
+          const uint64_t bad = __ballot(inR && c < 0);
+          const int lim = bad ? __builtin_ctzll(bad) : 64;
+          if (inR && lane < lim) {
+            const float f = d2 * distSqMaxInv;
+            const uint32_t v = (uint32_t)(uint16_t)(int)__builtin_roundf(f * 65535.0f);
+            atomicMin(&tile[tz + ftile[u]], v);
+          }
+          done = bad != 0;  // !isValueBlock -> goto loop_exit
         }
-        const uint64_t bad = __ballot(inR && c < 0);
-        const int lim = bad ? __builtin_ctzll(bad) : 64;
-        if (inR && lane < lim) {
-          const float f = d2 * distSqMaxInv;
-          const uint32_t v = (uint32_t)(uint16_t)(int)__builtin_roundf(f * 65535.0f);
-          atomicMin(&tile[((iz - tz0) * kTile + (iy - ty0)) * kTile + (ix - tx0)], v);
-        }
-        if (bad) break;  // !isValueBlock -> goto loop_exit
       }
     }
   }
