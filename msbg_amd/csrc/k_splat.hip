@@ -156,7 +156,7 @@ struct CellOfColour {
   }
 };
 
-constexpr int kSplatThreads = 512, kMergeBatch = 12;
+constexpr int kSplatThreads = 1024, kMergeBatch = 9;
 // footprint boxes of the tile path have at most 9 voxels a side
 // (2 rScan + 1 with rScan + 0.5 <= kHalo - 0.25): 81 <= 2 * 64 per face
 constexpr int kFaceSets = 2;

@@ -132,7 +132,7 @@ def test_splat_abandon_quirk_gpu():
 # FP32/FP64).
 @pytest.mark.parametrize("S,bsx,rscan,storage", [
     (64, 16, 4.0, M.U16), (48, 8, 1.5, M.U16), (64, 32, 4.25, M.F32), (64, 16, 6.0, M.U16),
-    (96, 32, 3.0, M.F64), (64, 16, 2.0, M.BF16), (64, 32, 4.5, M.F16)])
+    (96, 32, 3.0, M.F64), (64, 16, 2.0, M.BF16), (64, 32, 4.5, M.F16), (128, 64, 4.0, M.U16)])
 def test_splat_paths_bitexact(S, bsx, rscan, storage):
     note("splat paths", S, bsx, rscan, storage)
     nb = S // bsx
