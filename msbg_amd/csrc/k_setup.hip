@@ -42,9 +42,12 @@ __global__ void k_activate(const float *__restrict__ xyz, int64_t n, float scale
 }
 
 // The same counts over the particles in cell order (ensure_binned):
-// neighbouring lanes share most of their blocks, so each wave adds every
-// block of the union of its lanes' boxes once, with the number of lanes
-// whose box holds it. Waves whose union exceeds 64 blocks add per lane.
+// neighbouring lanes share most of their blocks. Each wave ballots every
+// block of the union of its lanes' boxes; when the workgroup's union holds
+// at most kAggBlocks blocks, the counts gather in LDS and the workgroup adds
+// each block once. Wider unions add per wave (up to 64 blocks) or per lane.
+constexpr int kAggBlocks = 512;
+
 __device__ inline int wave_min(int v) {
 #pragma unroll
   for (int o = 32; o; o >>= 1) v = min(v, __shfl_xor(v, o));
@@ -56,10 +59,18 @@ __device__ inline int wave_max(int v) {
   return v;
 }
 
-__global__ void k_activate_binned(const float *__restrict__ xyz, int64_t n, float scale,
-                                  float rScanBsx, int nbx, int nby, int nbz,
-                                  uint32_t *__restrict__ count) {
+__global__ __launch_bounds__(256) void k_activate_binned(const float *__restrict__ xyz, int64_t n,
+                                                         float scale, float rScanBsx, int nbx,
+                                                         int nby, int nbz,
+                                                         uint32_t *__restrict__ count) {
 #pragma clang fp contract(off)
+  __shared__ int glo[3], ghi[3];
+  __shared__ uint32_t agg[kAggBlocks];
+  if (threadIdx.x < 3) {
+    glo[threadIdx.x] = 0x7fffffff;
+    ghi[threadIdx.x] = -1;
+  }
+  for (int t = threadIdx.x; t < kAggBlocks; t += blockDim.x) agg[t] = 0;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
   if (i < n) {
@@ -80,9 +91,41 @@ __global__ void k_activate_binned(const float *__restrict__ xyz, int64_t n, floa
     uhi[k] = wave_max(has ? hi[k] : -1);
   }
   const int ex = uhi[0] - ulo[0] + 1, ey = uhi[1] - ulo[1] + 1, ez = uhi[2] - ulo[2] + 1;
-  if (ex <= 0 || ey <= 0 || ez <= 0) return;  // no lane of the wave has a block
+  const bool waveHas = ex > 0 && ey > 0 && ez > 0;
+  const int lane = threadIdx.x & 63;
+  __syncthreads();
+  if (waveHas && lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      atomicMin(&glo[k], ulo[k]);
+      atomicMax(&ghi[k], uhi[k]);
+    }
+  }
+  __syncthreads();
+  const int gx = ghi[0] - glo[0] + 1, gy = ghi[1] - glo[1] + 1, gz = ghi[2] - glo[2] + 1;
+  if (gx <= 0 || gy <= 0 || gz <= 0) return;  // no lane of the workgroup has a block
+  if (gx * gy * gz <= kAggBlocks) {
+    if (waveHas)
+      for (int bz = ulo[2]; bz <= uhi[2]; bz++)
+        for (int by = ulo[1]; by <= uhi[1]; by++)
+          for (int bx = ulo[0]; bx <= uhi[0]; bx++) {
+            const bool in = has && bx >= lo[0] && bx <= hi[0] && by >= lo[1] && by <= hi[1] &&
+                            bz >= lo[2] && bz <= hi[2];
+            const uint64_t m = __ballot(in);
+            if (m && lane == __builtin_ctzll(m))
+              atomicAdd(&agg[((bz - glo[2]) * gy + (by - glo[1])) * gx + (bx - glo[0])],
+                        (uint32_t)__builtin_popcountll(m));
+          }
+    __syncthreads();
+    for (int t = threadIdx.x; t < gx * gy * gz; t += blockDim.x) {
+      if (!agg[t]) continue;
+      const int bx = glo[0] + t % gx, by = glo[1] + (t / gx) % gy, bz = glo[2] + t / (gx * gy);
+      atomicAdd(&count[((int64_t)bz * nby + by) * nbx + bx], agg[t]);
+    }
+    return;
+  }
+  if (!waveHas) return;
   if (ex * ey * ez <= 64) {
-    const int lane = threadIdx.x & 63;
     for (int bz = ulo[2]; bz <= uhi[2]; bz++)
       for (int by = ulo[1]; by <= uhi[1]; by++)
         for (int bx = ulo[0]; bx <= uhi[0]; bx++) {
@@ -90,7 +133,8 @@ __global__ void k_activate_binned(const float *__restrict__ xyz, int64_t n, floa
                           bz >= lo[2] && bz <= hi[2];
           const uint64_t m = __ballot(in);
           if (m && lane == __builtin_ctzll(m))
-            atomicAdd(&count[((int64_t)bz * nby + by) * nbx + bx], (uint32_t)__builtin_popcountll(m));
+            atomicAdd(&count[((int64_t)bz * nby + by) * nbx + bx],
+                      (uint32_t)__builtin_popcountll(m));
         }
     return;
   }
