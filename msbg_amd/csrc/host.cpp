@@ -235,51 +235,14 @@ int msbg_synchronize(msbg_grid *g) {
 
 int msbg_points_upload(msbg_grid *g, const float *xyz, int64_t n, int64_t *nkept) {
   MSBG_CHECK(g && (xyz || n == 0) && n >= 0, MSBG_EINVAL, "msbg_points_upload: bad args");
-  float dmin[3], dmax[3];
-  domain_bounds(g->sx, g->sy, g->sz, dmin, dmax);
-  const int s[3] = {g->sx, g->sy, g->sz};
-  // msbg_demo.cpp:356-360: isInDomainRange, then trunc(pos) inRange.
-  std::vector<uint8_t> keep((size_t)n);
-  parallel_for(n, [&](int64_t i0, int64_t i1) {
-  for (int64_t i = i0; i < i1; i++) {
-    bool ok = true;
-    for (int k = 0; k < 3; k++) {
-      float p = xyz[3 * i + k];
-      if (p != p || p < dmin[k] || p > dmax[k]) {
-        ok = false;
-        break;
-      }
-      int ip = (int)p;  // truncate_to_int
-      if (ip < 0 || ip > s[k] - 1) {
-        ok = false;
-        break;
-      }
-    }
-    keep[i] = ok;
+  // msbg_demo.cpp:356-360 (isInDomainRange, then trunc(pos) inRange) runs on
+  // the device over the copied points
+  DevBuf<float> raw;
+  if (n) {
+    MSBG_TRY(raw.alloc((size_t)(3 * n)));
+    MSBG_HIP(hipMemcpyAsync(raw.p, xyz, sizeof(float) * 3 * n, hipMemcpyHostToDevice, g->stream));
   }
-  });
-  int64_t m = 0;
-  for (int64_t i = 0; i < n; i++) m += keep[i];
-  std::vector<float> kept;
-  const float *src = xyz;
-  if (m != n) {
-    kept.resize((size_t)(3 * m));
-    int64_t j = 0;
-    for (int64_t i = 0; i < n; i++)
-      if (keep[i]) {
-        kept[3 * j] = xyz[3 * i];
-        kept[3 * j + 1] = xyz[3 * i + 1];
-        kept[3 * j + 2] = xyz[3 * i + 2];
-        j++;
-      }
-    src = kept.data();
-  }
-  MSBG_TRY(g->d_xyz.ensure((size_t)(3 * std::max<int64_t>(m, 1))));
-  if (m) MSBG_HIP(hipMemcpy(g->d_xyz.p, src, sizeof(float) * 3 * m, hipMemcpyHostToDevice));
-  g->npoints = m;
-  g->bin_valid = false;
-  if (nkept) *nkept = m;
-  return 0;
+  return filter_points(g, raw.p, n, nkept);
 }
 
 int msbg_points_download(msbg_grid *g, float *xyz, int64_t cap) {

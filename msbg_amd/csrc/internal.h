@@ -228,6 +228,9 @@ int launch_activate(msbg_grid *g, float rScan, uint32_t *d_count);
 // builds g->bin_* from the current particles unless they are up to date;
 // returns MSBG_EUNSUPP when the point set is too large to bin
 int ensure_binned(msbg_grid *g);
+// the reference's domain filter over d_raw[n][3] (device), kept points into
+// g->d_xyz in their order
+int filter_points(msbg_grid *g, const float *d_raw, int64_t n, int64_t *nkept);
 // level bands: level = first band whose count is nonzero, else nLevels-1
 int launch_level_band(msbg_grid *g, const uint32_t *d_count, int band,
                       int nbands, int32_t *d_levels);

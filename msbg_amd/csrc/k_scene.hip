@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
@@ -70,7 +71,69 @@ __global__ void k_scene_points(const Obj *__restrict__ objs, int64_t nPts,
   keep[i] = ok ? 1 : 0;
 }
 
+// msbg_demo.cpp:356-360 for uploaded points: isInDomainRange, then
+// trunc(pos) inRange, per coordinate
+__global__ void k_points_keep(const float *__restrict__ xyz, int64_t n, float3 dmin, float3 dmax,
+                              int3 size, uint8_t *__restrict__ keep) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float lo[3] = {dmin.x, dmin.y, dmin.z}, hi[3] = {dmax.x, dmax.y, dmax.z};
+  const int sz[3] = {size.x, size.y, size.z};
+  bool ok = true;
+  for (int a = 0; a < 3 && ok; a++) {
+    const float p = xyz[3 * i + a];
+    if (p != p || p < lo[a] || p > hi[a]) {
+      ok = false;
+    } else {
+      const int ip = (int)p;  // truncate_to_int, p within the domain range
+      ok = ip >= 0 && ip <= sz[a] - 1;
+    }
+  }
+  keep[i] = ok ? 1 : 0;
+}
+
 }  // namespace
+
+namespace msbg {
+
+// The uploaded points d_raw[n][3] filtered into g->d_xyz, order kept.
+int filter_points(msbg_grid *g, const float *d_raw, int64_t n, int64_t *nkept) {
+  hipStream_t s = g->stream;
+  float dmin[3], dmax[3];
+  domain_bounds(g->sx, g->sy, g->sz, dmin, dmax);
+  MSBG_CHECK(n < ((int64_t)1 << 31), MSBG_EINVAL, "points_upload: %lld points exceed 2^31",
+             (long long)n);
+  MSBG_TRY(g->d_xyz.ensure((size_t)(3 * std::max<int64_t>(n, 1))));
+  int64_t m = 0;
+  if (n) {
+    DevBuf<uint8_t> keep;
+    DevBuf<int> d_num;
+    MSBG_TRY(keep.alloc((size_t)n));
+    MSBG_TRY(d_num.alloc(1));
+    hipLaunchKernelGGL(k_points_keep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_raw, n,
+                       make_float3(dmin[0], dmin[1], dmin[2]),
+                       make_float3(dmax[0], dmax[1], dmax[2]), make_int3(g->sx, g->sy, g->sz),
+                       keep.p);
+    MSBG_HIP(hipGetLastError());
+    const P3 *src = (const P3 *)d_raw;
+    P3 *dst = (P3 *)g->d_xyz.p;
+    size_t bytes = 0;
+    MSBG_HIP(hipcub::DeviceSelect::Flagged(nullptr, bytes, src, keep.p, dst, d_num.p, (int)n, s));
+    DevBuf<uint8_t> work;
+    MSBG_TRY(work.alloc(std::max<size_t>(bytes, 1)));
+    MSBG_HIP(hipcub::DeviceSelect::Flagged(work.p, bytes, src, keep.p, dst, d_num.p, (int)n, s));
+    int kept = 0;
+    MSBG_HIP(hipMemcpyAsync(&kept, d_num.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    MSBG_HIP(hipStreamSynchronize(s));
+    m = kept;
+  }
+  g->npoints = m;
+  g->bin_valid = false;
+  if (nkept) *nkept = m;
+  return 0;
+}
+
+}  // namespace msbg
 
 extern "C" int msbg_points_generate(msbg_grid *g, uint64_t seed, int nObj, float cmin, float cmax,
                                     float rmin, float rmax, int64_t nPts, float sigma,
