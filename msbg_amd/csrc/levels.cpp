@@ -201,9 +201,12 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
     std::swap(cl.d_pages.p, np.p);
     std::swap(cl.d_pages.n, np.n);
   }
-  // per call: the level's pool is lent to the working level for the call
-  // (given back on every return path), ghosts from the current level l+1,
-  // side-cars of both
+  // per call: the ghosts from the current level l+1 behind the level's
+  // pages, then the pool is lent to the working level for the sweeps (given
+  // back on every return path) and the side-cars of both are refreshed
+  if (W.nghost)
+    MSBG_TRY(launch_fill_ghost_pages(g, ch, level, haloOpt, W.d_ghost.p, W.d_gmask.p,
+                                     W.nghost, cl.d_pages.p, cl.npages));
   struct Lend {
     ChanLevel &a, &b;
     Lend(ChanLevel &x, ChanLevel &y) : a(x), b(y) { swap(); }
@@ -213,9 +216,6 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
       std::swap(a.d_pages.n, b.d_pages.n);
     }
   } lend(cl, wl);
-  if (W.nghost)
-    MSBG_TRY(launch_fill_ghost_pages(g, ch, level, haloOpt, W.d_ghost.p, W.d_gmask.p,
-                                     W.nghost, wl.d_pages.p, cl.npages));
   wl.side_valid = false;
   MSBG_TRY(sweep_prepare_level(g, wl, ch.storage));
 
