@@ -52,6 +52,32 @@ def test_tables_bitexact(S, bsx, nLevels):
     assert np.array_equal(gp["active"], o["active"])
 
 
+def test_points_upload_filter_edges():
+    """The device-side domain filter (isInDomainRange, then trunc(pos)
+    inRange, msbg_demo.cpp:356-360) against the oracle's, on NaN, infinities,
+    values next to both ends of the range and the domain's own faces; kept
+    points stay in upload order."""
+    note("upload filter edges")
+    S = 64
+    rng = np.random.default_rng(21)
+    f = np.float32
+    special = np.array([0.0, -0.0, 1e-30, -1e-30, 0.5, np.nextafter(f(0.5), f(0)), S - 1.0,
+                        S - 0.5, np.nextafter(f(S - 0.5), f(S)), np.nextafter(f(S), f(0)), S,
+                        S + 0.5, -0.5, np.nan, np.inf, -np.inf, 1e30, -1e30], np.float32)
+    pts = rng.uniform(-4, S + 4, (4000, 3)).astype(np.float32)
+    for k in range(3):  # every special value on every axis, the others inside
+        blk = np.full((special.size, 3), S / 2, np.float32)
+        blk[:, k] = special
+        pts = np.concatenate([pts, blk])
+    pts = pts[rng.permutation(pts.shape[0])]
+    with M.MultiresSparseGrid.create("f", S, S, S, 16, options=M.OPT_SINGLE_LEVEL, nLevels=2) as g:
+        nk = g.points_upload(pts)
+        got = g.points_download()
+    keep = O.domain_filter(S, S, S, pts) == 1
+    assert nk == int(keep.sum()) and 0 < nk < pts.shape[0]
+    assert np.array_equal(got.view(np.uint32), pts[keep].view(np.uint32))
+
+
 def test_regularize_multilevel_bitexact():
     note("regularize nLevels=4")
     nb = 16
