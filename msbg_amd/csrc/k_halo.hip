@@ -83,42 +83,52 @@ __device__ inline double to_raw_type(int storage, float v) {
   return (double)v;
 }
 
-// SparseGrid<Data_T>::interpolateGhost (sbg.h:3002-3095), options 0.
-__device__ double interpolate_ghost_raw(const LevelView &Lo, float px, float py, float pz) {
+// SparseGrid<Data_T>::interpolateGhost (sbg.h:3002-3095), options 0: the
+// arithmetic, with the corners supplied by corner(X, Y, Z, &val0, &have0),
+// which returns the raw value the reference reads for that coarse voxel.
+template <typename Corner>
+__device__ inline double interpolate_ghost_with(int storage, float px, float py, float pz,
+                                                Corner corner) {
 #pragma clang fp contract(off)
   const float po = 0.5f;
   float x = px - po, y = py - po, z = pz - po;
   int ix = (int)floorf(x), iy = (int)floorf(y), iz = (int)floorf(z);
   float u = x - (float)ix, v = y - (float)iy, w = z - (float)iz;
   double bx[2], by[2], bzv[2];
-  double val0 = Lo.empty_raw;
-  bool have0 = false;
   for (int k = 0; k < 2; k++) {
     for (int j = 0; j < 2; j++) {
-      for (int i = 0; i < 2; i++) {
-        int X = ix + i, Y = iy + j, Z = iz + k;
-        double val;
-        if (!in_range(Lo, X, Y, Z)) {
-          clip3(Lo, X, Y, Z);
-          val = get_value_raw(Lo, X, Y, Z);
-        } else if (code_at(Lo, X, Y, Z) == MSBG_BLOCK_NULL) {
-          if (!have0) {
-            int qx = (int)floorf(px), qy = (int)floorf(py), qz = (int)floorf(pz);
-            clip3(Lo, qx, qy, qz);
-            val0 = get_value_raw(Lo, qx, qy, qz);
-            have0 = true;
-          }
-          val = val0;
-        } else {
-          val = block_raw(Lo, code_at(Lo, X, Y, Z), vid_at(Lo, X, Y, Z));
-        }
-        bx[i] = val;
-      }
-      by[j] = to_raw_type(Lo.storage, (float)bx[1] * u + (float)bx[0] * (1.0f - u));
+      for (int i = 0; i < 2; i++) bx[i] = corner(ix + i, iy + j, iz + k);
+      by[j] = to_raw_type(storage, (float)bx[1] * u + (float)bx[0] * (1.0f - u));
     }
-    bzv[k] = to_raw_type(Lo.storage, (float)by[1] * v + (float)by[0] * (1.0f - v));
+    bzv[k] = to_raw_type(storage, (float)by[1] * v + (float)by[0] * (1.0f - v));
   }
-  return to_raw_type(Lo.storage, (float)bzv[1] * w + (float)bzv[0] * (1.0f - w));
+  return to_raw_type(storage, (float)bzv[1] * w + (float)bzv[0] * (1.0f - w));
+}
+
+// The corners from the level's block map: out of range, the clipped
+// voxel's value; in a NULL block, the value of the voxel holding the sample
+// point (clipped); else the voxel itself.
+__device__ double interpolate_ghost_raw(const LevelView &Lo, float px, float py, float pz) {
+  double val0 = Lo.empty_raw;
+  bool have0 = false;
+  auto corner = [&](int X, int Y, int Z) -> double {
+    if (!in_range(Lo, X, Y, Z)) {
+      clip3(Lo, X, Y, Z);
+      return get_value_raw(Lo, X, Y, Z);
+    }
+    const int c = code_at(Lo, X, Y, Z);
+    if (c == MSBG_BLOCK_NULL) {
+      if (!have0) {
+        int qx = (int)floorf(px), qy = (int)floorf(py), qz = (int)floorf(pz);
+        clip3(Lo, qx, qy, qz);
+        val0 = get_value_raw(Lo, qx, qy, qz);
+        have0 = true;
+      }
+      return val0;
+    }
+    return block_raw(Lo, c, vid_at(Lo, X, Y, Z));
+  };
+  return interpolate_ghost_with(Lo.storage, px, py, pz, corner);
 }
 
 // fillHaloBlock_ per-voxel rule + getOutOfBlockValue.
@@ -514,6 +524,86 @@ __global__ void k_fill_ghost_pages(LevelView L, LevelView Lo, unsigned opt,
   ((R *)pages)[(firstPage + i) * ((int64_t)B * B * B) + v] = out;
 }
 
+// The INTERPOLATE fill with the coarse neighbourhood staged in LDS: one
+// workgroup per ghost page. Its voxels interpolate level Lo at
+// ((x+.5)/2, ...), whose corners lie in the page's coarse voxels grown by
+// one, a (B/2+2)^3 region. Each region voxel is staged once as the value
+// interpolateGhost reads there (out of range: the clipped voxel's value;
+// else getValue's), with a flag for voxels of NULL blocks, whose corners
+// take the value at the sample point's voxel. The arithmetic is
+// interpolate_ghost_with's, so the pages equal k_fill_ghost_pages'.
+template <int ST, int B>
+__global__ __launch_bounds__(256) void k_fill_ghost_interp(LevelView L, LevelView Lo,
+                                                           const int32_t *__restrict__ bids,
+                                                           const uint32_t *__restrict__ masks,
+                                                           int64_t firstPage,
+                                                           uint8_t *__restrict__ pages) {
+  using R = typename Stor<ST>::raw;
+  constexpr int HB = B / 2 + 2, HV = HB * HB * HB, M = B - 1, PER = 4 * B - 4;
+  constexpr int NSHELL = 2 * B * B + (B - 2) * PER;
+  __shared__ float sval[HV];
+  __shared__ uint8_t snull[HV];
+  const int64_t i = blockIdx.x;
+  const int bid = bids[i];
+  const int bx = bid % L.nbx, by = (bid / L.nbx) % L.nby, bz = bid / (L.nbx * L.nby);
+  const int cx0 = bx * (B / 2) - 1, cy0 = by * (B / 2) - 1, cz0 = bz * (B / 2) - 1;
+  for (int e = threadIdx.x; e < HV; e += blockDim.x) {
+    const int hx = e % HB, hy = (e / HB) % HB, hz = e / (HB * HB);
+    int X = cx0 + hx, Y = cy0 + hy, Z = cz0 + hz;
+    bool isNull = false;
+    if (in_range(Lo, X, Y, Z))
+      isNull = code_at(Lo, X, Y, Z) == MSBG_BLOCK_NULL;
+    else
+      clip3(Lo, X, Y, Z);
+    sval[e] = (float)get_value_raw(Lo, X, Y, Z);
+    snull[e] = isNull ? 1 : 0;
+  }
+  __syncthreads();
+  const uint32_t mk = masks ? masks[i] : 0xffffffffu;
+  for (int s = threadIdx.x; s < NSHELL; s += blockDim.x) {
+    int vx, vy, vz;
+    if (s < 2 * B * B) {
+      vz = s < B * B ? 0 : M;
+      const int r = s - (vz ? B * B : 0);
+      vy = r / B;
+      vx = r - vy * B;
+    } else {
+      const int t = s - 2 * B * B, q = t % PER;
+      vz = 1 + t / PER;
+      if (q < 2 * B) {
+        vy = q < B ? 0 : M;
+        vx = q < B ? q : q - B;
+      } else {
+        vy = 1 + (q - 2 * B) / 2;
+        vx = ((q - 2 * B) & 1) ? M : 0;
+      }
+    }
+    const int cx = vx == 0 ? -1 : (vx == M ? 1 : 0), cy = vy == 0 ? -1 : (vy == M ? 1 : 0),
+              cz = vz == 0 ? -1 : (vz == M ? 1 : 0);
+    bool need = false;
+    for (int k = 1; k < 8 && !need; k++) {
+      if (((k & 1) && !cx) || ((k & 2) && !cy) || ((k & 4) && !cz)) continue;
+      const int dx = (k & 1) ? cx : 0, dy = (k & 2) ? cy : 0, dz = (k & 4) ? cz : 0;
+      need = (mk >> ((dz + 1) * 9 + (dy + 1) * 3 + dx + 1)) & 1u;
+    }
+    if (!need) continue;
+    const int x = bx * B + vx, y = by * B + vy, z = bz * B + vz;
+    const float px = ((float)x + 0.5f) / 2, py = ((float)y + 0.5f) / 2, pz = ((float)z + 0.5f) / 2;
+    auto corner = [&](int X, int Y, int Z) -> double {
+      const int e = ((Z - cz0) * HB + (Y - cy0)) * HB + (X - cx0);
+      if (!snull[e]) return sval[e];
+      // the sample point's voxel lies in the domain and in the region
+      return sval[(((int)floorf(pz) - cz0) * HB + ((int)floorf(py) - cy0)) * HB +
+                  ((int)floorf(px) - cx0)];
+    };
+    const double raw = interpolate_ghost_with(Lo.storage, px, py, pz, corner);
+    R out;
+    if constexpr (ST == MSBG_STORAGE_U16) out = (R)(uint16_t)raw;
+    else out = Stor<ST>::enc((float)raw);
+    ((R *)pages)[(firstPage + i) * ((int64_t)B * B * B) + (((vz * B) + vy) * B + vx)] = out;
+  }
+}
+
 }  // namespace
 
 int launch_halo_fill(msbg_grid *g, Channel &ch, int level, int bid, int dmax, unsigned opt,
@@ -533,6 +623,27 @@ int launch_fill_ghost_pages(msbg_grid *g, Channel &ch, int level, unsigned opt,
                             uint8_t *d_pages, int64_t firstPage) {
   LevelView L = make_view(g, ch, level);
   LevelView Lo = make_view(g, ch, level + 1);
+  if ((opt & MSBG_OPT_INTERPOLATE) && n > 0 && (L.bsx == 8 || L.bsx == 16 || L.bsx == 32) &&
+      ch.storage != MSBG_STORAGE_F64) {
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3((unsigned)n), dim3(256), 0, g->stream, L, Lo, d_bids, d_mask,
+                         firstPage, d_pages);
+    };
+    auto by_size = [&](auto st) {
+      constexpr int ST = decltype(st)::value;
+      if (L.bsx == 8) go(k_fill_ghost_interp<ST, 8>);
+      else if (L.bsx == 16) go(k_fill_ghost_interp<ST, 16>);
+      else go(k_fill_ghost_interp<ST, 32>);
+    };
+    switch (ch.storage) {
+      case MSBG_STORAGE_U16: by_size(std::integral_constant<int, MSBG_STORAGE_U16>{}); break;
+      case MSBG_STORAGE_F32: by_size(std::integral_constant<int, MSBG_STORAGE_F32>{}); break;
+      case MSBG_STORAGE_F16: by_size(std::integral_constant<int, MSBG_STORAGE_F16>{}); break;
+      default: by_size(std::integral_constant<int, MSBG_STORAGE_BF16>{}); break;
+    }
+    MSBG_HIP(hipGetLastError());
+    return 0;
+  }
   const int B = L.bsx, nshell = 2 * B * B + (B - 2) * (4 * B - 4);
   for (int64_t off = 0; off < n; off += 65535) {
     const unsigned cnt = (unsigned)(n - off < 65535 ? n - off : 65535);
