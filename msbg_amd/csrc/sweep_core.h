@@ -271,6 +271,9 @@ __device__ inline float pde7_exact(const float *D, const float *M, const float *
 // ---- side-car refresh ---------------------------------------------------
 // faces [page][2][z][y] (x = 0 and x = bsx-1), then edges [page][4][z]
 // (edge k at x = (k&1) ? bsx-1 : 0, y = (k&2) ? bsx-1 : 0)
+// One thread per row (z, y) of the page: its first and last voxel go to the
+// x-faces [side][z][y], and rows y = 0 and y = M also give the z-edges
+// [k][z], k = (x == M) + 2 (y == M). Both reads share the row's cache line.
 template <int BSX, typename R>
 __global__ void k_refresh_side(const R *__restrict__ pages, R *__restrict__ side, int64_t npages) {
   constexpr int M = BSX - 1, PS = BSX * BSX, FS = 2 * PS, ES = 4 * BSX;
@@ -279,13 +282,15 @@ __global__ void k_refresh_side(const R *__restrict__ pages, R *__restrict__ side
   const R *pg = pages + p * PE;
   R *faces = side + p * FS;
   R *edges = side + npages * FS + p * ES;
-  for (int t = threadIdx.x; t < FS + ES; t += blockDim.x) {
-    if (t < FS) {
-      int sd = t / PS, z = (t % PS) / BSX, y = t % BSX;
-      faces[t] = pg[z * PS + y * BSX + (sd ? M : 0)];
-    } else {
-      int u = t - FS, k = u / BSX, z = u % BSX;
-      edges[u] = pg[z * PS + ((k & 2) ? M : 0) * BSX + ((k & 1) ? M : 0)];
+  for (int r = threadIdx.x; r < PS; r += blockDim.x) {
+    const R a = pg[r * BSX], b = pg[r * BSX + M];
+    faces[r] = a;
+    faces[PS + r] = b;
+    const int z = r / BSX, y = r % BSX;
+    if (y == 0 || y == M) {
+      const int k = y ? 2 : 0;
+      edges[k * BSX + z] = a;
+      edges[(k + 1) * BSX + z] = b;
     }
   }
 }
