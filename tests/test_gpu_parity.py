@@ -78,6 +78,39 @@ def test_points_upload_filter_edges():
     assert np.array_equal(got.view(np.uint32), pts[keep].view(np.uint32))
 
 
+@pytest.mark.parametrize("storage", [M.U16, M.F16])
+def test_empty_point_sets(storage):
+    """No particles (none uploaded, or all outside the domain): activation
+    counts nothing, every band leaves the coarsest level, the splat leaves
+    the codes at 65535 (UNORM 1.0), and a later non-empty upload replaces
+    the cached cell sort."""
+    note("empty point sets", storage)
+    S, bsx = 64, 16
+    nb = S // bsx
+    with M.MultiresSparseGrid.create("e", S, S, S, bsx, options=M.OPT_SINGLE_LEVEL, nLevels=2) as g:
+        kind = np.full(nb ** 3, 3, np.uint8)
+        g.channel_define(M.CH_UINT16_1, storage, 0, kind, 0.0, None,
+                         65535.0 if storage == M.U16 else 0.0)
+        for pts in (np.zeros((0, 3), np.float32), np.full((10, 3), -5.0, np.float32)):
+            assert g.points_upload(pts) == 0
+            assert g.points_download().shape == (0, 3)
+            assert not g.activate(4.0).any()
+            g.splat(M.CH_UINT16_1, 2.0, 2.0)
+            raw = g.download(M.CH_UINT16_1).view(np.uint16)
+            assert (raw == 65535).all()
+        # a non-empty set after the empty ones: the splat sees it
+        pts = np.array([[20.5, 20.5, 20.5]], np.float32)
+        assert g.points_upload(pts) == 1
+        assert g.activate(4.0).sum() > 0
+        g.splat(M.CH_UINT16_1, 2.0, 2.0)
+        raw = g.download(M.CH_UINT16_1).view(np.uint16)
+        pm, npg = g.pagemap(M.CH_UINT16_1, 0)
+        L = O.Level(S, S, S, bsx, pm, np.full((npg, bsx, bsx, bsx), 65535, np.uint16), O.U16)
+        O.splat_u16(L, pts, 4.0)
+        assert np.array_equal(raw.reshape(L.pages.shape), L.pages)
+        assert (raw < 65535).sum() > 100
+
+
 def test_regularize_multilevel_bitexact():
     note("regularize nLevels=4")
     nb = 16
