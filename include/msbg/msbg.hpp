@@ -168,6 +168,20 @@ class MultiresSparseGrid {
         densThrTol1, densThrTol2, numIter, gridSpacing, timeStep, residual));
   }
 
+  // The same 8-colour MCF sweep on the blocks of one coarser level, on that
+  // level's own grid (no counterpart: the reference sweeps level 0 only,
+  // msbg4.cpp:467-470). haloOpt is OPT_BC_NEUMANN, OPT_BC_COARSE_LEVEL or
+  // OPT_BC_COARSE_LEVEL | OPT_INTERPOLATE; residual (optional) gets one
+  // FP64 RMS update per sweep.
+  void applyChannelPdeLevel(int chan, int level, unsigned haloOpt, std::vector<int> *blockList,
+                            int numIter, float timeStep, int doConstrZeroOne,
+                            double *residual = nullptr) {
+    check(msbg_apply_channel_pde_level(g_, chan, level, haloOpt,
+                                       blockList ? blockList->data() : nullptr,
+                                       blockList ? (int64_t)blockList->size() : 0, numIter,
+                                       timeStep, doConstrZeroOne, residual));
+  }
+
   // ---- demo phases (msbg_demo.cpp:339-680) ----
   int64_t uploadParticles(const float *xyz, int64_t n) {
     int64_t kept = 0;
