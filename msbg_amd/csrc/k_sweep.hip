@@ -119,7 +119,9 @@ int prepare_bsx(msbg_grid *g, Channel &ch) {
 
 }  // namespace
 
-int default_tile(int bsx) { return bsx >= 32 ? 1 : 0; }
+// 4x2 voxel columns from 16^3 up: measured on the 4096^3 scene's level 1
+// (16^3 blocks, FP16) at 6.1e11 voxel-updates/s against 5.8e11 with 2x2
+int default_tile(int bsx) { return bsx >= 16 ? 1 : 0; }
 
 int sweep_prepare_level(msbg_grid *g, ChanLevel &cl, int storage) {
   auto st = [&](auto bsxc) -> int {

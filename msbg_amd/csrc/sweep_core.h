@@ -310,11 +310,13 @@ struct Src {
   }
 };
 
-// Voxel column per thread (PX x PY): the default is 4 x 2 for 32^3 blocks
-// (128 threads) and 2 x 2 for smaller blocks. Measured on MI355X (FP16
-// pages, 32^3 blocks): 4x2 with one plane of prefetch ran 10% faster than
-// 2x2; 2x4, 4x4, 8x2 (register pressure) and a two-plane prefetch (lower
-// occupancy) were slower. Batches pick per block size with an autotuner.
+// Voxel column per thread (PX x PY): the fast default is 4 x 2 for 32^3 and
+// 16^3 blocks (128 and 32 threads) and 2 x 2 for 8^3 blocks; exact
+// arithmetic keeps 2 x 2 below 32^3. Measured on MI355X (FP16 pages, 32^3
+// blocks): 4x2 with one plane of prefetch ran 10% faster than 2x2; 2x4,
+// 4x4, 8x2 (register pressure) and a two-plane prefetch (lower occupancy)
+// were slower. At 16^3, 4x2 ran 6% faster than 2x2. Batches pick per block
+// size with an autotuner.
 template <int BSX, int PX, int PY>
 struct Tile {
   static constexpr int NT = (BSX / PX) * (BSX / PY);
