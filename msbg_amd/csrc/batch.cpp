@@ -275,7 +275,12 @@ int msbg_batch_get_tiles(msbg_batch *b, int *bsx, int *storage, int *tile, int c
     if (k < cap) {
       if (bsx) bsx[k] = G.bsx;
       if (storage) storage[k] = G.storage;
-      if (tile) tile[k] = G.tile < 0 ? default_tile(G.bsx) : G.tile;
+      if (tile) {
+        // the default follows the largest colour pass of the group
+        int64_t nmax = 0;
+        for (int c = 0; c < 8; c++) nmax = std::max<int64_t>(nmax, G.off[c + 1] - G.off[c]);
+        tile[k] = G.tile < 0 ? default_tile(G.bsx, nmax) : G.tile;
+      }
     }
     k++;
   }

@@ -276,7 +276,7 @@ int launch_bilap_items(hipStream_t s, int bsx, int storage, bool hard, const Swe
 // copy the listed pages and side-cars from pages to dst_pages of *d_grid
 int launch_copy_items(hipStream_t s, int bsx, int storage, const SweepGrid *d_grid,
                       const int2 *d_items, int64_t n);
-int default_tile(int bsx);
+int default_tile(int bsx, int64_t nItems);
 constexpr int kNumTiles = 2;  // voxel-column variants: 0 = 2x2, 1 = 4x2
 int ensure_nbr(msbg_grid *g, ChanLevel &cl, int level);
 int launch_reduce_residual(msbg_grid *g, const double *d_partial, int64_t n,

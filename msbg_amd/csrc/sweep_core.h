@@ -310,9 +310,10 @@ struct Src {
   }
 };
 
-// Voxel column per thread (PX x PY): the fast default is 4 x 2 for 32^3 and
-// 16^3 blocks (128 and 32 threads) and 2 x 2 for 8^3 blocks; exact
-// arithmetic keeps 2 x 2 below 32^3. Measured on MI355X (FP16 pages, 32^3
+// Voxel column per thread (PX x PY): the fast default is 4 x 2 for 32^3
+// blocks (128 threads), for 16^3 blocks in launches of at least 4096 (32
+// threads, two blocks per wave) and 2 x 2 otherwise; exact arithmetic keeps
+// 2 x 2 below 32^3. Measured on MI355X (FP16 pages, 32^3
 // blocks): 4x2 with one plane of prefetch ran 10% faster than 2x2; 2x4,
 // 4x4, 8x2 (register pressure) and a two-plane prefetch (lower occupancy)
 // were slower. At 16^3, 4x2 ran 6% faster than 2x2. Batches pick per block
