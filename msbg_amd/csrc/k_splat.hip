@@ -111,9 +111,26 @@ constexpr int kCell = 16, kHalo = 5, kTile = kCell + 2 * kHalo, kTileVox = kTile
 constexpr int kTileBlk = 125;  // blocks a tile can touch: (kTile/8 + 1)^3 at 8^3 blocks
 constexpr uint32_t kNoCode = 0xffffu;
 
+// Cells per axis (ncx..) and the cell keys: cells are numbered in 4x4x4
+// groups (nsx.. groups per axis, x fastest within and across groups), so
+// consecutive keys, and the particles sorted by them, stay close in space.
+// Keys of the partial groups at the domain's far faces are never used.
 struct CellGrid {
-  int ncx, ncy, ncz, sx, sy, sz;
+  int ncx, ncy, ncz, sx, sy, sz, nsx, nsy, nsz;
+  int64_t nkeys;
 };
+
+__host__ __device__ inline int cell_key(const CellGrid &G, int cx, int cy, int cz) {
+  return ((((cz >> 2) * G.nsy + (cy >> 2)) * G.nsx + (cx >> 2)) << 6) | ((cz & 3) << 4) |
+         ((cy & 3) << 2) | (cx & 3);
+}
+
+__device__ inline void key_cell(const CellGrid &G, int key, int &cx, int &cy, int &cz) {
+  const int s = key >> 6, l = key & 63;
+  cx = (s % G.nsx) * 4 + (l & 3);
+  cy = ((s / G.nsx) % G.nsy) * 4 + ((l >> 2) & 3);
+  cz = (s / (G.nsx * G.nsy)) * 4 + (l >> 4);
+}
 
 __device__ inline int cell_of(const CellGrid &G, float x, float y, float z) {
   auto ax = [](float p, int s, int n) {
@@ -122,7 +139,7 @@ __device__ inline int cell_of(const CellGrid &G, float x, float y, float z) {
     int c = (int)f / kCell;
     return c < n ? c : n - 1;
   };
-  return (ax(z, G.sz, G.ncz) * G.ncy + ax(y, G.sy, G.ncy)) * G.ncx + ax(x, G.sx, G.ncx);
+  return cell_key(G, ax(x, G.sx, G.ncx), ax(y, G.sy, G.ncy), ax(z, G.sz, G.ncz));
 }
 
 __global__ void k_cell_count(const float *__restrict__ xyz, int64_t n, CellGrid G,
@@ -151,7 +168,8 @@ struct CellOfColour {
   CellGrid G;
   int col;
   __device__ bool operator()(int c) const {
-    const int cx = c % G.ncx, cy = (c / G.ncx) % G.ncy, cz = c / (G.ncx * G.ncy);
+    int cx, cy, cz;
+    key_cell(G, c, cx, cy, cz);
     return ((cx & 1) | ((cy & 1) << 1) | ((cz & 1) << 2)) == col && cnt[c] != 0;
   }
 };
@@ -172,7 +190,8 @@ __global__ __launch_bounds__(kSplatThreads) void k_splat_tile(
   __shared__ uint32_t tile[kTileVox];
   __shared__ int32_t bcode[kTileBlk];
   const int cell = list[blockIdx.x];
-  const int cx = cell % G.ncx, cy = (cell / G.ncx) % G.ncy, cz = cell / (G.ncx * G.ncy);
+  int cx, cy, cz;
+  key_cell(G, cell, cx, cy, cz);
   const int tx0 = cx * kCell - kHalo, ty0 = cy * kCell - kHalo, tz0 = cz * kCell - kHalo;
   // blocks under the in-domain part of the tile
   const int bxa = (tx0 > 0 ? tx0 : 0) >> lg, bya = (ty0 > 0 ? ty0 : 0) >> lg,
@@ -399,8 +418,18 @@ inline unsigned grid_cap(int64_t n) {
 }
 
 CellGrid cell_grid(const msbg_grid *g) {
-  return CellGrid{(g->sx + kCell - 1) / kCell, (g->sy + kCell - 1) / kCell,
-                  (g->sz + kCell - 1) / kCell, g->sx, g->sy, g->sz};
+  CellGrid G{};
+  G.ncx = (g->sx + kCell - 1) / kCell;
+  G.ncy = (g->sy + kCell - 1) / kCell;
+  G.ncz = (g->sz + kCell - 1) / kCell;
+  G.sx = g->sx;
+  G.sy = g->sy;
+  G.sz = g->sz;
+  G.nsx = (G.ncx + 3) / 4;
+  G.nsy = (G.ncy + 3) / 4;
+  G.nsz = (G.ncz + 3) / 4;
+  G.nkeys = (int64_t)G.nsx * G.nsy * G.nsz * 64;
+  return G;
 }
 
 int splat_binned(msbg_grid *g, ChanLevel &cl, int esz, float rScan, float distSqMax,
@@ -408,7 +437,7 @@ int splat_binned(msbg_grid *g, ChanLevel &cl, int esz, float rScan, float distSq
   hipStream_t s = g->stream;
   MSBG_TRY(ensure_binned(g));
   const CellGrid G = cell_grid(g);
-  const int64_t nc = (int64_t)G.ncx * G.ncy * G.ncz;
+  const int64_t nc = G.nkeys;
   // the even cells (colour 0) are the most numerous colour
   const int64_t cap = (int64_t)((G.ncx + 1) / 2) * ((G.ncy + 1) / 2) * ((G.ncz + 1) / 2);
   DevBuf<uint32_t> ncol;
@@ -458,7 +487,7 @@ int ensure_binned(msbg_grid *g) {
   hipStream_t s = g->stream;
   const int64_t n = g->npoints;
   const CellGrid G = cell_grid(g);
-  const int64_t nc = (int64_t)G.ncx * G.ncy * G.ncz;
+  const int64_t nc = G.nkeys;
   if (n >= ((int64_t)1 << 31) || nc >= ((int64_t)1 << 31)) {
     set_error("binning: %lld particles / %lld cells exceed the 32-bit offsets", (long long)n,
               (long long)nc);
