@@ -419,7 +419,7 @@ int msbg_unet_scatter(msbg_grid *g, int chan, const int32_t *blocks, int n, cons
   if (!n) return 0;
   MSBG_TRY(launch_scatter32(g->stream, st, cl->d_page.p, cl->d_pages.p, d.p, n, y));
   MSBG_HIP(hipStreamSynchronize(g->stream));
-  cl->side_valid = false;
+  pages_written(*cl);
   return 0;
 }
 
@@ -480,7 +480,7 @@ int msbg_warm_start(msbg_grid *g, int chan, msbg_unet *u, int batch, double *ms)
   if (!rc && ms && hipEventElapsedTime(&t, e0, e1) == hipSuccess) *ms = t;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  cl.side_valid = false;
+  pages_written(cl);
   return rc;
 }
 

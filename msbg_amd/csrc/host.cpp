@@ -113,7 +113,7 @@ static int upload_pagemap(msbg_grid *g, ChanLevel &cl) {
   }
   MSBG_HIP(hipStreamSynchronize(g->stream));
   cl.nbr_valid = false;
-  cl.side_valid = false;
+  pages_written(cl);
   g->topo_version++;
   return 0;
 }
@@ -512,7 +512,7 @@ int msbg_channel_upload(msbg_grid *g, int chan, int level, const void *pages, in
              (long long)need);
   MSBG_HIP(hipStreamSynchronize(g->stream));
   if (need) MSBG_HIP(hipMemcpy(cl->d_pages.p, pages, (size_t)need, hipMemcpyHostToDevice));
-  cl->side_valid = false;
+  pages_written(*cl);
   return 0;
 }
 
@@ -530,7 +530,7 @@ static int block_io(msbg_grid *g, int chan, int level, int bid, void *data, bool
     MSBG_HIP(hipMemcpy(data, dev, sz, hipMemcpyDeviceToHost));
   } else {
     MSBG_HIP(hipMemcpy(dev, data, sz, hipMemcpyHostToDevice));
-    cl->side_valid = false;
+    pages_written(*cl);
   }
   return 0;
 }
@@ -572,7 +572,7 @@ int msbg_channel_copy(msbg_grid *g, int chanSrc, int chanDst, int dstStorage, in
   };
   MSBG_TRY(define_level(g, dst, level, kind.data(), conv(cs->empty_raw), conv(cs->full_raw), 0.0));
   MSBG_TRY(launch_convert(g, *cs, src->storage, dst.lv[level], dstStorage));
-  dst.lv[level].side_valid = false;
+  pages_written(dst.lv[level]);
   MSBG_HIP(hipStreamSynchronize(g->stream));
   return 0;
 }
@@ -621,7 +621,7 @@ int msbg_splat_points(msbg_grid *g, int chan, float rParticle, float nbDist) {
              "(the reference's lock-free colour scheme needs it to fit)",
              2 * (rScan + 0.5f), g->bsx0);
   MSBG_TRY(launch_splat(g, *ch, rScan));
-  cl->side_valid = false;
+  pages_written(*cl);
   MSBG_HIP(hipStreamSynchronize(g->stream));
   return 0;
 }
@@ -631,7 +631,7 @@ int msbg_finalize_density(msbg_grid *g, int chan, float rParticle, float nbDist)
   ChanLevel *cl;
   MSBG_TRY(get_level(g, chan, 0, &ch, &cl));
   MSBG_TRY(launch_finalize(g, *ch, rParticle, nbDist));
-  cl->side_valid = false;
+  pages_written(*cl);
   MSBG_HIP(hipStreamSynchronize(g->stream));
   return 0;
 }
@@ -708,7 +708,7 @@ int msbg_prepare_ghost_blocks(msbg_grid *g, int chan) {
       MSBG_HIP(hipMemcpyAsync(d.p, todo.data(), sizeof(int32_t) * todo.size(),
                               hipMemcpyHostToDevice, g->stream));
       MSBG_TRY(launch_ghost(g, *ch, l, d.p, (int64_t)todo.size()));
-      lo.side_valid = false;
+      pages_written(lo);
       MSBG_HIP(hipStreamSynchronize(g->stream));
     }
   }

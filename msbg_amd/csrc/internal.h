@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <functional>
 #include <map>
@@ -150,8 +151,21 @@ struct ChanLevel {
   DevBuf<uint8_t> d_side;
   DevBuf<uint8_t> d_consts;          // 4 x empty value, 4 x full value (raw)
   bool side_valid = false;
+  // renewed by every write to the pages that leaves the side-cars stale (the
+  // sweeps that keep them current do not renew it): a coarse-level sweep
+  // compares it with the value after its own last call
+  uint64_t wstamp = 0;
   double empty_raw = 0, full_raw = 0;
 };
+
+// the pages of `cl` were written by something that did not update its
+// side-cars; stamps come from one process-wide counter, so a level that is
+// defined afresh (wstamp 0) never repeats a stamp another set-up recorded
+inline void pages_written(ChanLevel &cl) {
+  static std::atomic<uint64_t> clock{0};
+  cl.side_valid = false;
+  cl.wstamp = ++clock;
+}
 
 struct Channel {
   int id = 0;
@@ -172,6 +186,11 @@ struct LevelWork {
   DevBuf<int2> d_items;
   int64_t off[9] = {0};
   DevBuf<SweepGrid> d_desc;
+  // the level's wstamp when this set-up's last call ended: while it still
+  // holds (levels >= 1, whose every other writer renews it), wl's side-cars
+  // of the level's own pages are current and only the ghost pages' are
+  // rebuilt
+  uint64_t side_stamp = ~0ull;
 };
 
 }  // namespace msbg
@@ -249,8 +268,9 @@ int launch_ghost(msbg_grid *g, Channel &ch, int level, const int32_t *d_bids,
                  int64_t n);
 // sweep kernel of one colour pass over work items {grid, page}
 int sweep_prepare(msbg_grid *g, Channel &ch);  // side-car faces/edges + consts
-// the same for any channel level (its own block size) or a working copy
-int sweep_prepare_level(msbg_grid *g, ChanLevel &cl, int storage);
+// the same for any channel level (its own block size) or a working copy;
+// when the side-cars are stale, pages [fromPage, npages) are refreshed
+int sweep_prepare_level(msbg_grid *g, ChanLevel &cl, int storage, int64_t fromPage = 0);
 // ghost pages of a coarse-level sweep (level `level`, values from level+1)
 int launch_fill_ghost_pages(msbg_grid *g, Channel &ch, int level, unsigned opt,
                             const int32_t *d_bids, const uint32_t *d_mask, int64_t n,

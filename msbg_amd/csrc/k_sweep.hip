@@ -34,7 +34,7 @@ namespace {
 using namespace sweep;
 
 template <int BSX, int ST>
-int ensure_side(msbg_grid *g, ChanLevel &cl) {
+int ensure_side(msbg_grid *g, ChanLevel &cl, int64_t fromPage = 0) {
   using R = typename Stor<ST>::raw;
   if (cl.side_valid) return 0;
   const int64_t per = 2 * BSX * BSX + 4 * BSX;
@@ -46,9 +46,10 @@ int ensure_side(msbg_grid *g, ChanLevel &cl) {
     c[4 + k] = raw_value<ST>(cl.full_raw);
   }
   MSBG_HIP(hipMemcpyAsync(cl.d_consts.p, c, sizeof c, hipMemcpyHostToDevice, g->stream));
-  if (cl.npages)
-    hipLaunchKernelGGL((k_refresh_side<BSX, R>), dim3((unsigned)cl.npages), dim3(256), 0, g->stream,
-                       (const R *)cl.d_pages.p, (R *)cl.d_side.p, cl.npages);
+  if (cl.npages > fromPage)
+    hipLaunchKernelGGL((k_refresh_side<BSX, R>), dim3((unsigned)(cl.npages - fromPage)), dim3(256),
+                       0, g->stream, (const R *)cl.d_pages.p, (R *)cl.d_side.p, cl.npages,
+                       fromPage);
   MSBG_HIP(hipGetLastError());
   MSBG_HIP(hipStreamSynchronize(g->stream));
   cl.side_valid = true;
@@ -56,8 +57,8 @@ int ensure_side(msbg_grid *g, ChanLevel &cl) {
 }
 
 template <int BSX, int ST>
-int prepare_t(msbg_grid *g, ChanLevel &cl) {
-  return ensure_side<BSX, ST>(g, cl);
+int prepare_t(msbg_grid *g, ChanLevel &cl, int64_t fromPage) {
+  return ensure_side<BSX, ST>(g, cl, fromPage);
 }
 
 template <int BSX, int ST, bool EXACT, int PX, int PY>
@@ -107,11 +108,11 @@ template <int BSX>
 int prepare_bsx(msbg_grid *g, Channel &ch) {
   ChanLevel &cl = ch.lv[0];
   switch (ch.storage) {
-    case MSBG_STORAGE_U16: return prepare_t<BSX, MSBG_STORAGE_U16>(g, cl);
-    case MSBG_STORAGE_F32: return prepare_t<BSX, MSBG_STORAGE_F32>(g, cl);
-    case MSBG_STORAGE_F16: return prepare_t<BSX, MSBG_STORAGE_F16>(g, cl);
-    case MSBG_STORAGE_BF16: return prepare_t<BSX, MSBG_STORAGE_BF16>(g, cl);
-    case MSBG_STORAGE_F64: return prepare_t<BSX, MSBG_STORAGE_F64>(g, cl);
+    case MSBG_STORAGE_U16: return prepare_t<BSX, MSBG_STORAGE_U16>(g, cl, 0);
+    case MSBG_STORAGE_F32: return prepare_t<BSX, MSBG_STORAGE_F32>(g, cl, 0);
+    case MSBG_STORAGE_F16: return prepare_t<BSX, MSBG_STORAGE_F16>(g, cl, 0);
+    case MSBG_STORAGE_BF16: return prepare_t<BSX, MSBG_STORAGE_BF16>(g, cl, 0);
+    case MSBG_STORAGE_F64: return prepare_t<BSX, MSBG_STORAGE_F64>(g, cl, 0);
   }
   set_error("sweep: bad storage %d", ch.storage);
   return MSBG_EINVAL;
@@ -126,15 +127,15 @@ int prepare_bsx(msbg_grid *g, Channel &ch) {
 // time lose 6 % with half the workgroups
 int default_tile(int bsx, int64_t n) { return bsx >= 32 || (bsx == 16 && n >= 4096) ? 1 : 0; }
 
-int sweep_prepare_level(msbg_grid *g, ChanLevel &cl, int storage) {
+int sweep_prepare_level(msbg_grid *g, ChanLevel &cl, int storage, int64_t fromPage) {
   auto st = [&](auto bsxc) -> int {
     constexpr int B = decltype(bsxc)::value;
     switch (storage) {
-      case MSBG_STORAGE_U16: return prepare_t<B, MSBG_STORAGE_U16>(g, cl);
-      case MSBG_STORAGE_F32: return prepare_t<B, MSBG_STORAGE_F32>(g, cl);
-      case MSBG_STORAGE_F16: return prepare_t<B, MSBG_STORAGE_F16>(g, cl);
-      case MSBG_STORAGE_BF16: return prepare_t<B, MSBG_STORAGE_BF16>(g, cl);
-      case MSBG_STORAGE_F64: return prepare_t<B, MSBG_STORAGE_F64>(g, cl);
+      case MSBG_STORAGE_U16: return prepare_t<B, MSBG_STORAGE_U16>(g, cl, fromPage);
+      case MSBG_STORAGE_F32: return prepare_t<B, MSBG_STORAGE_F32>(g, cl, fromPage);
+      case MSBG_STORAGE_F16: return prepare_t<B, MSBG_STORAGE_F16>(g, cl, fromPage);
+      case MSBG_STORAGE_BF16: return prepare_t<B, MSBG_STORAGE_BF16>(g, cl, fromPage);
+      case MSBG_STORAGE_F64: return prepare_t<B, MSBG_STORAGE_F64>(g, cl, fromPage);
     }
     set_error("sweep: bad storage %d", storage);
     return MSBG_EINVAL;

@@ -216,8 +216,13 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
       std::swap(a.d_pages.n, b.d_pages.n);
     }
   } lend(cl, wl);
+  // wl's side-cars of the level's own pages are still the ones the last call
+  // left if nothing has written the level since; then only the ghost pages'
+  // are rebuilt. Level 0 has sweeps that keep its own side-cars current
+  // without renewing the stamp, so it is always rebuilt whole.
+  const int64_t sideFrom = (level >= 1 && W.side_stamp == cl.wstamp) ? cl.npages : 0;
   wl.side_valid = false;
-  MSBG_TRY(sweep_prepare_level(g, wl, ch.storage));
+  MSBG_TRY(sweep_prepare_level(g, wl, ch.storage, sideFrom));
 
   SweepGrid desc{};
   desc.pages = wl.d_pages.p;
@@ -240,6 +245,9 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
   }
   const bool exact = g->arith == MSBG_ARITH_EXACT;
   int64_t npasses = 0;
+  // the level's pages change in place from here on: its own side-cars
+  // (which level-0 sweeps read) are stale, also if a launch fails
+  pages_written(cl);
   MSBG_HIP(hipEventRecord(g->ev[0], g->stream));
   for (int iter = 0; iter < nMaxIter; iter++) {
     for (int pass = 0; pass < 8; pass++) {
@@ -253,8 +261,8 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
     if (residual) MSBG_TRY(launch_reduce_residual(g, d_partial.p, ntot, d_resid.p + iter));
   }
   MSBG_HIP(hipEventRecord(g->ev[1], g->stream));
-  // the level's pages changed in place; its side-cars are stale now
-  cl.side_valid = false;
+  // the sweeps kept wl's side-cars of the pages they wrote current
+  W.side_stamp = cl.wstamp;
   if (residual) {
     MSBG_HIP(hipMemcpyAsync(residual, d_resid.p, sizeof(double) * nMaxIter,
                             hipMemcpyDeviceToHost, g->stream));

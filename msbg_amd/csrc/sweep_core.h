@@ -274,11 +274,13 @@ __device__ inline float pde7_exact(const float *D, const float *M, const float *
 // One thread per row (z, y) of the page: its first and last voxel go to the
 // x-faces [side][z][y], and rows y = 0 and y = M also give the z-edges
 // [k][z], k = (x == M) + 2 (y == M). Both reads share the row's cache line.
+// Workgroup i refreshes page first + i of the pool's npages.
 template <int BSX, typename R>
-__global__ void k_refresh_side(const R *__restrict__ pages, R *__restrict__ side, int64_t npages) {
+__global__ void k_refresh_side(const R *__restrict__ pages, R *__restrict__ side, int64_t npages,
+                               int64_t first) {
   constexpr int M = BSX - 1, PS = BSX * BSX, FS = 2 * PS, ES = 4 * BSX;
   constexpr int64_t PE = (int64_t)PS * BSX;
-  const int64_t p = blockIdx.x;
+  const int64_t p = first + blockIdx.x;
   const R *pg = pages + p * PE;
   R *faces = side + p * FS;
   R *edges = side + npages * FS + p * ES;

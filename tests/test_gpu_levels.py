@@ -170,6 +170,17 @@ def test_level_sweep_cached_setup_follows_changes():
         O.mcf_sweeps_level(L[1], L[2], b1, levels, 1, opt, 2, 0.1, 1)
         g.applyChannelPdeLevel(CH, 1, opt, None, 2, 0.1, 1)
         assert np.array_equal(g.download(CH, 1), L[1].pages)
+        # the level's own side-cars carry over between calls only while
+        # nothing else writes the level: an upload, or a sweep of it under
+        # another halo rule (a set-up of its own), makes the next call
+        # rebuild them
+        new1 = np.random.default_rng(2).random(L[1].pages.shape).astype(np.float32)
+        g.upload(CH, 1, new1)
+        L[1].pages[...] = new1
+        for o in (opt, M.OPT_BC_NEUMANN, opt, opt):
+            O.mcf_sweeps_level(L[1], L[2], b1, levels, 1, o, 2, 0.1, 1)
+            g.applyChannelPdeLevel(CH, 1, o, None, 2, 0.1, 1)
+            assert np.array_equal(g.download(CH, 1), L[1].pages), o
         # a topology change (ghost pages of finer blocks appear) rebuilds it
         g.prepareGhostBlocks(CH)
         L = _olevels(g, S, bsx, M.F32, empty, full)
