@@ -539,8 +539,7 @@ __global__ __launch_bounds__(256) void k_fill_ghost_interp(LevelView L, LevelVie
                                                            int64_t firstPage,
                                                            uint8_t *__restrict__ pages) {
   using R = typename Stor<ST>::raw;
-  constexpr int HB = B / 2 + 2, HV = HB * HB * HB, M = B - 1, PER = 4 * B - 4;
-  constexpr int NSHELL = 2 * B * B + (B - 2) * PER;
+  constexpr int HB = B / 2 + 2, HV = HB * HB * HB, M = B - 1;
   __shared__ float sval[HV];
   __shared__ uint8_t snull[HV];
   const int64_t i = blockIdx.x;
@@ -560,47 +559,38 @@ __global__ __launch_bounds__(256) void k_fill_ghost_interp(LevelView L, LevelVie
   }
   __syncthreads();
   const uint32_t mk = masks ? masks[i] : 0xffffffffu;
-  for (int s = threadIdx.x; s < NSHELL; s += blockDim.x) {
-    int vx, vy, vz;
-    if (s < 2 * B * B) {
-      vz = s < B * B ? 0 : M;
-      const int r = s - (vz ? B * B : 0);
-      vy = r / B;
-      vx = r - vy * B;
-    } else {
-      const int t = s - 2 * B * B, q = t % PER;
-      vz = 1 + t / PER;
-      if (q < 2 * B) {
-        vy = q < B ? 0 : M;
-        vx = q < B ? q : q - B;
-      } else {
-        vy = 1 + (q - 2 * B) / 2;
-        vx = ((q - 2 * B) & 1) ? M : 0;
-      }
+  // The needed voxels are the union of the layers facing the swept
+  // neighbours: for direction d, the voxels with v = 0 on the axes where
+  // d < 0 and v = M where d > 0, free on the others. The workgroup walks the
+  // set directions in turn (a uniform loop) and each layer's voxels with its
+  // threads; a voxel shared by two layers is written twice, with one value.
+  for (int bit = 0; bit < 27; bit++) {
+    if (bit == 13 || !((mk >> bit) & 1u)) continue;
+    const int dx = bit % 3 - 1, dy = (bit / 3) % 3 - 1, dz = bit / 9 - 1;
+    const int size = (dx ? 1 : B) * (dy ? 1 : B) * (dz ? 1 : B);
+    for (int t = threadIdx.x; t < size; t += blockDim.x) {
+      int r = t, vx, vy, vz;
+      if (dx) vx = dx < 0 ? 0 : M;
+      else { vx = r % B; r /= B; }
+      if (dy) vy = dy < 0 ? 0 : M;
+      else { vy = r % B; r /= B; }
+      if (dz) vz = dz < 0 ? 0 : M;
+      else vz = r;
+      const int x = bx * B + vx, y = by * B + vy, z = bz * B + vz;
+      const float px = ((float)x + 0.5f) / 2, py = ((float)y + 0.5f) / 2, pz = ((float)z + 0.5f) / 2;
+      auto corner = [&](int X, int Y, int Z) -> double {
+        const int e = ((Z - cz0) * HB + (Y - cy0)) * HB + (X - cx0);
+        if (!snull[e]) return sval[e];
+        // the sample point's voxel lies in the domain and in the region
+        return sval[(((int)floorf(pz) - cz0) * HB + ((int)floorf(py) - cy0)) * HB +
+                    ((int)floorf(px) - cx0)];
+      };
+      const double raw = interpolate_ghost_with(Lo.storage, px, py, pz, corner);
+      R out;
+      if constexpr (ST == MSBG_STORAGE_U16) out = (R)(uint16_t)raw;
+      else out = Stor<ST>::enc((float)raw);
+      ((R *)pages)[(firstPage + i) * ((int64_t)B * B * B) + (((vz * B) + vy) * B + vx)] = out;
     }
-    const int cx = vx == 0 ? -1 : (vx == M ? 1 : 0), cy = vy == 0 ? -1 : (vy == M ? 1 : 0),
-              cz = vz == 0 ? -1 : (vz == M ? 1 : 0);
-    bool need = false;
-    for (int k = 1; k < 8 && !need; k++) {
-      if (((k & 1) && !cx) || ((k & 2) && !cy) || ((k & 4) && !cz)) continue;
-      const int dx = (k & 1) ? cx : 0, dy = (k & 2) ? cy : 0, dz = (k & 4) ? cz : 0;
-      need = (mk >> ((dz + 1) * 9 + (dy + 1) * 3 + dx + 1)) & 1u;
-    }
-    if (!need) continue;
-    const int x = bx * B + vx, y = by * B + vy, z = bz * B + vz;
-    const float px = ((float)x + 0.5f) / 2, py = ((float)y + 0.5f) / 2, pz = ((float)z + 0.5f) / 2;
-    auto corner = [&](int X, int Y, int Z) -> double {
-      const int e = ((Z - cz0) * HB + (Y - cy0)) * HB + (X - cx0);
-      if (!snull[e]) return sval[e];
-      // the sample point's voxel lies in the domain and in the region
-      return sval[(((int)floorf(pz) - cz0) * HB + ((int)floorf(py) - cy0)) * HB +
-                  ((int)floorf(px) - cx0)];
-    };
-    const double raw = interpolate_ghost_with(Lo.storage, px, py, pz, corner);
-    R out;
-    if constexpr (ST == MSBG_STORAGE_U16) out = (R)(uint16_t)raw;
-    else out = Stor<ST>::enc((float)raw);
-    ((R *)pages)[(firstPage + i) * ((int64_t)B * B * B) + (((vz * B) + vy) * B + vx)] = out;
   }
 }
 
