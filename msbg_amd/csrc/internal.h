@@ -271,10 +271,13 @@ int sweep_prepare(msbg_grid *g, Channel &ch);  // side-car faces/edges + consts
 // the same for any channel level (its own block size) or a working copy;
 // when the side-cars are stale, pages [fromPage, npages) are refreshed
 int sweep_prepare_level(msbg_grid *g, ChanLevel &cl, int storage, int64_t fromPage = 0);
-// ghost pages of a coarse-level sweep (level `level`, values from level+1)
+// ghost pages of a coarse-level sweep (level `level`, values from level+1);
+// with d_side (side-cars of a pool of nTotal pages) the INTERPOLATE fill also
+// writes the side-car entries of what it fills, and *sideDone says it did
 int launch_fill_ghost_pages(msbg_grid *g, Channel &ch, int level, unsigned opt,
                             const int32_t *d_bids, const uint32_t *d_mask, int64_t n,
-                            uint8_t *d_pages, int64_t firstPage);
+                            uint8_t *d_pages, int64_t firstPage, uint8_t *d_side,
+                            int64_t nTotal, bool *sideDone);
 SweepGrid sweep_desc(Channel &ch, float dt, int doConstr01);
 int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, int tile,
                        const SweepGrid *d_grids, const int2 *d_items, int64_t n,

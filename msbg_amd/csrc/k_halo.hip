@@ -537,9 +537,12 @@ __global__ __launch_bounds__(256) void k_fill_ghost_interp(LevelView L, LevelVie
                                                            const int32_t *__restrict__ bids,
                                                            const uint32_t *__restrict__ masks,
                                                            int64_t firstPage,
-                                                           uint8_t *__restrict__ pages) {
+                                                           uint8_t *__restrict__ pages,
+                                                           uint8_t *__restrict__ side,
+                                                           int64_t nTotal) {
   using R = typename Stor<ST>::raw;
   constexpr int HB = B / 2 + 2, HV = HB * HB * HB, M = B - 1;
+  constexpr int PS = B * B, FS = 2 * PS, ES = 4 * B;
   __shared__ float sval[HV];
   __shared__ uint8_t snull[HV];
   const int64_t i = blockIdx.x;
@@ -590,6 +593,16 @@ __global__ __launch_bounds__(256) void k_fill_ghost_interp(LevelView L, LevelVie
       if constexpr (ST == MSBG_STORAGE_U16) out = (R)(uint16_t)raw;
       else out = Stor<ST>::enc((float)raw);
       ((R *)pages)[(firstPage + i) * ((int64_t)B * B * B) + (((vz * B) + vy) * B + vx)] = out;
+      // side-cars of the pool's nTotal pages (when given): the voxel's entry
+      // in its page's x-faces [2][z][y] and z-edges [4][z], as the refresh
+      // kernel lays them out
+      if (side && (vx == 0 || vx == M)) {
+        R *sd = (R *)side;
+        const int64_t p = firstPage + i;
+        sd[p * FS + (vx ? PS : 0) + vz * B + vy] = out;
+        if (vy == 0 || vy == M)
+          sd[nTotal * FS + p * ES + ((vx == M) + 2 * (vy == M)) * B + vz] = out;
+      }
     }
   }
 }
@@ -610,14 +623,16 @@ int launch_halo_fill(msbg_grid *g, Channel &ch, int level, int bid, int dmax, un
 
 int launch_fill_ghost_pages(msbg_grid *g, Channel &ch, int level, unsigned opt,
                             const int32_t *d_bids, const uint32_t *d_mask, int64_t n,
-                            uint8_t *d_pages, int64_t firstPage) {
+                            uint8_t *d_pages, int64_t firstPage, uint8_t *d_side,
+                            int64_t nTotal, bool *sideDone) {
+  *sideDone = false;
   LevelView L = make_view(g, ch, level);
   LevelView Lo = make_view(g, ch, level + 1);
   if ((opt & MSBG_OPT_INTERPOLATE) && n > 0 && (L.bsx == 8 || L.bsx == 16 || L.bsx == 32) &&
       ch.storage != MSBG_STORAGE_F64) {
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3((unsigned)n), dim3(256), 0, g->stream, L, Lo, d_bids, d_mask,
-                         firstPage, d_pages);
+                         firstPage, d_pages, d_side, nTotal);
     };
     auto by_size = [&](auto st) {
       constexpr int ST = decltype(st)::value;
@@ -632,6 +647,7 @@ int launch_fill_ghost_pages(msbg_grid *g, Channel &ch, int level, unsigned opt,
       default: by_size(std::integral_constant<int, MSBG_STORAGE_BF16>{}); break;
     }
     MSBG_HIP(hipGetLastError());
+    *sideDone = d_side != nullptr;
     return 0;
   }
   const int B = L.bsx, nshell = 2 * B * B + (B - 2) * (4 * B - 4);

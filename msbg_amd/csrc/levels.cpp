@@ -204,9 +204,18 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
   // per call: the ghosts from the current level l+1 behind the level's
   // pages, then the pool is lent to the working level for the sweeps (given
   // back on every return path) and the side-cars of both are refreshed
+  // wl's side-cars of the level's own pages are still the ones the last call
+  // left if nothing has written the level since. Level 0 has sweeps that
+  // keep its own side-cars current without renewing the stamp, so it is
+  // always rebuilt whole. With the own side-cars current, the INTERPOLATE
+  // fill writes the ghost pages' side-car entries as it fills them (the
+  // sweep reads no others), else the ghost pages' are rebuilt.
+  const bool ownSide = level >= 1 && W.side_stamp == cl.wstamp;
+  bool ghostSide = false;
   if (W.nghost)
     MSBG_TRY(launch_fill_ghost_pages(g, ch, level, haloOpt, W.d_ghost.p, W.d_gmask.p,
-                                     W.nghost, cl.d_pages.p, cl.npages));
+                                     W.nghost, cl.d_pages.p, cl.npages,
+                                     ownSide ? wl.d_side.p : nullptr, wl.npages, &ghostSide));
   struct Lend {
     ChanLevel &a, &b;
     Lend(ChanLevel &x, ChanLevel &y) : a(x), b(y) { swap(); }
@@ -216,11 +225,7 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
       std::swap(a.d_pages.n, b.d_pages.n);
     }
   } lend(cl, wl);
-  // wl's side-cars of the level's own pages are still the ones the last call
-  // left if nothing has written the level since; then only the ghost pages'
-  // are rebuilt. Level 0 has sweeps that keep its own side-cars current
-  // without renewing the stamp, so it is always rebuilt whole.
-  const int64_t sideFrom = (level >= 1 && W.side_stamp == cl.wstamp) ? cl.npages : 0;
+  const int64_t sideFrom = !ownSide ? 0 : (ghostSide ? wl.npages : cl.npages);
   wl.side_valid = false;
   MSBG_TRY(sweep_prepare_level(g, wl, ch.storage, sideFrom));
 
