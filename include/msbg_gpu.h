@@ -415,7 +415,8 @@ int msbg_batch_add(msbg_batch *b, msbg_grid *g, int chan, float timeStep,
  * blocks (`passes` launches of every colour; the pages are restored after)
  * and keep the fastest; member grids adopt it for their own sweeps. */
 int msbg_batch_autotune(msbg_batch *b, int passes);
-/* Kernel groups of the batch and their variant (0 = 2x2, 1 = 4x2 columns). */
+/* Kernel groups of the batch and their variant (0 = 2x2, 1 = 4x2, 2 = 4x4
+ * voxel columns). */
 int msbg_batch_get_tiles(msbg_batch *b, int *bsx, int *storage, int *tile,
                          int cap, int *n);
 /* numIter rounded up to even sweeps of every member grid; useGraph != 0

@@ -32,6 +32,7 @@ struct msbg_batch {
   struct Group {
     int bsx = 0, storage = 0;
     bool exact = false;
+    bool c01 = false;  // every member clamps to [0,1]
     int tile = -1;
     std::vector<int> grids;  // entry indices
     DevBuf<int2> items;
@@ -81,7 +82,7 @@ int build(msbg_batch *b) {
   if (!stale(b)) return 0;
   b->built = false;
   drop_graph(b);
-  std::map<std::tuple<int, int, bool>, int> gi;
+  std::map<std::tuple<int, int, bool, bool>, int> gi;
   std::vector<SweepGrid> desc(b->entries.size());
   std::vector<std::vector<std::vector<int2>>> per;  // group -> colour -> items
   std::vector<msbg_batch::Group> groups;
@@ -98,13 +99,15 @@ int build(msbg_batch *b) {
     if (cl.npages) MSBG_TRY(sweep_prepare(g, ch));
     desc[e] = sweep_desc(ch, E.dt, E.c01);
     bool exact = g->arith == MSBG_ARITH_EXACT || ch.storage == MSBG_STORAGE_F64;
-    auto key = std::make_tuple(g->bsx0, ch.storage, exact);
+    const bool c01 = E.c01 != 0;
+    auto key = std::make_tuple(g->bsx0, ch.storage, exact, c01);
     if (!gi.count(key)) {
       gi[key] = (int)groups.size();
       groups.emplace_back();
       groups.back().bsx = g->bsx0;
       groups.back().storage = ch.storage;
       groups.back().exact = exact;
+      groups.back().c01 = c01;
       per.emplace_back(8);
     }
     int k = gi[key];
@@ -155,7 +158,7 @@ int enqueue(msbg_batch *b, int sweeps) {
       for (auto &G : b->groups) {
         int64_t n = G.off[pass + 1] - G.off[pass];
         if (!n) continue;
-        MSBG_TRY(launch_sweep_items(b->stream, G.bsx, G.storage, G.exact, G.tile, b->d_desc.p,
+        MSBG_TRY(launch_sweep_items(b->stream, G.bsx, G.storage, G.exact, G.c01, G.tile, b->d_desc.p,
                                     G.items.p + G.off[pass], n, nullptr));
       }
   return 0;
@@ -243,7 +246,7 @@ int msbg_batch_autotune(msbg_batch *b, int passes) {
         for (int pass = 0; pass < 8; pass++) {
           int64_t n = G.off[pass + 1] - G.off[pass];
           if (n)
-            MSBG_TRY(launch_sweep_items(b->stream, G.bsx, G.storage, false, tile, b->d_desc.p,
+            MSBG_TRY(launch_sweep_items(b->stream, G.bsx, G.storage, false, G.c01, tile, b->d_desc.p,
                                         G.items.p + G.off[pass], n, nullptr));
         }
       }
@@ -279,7 +282,7 @@ int msbg_batch_get_tiles(msbg_batch *b, int *bsx, int *storage, int *tile, int c
         // the default follows the largest colour pass of the group
         int64_t nmax = 0;
         for (int c = 0; c < 8; c++) nmax = std::max<int64_t>(nmax, G.off[c + 1] - G.off[c]);
-        tile[k] = G.tile < 0 ? default_tile(G.bsx, nmax) : G.tile;
+        tile[k] = G.tile < 0 ? default_tile(G.bsx, G.storage, nmax) : G.tile;
       }
     }
     k++;

@@ -799,7 +799,8 @@ int mcf_colour_inplace(msbg_grid *g, int chPhi, const int32_t *blockList, int64_
     for (int pass = 0; pass < 8; pass++) {
       int64_t n = off[pass + 1] - off[pass];
       if (!n) continue;
-      MSBG_TRY(launch_sweep_items(g->stream, g->bsx0, ch->storage, exact, tile, g->d_desc.p,
+      MSBG_TRY(launch_sweep_items(g->stream, g->bsx0, ch->storage, exact, doConstrZeroOne != 0,
+                                  tile, g->d_desc.p,
                                   g->d_sweep_list.p + off[pass], n,
                                   residual ? g->d_partial.p + off[pass] : nullptr));
       npasses++;

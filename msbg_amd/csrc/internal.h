@@ -279,7 +279,10 @@ int launch_fill_ghost_pages(msbg_grid *g, Channel &ch, int level, unsigned opt,
                             uint8_t *d_pages, int64_t firstPage, uint8_t *d_side,
                             int64_t nTotal, bool *sideDone);
 SweepGrid sweep_desc(Channel &ch, float dt, int doConstr01);
-int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, int tile,
+// c01: every grid of d_grids clamps to [0,1] (doConstrZeroOne), which
+// selects the fast kernel whose last fma clamps; the other kernels clamp to
+// each grid's own [lo,hi]
+int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, bool c01, int tile,
                        const SweepGrid *d_grids, const int2 *d_items, int64_t n,
                        double *d_partial);
 // the other applyChannelPdeFast operators (1, 3, 14, 15, 17) and MCF with
@@ -299,8 +302,8 @@ int launch_bilap_items(hipStream_t s, int bsx, int storage, bool hard, const Swe
 // copy the listed pages and side-cars from pages to dst_pages of *d_grid
 int launch_copy_items(hipStream_t s, int bsx, int storage, const SweepGrid *d_grid,
                       const int2 *d_items, int64_t n);
-int default_tile(int bsx, int64_t nItems);
-constexpr int kNumTiles = 2;  // voxel-column variants: 0 = 2x2, 1 = 4x2
+int default_tile(int bsx, int storage, int64_t nItems);
+constexpr int kNumTiles = 3;  // voxel-column variants: 0 = 2x2, 1 = 4x2, 2 = 4x4
 int ensure_nbr(msbg_grid *g, ChanLevel &cl, int level);
 int launch_reduce_residual(msbg_grid *g, const double *d_partial, int64_t n,
                            double *d_out);

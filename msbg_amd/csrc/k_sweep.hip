@@ -61,14 +61,14 @@ int prepare_t(msbg_grid *g, ChanLevel &cl, int64_t fromPage) {
   return ensure_side<BSX, ST>(g, cl, fromPage);
 }
 
-template <int BSX, int ST, bool EXACT, int PX, int PY>
+template <int BSX, int ST, bool EXACT, int PX, int PY, bool C01 = false>
 int launch_one(hipStream_t s, const SweepGrid *d_grids, const int2 *d_items, int64_t n,
                double *d_partial) {
   constexpr int NT = Tile<BSX, PX, PY>::NT;
   constexpr int B = Pack<NT>::BPW;
   for (int64_t off = 0; off < n; off += 0x40000000) {
     const int cnt = (int)(n - off < 0x40000000 ? n - off : 0x40000000);
-    hipLaunchKernelGGL((k_sweep<BSX, ST, EXACT, PX, PY, OP_MCF, false, B>),
+    hipLaunchKernelGGL((k_sweep<BSX, ST, EXACT, PX, PY, OP_MCF, false, B, C01>),
                        dim3((unsigned)((cnt + B - 1) / B)), dim3(NT * B), 0, s, d_grids,
                        d_items + off, d_partial ? d_partial + off : nullptr, 0, cnt);
     MSBG_HIP(hipGetLastError());
@@ -77,28 +77,34 @@ int launch_one(hipStream_t s, const SweepGrid *d_grids, const int2 *d_items, int
 }
 
 template <int BSX, int ST>
-int launch_st(hipStream_t s, bool exact, int tile, const SweepGrid *d_grids, const int2 *d_items,
-              int64_t n, double *d_partial) {
+int launch_st(hipStream_t s, bool exact, bool c01, int tile, const SweepGrid *d_grids,
+              const int2 *d_items, int64_t n, double *d_partial) {
   if constexpr (ST == MSBG_STORAGE_F64) exact = true;  // the FP64 run is always exact
   if (exact)  // the exact path keeps the default column shape
     return BSX >= 32 ? launch_one<BSX, ST, true, 4, 2>(s, d_grids, d_items, n, d_partial)
                      : launch_one<BSX, ST, true, 2, 2>(s, d_grids, d_items, n, d_partial);
   if constexpr (ST != MSBG_STORAGE_F64) {
-    if (tile == 1) return launch_one<BSX, ST, false, 4, 2>(s, d_grids, d_items, n, d_partial);
-    return launch_one<BSX, ST, false, 2, 2>(s, d_grids, d_items, n, d_partial);
+    if (tile == 2)
+      return c01 ? launch_one<BSX, ST, false, 4, 4, true>(s, d_grids, d_items, n, d_partial)
+                 : launch_one<BSX, ST, false, 4, 4>(s, d_grids, d_items, n, d_partial);
+    if (tile == 1)
+      return c01 ? launch_one<BSX, ST, false, 4, 2, true>(s, d_grids, d_items, n, d_partial)
+                 : launch_one<BSX, ST, false, 4, 2>(s, d_grids, d_items, n, d_partial);
+    return c01 ? launch_one<BSX, ST, false, 2, 2, true>(s, d_grids, d_items, n, d_partial)
+               : launch_one<BSX, ST, false, 2, 2>(s, d_grids, d_items, n, d_partial);
   }
   return 0;
 }
 
 template <int BSX>
-int launch_bsx(hipStream_t s, int storage, bool exact, int tile, const SweepGrid *d_grids,
-               const int2 *d_items, int64_t n, double *d_partial) {
+int launch_bsx(hipStream_t s, int storage, bool exact, bool c01, int tile,
+               const SweepGrid *d_grids, const int2 *d_items, int64_t n, double *d_partial) {
   switch (storage) {
-    case MSBG_STORAGE_U16: return launch_st<BSX, MSBG_STORAGE_U16>(s, exact, tile, d_grids, d_items, n, d_partial);
-    case MSBG_STORAGE_F32: return launch_st<BSX, MSBG_STORAGE_F32>(s, exact, tile, d_grids, d_items, n, d_partial);
-    case MSBG_STORAGE_F16: return launch_st<BSX, MSBG_STORAGE_F16>(s, exact, tile, d_grids, d_items, n, d_partial);
-    case MSBG_STORAGE_BF16: return launch_st<BSX, MSBG_STORAGE_BF16>(s, exact, tile, d_grids, d_items, n, d_partial);
-    case MSBG_STORAGE_F64: return launch_st<BSX, MSBG_STORAGE_F64>(s, exact, tile, d_grids, d_items, n, d_partial);
+    case MSBG_STORAGE_U16: return launch_st<BSX, MSBG_STORAGE_U16>(s, exact, c01, tile, d_grids, d_items, n, d_partial);
+    case MSBG_STORAGE_F32: return launch_st<BSX, MSBG_STORAGE_F32>(s, exact, c01, tile, d_grids, d_items, n, d_partial);
+    case MSBG_STORAGE_F16: return launch_st<BSX, MSBG_STORAGE_F16>(s, exact, c01, tile, d_grids, d_items, n, d_partial);
+    case MSBG_STORAGE_BF16: return launch_st<BSX, MSBG_STORAGE_BF16>(s, exact, c01, tile, d_grids, d_items, n, d_partial);
+    case MSBG_STORAGE_F64: return launch_st<BSX, MSBG_STORAGE_F64>(s, exact, c01, tile, d_grids, d_items, n, d_partial);
   }
   set_error("sweep: bad storage %d", storage);
   return MSBG_EINVAL;
@@ -120,12 +126,18 @@ int prepare_bsx(msbg_grid *g, Channel &ch) {
 
 }  // namespace
 
-// 4x2 voxel columns for 32^3 blocks; for 16^3 blocks when the launch holds
-// at least 4096 of them (two per workgroup, about 8 workgroups per CU):
-// measured on the 4096^3 scene's level 1 (16^3, FP16) at 6.1e11
+// Voxel columns of the fast path. 32^3 blocks: 4x4 for FP16 pages (one wave
+// per block, FP16 patches), 4x2 otherwise; measured on the 4096^3 scene
+// (FP16, fast): 3.25 ms per colour pass with 4x4 against 3.38 ms with 4x2,
+// 3.36 ms with 8x2 and 3.84 ms with 4x8 (register pressure). 16^3 blocks:
+// 4x2 when the launch holds at least 4096 of them (two per workgroup, about
+// 8 workgroups per CU): on the scene's level 1 (16^3, FP16) 6.1e11
 // voxel-updates/s against 5.8e11 with 2x2, while small grids swept one at a
-// time lose 6 % with half the workgroups
-int default_tile(int bsx, int64_t n) { return bsx >= 32 || (bsx == 16 && n >= 4096) ? 1 : 0; }
+// time lose 6 % with half the workgroups.
+int default_tile(int bsx, int storage, int64_t n) {
+  if (bsx >= 32) return storage == MSBG_STORAGE_F16 ? 2 : 1;
+  return bsx == 16 && n >= 4096 ? 1 : 0;
+}
 
 int sweep_prepare_level(msbg_grid *g, ChanLevel &cl, int storage, int64_t fromPage) {
   auto st = [&](auto bsxc) -> int {
@@ -175,14 +187,14 @@ SweepGrid sweep_desc(Channel &ch, float dt, int doConstr01) {
   return d;
 }
 
-int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, int tile,
+int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, bool c01, int tile,
                        const SweepGrid *d_grids, const int2 *d_items, int64_t n,
                        double *d_partial) {
-  if (tile < 0) tile = default_tile(bsx, n);
+  if (tile < 0) tile = default_tile(bsx, storage, n);
   switch (bsx) {
-    case 8: return launch_bsx<8>(s, storage, exact, tile, d_grids, d_items, n, d_partial);
-    case 16: return launch_bsx<16>(s, storage, exact, tile, d_grids, d_items, n, d_partial);
-    case 32: return launch_bsx<32>(s, storage, exact, tile, d_grids, d_items, n, d_partial);
+    case 8: return launch_bsx<8>(s, storage, exact, c01, tile, d_grids, d_items, n, d_partial);
+    case 16: return launch_bsx<16>(s, storage, exact, c01, tile, d_grids, d_items, n, d_partial);
+    case 32: return launch_bsx<32>(s, storage, exact, c01, tile, d_grids, d_items, n, d_partial);
   }
   set_error("sweep: block size %d not supported by the sweep kernel (8, 16, 32)", bsx);
   return MSBG_EUNSUPP;

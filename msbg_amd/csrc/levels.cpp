@@ -258,7 +258,8 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
     for (int pass = 0; pass < 8; pass++) {
       const int64_t n = off[pass + 1] - off[pass];
       if (!n) continue;
-      MSBG_TRY(launch_sweep_items(g->stream, cl.bsx, ch.storage, exact, -1, W.d_desc.p,
+      MSBG_TRY(launch_sweep_items(g->stream, cl.bsx, ch.storage, exact, doConstrZeroOne != 0, -1,
+                                  W.d_desc.p,
                                   W.d_items.p + off[pass], n,
                                   residual ? d_partial.p + off[pass] : nullptr));
       npasses++;

@@ -9,37 +9,20 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "internal.h"
 #include "storage.h"
 
 namespace msbg {
 namespace sweep {
 
-// compile-time configuration of the sweep kernel
-#ifndef MSBG_SWEEP_ALG
-#define MSBG_SWEEP_ALG 1  // fold -2 f0 of fxx/fyy/fzz into H = Np/G - 4 f0
-#endif
-#ifndef MSBG_SWEEP_PD
-#define MSBG_SWEEP_PD 1  // plane prefetch distance (1 or 2)
-#endif
-#ifndef MSBG_SWEEP_WAVES
-#define MSBG_SWEEP_WAVES 0  // minimum waves per SIMD requested (0 = none)
-#endif
-
-
 // ---- vector raw loads/stores -------------------------------------------
 // Page, side-car and const pointers come out of the grid descriptor as
 // generic pointers; casting them to the global address space at each access
 // makes the compiler emit global (not flat) loads and stores, which wait on
 // the vector-memory counter only.
-#ifndef MSBG_SWEEP_GLOBAL
-#define MSBG_SWEEP_GLOBAL 1
-#endif
-#if MSBG_SWEEP_GLOBAL
 #define MSBG_GLOBAL __attribute__((address_space(1)))
-#else
-#define MSBG_GLOBAL
-#endif
 template <typename T>
 __device__ inline T ldg(const T *p) {
   return *(const MSBG_GLOBAL T *)p;
@@ -117,39 +100,86 @@ __device__ inline C mcf_exact(const C *D, const C *M, const C *U, int c, C dt, b
   return phi;
 }
 
-// Fast evaluation of the same formula for a PX x PY column step. With
-// d* = 2 f* (central differences), c** = 4 f** (mixed derivatives) and
-// s** = the two neighbours' sum along an axis, the reference's
-// H / |grad|^2 equals
+// ---- the fast MCF update ----------------------------------------------------
+// One plane's (PX+2) x (PY+2) patch around a thread's PX x PY voxel column
+// (halo-local x = i-1, y = j-1). Float patches hold decoded values; FP16
+// patches keep the storage bits, two to a register, so that the arithmetic
+// can take them as FP16 operands of v_fma_mix_f32 without a conversion.
+typedef uint32_t __attribute__((may_alias)) u32a;
+typedef uint64_t __attribute__((may_alias)) u64a;
+typedef uint16_t __attribute__((may_alias)) u16a;
+typedef _Float16 h2x_t __attribute__((ext_vector_type(2)));
+
+template <typename L, int PX, int PY>
+struct Patch {
+  static constexpr int W = PX + 2;
+  L v[(PY + 2) * W];
+  __device__ inline L at(int j, int i) const { return v[j * W + i]; }
+};
+template <int PX, int PY>
+struct Patch<_Float16, PX, PY> {
+  static constexpr int WW = (PX + 2) / 2;
+  uint32_t w[PY + 2][WW];
+  __device__ inline _Float16 at(int j, int i) const {
+    return __builtin_bit_cast(_Float16, (uint16_t)(w[j][i >> 1] >> ((i & 1) * 16)));
+  }
+};
+
+// a + b and a - b of two patch values, one rounding in FP32. FP16 operands
+// go into v_fma_mix_f32 as a * one + b, each widened exactly inside the
+// instruction; `one` is 1.0 in a register the compiler does not look into,
+// so that it keeps the fma form instead of an add of two conversions.
+__device__ inline float vadd(float a, float b, float) { return a + b; }
+__device__ inline float vsub(float a, float b, float) { return a - b; }
+__device__ inline float vadd(_Float16 a, _Float16 b, float one) {
+  return __builtin_fmaf((float)a, one, (float)b);
+}
+__device__ inline float vsub(_Float16 a, _Float16 b, float one) {
+  return __builtin_fmaf((float)a, one, -(float)b);
+}
+__device__ inline float opaque_one() {
+  float one;
+  asm("s_mov_b32 %0, 1.0" : "=s"(one));
+  return one;
+}
+
+// Fast evaluation of the reference's update (msbg4.cpp:559-637) for a PX x PY
+// column step. With d* = 2 f* (central differences), c** = 4 f** (mixed
+// derivatives) and s** = the two neighbours' sum along an axis, the
+// reference's H / |grad|^2 equals
 //   Np / G - 4 f0,  Np = A sxx + B syy + C szz - 0.5 (dx dy cyx + dx dz czx + dy dz czy),
 //   A = dy^2+dz^2, B = dx^2+dz^2, C = dx^2+dy^2, G = dx^2+dy^2+dz^2,
 // because A+B+C = 2G folds the -2 f0 of fxx, fyy, fzz into one term; the
-// threshold |grad|^2 > 1e-7 becomes G > 4e-7. Central differences shared by
-// neighbouring voxels of the column are formed once.
-template <int PX, int PY>
-__device__ inline void mcf_fast(const float *D, const float *M, const float *U, float dt, float lo,
-                                float hi, float *out) {
+// threshold |grad|^2 > 1e-7 becomes G > 4e-7. The central differences dy and
+// dz of the column and its x-neighbours (colY, dzc) are formed once per step
+// and shared by neighbouring voxels. The limiter is applied as
+// dt * clamp(u, +-0.1/dt) (lim = 0.1/dt), so that the last step is one fma
+// onto f0 whose [0,1] clamp is the instruction's clamp bit.
+template <int PX, int PY, bool C01, typename PT>
+__device__ inline void mcf_fast(const PT &D, const PT &M, const PT &U, float one, float dt,
+                                    float lim, float lo, float hi, float *out) {
   constexpr int W = PX + 2;
-  float colY[PY + 2][PX + 2], dzc[PY + 2][PX + 2];
+  float colY[PY][W], dzc[PY + 2][W];
 #pragma unroll
-  for (int j = 1; j <= PY; j++)
+  for (int b = 0; b < PY; b++)
 #pragma unroll
-    for (int i = 0; i < W; i++) colY[j][i] = M[(j + 1) * W + i] - M[(j - 1) * W + i];
+    for (int i = 0; i < W; i++) colY[b][i] = vsub(M.at(b + 2, i), M.at(b, i), one);
 #pragma unroll
   for (int j = 0; j < PY + 2; j++)
 #pragma unroll
     for (int i = 0; i < W; i++)
-      if ((j >= 1 && j <= PY) || (i >= 1 && i <= PX)) dzc[j][i] = U[j * W + i] - D[j * W + i];
-  const float m4dt = -4.0f * dt;
+      if ((j >= 1 && j <= PY) || (i >= 1 && i <= PX)) dzc[j][i] = vsub(U.at(j, i), D.at(j, i), one);
 #pragma unroll
   for (int b = 0; b < PY; b++)
 #pragma unroll
     for (int a = 0; a < PX; a++) {
-      const int i = a + 1, j = b + 1, c = j * W + i;
-      const float f0 = M[c];
-      const float dx = M[c + 1] - M[c - 1], dy = colY[j][i], dz = dzc[j][i];
-      const float sxx = M[c - 1] + M[c + 1], syy = M[c - W] + M[c + W], szz = D[c] + U[c];
-      const float cyx = colY[j][i + 1] - colY[j][i - 1];
+      const int i = a + 1, j = b + 1;
+      const auto f0 = M.at(j, i);
+      const float sxx = vadd(M.at(j, i - 1), M.at(j, i + 1), one);
+      const float syy = vadd(M.at(j - 1, i), M.at(j + 1, i), one);
+      const float szz = vadd(D.at(j, i), U.at(j, i), one);
+      const float dx = vsub(M.at(j, i + 1), M.at(j, i - 1), one), dy = colY[b][i], dz = dzc[j][i];
+      const float cyx = colY[b][i + 1] - colY[b][i - 1];
       const float czx = dzc[j][i + 1] - dzc[j][i - 1];
       const float czy = dzc[j + 1][i] - dzc[j - 1][i];
       const float dz2 = dz * dz, dx2 = dx * dx;
@@ -159,24 +189,17 @@ __device__ inline void mcf_fast(const float *D, const float *M, const float *U, 
       p1 = __builtin_fmaf(dz, czx, p1);
       float t = (dy * dz) * czy;
       t = __builtin_fmaf(dx, p1, t);
-      const float r = __builtin_amdgcn_rcpf(G);
-#if MSBG_SWEEP_ALG
       float Np = A * sxx;
       Np = __builtin_fmaf(B, syy, Np);
       Np = __builtin_fmaf(Cq, szz, Np);
       Np = __builtin_fmaf(-0.5f, t, Np);
-      float Dv = __builtin_fmaf(Np * dt, r, m4dt * f0);  // dt * (Np/G - 4 f0)
-#else
-      float N = A * __builtin_fmaf(-2.0f, f0, sxx);
-      N = __builtin_fmaf(B, __builtin_fmaf(-2.0f, f0, syy), N);
-      N = __builtin_fmaf(Cq, __builtin_fmaf(-2.0f, f0, szz), N);
-      N = __builtin_fmaf(-0.5f, t, N);
-      float Dv = (N * dt) * r;
-#endif
-      Dv = G > 4.0f * 1e-7f ? Dv : 0.0f;
-      Dv = __builtin_amdgcn_fmed3f(Dv, -0.1f, 0.1f);
-      // doConstrZeroOne: [lo,hi] = [0,1], else (-inf,inf)
-      out[b * PX + a] = __builtin_amdgcn_fmed3f(f0 + Dv, lo, hi);
+      const float r = __builtin_amdgcn_rcpf(G);
+      float u = __builtin_fmaf((float)f0, -4.0f, Np * r);
+      u = G > 4.0f * 1e-7f ? u : 0.0f;
+      const float m = __builtin_amdgcn_fmed3f(u, -lim, lim);
+      const float phi = __builtin_fmaf(dt, m, (float)f0);
+      out[b * PX + a] = C01 ? __builtin_amdgcn_fmed3f(phi, 0.0f, 1.0f)
+                            : __builtin_amdgcn_fmed3f(phi, lo, hi);
     }
 }
 
@@ -325,11 +348,6 @@ struct Tile {
   static constexpr int NT = (BSX / PX) * (BSX / PY);
 };
 
-#if MSBG_SWEEP_WAVES > 0
-#define MSBG_SWEEP_LB(nt) __launch_bounds__(nt, MSBG_SWEEP_WAVES)
-#else
-#define MSBG_SWEEP_LB(nt) __launch_bounds__(nt)
-#endif
 // Blocks per workgroup: a block with fewer columns than a wave (8^3 blocks
 // have 16 with 2x2 columns) shares its workgroup with others, each group of
 // NT threads working on its own item with its own LDS planes.
@@ -341,18 +359,32 @@ struct Pack {
 // OP: the stencil (OP_MCF is the hot path); HARD: hard constraints around
 // the field at entry (G.phi0); iter: the sweep index (Taubin's parity);
 // nItems: the number of items of this launch (the last workgroup may hold
-// fewer than BPW).
+// fewer than BPW); C01 (fast path only): every grid of the launch clamps to
+// [0,1] (doConstrZeroOne), which the last fma of the update does with its
+// clamp bit.
+//
+// The fast MCF path (OP_MCF, not EXACT, not HARD, FP32 arithmetic) on FP16
+// pages keeps the planes in LDS and the patches in registers as FP16 bits
+// (LDS rows padded to a multiple of four elements so that a patch row of a
+// 4-wide column is one 8-byte and one 4-byte read): half the registers of
+// decoded patches, which keeps three waves per SIMD, at the price of sums and
+// differences in v_fma_mix_f32 instead of plain FP32 adds.
 template <int BSX, int ST, bool EXACT, int PX, int PY, int OP = OP_MCF, bool HARD = false,
-          int BPW = 1>
-__global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
+          int BPW = 1, bool C01 = false>
+__global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
     const SweepGrid *__restrict__ grids, const int2 *__restrict__ items,
     double *__restrict__ partial, int iter, int nItems) {
   using S = Stor<ST>;
   using R = typename S::raw;
   using C = typename S::ct;
+  constexpr bool FAST = OP == OP_MCF && !EXACT && !HARD && sizeof(C) == 4;
+  constexpr bool H16 = FAST && ST == MSBG_STORAGE_F16;
+  using L = std::conditional_t<H16, _Float16, C>;  // element of the LDS planes and patches
+  using PT = Patch<L, PX, PY>;
   constexpr int NT = Tile<BSX, PX, PY>::NT;
-  constexpr int W = PX + 2, PN = (PX + 2) * (PY + 2);  // patch row stride / size
-  constexpr int P = BSX + 2;                            // LDS row pitch
+  constexpr int W = PX + 2;          // patch row stride
+  constexpr int P = H16 ? BSX + 4 : BSX + 2;  // LDS row pitch
+  constexpr int PL = (BSX + 2) * P;  // LDS plane size
   constexpr int PS = BSX * BSX;
   constexpr int64_t PE = (int64_t)BSX * BSX * BSX;
   constexpr int FS = 2 * PS, ES = 4 * BSX;
@@ -361,6 +393,7 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
   constexpr int LPT = PS / (4 * NT);  // 4-wide interior chunks per thread and plane
   constexpr int M = BSX - 1;
   static_assert(LPT * 4 * NT == PS, "plane does not split into 4-wide chunks");
+  static_assert(!H16 || PX % 2 == 0, "FP16 patches hold pairs of x");
 
   static_assert(BPW == 1 || BPW * NT <= 64, "packed blocks must share one wave");
   // work item: which grid, which page of it. A group past the end of the
@@ -380,10 +413,10 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
   const float dtf = G.dt, lo01 = G.lo, hi01 = G.hi;
   (void)iter;
 
-  __shared__ C bufs[BPW][2][P * P];
+  __shared__ __align__(16) L bufs[BPW][2][PL];
   __shared__ int codesAll[BPW][27];
   __shared__ double red[(NT + 63) / 64];
-  C(&buf)[2][P * P] = bufs[sub];
+  L(&buf)[2][PL] = bufs[sub];
   int *codes = codesAll[sub];
 
   const int page = item.y;
@@ -520,18 +553,16 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
     rs[q].stride = st0;
   }
 
-  // two register sets of staged plane values (prefetch distance 2 planes)
+  // one plane's values held between its load and its LDS store: decoded,
+  // or the FP16 bits (two per register) on the FP16 fast path
   struct Staged {
-    C in[LPT * 4];
-    C ring[RPT];
+    std::conditional_t<H16, uint32_t, C> in[H16 ? LPT * 2 : LPT * 4];
+    std::conditional_t<H16, uint16_t, C> ring[RPT];
   };
   auto dec = [](R r) -> C { return (C)S::dec(r); };
 
-  // chunk k of this thread: 4 consecutive x of one row
-  auto chunk_off = [&](int k) -> int {
-    int e = (tid + k * NT) * 4;
-    return e;  // = y * BSX + x with x multiple of 4
-  };
+  // chunk k of this thread: 4 consecutive x of one row, e = y * BSX + x
+  auto chunk_off = [&](int k) -> int { return (tid + k * NT) * 4; };
 
   auto load_plane = [&](int zz, Staged &st) {
 #pragma unroll
@@ -539,60 +570,147 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
       const int e = chunk_off(k);
       const R *src = zz < 0 ? zsrcM + (constM ? 0 : e)
                             : (zz >= BSX ? zsrcP + (constP ? 0 : e) : own + zz * PS + e);
-      R r[4];
-      load4(src, r);
+      if constexpr (H16) {
+        const uint64_t u = *(const MSBG_GLOBAL uint64_t *)src;
+        st.in[2 * k] = (uint32_t)u;
+        st.in[2 * k + 1] = (uint32_t)(u >> 32);
+      } else {
+        R r[4];
+        load4(src, r);
 #pragma unroll
-      for (int u = 0; u < 4; u++) st.in[k * 4 + u] = dec(r[u]);
+        for (int u = 0; u < 4; u++) st.in[k * 4 + u] = dec(r[u]);
+      }
     }
 #pragma unroll
-    for (int q = 0; q < RPT; q++) st.ring[q] = dec(ldg(rs[q].at(zz, BSX)));
+    for (int q = 0; q < RPT; q++) {
+      if constexpr (H16)
+        st.ring[q] = ldg(rs[q].at(zz, BSX));
+      else
+        st.ring[q] = dec(ldg(rs[q].at(zz, BSX)));
+    }
   };
 
-  auto put_plane = [&](C *b, const Staged &st) {
+  auto put_plane = [&](L *b, const Staged &st) {
 #pragma unroll
     for (int k = 0; k < LPT; k++) {
       const int e = chunk_off(k);
-      C *row = b + (e / BSX + 1) * P + (e % BSX) + 1;
+      L *row = b + (e / BSX + 1) * P + (e % BSX) + 1;
+      if constexpr (H16) {
+        // x..x+3 land at the odd LDS columns x+1..x+4: one 2-byte, one
+        // 4-byte and one 2-byte store
+        const uint32_t w0 = st.in[2 * k], w1 = st.in[2 * k + 1];
+        *(u16a *)row = (uint16_t)w0;
+        *(u32a *)(row + 1) = __builtin_amdgcn_alignbit(w1, w0, 16);
+        *(u16a *)(row + 3) = (uint16_t)(w1 >> 16);
+      } else {
 #pragma unroll
-      for (int u = 0; u < 4; u++) row[u] = st.in[k * 4 + u];
+        for (int u = 0; u < 4; u++) row[u] = st.in[k * 4 + u];
+      }
     }
 #pragma unroll
     for (int q = 0; q < RPT; q++)
-      if (tid + q * NT < NR) b[rlds[q]] = st.ring[q];
+      if (tid + q * NT < NR) {
+        if constexpr (H16)
+          *(u16a *)(b + rlds[q]) = st.ring[q];
+        else
+          b[rlds[q]] = st.ring[q];
+      }
   };
 
-  auto get_patch = [&](const C *b, C (&p)[PN]) {
+  auto get_patch = [&](const L *b, PT &p) {
 #pragma unroll
-    for (int j = 0; j < PY + 2; j++)
+    for (int j = 0; j < PY + 2; j++) {
+      const L *row = b + (y0 + j) * P + x0;
+      if constexpr (H16) {
+        if constexpr (PX % 4 == 0) {  // 8-byte aligned rows
 #pragma unroll
-      for (int i = 0; i < PX + 2; i++) p[j * W + i] = b[(y0 + j) * P + x0 + i];
+          for (int k = 0; k + 1 < PT::WW; k += 2) {
+            const uint64_t q = *(const u64a *)(row + 2 * k);
+            p.w[j][k] = (uint32_t)q;
+            p.w[j][k + 1] = (uint32_t)(q >> 32);
+          }
+          if constexpr (PT::WW % 2) p.w[j][PT::WW - 1] = *(const u32a *)(row + 2 * (PT::WW - 1));
+        } else {
+#pragma unroll
+          for (int k = 0; k < PT::WW; k++) p.w[j][k] = *(const u32a *)(row + 2 * k);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < PX + 2; i++) p.v[j * W + i] = row[i];
+      }
+    }
   };
 
   double acc = 0.0;
   const bool wFace0 = x0 == 0, wFace1 = x0 == BSX - PX, wY0 = y0 == 0, wY1 = y0 == BSX - PY;
+  const float fone = FAST ? opaque_one() : 1.0f;
+  const float lim = FAST ? 0.1f / dtf : 0.0f;  // limiter bound of the fast path, in units of dt
 
-  auto compute = [&](int z, const C (&Dp)[PN], const C (&Mp)[PN], const C (&Up)[PN]) {
+  // encode and store the new values of plane z, keep the side-cars equal to
+  // the page, accumulate the residual
+  auto store = [&](int z, const C (&o)[PX * PY], const PT &Mp) {
+    R e[PX * PY];
+    if constexpr (H16) {
+#pragma unroll
+      for (int k = 0; k < PX * PY; k += 2) {  // v_cvt_pk_f16_f32
+        const uint32_t pk =
+            __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2_t{o[k], o[k + 1]}, h2x_t));
+        e[k] = (R)pk;
+        e[k + 1] = (R)(pk >> 16);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < PX * PY; k++) e[k] = S::enc(o[k]);
+    }
+#pragma unroll
+    for (int b = 0; b < PY; b++)
+#pragma unroll
+      for (int a = 0; a < PX; a += 2)
+        store2(dOwn + z * PS + (y0 + b) * BSX + x0 + a, e[b * PX + a], e[b * PX + a + 1]);
+    if (wFace0) {
+#pragma unroll
+      for (int b = 0; b < PY; b++) stg(dFaces + z * BSX + y0 + b, e[b * PX]);
+    }
+    if (wFace1) {
+#pragma unroll
+      for (int b = 0; b < PY; b++) stg(dFaces + PS + z * BSX + y0 + b, e[b * PX + PX - 1]);
+    }
+    if (wFace0 && wY0) stg(dEdges + 0 * BSX + z, e[0]);
+    if (wFace1 && wY0) stg(dEdges + 1 * BSX + z, e[PX - 1]);
+    if (wFace0 && wY1) stg(dEdges + 2 * BSX + z, e[(PY - 1) * PX]);
+    if (wFace1 && wY1) stg(dEdges + 3 * BSX + z, e[PY * PX - 1]);
+    if (partial) {
+#pragma unroll
+      for (int b = 0; b < PY; b++)
+#pragma unroll
+        for (int a = 0; a < PX; a++) {
+          double d = (double)S::dec(e[b * PX + a]) - (double)(C)Mp.at(b + 1, a + 1);
+          acc += d * d;
+        }
+    }
+  };
+
+  // plane z's update from the patches of planes z-1, z, z+1
+  auto compute = [&](int z, const PT &Dp, const PT &Mp, const PT &Up) {
     if (BPW > 1 && !active) return;
     C o[PX * PY];
-    if constexpr (OP == OP_MCF) {
-      if constexpr (EXACT || HARD || sizeof(C) == 8) {
-        const bool c01 = !HARD && lo01 == 0.0f;
+    if constexpr (FAST) {
+      mcf_fast<PX, PY, C01>(Dp, Mp, Up, fone, dtf, lim, lo01, hi01, o);
+    } else if constexpr (OP == OP_MCF) {
+      const bool c01x = !HARD && lo01 == 0.0f;
 #pragma unroll
-        for (int b = 0; b < PY; b++)
+      for (int b = 0; b < PY; b++)
 #pragma unroll
-          for (int a = 0; a < PX; a++)
-            o[b * PX + a] = mcf_exact<C, W>(Dp, Mp, Up, (b + 1) * W + a + 1, dt, c01);
-      } else {
-        mcf_fast<PX, PY>(Dp, Mp, Up, dt, lo01, hi01, o);
-      }
+        for (int a = 0; a < PX; a++)
+          o[b * PX + a] = mcf_exact<C, W>(Dp.v, Mp.v, Up.v, (b + 1) * W + a + 1, dt, c01x);
     } else {
       static_assert(sizeof(C) == 4, "the 7-point stencils run in FP32");
 #pragma unroll
       for (int b = 0; b < PY; b++)
 #pragma unroll
         for (int a = 0; a < PX; a++) {
-          float ph = pde7_exact<OP, W>(Dp, Mp, Up, (b + 1) * W + a + 1, dtf, G.taubin, G.alpha,
-                                       iter);
+          float ph = pde7_exact<OP, W>(Dp.v, Mp.v, Up.v, (b + 1) * W + a + 1, dtf, G.taubin,
+                                       G.alpha, iter);
           if (!HARD && lo01 == 0.0f) {  // doConstrZeroOne: min(max(phi,0),1)
             ph = ph > 0.0f ? ph : 0.0f;
             ph = ph < 1.0f ? ph : 1.0f;
@@ -616,73 +734,10 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
           o[b * PX + a] = ph;
         }
     }
-    R e[PX * PY];
-#pragma unroll
-    for (int k = 0; k < PX * PY; k++) e[k] = S::enc(o[k]);
-#pragma unroll
-    for (int b = 0; b < PY; b++)
-#pragma unroll
-      for (int a = 0; a < PX; a += 2)
-        store2(dOwn + z * PS + (y0 + b) * BSX + x0 + a, e[b * PX + a], e[b * PX + a + 1]);
-    // keep the side-car faces and edges equal to the page
-    if (wFace0) {
-#pragma unroll
-      for (int b = 0; b < PY; b++) stg(dFaces + z * BSX + y0 + b, e[b * PX]);
-    }
-    if (wFace1) {
-#pragma unroll
-      for (int b = 0; b < PY; b++) stg(dFaces + PS + z * BSX + y0 + b, e[b * PX + PX - 1]);
-    }
-    if (wFace0 && wY0) stg(dEdges + 0 * BSX + z, e[0]);
-    if (wFace1 && wY0) stg(dEdges + 1 * BSX + z, e[PX - 1]);
-    if (wFace0 && wY1) stg(dEdges + 2 * BSX + z, e[(PY - 1) * PX]);
-    if (wFace1 && wY1) stg(dEdges + 3 * BSX + z, e[PY * PX - 1]);
-    if (partial) {
-#pragma unroll
-      for (int b = 0; b < PY; b++)
-#pragma unroll
-        for (int a = 0; a < PX; a++) {
-          double d = (double)S::dec(e[b * PX + a]) - (double)Mp[(b + 1) * W + a + 1];
-          acc += d * d;
-        }
-    }
+    store(z, o, Mp);
   };
 
-  C pA[PN], pB[PN], pC[PN];
-#if MSBG_SWEEP_PD == 2
-  Staged s0, s1;  // plane zz lives in s[zz & 1]
-
-  // prologue: planes -1 and 0 into pA, pB; planes 1 and 2 prefetched
-  load_plane(-1, s1);
-  load_plane(0, s0);
-  put_plane(buf[1], s1);
-  load_plane(1, s1);
-  __syncthreads();
-  get_patch(buf[1], pA);
-  put_plane(buf[0], s0);
-  load_plane(2, s0);
-  __syncthreads();
-  get_patch(buf[0], pB);
-
-  // step z: stage plane z+1, prefetch z+3 into its register set, compute z
-  auto step = [&](int z, Staged &st, C (&Dp)[PN], C (&Mp)[PN], C (&Up)[PN]) {
-    put_plane(buf[(z + 1) & 1], st);
-    if (z + 3 <= BSX) load_plane(z + 3, st);
-    __syncthreads();
-    get_patch(buf[(z + 1) & 1], Up);
-    compute(z, Dp, Mp, Up);
-  };
-
-  // unrolled by 6: patches rotate with period 3, register sets with period 2
-  for (int z = 0; z < BSX; z += 6) {
-    step(z, s1, pA, pB, pC);
-    if (z + 1 < BSX) step(z + 1, s0, pB, pC, pA);
-    if (z + 2 < BSX) step(z + 2, s1, pC, pA, pB);
-    if (z + 3 < BSX) step(z + 3, s0, pA, pB, pC);
-    if (z + 4 < BSX) step(z + 4, s1, pB, pC, pA);
-    if (z + 5 < BSX) step(z + 5, s0, pC, pA, pB);
-  }
-#else
+  PT pA, pB, pC;
   Staged s0;
 
   // prologue: planes -1 and 0 into pA, pB; plane 1 prefetched
@@ -697,7 +752,7 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
   get_patch(buf[0], pB);
 
   // step z: stage plane z+1 (prefetched), prefetch z+2, compute plane z
-  auto step = [&](int z, C (&Dp)[PN], C (&Mp)[PN], C (&Up)[PN]) {
+  auto step = [&](int z, const PT &Dp, const PT &Mp, PT &Up) {
     put_plane(buf[(z + 1) & 1], s0);
     if (z + 2 <= BSX) load_plane(z + 2, s0);
     __syncthreads();
@@ -710,7 +765,6 @@ __global__ MSBG_SWEEP_LB((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
     if (z + 1 < BSX) step(z + 1, pB, pC, pA);
     if (z + 2 < BSX) step(z + 2, pC, pA, pB);
   }
-#endif
 
   if (partial) {
     // deterministic block reduction: wave shuffles, then waves in order
