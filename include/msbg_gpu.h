@@ -106,6 +106,17 @@ int msbg_set_arith(msbg_grid *g, int mode);
 /* Device synchronisation of the grid's stream. */
 int msbg_synchronize(msbg_grid *g);
 
+/* Device memory for the entry points that take DEVICE pointers (msbg_unet_*,
+ * msbg_cnn_*), so that a caller needs no other GPU runtime to hold their
+ * tensors: msbg_dev_alloc returns `bytes` of device memory on the current
+ * device (NULL for 0 bytes), msbg_dev_free releases it, and the copies are
+ * synchronous host <-> device transfers. No counterpart in the reference,
+ * whose buffers are host memory. */
+int msbg_dev_alloc(int64_t bytes, void **out);
+int msbg_dev_free(void *p);
+int msbg_dev_upload(void *dst, const void *src, int64_t bytes);
+int msbg_dev_download(void *dst, const void *src, int64_t bytes);
+
 /* ======================================================================== */
 /* particles, activation, refinement map (msbg_demo.cpp:339-441)            */
 /* ======================================================================== */

@@ -57,6 +57,10 @@ SIGNATURES = {
     "msbg_last_error": (C.c_char_p, []),
     "msbg_create": (_I, [C.c_char_p, _I, _I, _I, _I, C.c_uint, _I, C.POINTER(_P)]),
     "msbg_destroy": (_I, [_P]),
+    "msbg_dev_alloc": (_I, [_I64, C.POINTER(_P)]),
+    "msbg_dev_free": (_I, [_P]),
+    "msbg_dev_upload": (_I, [_P, _P, _I64]),
+    "msbg_dev_download": (_I, [_P, _P, _I64]),
     "msbg_get_dims": (_I, [_P, _P, _P]),
     "msbg_set_arith": (_I, [_P, _I]),
     "msbg_synchronize": (_I, [_P]),

@@ -230,6 +230,44 @@ int msbg_synchronize(msbg_grid *g) {
 }
 
 // ---------------------------------------------------------------------------
+// device memory for callers of the device-pointer entry points
+// ---------------------------------------------------------------------------
+int msbg_dev_alloc(int64_t bytes, void **out) {
+  MSBG_CHECK(out && bytes >= 0, MSBG_EINVAL, "msbg_dev_alloc: bad args");
+  *out = nullptr;
+  if (bytes == 0) return 0;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_error("msbg_dev_alloc: no HIP device available");
+    return MSBG_EDEVICE;
+  }
+  if (hipMalloc(out, (size_t)bytes) != hipSuccess) {
+    *out = nullptr;
+    set_error("msbg_dev_alloc: %lld bytes: out of device memory", (long long)bytes);
+    return MSBG_ENOMEM;
+  }
+  return 0;
+}
+
+int msbg_dev_free(void *p) {
+  if (p) MSBG_HIP(hipFree(p));
+  return 0;
+}
+
+int msbg_dev_upload(void *dst, const void *src, int64_t bytes) {
+  MSBG_CHECK(bytes >= 0 && (bytes == 0 || (dst && src)), MSBG_EINVAL, "msbg_dev_upload: bad args");
+  if (bytes) MSBG_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int msbg_dev_download(void *dst, const void *src, int64_t bytes) {
+  MSBG_CHECK(bytes >= 0 && (bytes == 0 || (dst && src)), MSBG_EINVAL,
+             "msbg_dev_download: bad args");
+  if (bytes) MSBG_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // particles
 // ---------------------------------------------------------------------------
 

@@ -47,6 +47,56 @@ def scene_objects(seed, nObj, cmin, cmax, rmin, rmax, nPts, sigma=0.5, ellipsoid
     return out
 
 
+class DeviceArray:
+    """Device memory held through the library (msbg_dev_alloc) with a numpy
+    shape and dtype. data_ptr() is the device address the msbg_unet_* and
+    msbg_cnn_* entry points take, so their callers need no other GPU runtime."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        self._p = C.c_void_p()
+        check(lib().msbg_dev_alloc(self.nbytes, C.byref(self._p)))
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def data_ptr(self):
+        return self._p.value or 0
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.shape, self.shape)
+        check(lib().msbg_dev_upload(self._p, _ptr(a), a.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        check(lib().msbg_dev_download(_ptr(out), self._p, out.nbytes))
+        return out
+
+    def free(self):
+        if self._p.value:
+            check(lib().msbg_dev_free(self._p))
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+
+
 def _dptr(t):
     """Device address of a tensor-like object with data_ptr() (e.g. a torch
     CUDA tensor); the library only sees the address. Work already queued on
