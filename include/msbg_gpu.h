@@ -518,6 +518,16 @@ int msbg_warm_start(msbg_grid *g, int chan, msbg_unet *u, int batch,
 int msbg_cnn_conv3d(const uint16_t *x, int n, int cin, int cout, int dout,
                     int pad, const float *w, const float *b, int relu,
                     uint16_t *y);
+/* msbg_cnn_conv3d without any allocation: the same kernels, with the
+ * swizzled weights in a fixed per-device scratch array of 256 KiB (every
+ * layer shape of the network fits; weight sets that do not return
+ * MSBG_EINVAL). No counterpart in the reference: it is for callers that
+ * cannot allocate device memory, such as code under HIP graph capture or a
+ * tight serving loop. All calls on one GPU share that GPU's array: calls
+ * that overlap on one GPU race. */
+int msbg_cnn_conv3d_noalloc(const uint16_t *x, int n, int cin, int cout,
+                            int dout, int pad, const float *w, const float *b,
+                            int relu, uint16_t *y);
 int msbg_cnn_conv3d_bwd_data(const uint16_t *dy, int n, int cin, int cout,
                              int din, int pad, const float *w, uint16_t *dx);
 int msbg_cnn_conv3d_bwd_weights(const uint16_t *x, const uint16_t *dy, int n,

@@ -121,6 +121,7 @@ SIGNATURES = {
     "msbg_unet_extract": (_I, [_P, _I, _P, _I, _P]),
     "msbg_warm_start": (_I, [_P, _I, _P, _I, _P]),
     "msbg_cnn_conv3d": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _I, _P]),
+    "msbg_cnn_conv3d_noalloc": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _I, _P]),
     "msbg_cnn_conv3d_bwd_data": (_I, [_P, _I, _I, _I, _I, _I, _P, _P]),
     "msbg_cnn_conv3d_bwd_weights": (_I, [_P, _P, _I, _I, _I, _I, _I, _P, _P]),
 }

@@ -20,6 +20,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "cnn_kernels.h"
@@ -383,6 +384,35 @@ int block_list(msbg_grid *g, int chan, const int32_t *blocks, int n, const char 
 
 }  // namespace
 
+// ---- msbg_cnn_conv3d_noalloc's scratch ------------------------------------
+// The swizzled weights of one convolution, in a file-scope device array
+// instead of a per-call allocation: 16384 uint4 (256 KiB) per device, above
+// the 13824 of the largest layer shape (64 -> 64 channels). Every call on a
+// device shares that device's copy. hipGetSymbolAddress gives each device's
+// own address, once per device (with the device current), and only that
+// address goes to the kernels: the array's name is a host-side stand-in.
+static constexpr int64_t kConvScratchElems = 16384;
+static __device__ __align__(256) uint4 scratch[kConvScratchElems];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static uint4 *g_scratch_ptr[kMaxDevices] = {};
+
+static int conv_scratch(uint4 **out) {
+  int dev = -1;
+  MSBG_HIP(hipGetDevice(&dev));
+  MSBG_CHECK(dev >= 0 && dev < kMaxDevices, MSBG_EDEVICE, "conv scratch: device %d beyond %d",
+             dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    MSBG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MSBG_CHECK(p, MSBG_EDEVICE, "conv scratch: no device address for device %d", dev);
+    g_scratch_ptr[dev] = (uint4 *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 extern "C" {
 
 int msbg_unet_gather(msbg_grid *g, int chan, const int32_t *blocks, int n, uint16_t *x) {
@@ -494,6 +524,23 @@ int msbg_cnn_conv3d(const uint16_t *x, int n, int cin, int cout, int dout, int p
   MSBG_TRY(sw.alloc((size_t)swizzled_size(cin, cout)));
   MSBG_TRY(launch_swizzle(nullptr, w, cin, cin, cout, false, sw.p));
   MSBG_TRY(launch_conv(nullptr, x, n, cin, cout, dout, pad, sw.p, b, relu != 0, y));
+  MSBG_HIP(hipStreamSynchronize(nullptr));
+  return 0;
+}
+
+int msbg_cnn_conv3d_noalloc(const uint16_t *x, int n, int cin, int cout, int dout, int pad,
+                            const float *w, const float *b, int relu, uint16_t *y) {
+  MSBG_CHECK(x && w && y && (cin == 1 || cin % 8 == 0) && cout > 0 && cout % 16 == 0,
+             MSBG_EINVAL, "msbg_cnn_conv3d_noalloc: bad args");
+  const int64_t need = swizzled_size(cin, cout);
+  MSBG_CHECK(need <= kConvScratchElems, MSBG_EINVAL,
+             "msbg_cnn_conv3d_noalloc: the swizzled weights of %d -> %d channels take %lld bytes, "
+             "above the %lld-byte scratch",
+             cin, cout, (long long)(need * 16), (long long)(kConvScratchElems * 16));
+  uint4 *sw = nullptr;
+  MSBG_TRY(conv_scratch(&sw));
+  MSBG_TRY(launch_swizzle(nullptr, w, cin, cin, cout, false, sw));
+  MSBG_TRY(launch_conv(nullptr, x, n, cin, cout, dout, pad, sw, b, relu != 0, y));
   MSBG_HIP(hipStreamSynchronize(nullptr));
   return 0;
 }
