@@ -15,6 +15,13 @@ colour-pass kernel at 2 * sizeof(storage) algorithmic bytes per voxel update
 against MI355X's 8 TB/s, from HIP events recorded on the kernel's stream.
 cpu_baseline times the same sweep on the host cores (CPU oracle built with
 the reference's flags + OpenMP) on a bounded sub-box sample.
+
+Beside the rate, the line carries the rest of the metric and the run's
+precision statement: the other arithmetic mode's kernel rate on the same
+pages (exact beside fast), error_vs_fp64 (the FP16 fast sweep against an FP64
+exact run of the same field on the 512^3 centre sub-box, same sweep count)
+and time_to_tol (a held-out 2048^3 scene solved to FP64 residual 1e-4 by
+plain sweeps and by the learned warm start + plain sweeps).
 """
 import argparse
 import json
@@ -162,6 +169,73 @@ def cpu_baseline(cfg, xyz, sweeps_budget_s=15.0):
                   f"{total_t:.1f} s on {threads} threads (oracle built -O3 -ffast-math "
                   f"-march=x86-64-v3 -fopenmp)",
     }
+
+
+def error_vs_fp64(cfg, xyz, sweeps, seed_note=""):
+    """Error of the FP16-storage fast sweep against an FP64 exact run of the
+    same field on the 512^3 centre sub-box (same level bands, 32^3 blocks):
+    max and RMS difference after `sweeps` sweeps, accumulated in FP64 on the
+    device, with both runs' FP64 residual of the last sweep."""
+    import msbg_amd as M
+    S, bsx = cfg["S"], cfg["bsx"]
+    sub = min(512, S)
+    lo = np.float32((S - sub) // 2 // bsx * bsx)
+    p = xyz - lo
+    p = np.ascontiguousarray(p[np.all((p >= 0) & (p < sub), axis=1)], np.float32)
+    CH, CH64, CH16 = 30, 31, 32
+    with M.MultiresSparseGrid.create("err", sub, sub, sub, bsx, options=0,
+                                     nLevels=cfg["nLevels"]) as g:
+        g.points_upload(p)
+        lv = g.regularizeRefinementMap(g.levels_from_points(cfg["bands"]))
+        g.setRefinementMap(lv)
+        blv, _ = g.getBlockInfo()
+        kind = np.where(blv == 0, 3, 1).astype(np.uint8)  # level 0 values, EMPTY elsewhere
+        g.channel_define(CH, M.U16, 0, kind, 0.0, None, 65535.0)
+        g.splat(CH, 2.0, 2.0)
+        g.finalize(CH, 2.0, 2.0)
+        g.copyDataTo(CH, CH64, M.F64)
+        g.copyDataTo(CH, CH16, M.F16)
+        q_max, q_rms = g.diff(CH16, CH64)
+        g.set_arith(M.ARITH_EXACT)
+        r64 = g.applyChannelPdeFast(CH64, 0, 0, None, M.MCF_8COLOR, 1, 0, 0, 0, sweeps, 1.0, 0.1,
+                                    residual=True)
+        g.set_arith(M.ARITH_FAST)
+        r16 = g.applyChannelPdeFast(CH16, 0, 0, None, M.MCF_8COLOR, 1, 0, 0, 0, sweeps, 1.0, 0.1,
+                                    residual=True)
+        e_max, e_rms = g.diff(CH16, CH64)
+        n0 = int((blv == 0).sum())
+    return {"sample": f"{sub}^3 centre sub-box of the same scene, {n0} level-0 {bsx}^3 blocks, "
+                      f"UNORM16 splat converted to FP16 (fast arithmetic) and FP64 (exact)",
+            "sweeps": int(len(r16)), "max_abs": e_max, "rms": e_rms,
+            "quantisation_max_abs": q_max, "quantisation_rms": q_rms,
+            "residual_last_fp16": float(r16[-1]), "residual_last_fp64": float(r64[-1])}
+
+
+def time_to_tol(params_path, seeds=(777,), S=2048, nObj=128):
+    """The metric's second part: a held-out scene solved to FP64 residual 1e-4
+    by plain sweeps and by the learned warm start + plain sweeps (the
+    committed K = 1024 network), each with its device time, and the two
+    solutions' distance beside the plain field's own drift in 300 more
+    sweeps."""
+    import msbg_amd as M
+    from msbg_amd import warmstart as WS
+    out = []
+    with M.UNet(seed=0) as net:
+        net.set_params(np.load(params_path, allow_pickle=False))
+        for sd in seeds:
+            r = WS.time_to_tolerance(net, S, nObj, sd)
+            out.append({"seed": sd, "S": S, "objects": nObj, "level0_blocks": r["level0_blocks"],
+                        "tol": r["tol"], "speedup": r["speedup"],
+                        "plain_ms": 0.5 * (r["plain"]["ms"] + r["plain"]["ms_repeat"]),
+                        "plain_sweeps": r["plain"]["sweeps"],
+                        "learned_ms": r["learned"]["total_ms"],
+                        "warm_start_ms": r["learned"]["warm_start_ms"],
+                        "learned_sweeps": r["learned"]["sweeps"],
+                        "fallback": r["learned"]["fallback"] is not None,
+                        "learned_vs_plain": r["learned_vs_plain_solution"],
+                        "plain_drift_300_sweeps": {k: r["plain_drift"][k]
+                                                   for k in ("max_abs", "rms")}})
+    return out
 
 
 # RaymarchRenderer as msbg_demo.cpp:122-166 sets it up
@@ -436,6 +510,10 @@ def main():
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--other-steps", type=int, default=3,
+                    help="steps of the other arithmetic mode timed after the run")
+    ap.add_argument("--no-error", action="store_true", help="skip the FP64 error run")
+    ap.add_argument("--no-tol", action="store_true", help="skip the time-to-tolerance run")
     ap.add_argument("--calib", action="store_true",
                     help="more copy-kernel launches for profiler byte-counter calibration")
     args = ap.parse_args()
@@ -485,12 +563,36 @@ def main():
     res = g.applyChannelPdeFast(chan, 0, 0, None, M.MCF_8COLOR, 1, 0, 0, 0, 2, 1.0, 0.1,
                                 residual=True)
     log(f"residual after {args.warmup * 2 + sweeps + 2} sweeps: {res}")
+    # the other arithmetic's kernel rate on the same pages (device time)
+    other = "exact" if args.arith == "fast" else "fast"
+    g.set_arith(M.ARITH_EXACT if other == "exact" else M.ARITH_FAST)
+    o_ms, o_n = 0.0, 0
+    for _ in range(args.other_steps):
+        g.applyChannelPdeFast(chan, 0, 0, None, M.MCF_8COLOR, 1, 0, 0, 0, 2, 1.0, 0.1)
+        tot, _, n = g.last_sweep_timing()
+        o_ms += tot
+        o_n += n
+    other_rate = nvox * 2 * args.other_steps / (o_ms * 1e-3) if o_ms > 0 else None
+    log(f"{other} arithmetic: {other_rate:.4g} voxel-updates/s, "
+        f"{o_ms / max(o_n, 1):.3f} ms per colour pass")
+    g.destroy()
+    err = None
+    if not args.no_error and storage != 5:
+        t0 = time.perf_counter()
+        err = error_vs_fp64(cfg, xyz, args.warmup * 2 + sweeps + 2)
+        log(f"error vs FP64 ({time.perf_counter() - t0:.1f}s): {err}")
+    ttt = None
+    if not args.no_tol:
+        t0 = time.perf_counter()
+        ttt = time_to_tol(os.path.join(ROOT, "msbg_amd", "data", "unet_k1024.npy"))
+        log(f"time to tolerance ({time.perf_counter() - t0:.1f}s): {ttt}")
     cpu = None
     if not args.no_cpu:
         try:
             cpu = cpu_baseline(cfg, xyz, args.cpu_seconds)
         except OSError as e:
             log(f"cpu baseline unavailable: {e}")
+    ST_NAME = {1: "u16", 2: "f32", 3: "f16", 4: "bf16", 5: "f64"}
     out = {
         "metric": "voxel-updates/s/sweep",
         "value": value,
@@ -503,11 +605,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if storage == 5 else "f32",
+        "storage_dtype": ST_NAME[storage],
+        "compute_dtype": "f64" if storage == 5 else "f32",
         "data": "synthetic",
         "config": {
             "workload": f"{args.config}: {cfg['desc']}; step = 2 sweeps of 8 colour passes",
             "storage": args.storage,
             "arith": args.arith,
+            f"{other}_arith_rate": other_rate,
+            f"{other}_arith_avg_pass_ms": o_ms / max(o_n, 1) if o_n else None,
             "particles": int(xyz.shape[0]),
             "particles_kept": int(nk),
             "level0_blocks": n0,
@@ -517,7 +623,12 @@ def main():
             "setup_s": {k: round(v, 4) for k, v in tsetup.items() if k.endswith("_s")},
             "splat_particles_per_s": nk / tsetup["splat_s"] if tsetup.get("splat_s") else None,
             "ghost_blocks": tsetup.get("ghost_blocks"),
+            "warm_start_network": "3-stage U-Net of the learned warm start trained with horizon "
+                                  "K = 1024 on 2000 scenes (cfg5 as written, K = 64 on 10k "
+                                  "scenes, gives 0.91x)",
         },
+        "error_vs_fp64": err,
+        "time_to_tol": ttt,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -532,7 +643,6 @@ def main():
         },
         "cpu_baseline": cpu,
     }
-    g.destroy()
     # measured copy bandwidth on this box (1 GiB read + 1 GiB written per
     # launch): the reference for the profiler's byte counters
     copy_gbs = M.copy_bandwidth(1 << 30, 5 if args.calib else 3)

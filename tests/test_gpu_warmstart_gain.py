@@ -3,8 +3,9 @@
 The committed parameters (msbg_amd/data/unet_k1024.npy) were trained by the
 project's own kernels on the change over 1024 plain sweeps. On a held-out
 scene (a seed outside the training range), learned + plain must reach the
-same FP64 residual as plain sweeps alone in at most half the sweeps, without
-the fallback; an untrained network (zero head: no change) gains nothing."""
+same FP64 residual as plain sweeps alone in at most half the sweeps and at
+least 2x faster, without the fallback; an untrained network (zero head: no
+change) gains nothing."""
 import os
 import sys
 
@@ -23,8 +24,6 @@ def note(*a):
 
 
 def test_trained_network_halves_the_sweeps_to_tolerance():
-    import torch
-    torch.cuda.init()
     from msbg_amd import warmstart as W
     p = np.load(PARAMS, allow_pickle=False)
     assert p.dtype == np.float32
@@ -36,12 +35,11 @@ def test_trained_network_halves_the_sweeps_to_tolerance():
     note({k: r[k] for k in ("speedup", "sweeps_saved")}, r["plain"]["sweeps"], r["learned"]["sweeps"])
     assert r["learned"]["fallback"] is None
     assert r["learned"]["sweeps"] <= 0.5 * r["plain"]["sweeps"]
-    assert r["speedup"] > 1.0
+    # measured 2.88x on this scene: the target of the learned path is 2x
+    assert r["speedup"] >= 2.0
 
 
 def test_untrained_network_gains_nothing():
-    import torch
-    torch.cuda.init()
     from msbg_amd import warmstart as W
     with W.UNet(seed=0) as net:  # the head starts at zero: no change
         note("held-out 1024^3 scene, untrained network")
