@@ -65,6 +65,7 @@ def load(kind="strict"):
             "or_mcf_voxel_f64": (D, [P, I, I, D, I]),
             "or_ghost_downsample": (None, [P, I, P]),
             "or_prepare_ghost_blocks": (None, [P, I, P, P, P]),
+            "or_interpolate_ghost": (C.c_double, [P, F, F, F]),
             "or_interp_linear": (F, [P, P]),
             "or_bspline3_weights": (None, [F, P, P]),
             "or_interp_points": (None, [P, I, P, I64, P, P]),
@@ -264,6 +265,13 @@ def ghost_downsample(halo, bsxHi):
     out = np.zeros((bsxHi // 2,) * 3, np.float32)
     load().or_ghost_downsample(_p(halo), bsxHi, _p(out))
     return out
+
+
+def interpolate_ghost(level, p):
+    """or_interpolate_ghost: interpolateGhost of `level` at p (voxel
+    coordinates, cell centres at i + 0.5), in the level's raw domain."""
+    s = level.struct()
+    return load().or_interpolate_ghost(C.byref(s), float(p[0]), float(p[1]), float(p[2]))
 
 
 def prepare_ghost_blocks(levels, fc_lists, flags):

@@ -526,6 +526,12 @@ static double interpolate_ghost_raw(const or_level *Lo, float px, float py,
   return to_raw_type(Lo->storage, (float)bzv[1] * w + (float)bzv[0] * (1.0f - w));
 }
 
+/* interpolate_ghost_raw as an entry point (the tests' handle on it); the
+ * result is in the level's raw domain. */
+double or_interpolate_ghost(const or_level *Lo, float px, float py, float pz) {
+  return interpolate_ghost_raw(Lo, px, py, pz);
+}
+
 /* HaloBlockSet::getOutOfBlockValue (halo.h:267-355): the value of halo voxel
  * (x,y,z) whose level-L block is NULL or which lies outside the domain. */
 static float out_of_block_value(const or_level *L, const or_level *Lo,

@@ -113,6 +113,10 @@ void or_finalize_u16(uint16_t *pages, int64_t nvox, float rParticle,
 /* ---- A11/A13: halo fill of block bid into a float (or double) buffer of
  * (bsx+2*dmax)^3 (halo.cpp:267-479, halo.h:267-355,422-501,
  * sbg.h:3002-3095). Lo may be NULL unless OPT_BC_COARSE_LEVEL is set. */
+/* SparseGrid<Data_T>::interpolateGhost(p, 0) (sbg.h:3002-3095) of level Lo
+ * at the position (px,py,pz) in Lo's voxel coordinates, in Lo's raw domain
+ * (UNORM16: the truncated integer of each interpolation stage). */
+double or_interpolate_ghost(const or_level *Lo, float px, float py, float pz);
 void or_halo_fill(const or_level *L, const or_level *Lo, int bid, int dmax,
                   unsigned opt, float *out);
 void or_halo_fill_f64(const or_level *L, int bid, int dmax, unsigned opt,

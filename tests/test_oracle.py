@@ -289,6 +289,29 @@ def test_finalize_table_closed_form():
     assert np.all(t[cut:] == 0) and t[cut - 1] > 0
 
 
+def test_finalize_density_float_closed_form():
+    # the float variant behind the FP32/FP16/BF16 finalize: with rParticle =
+    # nbDist = 2 the ramp never saturates, so f = 1 - sqrt(q) * 4 / 6 below
+    # the 0.999 cut and 0 above it, for every code
+    f = O.finalize_density_table(2.0, 2.0).astype(np.float64)
+    u = np.arange(65536)
+    q32 = u.astype(np.float32) * np.float32(1 / 65535)
+    below = q32 < np.float32(0.999)
+    want = 1 - np.sqrt(q32.astype(np.float64)) * 4 / 6
+    assert np.abs(f[below] - want[below]).max() < 3e-7
+    assert np.all(f[~below] == 0.0) and f[0] == 1.0
+    assert np.all(np.diff(f[below]) <= 0)  # farther from a particle, less density
+    # the UNORM16 finalize is the same value rounded half away from zero
+    f32 = O.finalize_density_table(2.0, 2.0)
+    prod = (f32 * np.float32(65535)).astype(np.float64)  # the FP32 product, then roundf
+    assert np.array_equal(O.finalize_table(2.0, 2.0), np.floor(prod + 0.5).astype(np.uint16))
+    # other radii move the ramp: rParticle 1, nbDist 3 (rScan 4, ramp [-1, 4])
+    g = O.finalize_density_table(1.0, 3.0).astype(np.float64)
+    d = np.sqrt(q32.astype(np.float64) * 16) - 1
+    want = 1 - np.clip((d + 1) / 5, 0, 1)
+    assert np.abs(g[below] - want[below]).max() < 3e-7
+
+
 # --------------------------------------------------------------------------
 # A11 halo (halo.cpp:267-479): tag test against a dense embedding
 # --------------------------------------------------------------------------
@@ -440,7 +463,9 @@ def test_sweep_driver_matches_dense_colour_restatement():
     n = O.mcf_sweeps(lvl, vb, levels, 3, 0.1, 1, residual=True)
     assert n.size == 4  # numIter 3 rounds up to 4 sweeps
     ref = pages.copy()
+    rms = []
     for _ in range(4):
+        before = ref.astype(np.float64)
         for c in range(8):
             for bid in vb:
                 bz, rem = divmod(int(bid), nb * nb)
@@ -449,7 +474,36 @@ def test_sweep_driver_matches_dense_colour_restatement():
                     continue
                 h = dense_halo(pm, ref, S, bsx, int(bid))
                 ref[pm[bid]] = _np_mcf(h, 0.1)
+        # the residual: RMS over the swept voxels of this sweep's change
+        rms.append(np.sqrt(np.mean((ref.astype(np.float64) - before) ** 2)))
     assert np.array_equal(lvl.pages, ref)
+    np.testing.assert_allclose(n, rms, rtol=1e-12, atol=0)
+    assert np.all(np.diff(n) < 0)  # the flow settles
+
+
+def test_residual_zero_for_fields_the_flow_keeps():
+    # constant and linear fields are fixed points of the update (the linear
+    # one away from the domain faces, where the Neumann halo bends it): every
+    # sweep's residual over the interior blocks is exactly 0
+    S, bsx = 32, 8
+    z, y, x = np.meshgrid(*[np.arange(S, dtype=np.float64)] * 3, indexing="ij")
+    nb = S // bsx
+    for f in (np.full((S,) * 3, 0.375), 0.25 + x / 128 + y / 256):
+        for st in (O.F32, O.F64):
+            dt_ = np.float64 if st == O.F64 else np.float32
+            pages = np.stack([f[(b // (nb * nb)) * bsx:(b // (nb * nb) + 1) * bsx,
+                                ((b // nb) % nb) * bsx:((b // nb) % nb + 1) * bsx,
+                                (b % nb) * bsx:(b % nb + 1) * bsx] for b in range(nb ** 3)])
+            lvl = O.Level(S, S, S, bsx, np.arange(nb ** 3, dtype=np.int32), pages.astype(dt_), st)
+            inner = np.array([b for b in range(nb ** 3)
+                              if all(0 < c < nb - 1 for c in (b % nb, (b // nb) % nb, b // nb ** 2))],
+                             np.int32)
+            r = O.mcf_sweeps(lvl, inner, np.zeros(nb ** 3, np.uint16), 2, 0.1, 1, residual=True)
+            assert r.size == 2 and np.all(r == 0.0), (st, r)
+            # all blocks: the faces move, and the residual sees it
+            r = O.mcf_sweeps(lvl, np.arange(nb ** 3, dtype=np.int32), np.zeros(nb ** 3, np.uint16),
+                             2, 0.1, 1, residual=True)
+            assert np.all(r == 0.0) == (f.std() == 0), (st, r)
 
 
 def test_ghost_downsample_closed_forms():
@@ -537,7 +591,9 @@ def test_coarse_level_sweep_matches_dense_restatement():
                              residual=True)
     assert res.size == 2 and np.all(res > 0)
     ref = pages1.copy()
+    rms = []
     for _ in range(2):
+        before = ref[pm1[own]].astype(np.float64)
         for c in range(8):
             for bid in own:
                 bz, rem = divmod(int(bid), nb * nb)
@@ -546,7 +602,10 @@ def test_coarse_level_sweep_matches_dense_restatement():
                     continue
                 h = _coarse_halo(pm1, ref, S1, b1, int(bid), lo_dense)
                 ref[pm1[bid]] = _np_mcf(h, 0.1)
+        # RMS over the level's own (swept) blocks only: ghost pages do not count
+        rms.append(np.sqrt(np.mean((ref[pm1[own]].astype(np.float64) - before) ** 2)))
     assert np.array_equal(L1.pages, ref)
+    np.testing.assert_allclose(res, rms, rtol=1e-12, atol=0)
     # ghost pages of finer blocks are read, never written
     gi = pm1[ghost]
     assert np.array_equal(L1.pages[gi], pages1[gi])
@@ -554,3 +613,153 @@ def test_coarse_level_sweep_matches_dense_restatement():
     L1n = O.Level(S1, S1, S1, b1, pm1, pages1.copy(), O.F32)
     O.mcf_sweeps_level(L1n, None, vals, levels, 1, O.OPT_BC_NEUMANN, 2, 0.1, 1)
     assert not np.array_equal(L1n.pages, L1.pages)
+
+
+# --------------------------------------------------------------------------
+# A13 interpolateGhost (sbg.h:3002-3095): closed forms and hand KATs
+# --------------------------------------------------------------------------
+def _level_from_dense(f, bsx, storage=O.F32, null=(), empty_raw=0.0):
+    """A level whose value blocks hold the dense array f[z, y, x], with the
+    blocks in `null` left NULL."""
+    S = f.shape[0]
+    nb = S // bsx
+    pm = np.full(nb ** 3, O.NULL, np.int32)
+    vb = [b for b in range(nb ** 3) if b not in set(null)]
+    pm[vb] = np.arange(len(vb))
+    pages = np.stack([f[(b // (nb * nb)) * bsx:(b // (nb * nb) + 1) * bsx,
+                        ((b // nb) % nb) * bsx:((b // nb) % nb + 1) * bsx,
+                        (b % nb) * bsx:(b % nb + 1) * bsx] for b in vb])
+    return O.Level(S, S, S, bsx, pm, pages, storage, empty_raw)
+
+
+def _grid(S):
+    return np.meshgrid(*[np.arange(S, dtype=np.float64)] * 3, indexing="ij")  # z, y, x
+
+
+def test_interpolate_ghost_linear_field_is_exact():
+    # a linear field sampled at the cell centres is reproduced exactly at any
+    # position whose eight corners lie in the domain (dyadic values, so FP32
+    # is exact for positions on the half-voxel lattice)
+    S, bsx = 16, 4
+    a, bx, by, bz = 0.25, 1 / 32, -1 / 64, 1 / 16
+    z, y, x = _grid(S)
+    f = (a + bx * (x + 0.5) + by * (y + 0.5) + bz * (z + 0.5)).astype(np.float32)
+    lvl = _level_from_dense(f, bsx)
+    for g in [(1, 1, 1), (5, 20, 9), (2 * S - 2, 3, 17), (14, 15, 2 * S - 2)]:
+        p = [(c + 0.5) / 2 for c in g]  # the fine-voxel centre in coarse coordinates
+        want = a + bx * p[0] + by * p[1] + bz * p[2]
+        assert O.interpolate_ghost(lvl, p) == want, g
+    rng = np.random.default_rng(2)
+    for p in rng.uniform(0.5, S - 0.5, (200, 3)):
+        p32 = p.astype(np.float32).astype(np.float64)
+        want = a + bx * p32[0] + by * p32[1] + bz * p32[2]
+        assert abs(O.interpolate_ghost(lvl, p32) - want) < 2e-6
+
+
+def test_interpolate_ghost_constant_and_clipping():
+    S, bsx = 8, 4
+    z, y, x = _grid(S)
+    const = _level_from_dense(np.full((S,) * 3, 0.625, np.float32), bsx)
+    for p in [(0.1, 0.2, 0.3), (7.9, 7.75, 0.0), (4.0, 4.0, 4.0), (-0.5, 9.0, 3.3)]:
+        assert O.interpolate_ghost(const, p) == 0.625
+    # clamp to edge: outside the cell centres of the first and last voxel the
+    # value is that voxel's, whatever the fraction
+    ramp = _level_from_dense((0.25 + x / 8).astype(np.float32), bsx)
+    assert O.interpolate_ghost(ramp, (0.25, 3.5, 3.5)) == 0.25
+    assert O.interpolate_ghost(ramp, (0.0, 0.3, 7.9)) == 0.25
+    assert O.interpolate_ghost(ramp, (S - 0.25, 3.5, 3.5)) == 0.25 + (S - 1) / 8
+    # between the centres it is the linear ramp
+    assert O.interpolate_ghost(ramp, (2.25, 3.5, 3.5)) == 0.25 + 1.75 / 8
+
+
+def test_interpolate_ghost_null_corners_take_the_value_at_p():
+    # Corners in NULL blocks take getValue(p) (floor(p), clamped) once; if
+    # that voxel's block is NULL too, the empty value. Hand-evaluated:
+    # block 0 holds 1 except voxel (3,1,1) = 5; block 1 (x = 4..7) is NULL.
+    S, bsx = 8, 4
+    f = np.ones((S,) * 3, np.float32)
+    f[1, 1, 3] = 5.0
+    lvl = _level_from_dense(f, bsx, null=(1,), empty_raw=0.5)
+    # p = (3.75, 1.75, 1.75): u = v = w = 1/4, getValue(p) = f(3,1,1) = 5
+    #   x: (5,5)->5, (1,5)->2, (1,5)->2, (1,5)->2;  y: (5,2)->4.25, (2,2)->2;
+    #   z: (4.25, 2) -> 3.6875
+    assert O.interpolate_ghost(lvl, (3.75, 1.75, 1.75)) == 3.6875
+    # p = (4.25, 1.75, 1.75): u = 3/4, floor(p) = (4,1,1) lies in the NULL
+    # block, so the NULL corners read the empty value 0.5
+    #   x: (5,.5)->1.625, (1,.5)->0.625 (x3);  y: (1.625,.625)->1.375, ->.625;
+    #   z: (1.375, .625) -> 1.1875
+    assert O.interpolate_ghost(lvl, (4.25, 1.75, 1.75)) == 1.1875
+
+
+def test_interpolate_ghost_unorm16_truncates_each_stage():
+    # UNORM16 intermediates are uint16: each stage truncates toward zero
+    S, bsx = 8, 4
+    z, y, x = _grid(S)
+    lvl = _level_from_dense((3 * (x.astype(np.int64) % 2)).astype(np.uint16), bsx, O.U16)
+    assert O.interpolate_ghost(lvl, (0.75, 2.5, 2.5)) == 0.0  # 3 * 0.25 = 0.75 -> 0
+    assert O.interpolate_ghost(lvl, (1.25, 2.5, 2.5)) == 2.0  # 3 * 0.75 = 2.25 -> 2
+    flt = _level_from_dense((3 * (x.astype(np.int64) % 2)).astype(np.float32), bsx)
+    assert O.interpolate_ghost(flt, (0.75, 2.5, 2.5)) == 0.75
+
+
+def test_coarse_interpolate_halo_is_the_linear_field():
+    # OPT_BC_COARSE_LEVEL|OPT_INTERPOLATE (halo.h:288-310): a NULL neighbour's
+    # halo voxel g reads the coarser level at (g + .5)/2, after clipping g
+    # into the domain (a clipped voxel in a non-NULL block reads that block).
+    # With a linear field on both levels (fine voxel g holds
+    # F(g) = f((g + .5)/2), coarse voxel X holds f(X + .5)) the halo is F,
+    # except that the coarse interpolation clamps to its edge cell centres
+    # (0.5 and S/2 - 0.5 per axis).
+    S, bsx = 32, 8
+    nb = S // bsx
+    a, bx, by, bz = 0.25, 1 / 16, 1 / 32, -1 / 64
+    fz = lambda z_, y_, x_: a + bx * x_ + by * y_ + bz * z_  # noqa: E731
+    z, y, x = _grid(S // 2)
+    lo = _level_from_dense(fz(z + 0.5, y + 0.5, x + 0.5).astype(np.float32), bsx // 2)
+    z, y, x = _grid(S)
+    Fd = fz((z + 0.5) / 2, (y + 0.5) / 2, (x + 0.5) / 2).astype(np.float32)
+    for b in (((1 * nb) + 1) * nb + 2, 0, nb ** 3 - 1):
+        fine = _level_from_dense(Fd, bsx, null=[i for i in range(nb ** 3) if i != b])
+        h = O.halo_fill(fine, b, 1, O.OPT_BC_COARSE_LEVEL | O.OPT_INTERPOLATE, lo)
+        h = h.reshape((bsx + 2,) * 3)
+        bz_, rem = divmod(b, nb * nb)
+        by_, bx_ = divmod(rem, nb)
+        o = np.array([bz_, by_, bx_]) * bsx
+        for k in np.ndindex(*h.shape):
+            gc = np.clip(o + np.array(k) - 1, 0, S - 1)
+            if (gc // bsx == o // bsx).all():
+                want = Fd[tuple(gc)]  # the block itself (in the domain or clipped into it)
+            else:
+                pc = np.clip((gc + 0.5) / 2, 0.5, S / 2 - 0.5)
+                want = np.float32(fz(*pc))
+            assert h[k] == want, (b, k)
+
+
+# --------------------------------------------------------------------------
+# prepareGhostBlocks' block selection (msbg.cpp:3100-3149)
+# --------------------------------------------------------------------------
+def test_prepare_ghost_blocks_selects_fine_coarse_existing_blocks():
+    # Only blocks of the fine-coarse lists of levels 0..nLevels-2 that carry
+    # BLK_EXISTS are downsampled into their level+1 slot; the others (no
+    # BLK_EXISTS: the reference sets the EMPTY block there) and every block
+    # outside the lists keep what the slot held, and the list of the last
+    # level is not used.
+    S0, b0 = 32, 8
+    nb = S0 // b0
+    pages0 = np.full((nb ** 3, b0, b0, b0), 0.5, np.float32)
+    L0 = O.Level(S0, S0, S0, b0, np.arange(nb ** 3, dtype=np.int32), pages0, O.F32)
+    pm1 = np.full(nb ** 3, O.NULL, np.int32)
+    slots = [3, 17, 40, 41]
+    pm1[slots] = np.arange(len(slots))
+    L1 = O.Level(S0 // 2, S0 // 2, S0 // 2, b0 // 2, pm1,
+                 np.full((len(slots), b0 // 2, b0 // 2, b0 // 2), 0.125, np.float32), O.F32)
+    flags = np.zeros(nb ** 3, np.uint16)
+    EXISTS, FC = 1 << 12, 1 << 3
+    flags[[3, 40, 41]] |= EXISTS
+    flags[[3, 17, 40]] |= FC
+    O.prepare_ghost_blocks([L0, L1], [[3, 17, 40], [41]], flags)
+    got = {b: L1.pages[pm1[b]] for b in slots}
+    assert np.all(got[3] == 0.5) and np.all(got[40] == 0.5)  # downsampled (constant field)
+    assert np.all(got[17] == 0.125)  # in the list but without BLK_EXISTS
+    assert np.all(got[41] == 0.125)  # only in the last level's list
+    assert np.all(L0.pages == 0.5)  # the fine level is only read
