@@ -625,6 +625,74 @@ def test_batch_graph_replays_match_eager():
                                       gb.download(BCH).view(np.uint16)), rep
 
 
+def test_batch_cfg4_grids_against_oracle():
+    """cfg4's generator at test sizes: 16 grids of 128^3-256^3, 16^3 or 32^3
+    blocks, 1-3 levels, 1-20 % active, FP16 and FP32. Exact arithmetic
+    through one batched launch, as a captured graph and eagerly: every grid
+    bitwise equal to the oracle's own sweeps (msbg4.cpp:384-900). Fast
+    arithmetic through the batch: within the per-storage sweep bound."""
+    from msbg_amd.studies import _scene_grid
+    rng = np.random.default_rng(2024)
+    CH, CHE, CHF = 50, 51, 52
+    grids, want, want_fast, st = [], [], [], []
+    try:
+        for k in range(16):
+            S = int(rng.choice([128, 256]))
+            bsx = int(rng.choice([16, 32]))
+            nl = int(rng.choice([1, 2, 3]))
+            frac = float(rng.uniform(0.01, 0.20))
+            nObj = int(rng.integers(8, 129))
+            storage = (M.F16, M.F32)[k % 2]
+            note("cfg4 grid", k, S, bsx, nl, round(frac, 3), nObj, storage)
+            g, _ = _scene_grid(rng, S, bsx, nl, frac, nObj, CH, storage)
+            grids.append(g)
+            st.append(storage)
+            g.copyDataTo(CH, CHE, storage)
+            g.copyDataTo(CH, CHF, storage)
+            pm, _ = g.pagemap(CH)
+            pages = g.download(CH)
+            if storage == M.F16:
+                pages = pages.view(np.uint16)
+            levels, _ = g.getBlockInfo()
+            blocks = np.nonzero(pm >= 0)[0].astype(np.int32)
+            ost = O.F16 if storage == M.F16 else O.F32
+            Lx = O.Level(S, S, S, bsx, pm, pages.copy(), ost)
+            O.mcf_sweeps(Lx, blocks, levels, 3, 0.1, 1)  # 4 sweeps
+            want.append(Lx.pages)
+            Lf = O.Level(S, S, S, bsx, pm, pages.copy(), ost)
+            O.mcf_sweeps(Lf, blocks, levels, 2, 0.1, 1)
+            want_fast.append(Lf.pages)
+        for g in grids:
+            g.set_arith(M.ARITH_EXACT)
+        with M.Batch() as bg, M.Batch() as be:
+            for g in grids:
+                bg.add(g, CH, 0.1, 1)
+                be.add(g, CHE, 0.1, 1)
+            bg.run(3, useGraph=True)
+            be.run(3, useGraph=False)
+        for k, g in enumerate(grids):
+            for ch in (CH, CHE):
+                got = g.download(ch)
+                if st[k] == M.F16:
+                    got = got.view(np.uint16)
+                assert np.array_equal(got, want[k]), (k, ch)
+        for g in grids:
+            g.set_arith(M.ARITH_FAST)
+        with M.Batch() as bf:
+            for g in grids:
+                bf.add(g, CHF, 0.1, 1)
+            bf.run(2, useGraph=True)
+        for k, g in enumerate(grids):
+            got = g.download(CHF).astype(np.float64)
+            w = want_fast[k]
+            w = w.view(np.float16).astype(np.float64) if st[k] == M.F16 else w.astype(np.float64)
+            d = np.abs(got.reshape(w.shape) - w)
+            assert d.max() <= _FAST_MAX[st[k]], (k, d.max())
+    finally:
+        for g in grids:
+            g.destroy()
+
+
 def test_batch_autotune_keeps_results():
     note("batch autotune")
     specs = [(128, 32), (128, 16)]
