@@ -29,7 +29,29 @@ enum : unsigned {
   OPT_8_COLOR_SCHEME = MSBG_OPT_8_COLOR_SCHEME,
 };
 enum { PDE_MEAN_CURVATURE = MSBG_PDE_MEAN_CURVATURE };
-enum { CH_NULL = MSBG_CH_NULL, CH_FLOAT_1 = MSBG_CH_FLOAT_1, CH_UINT16_1 = MSBG_CH_UINT16_1 };
+// channel ids (msbg.h:274-343)
+enum {
+  CH_NULL = MSBG_CH_NULL,
+  CH_FLOAT_1 = MSBG_CH_FLOAT_1,
+  CH_FLOAT_2 = 6,
+  CH_FLOAT_3 = 7,
+  CH_UINT16_1 = MSBG_CH_UINT16_1,
+};
+
+// WTURB_LapSmConf (wturbglob.h:217-233): the smoothing pass that
+// applyChannelPdeCnf runs
+typedef struct {
+  int typ, use_pres_grad_mag;
+  int iter;
+  float thr, thr_tol1, thr_tol2;
+  float dt;
+  float velmag_thr;
+  float lambda;
+  float unused;
+  float taubin;
+  float lambda_sdf;
+  float redist;
+} WTURB_LapSmConf;
 
 struct BlockInfo {  // msbg.h:673-678
   uint16_t level, flags;
@@ -58,6 +80,22 @@ struct StorageOf<float> {
 template <>
 struct StorageOf<double> {
   static constexpr int value = MSBG_STORAGE_F64;
+};
+// The 2-byte float storages have no type of their own in the reference
+// (its channels are float, uint16_t or uint8_t): these tags name them.
+struct half16 {
+  uint16_t bits;
+};
+struct bfloat16 {
+  uint16_t bits;
+};
+template <>
+struct StorageOf<half16> {
+  static constexpr int value = MSBG_STORAGE_F16;
+};
+template <>
+struct StorageOf<bfloat16> {
+  static constexpr int value = MSBG_STORAGE_BF16;
 };
 
 // getBlockColor8 (msbg.h:642-646)
@@ -149,23 +187,47 @@ class MultiresSparseGrid {
 
   void prepareGhostBlocks(int chan) { check(msbg_prepare_ghost_blocks(g_, chan)); }
 
-  // applyChannelPdeFast<T> (msbg.h:2150-2172); laplTyp must be
-  // -(PDE_MEAN_CURVATURE + OPT_8_COLOR_SCHEME).
+  // applyChannelPdeFast<T> (msbg.h:2150-2172, msbg4.cpp:60-957) with the
+  // reference's parameter list. Data_T must be the channel's storage
+  // (StorageOf); laplTyp, lambda, taubin and anisDiffAlpha go to
+  // msbg_apply_channel_pde, which rejects the operators and modes outside
+  // this path. The velocity-dependent, SDF and redistancing terms
+  // (velMagThr, lambdaSDF, redist, veldep_on) are outside it too: non-zero
+  // values throw MSBG_EUNSUPP instead of being dropped. residual (an
+  // addition) receives one FP64 RMS update per sweep.
   template <typename Data_T>
   void applyChannelPdeFast(int chPhi, int chTmp, int chTmp2, std::vector<int> *blockList,
                            int laplTyp, int doConstrZeroOne, float densThr, float densThrTol1,
                            float densThrTol2, int numIter, float gridSpacing = 1,
-                           float timeStep = 0.025f, float /*velMagThr*/ = 0,
-                           MultiresSparseGrid * /*msbgVel*/ = nullptr, float /*lambda*/ = 0,
-                           float /*lambdaSDF*/ = 0, float /*taubin*/ = 0, float /*redist*/ = 0,
-                           float /*anisDiffAlpha*/ = 0, int /*veldep_on*/ = 0,
+                           float timeStep = 0.025f, float velMagThr = 0,
+                           MultiresSparseGrid * /*msbgVel*/ = nullptr, float lambda = 0,
+                           float lambdaSDF = 0, float taubin = 0, float redist = 0,
+                           float anisDiffAlpha = 0, int veldep_on = 0,
                            float * /*veldep_fit*/ = nullptr, int /*visOpt*/ = 0,
                            double *residual = nullptr) {
-    (void)StorageOf<Data_T>::value;
-    check(msbg_apply_channel_pde_fast(
-        g_, chPhi, chTmp, chTmp2, blockList ? blockList->data() : nullptr,
-        blockList ? (int64_t)blockList->size() : 0, laplTyp, doConstrZeroOne, densThr,
-        densThrTol1, densThrTol2, numIter, gridSpacing, timeStep, residual));
+    int st = 0;
+    check(msbg_channel_storage(g_, chPhi, &st));
+    if (st != StorageOf<Data_T>::value)
+      throw Error(MSBG_EINVAL, "applyChannelPdeFast<T>: channel " + std::to_string(chPhi) +
+                                   " has storage " + std::to_string(st) + ", T names storage " +
+                                   std::to_string(StorageOf<Data_T>::value));
+    if (velMagThr != 0 || lambdaSDF != 0 || redist != 0 || veldep_on != 0)
+      throw Error(MSBG_EUNSUPP,
+                  "applyChannelPdeFast: velMagThr, lambdaSDF, redist and veldep_on are outside "
+                  "this path (must be 0)");
+    check(msbg_apply_channel_pde(g_, chPhi, chTmp, chTmp2, blockList ? blockList->data() : nullptr,
+                                 blockList ? (int64_t)blockList->size() : 0, laplTyp,
+                                 doConstrZeroOne, densThr, densThrTol1, densThrTol2, numIter,
+                                 gridSpacing, timeStep, lambda, taubin, anisDiffAlpha, residual));
+  }
+
+  // applyChannelPdeCnf (msbg.h:2174-2195): the configured smoothing pass on
+  // a float channel with CH_FLOAT_2 / CH_FLOAT_3 as its temporaries
+  void applyChannelPdeCnf(int chan, WTURB_LapSmConf *cnf) {
+    applyChannelPdeFast<float>(chan, CH_FLOAT_2, CH_FLOAT_3, nullptr, cnf->typ, 1, cnf->thr,
+                               cnf->thr_tol1, cnf->thr_tol2, cnf->iter, 1.0f, cnf->dt,
+                               cnf->velmag_thr, this, cnf->lambda, cnf->lambda_sdf, cnf->taubin,
+                               cnf->redist, 0, 0);
   }
 
   // The same 8-colour MCF sweep on the blocks of one coarser level, on that
@@ -208,6 +270,12 @@ class MultiresSparseGrid {
   }
   void finalizeDensity(int chan, float rParticle, float nbDist) {
     check(msbg_finalize_density(g_, chan, rParticle, nbDist));
+  }
+  // SparseGrid<T>::copyDataTo (sbg.h:550-557) into channel chDst of storage
+  // Dst_T (values decoded exactly, re-encoded with round-to-nearest-even)
+  template <typename Dst_T>
+  void copyChannel(int chSrc, int chDst, int level = 0) {
+    check(msbg_channel_copy(g_, chSrc, chDst, StorageOf<Dst_T>::value, level));
   }
   template <typename Data_T>
   void getBlockData(int chan, int bid, Data_T *out, int level = 0) {

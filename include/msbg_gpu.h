@@ -203,6 +203,11 @@ int msbg_channel_alloc_demo(msbg_grid *g, int chan, int storage,
                             int32_t *activeBlocks, int64_t cap,
                             int64_t *nactive);
 
+/* Storage format (MSBG_STORAGE_*) of a defined channel. No counterpart in the
+ * reference, where it is the channel's template type Data_T
+ * (getChannel<Data_T>, msbg4.cpp:236-240). */
+int msbg_channel_storage(msbg_grid *g, int chan, int *storage);
+
 /* Page map of (chan, level): codes MSBG_BLOCK_* or page index per block. */
 int msbg_channel_get_pagemap(msbg_grid *g, int chan, int level,
                              int32_t *pagemap, int64_t *npages);

@@ -75,6 +75,7 @@ SIGNATURES = {
     "msbg_get_fine_coarse_blocks": (_I, [_P, _I, _P, _I64, _P]),
     "msbg_channel_define": (_I, [_P, _I, _I, _I, _P, _D, _D, _D]),
     "msbg_channel_alloc_demo": (_I, [_P, _I, _I, _P, _I64, _P]),
+    "msbg_channel_storage": (_I, [_P, _I, C.POINTER(_I)]),
     "msbg_channel_get_pagemap": (_I, [_P, _I, _I, _P, _P]),
     "msbg_channel_download": (_I, [_P, _I, _I, _P, _I64]),
     "msbg_channel_upload": (_I, [_P, _I, _I, _P, _I64]),

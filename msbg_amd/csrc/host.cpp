@@ -519,6 +519,14 @@ static int get_level(msbg_grid *g, int chan, int level, Channel **pch, ChanLevel
   return 0;
 }
 
+int msbg_channel_storage(msbg_grid *g, int chan, int *storage) {
+  MSBG_CHECK(g && storage, MSBG_EINVAL, "msbg_channel_storage: bad args");
+  auto it = g->channels.find(chan);
+  MSBG_CHECK(it != g->channels.end(), MSBG_ESTATE, "channel %d is not defined", chan);
+  *storage = it->second.storage;
+  return 0;
+}
+
 int msbg_channel_get_pagemap(msbg_grid *g, int chan, int level, int32_t *pagemap,
                              int64_t *npages) {
   Channel *ch;

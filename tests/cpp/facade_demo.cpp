@@ -4,7 +4,10 @@
 // ray-march render (msbg_demo.cpp:129-185). Writes the final UNORM16 pages
 // of the active blocks to argv[1] (raw, block order of the active list)
 // and, given argv[2], a 320x180 frame of BMP_MKRGB words to it, for
-// comparison with the Python binding.
+// comparison with the Python binding. Given argv[3], it also copies the
+// field into a float channel, runs applyChannelPdeCnf's Taubin-smoothed
+// laplacian flow on it and writes those pages there, and checks that a
+// storage type other than the channel's is rejected.
 #include <cstdio>
 #include <vector>
 
@@ -55,6 +58,34 @@ int main(int argc, char **argv) {
       FILE *fi = std::fopen(argv[2], "wb");
       std::fwrite(img.data(), 4, img.size(), fi);
       std::fclose(fi);
+    }
+    if (argc > 3) {
+      // applyChannelPdeCnf: laplacian flow (laplTyp 1) in 8 colours with
+      // Taubin steps, the reference's configured smoothing pass
+      msbg->copyChannel<float>(MSBG::CH_UINT16_1, MSBG::CH_FLOAT_1);
+      MSBG::WTURB_LapSmConf cnf{};
+      cnf.typ = -(1 + (int)MSBG::OPT_8_COLOR_SCHEME);
+      cnf.iter = 4;
+      cnf.dt = 0.05f;
+      cnf.taubin = 0.5f;
+      msbg->applyChannelPdeCnf(MSBG::CH_FLOAT_1, &cnf);
+      FILE *ff = std::fopen(argv[3], "wb");
+      std::vector<float> fb(bsx * bsx * bsx);
+      for (int bid : activeBlocks) {
+        msbg->getBlockData<float>(MSBG::CH_FLOAT_1, bid, fb.data());
+        std::fwrite(fb.data(), 4, fb.size(), ff);
+      }
+      std::fclose(ff);
+      try {  // float is not the UNORM16 channel's type
+        msbg->applyChannelPdeFast<float>(MSBG::CH_UINT16_1, MSBG::CH_NULL, MSBG::CH_NULL,
+                                         nullptr,
+                                         -(MSBG::PDE_MEAN_CURVATURE + MSBG::OPT_8_COLOR_SCHEME),
+                                         1, 0, 0, 0, 2, 1.0f, 0.1f);
+        std::fprintf(stderr, "storage mismatch not rejected\n");
+        return 1;
+      } catch (const MSBG::Error &e) {
+        if (e.code != MSBG_EINVAL) return 1;
+      }
     }
     std::printf("facade_demo: %zu active blocks\n", activeBlocks.size());
     MSBG::MultiresSparseGrid::destroy(msbg);

@@ -516,8 +516,9 @@ def test_cpp_facade_demo_matches_binding(tmp_path):
     exe = os.path.join(os.path.dirname(__file__), "cpp", "facade_demo")
     if not os.path.exists(exe):
         pytest.fail("tests/cpp/facade_demo not built (run the build step)")
-    out, frame = tmp_path / "pages.bin", tmp_path / "frame.bin"
-    r = subprocess.run([exe, str(out), str(frame)], capture_output=True, text=True, timeout=120)
+    out, frame, cnf = tmp_path / "pages.bin", tmp_path / "frame.bin", tmp_path / "cnf.bin"
+    r = subprocess.run([exe, str(out), str(frame), str(cnf)], capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stderr
     got = np.fromfile(out, dtype=np.uint16)
     S, bsx = 128, 16
@@ -530,6 +531,17 @@ def test_cpp_facade_demo_matches_binding(tmp_path):
                              sunPos=(.5, 5, 5), color=(0.8, 0.6, 0.4))
     assert np.array_equal(img, want)
     assert ((img & 0xffffff) != 0).sum() > 1000
+    # applyChannelPdeCnf (Taubin laplacian flow on a float copy) through the
+    # facade = msbg_apply_channel_pde with lambda/taubin through ctypes
+    g = gp["grid"]
+    g.copyDataTo(30, M.CH_FLOAT_1, M.F32)
+    g.applyChannelPdeFast(M.CH_FLOAT_1, 6, 7, None, -(1 + M.OPT_8_COLOR_SCHEME), 1, 0, 0, 0, 4,
+                          1.0, 0.05, taubinStrength=0.5)
+    got = np.fromfile(cnf, dtype=np.float32)
+    assert np.array_equal(got, g.download(M.CH_FLOAT_1).reshape(-1))
+    g.copyDataTo(30, 8, M.F32)  # without Taubin the result differs
+    g.applyChannelPdeFast(8, 6, 7, None, -(1 + M.OPT_8_COLOR_SCHEME), 1, 0, 0, 0, 4, 1.0, 0.05)
+    assert not np.array_equal(got, g.download(8).reshape(-1))
 
 
 # --------------------------------------------------------------------------
