@@ -36,6 +36,7 @@ enum {
   CH_FLOAT_2 = 6,
   CH_FLOAT_3 = 7,
   CH_UINT16_1 = MSBG_CH_UINT16_1,
+  CH_UINT8_1 = 38,
 };
 
 // WTURB_LapSmConf (wturbglob.h:217-233): the smoothing pass that
@@ -72,6 +73,10 @@ struct StorageOf;
 template <>
 struct StorageOf<uint16_t> {
   static constexpr int value = MSBG_STORAGE_U16;
+};
+template <>
+struct StorageOf<uint8_t> {
+  static constexpr int value = MSBG_STORAGE_U8;
 };
 template <>
 struct StorageOf<float> {

@@ -40,7 +40,10 @@ enum {
   MSBG_STORAGE_F32 = 2,
   MSBG_STORAGE_F16 = 3,  /* IEEE binary16, FP32 arithmetic */
   MSBG_STORAGE_BF16 = 4, /* bfloat16, FP32 arithmetic */
-  MSBG_STORAGE_F64 = 5   /* FP64 storage and arithmetic (oracle run) */
+  MSBG_STORAGE_F64 = 5,  /* FP64 storage and arithmetic (oracle run) */
+  MSBG_STORAGE_U8 = 6    /* UNORM8: value u/255 (the uint8_t RenderDensity of
+                            applyChannelPdeFast<uint8_t>, msbg4.cpp:1008-1030);
+                            the sweeps store it with stochastic rounding */
 };
 
 /* ---- option bits and ids with the reference's values (msbg.h:101-391) ---- */

@@ -10,9 +10,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmsbg_gpu.so")
 
 # storage formats
-U16, F32, F16, BF16, F64 = 1, 2, 3, 4, 5
-STORAGE_BYTES = {U16: 2, F32: 4, F16: 2, BF16: 2, F64: 8}
-STORAGE_NAMES = {U16: "u16", F32: "f32", F16: "f16", BF16: "bf16", F64: "f64"}
+U16, F32, F16, BF16, F64, U8 = 1, 2, 3, 4, 5, 6
+STORAGE_BYTES = {U16: 2, F32: 4, F16: 2, BF16: 2, F64: 8, U8: 1}
+STORAGE_NAMES = {U16: "u16", F32: "f32", F16: "f16", BF16: "bf16", F64: "f64", U8: "u8"}
 
 # reference option values (msbg.h:101-391)
 OPT_INTERPOLATE = 1 << 7
@@ -22,7 +22,7 @@ OPT_BC_COARSE_LEVEL = 1 << 26
 OPT_8_COLOR_SCHEME = 1 << 29
 PDE_MEAN_CURVATURE = 4
 IP_LINEAR, IP_BSPLINE3 = 0, 1  # msbg_interpolate kinds
-CH_NULL, CH_FLOAT_1, CH_UINT16_1 = 0, 1, 30
+CH_NULL, CH_FLOAT_1, CH_FLOAT_2, CH_FLOAT_3, CH_UINT16_1, CH_UINT8_1 = 0, 1, 6, 7, 30, 38
 
 BLK_DOM_BORDER = 1 << 1
 BLK_COARSE_FINE = 1 << 2

@@ -159,7 +159,7 @@ int enqueue(msbg_batch *b, int sweeps) {
         int64_t n = G.off[pass + 1] - G.off[pass];
         if (!n) continue;
         MSBG_TRY(launch_sweep_items(b->stream, G.bsx, G.storage, G.exact, G.c01, G.tile, b->d_desc.p,
-                                    G.items.p + G.off[pass], n, nullptr));
+                                    G.items.p + G.off[pass], n, nullptr, it));
       }
   return 0;
 }
@@ -247,7 +247,7 @@ int msbg_batch_autotune(msbg_batch *b, int passes) {
           int64_t n = G.off[pass + 1] - G.off[pass];
           if (n)
             MSBG_TRY(launch_sweep_items(b->stream, G.bsx, G.storage, false, G.c01, tile, b->d_desc.p,
-                                        G.items.p + G.off[pass], n, nullptr));
+                                        G.items.p + G.off[pass], n, nullptr, rep));
         }
       }
       MSBG_HIP(hipEventRecord(b->ev[1], b->stream));

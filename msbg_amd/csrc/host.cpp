@@ -54,6 +54,7 @@ int storage_bytes(int storage) {
     case MSBG_STORAGE_BF16: return 2;
     case MSBG_STORAGE_F32: return 4;
     case MSBG_STORAGE_F64: return 8;
+    case MSBG_STORAGE_U8: return 1;
   }
   return 0;
 }
@@ -492,8 +493,9 @@ int msbg_channel_alloc_demo(msbg_grid *g, int chan, int storage, int32_t *active
       na++;
     }
   }
-  // renderDensFromFloat_(0) / (1): UNORM 0 / 65535; the float formats 0 / 1
-  double full = storage == MSBG_STORAGE_U16 ? 65535.0 : 1.0;
+  // renderDensFromFloat_(0) / (1): UNORM16 0 / 65535, UNORM8 0 / 255; the
+  // float formats 0 / 1
+  double full = storage == MSBG_STORAGE_U16 ? 65535.0 : (storage == MSBG_STORAGE_U8 ? 255.0 : 1.0);
   Channel &ch = g->channels[chan];
   for (int l = 0; l < kMaxLevels; l++) ch.lv[l] = ChanLevel();
   ch.id = chan;
@@ -661,6 +663,9 @@ int msbg_splat_points(msbg_grid *g, int chan, float rParticle, float nbDist) {
   Channel *ch;
   ChanLevel *cl;
   MSBG_TRY(get_level(g, chan, 0, &ch, &cl));
+  MSBG_CHECK(ch->storage != MSBG_STORAGE_U8, MSBG_EUNSUPP,
+             "splat: UNORM8 channels are converted from the UNORM16 splat (msbg_channel_copy); "
+             "the demo splats into UNORM16 (msbg_demo.cpp:451-458)");
   float rScan = rParticle + nbDist;
   MSBG_CHECK(rScan > 0 && 2 * (rScan + 0.5f) <= (float)g->bsx0, MSBG_EINVAL,
              "splat: footprint 2*(rScan+0.5)=%g exceeds block size %d "
@@ -676,6 +681,8 @@ int msbg_finalize_density(msbg_grid *g, int chan, float rParticle, float nbDist)
   Channel *ch;
   ChanLevel *cl;
   MSBG_TRY(get_level(g, chan, 0, &ch, &cl));
+  MSBG_CHECK(ch->storage != MSBG_STORAGE_U8, MSBG_EUNSUPP,
+             "finalize: UNORM8 channels are converted from the UNORM16 field (msbg_channel_copy)");
   MSBG_TRY(launch_finalize(g, *ch, rParticle, nbDist));
   pages_written(*cl);
   MSBG_HIP(hipStreamSynchronize(g->stream));
@@ -711,7 +718,9 @@ int msbg_prepare_ghost_blocks(msbg_grid *g, int chan) {
   MSBG_CHECK(g, MSBG_EINVAL, "msbg_prepare_ghost_blocks: grid is NULL");
   Channel *ch = find_channel(g, chan);
   MSBG_CHECK(ch, MSBG_ESTATE, "channel %d not defined", chan);
-  MSBG_CHECK(ch->storage != MSBG_STORAGE_U16 && ch->storage != MSBG_STORAGE_F64, MSBG_EUNSUPP,
+  MSBG_CHECK(ch->storage != MSBG_STORAGE_U16 && ch->storage != MSBG_STORAGE_F64 &&
+                 ch->storage != MSBG_STORAGE_U8,
+             MSBG_EUNSUPP,
              "prepareGhostBlocks supports float channels (msbg.cpp:3113-3128); storage %d",
              ch->storage);
   MSBG_CHECK(!(g->options & MSBG_OPT_SINGLE_LEVEL), MSBG_ESTATE,
@@ -848,7 +857,8 @@ int mcf_colour_inplace(msbg_grid *g, int chPhi, const int32_t *blockList, int64_
       MSBG_TRY(launch_sweep_items(g->stream, g->bsx0, ch->storage, exact, doConstrZeroOne != 0,
                                   tile, g->d_desc.p,
                                   g->d_sweep_list.p + off[pass], n,
-                                  residual ? g->d_partial.p + off[pass] : nullptr));
+                                  residual ? g->d_partial.p + off[pass] : nullptr,
+                                  (int)(g->sweep_step + (uint64_t)it)));
       npasses++;
     }
     if (residual)
@@ -860,6 +870,7 @@ int mcf_colour_inplace(msbg_grid *g, int chPhi, const int32_t *blockList, int64_
   MSBG_HIP(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]));
   g->last_total_ms = ms;
   g->last_npasses = npasses;
+  g->sweep_step += (uint64_t)nMaxIter;
   g->last_avg_pass_ms = npasses ? ms / (double)npasses : 0.0;
   if (residual && nMaxIter > 0) {
     if (ntot) {
@@ -1015,7 +1026,8 @@ int pde_general(msbg_grid *g, int chPhi, int chTmp, int chTmp2, const int32_t *b
       } else {
         MSBG_TRY(launch_pde_items(g->stream, g->bsx0, storage, op, hard, dd,
                                   items.p + off[(size_t)p], n,
-                                  residual ? g->d_partial.p + off[(size_t)p] : nullptr, it));
+                                  residual ? g->d_partial.p + off[(size_t)p] : nullptr,
+                                  (int)(g->sweep_step + (uint64_t)it)));
       }
       npasses++;
     }
@@ -1027,6 +1039,7 @@ int pde_general(msbg_grid *g, int chPhi, int chTmp, int chTmp2, const int32_t *b
   MSBG_HIP(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]));
   g->last_total_ms = ms;
   g->last_npasses = npasses;
+  g->sweep_step += (uint64_t)nMaxIter;
   g->last_avg_pass_ms = npasses ? ms / (double)npasses : 0.0;
   if (residual && nMaxIter > 0) {
     if (ntot) {

@@ -232,6 +232,9 @@ struct msbg_grid {
   msbg::DevBuf<double> d_partial;       // per-block residual partials
   msbg::DevBuf<double> d_resid;         // per-sweep residuals
   uint64_t topo_version = 1;             // bumped when block maps or levels change
+  // sweeps run on this grid by msbg_apply_channel_pde*: the step the UNORM8
+  // store's stochastic rounding draws from (always even between calls)
+  uint64_t sweep_step = 0;
   uint64_t sweep_key = 0;                // key of the cached colour lists
   int64_t sweep_off[9] = {0};
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -282,9 +285,11 @@ SweepGrid sweep_desc(Channel &ch, float dt, int doConstr01);
 // c01: every grid of d_grids clamps to [0,1] (doConstrZeroOne), which
 // selects the fast kernel whose last fma clamps; the other kernels clamp to
 // each grid's own [lo,hi]
+// iter: the sweep step (per grid, counted across calls) that the UNORM8
+// store's stochastic rounding draws from
 int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, bool c01, int tile,
                        const SweepGrid *d_grids, const int2 *d_items, int64_t n,
-                       double *d_partial);
+                       double *d_partial, int iter);
 // the other applyChannelPdeFast operators (1, 3, 14, 15, 17) and MCF with
 // hard constraints, exact arithmetic, on the same item lists
 int launch_pde_items(hipStream_t s, int bsx, int storage, int op, bool hard,

@@ -19,11 +19,13 @@ __device__ inline double load_as_double(const uint8_t *p, int64_t i, int st) {
     case MSBG_STORAGE_F32: return ((const float *)p)[i];
     case MSBG_STORAGE_F16: return f16_bits_to_float(((const uint16_t *)p)[i]);
     case MSBG_STORAGE_BF16: return bf16_bits_to_float(((const uint16_t *)p)[i]);
+    case MSBG_STORAGE_U8: return (double)((const uint8_t *)p)[i] * (double)(1.0f / 255.0f);
     default: return ((const double *)p)[i];
   }
 }
 
-// UNORM16 targets round half to even like the sweep's store
+// UNORM16 and UNORM8 targets round half to even (the reference's
+// round_to_int stores, sbg.h:319-323, 343-351)
 __device__ inline void store_from_double(uint8_t *p, int64_t i, int st, double v) {
   switch (st) {
     case MSBG_STORAGE_U16:
@@ -32,6 +34,7 @@ __device__ inline void store_from_double(uint8_t *p, int64_t i, int st, double v
     case MSBG_STORAGE_F32: ((float *)p)[i] = (float)v; break;
     case MSBG_STORAGE_F16: ((uint16_t *)p)[i] = f16_bits_from_float((float)v); break;
     case MSBG_STORAGE_BF16: ((uint16_t *)p)[i] = bf16_bits_from_float((float)v); break;
+    case MSBG_STORAGE_U8: ((uint8_t *)p)[i] = (uint8_t)(int)__builtin_rintf((float)v * 255.0f); break;
     default: ((double *)p)[i] = v; break;
   }
 }

@@ -56,6 +56,7 @@ __device__ inline double page_raw(const LevelView &L, int64_t page, int vid) {
     case MSBG_STORAGE_F32: return ((const float *)L.pages)[i];
     case MSBG_STORAGE_F16: return f16_bits_to_float(((const uint16_t *)L.pages)[i]);
     case MSBG_STORAGE_BF16: return bf16_bits_to_float(((const uint16_t *)L.pages)[i]);
+    case MSBG_STORAGE_U8: return ((const uint8_t *)L.pages)[i];
     default: return ((const double *)L.pages)[i];
   }
 }
@@ -68,6 +69,7 @@ __device__ inline double block_raw(const LevelView &L, int c, int vid) {
 // transferValue: UNORM16 -> u*(1.f/65535) (halo.h:484-501), floats as is
 __device__ inline float raw_to_float(int storage, double raw) {
   if (storage == MSBG_STORAGE_U16) return (float)(uint16_t)raw * (1.0f / 65535.0f);
+  if (storage == MSBG_STORAGE_U8) return (float)(uint8_t)raw * (1.0f / 255.0f);
   return (float)raw;
 }
 

@@ -72,6 +72,7 @@ int launch_bsx(hipStream_t s, int storage, int op, bool hard, const SweepGrid *d
     case MSBG_STORAGE_F16: return launch_st<BSX, MSBG_STORAGE_F16>(s, op, hard, d_grids, d_items, n, d_partial, iter);
     case MSBG_STORAGE_BF16: return launch_st<BSX, MSBG_STORAGE_BF16>(s, op, hard, d_grids, d_items, n, d_partial, iter);
     case MSBG_STORAGE_F64: return launch_st<BSX, MSBG_STORAGE_F64>(s, op, hard, d_grids, d_items, n, d_partial, iter);
+    case MSBG_STORAGE_U8: return launch_st<BSX, MSBG_STORAGE_U8>(s, op, hard, d_grids, d_items, n, d_partial, iter);
   }
   set_error("applyChannelPdeFast: bad storage %d", storage);
   return MSBG_EINVAL;
@@ -263,7 +264,10 @@ int launch_copy_items(hipStream_t s, int bsx, int storage, const SweepGrid *d_gr
   const int64_t pe = (int64_t)bsx * bsx * bsx, fs = 2 * (int64_t)bsx * bsx, es = 4 * bsx;
   const int esz = storage_bytes(storage);
   MSBG_CHECK(n < 0x7fffffff, MSBG_EINVAL, "copy: too many blocks");
-  if (esz == 2)
+  if (esz == 1)
+    hipLaunchKernelGGL(k_copy_items<uint8_t>, dim3((unsigned)n), dim3(256), 0, s, d_grid, d_items,
+                       pe, fs, es);
+  else if (esz == 2)
     hipLaunchKernelGGL(k_copy_items<uint16_t>, dim3((unsigned)n), dim3(256), 0, s, d_grid, d_items,
                        pe, fs, es);
   else if (esz == 4)

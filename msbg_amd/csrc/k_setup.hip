@@ -332,6 +332,10 @@ int launch_fill_pages(msbg_grid *g, ChanLevel &cl, int storage, double fill_raw,
       hipLaunchKernelGGL(k_fill<double>, dim3(grid), dim3(kThreads), 0, g->stream, (double *)base,
                          n, fill_raw);
       break;
+    case MSBG_STORAGE_U8:
+      hipLaunchKernelGGL(k_fill<uint8_t>, dim3(grid), dim3(kThreads), 0, g->stream, (uint8_t *)base,
+                         n, (uint8_t)fill_raw);
+      break;
     default: set_error("fill_pages: bad storage %d", storage); return MSBG_EINVAL;
   }
   MSBG_HIP(hipGetLastError());

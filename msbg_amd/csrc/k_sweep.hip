@@ -63,14 +63,14 @@ int prepare_t(msbg_grid *g, ChanLevel &cl, int64_t fromPage) {
 
 template <int BSX, int ST, bool EXACT, int PX, int PY, bool C01 = false>
 int launch_one(hipStream_t s, const SweepGrid *d_grids, const int2 *d_items, int64_t n,
-               double *d_partial) {
+               double *d_partial, int iter) {
   constexpr int NT = Tile<BSX, PX, PY>::NT;
   constexpr int B = Pack<NT>::BPW;
   for (int64_t off = 0; off < n; off += 0x40000000) {
     const int cnt = (int)(n - off < 0x40000000 ? n - off : 0x40000000);
     hipLaunchKernelGGL((k_sweep<BSX, ST, EXACT, PX, PY, OP_MCF, false, B, C01>),
                        dim3((unsigned)((cnt + B - 1) / B)), dim3(NT * B), 0, s, d_grids,
-                       d_items + off, d_partial ? d_partial + off : nullptr, 0, cnt);
+                       d_items + off, d_partial ? d_partial + off : nullptr, iter, cnt);
     MSBG_HIP(hipGetLastError());
   }
   return 0;
@@ -78,33 +78,35 @@ int launch_one(hipStream_t s, const SweepGrid *d_grids, const int2 *d_items, int
 
 template <int BSX, int ST>
 int launch_st(hipStream_t s, bool exact, bool c01, int tile, const SweepGrid *d_grids,
-              const int2 *d_items, int64_t n, double *d_partial) {
+              const int2 *d_items, int64_t n, double *d_partial, int iter) {
   if constexpr (ST == MSBG_STORAGE_F64) exact = true;  // the FP64 run is always exact
   if (exact)  // the exact path keeps the default column shape
-    return BSX >= 32 ? launch_one<BSX, ST, true, 4, 2>(s, d_grids, d_items, n, d_partial)
-                     : launch_one<BSX, ST, true, 2, 2>(s, d_grids, d_items, n, d_partial);
+    return BSX >= 32 ? launch_one<BSX, ST, true, 4, 2>(s, d_grids, d_items, n, d_partial, iter)
+                     : launch_one<BSX, ST, true, 2, 2>(s, d_grids, d_items, n, d_partial, iter);
   if constexpr (ST != MSBG_STORAGE_F64) {
     if (tile == 2)
-      return c01 ? launch_one<BSX, ST, false, 4, 4, true>(s, d_grids, d_items, n, d_partial)
-                 : launch_one<BSX, ST, false, 4, 4>(s, d_grids, d_items, n, d_partial);
+      return c01 ? launch_one<BSX, ST, false, 4, 4, true>(s, d_grids, d_items, n, d_partial, iter)
+                 : launch_one<BSX, ST, false, 4, 4>(s, d_grids, d_items, n, d_partial, iter);
     if (tile == 1)
-      return c01 ? launch_one<BSX, ST, false, 4, 2, true>(s, d_grids, d_items, n, d_partial)
-                 : launch_one<BSX, ST, false, 4, 2>(s, d_grids, d_items, n, d_partial);
-    return c01 ? launch_one<BSX, ST, false, 2, 2, true>(s, d_grids, d_items, n, d_partial)
-               : launch_one<BSX, ST, false, 2, 2>(s, d_grids, d_items, n, d_partial);
+      return c01 ? launch_one<BSX, ST, false, 4, 2, true>(s, d_grids, d_items, n, d_partial, iter)
+                 : launch_one<BSX, ST, false, 4, 2>(s, d_grids, d_items, n, d_partial, iter);
+    return c01 ? launch_one<BSX, ST, false, 2, 2, true>(s, d_grids, d_items, n, d_partial, iter)
+               : launch_one<BSX, ST, false, 2, 2>(s, d_grids, d_items, n, d_partial, iter);
   }
   return 0;
 }
 
 template <int BSX>
 int launch_bsx(hipStream_t s, int storage, bool exact, bool c01, int tile,
-               const SweepGrid *d_grids, const int2 *d_items, int64_t n, double *d_partial) {
+               const SweepGrid *d_grids, const int2 *d_items, int64_t n, double *d_partial,
+               int iter) {
   switch (storage) {
-    case MSBG_STORAGE_U16: return launch_st<BSX, MSBG_STORAGE_U16>(s, exact, c01, tile, d_grids, d_items, n, d_partial);
-    case MSBG_STORAGE_F32: return launch_st<BSX, MSBG_STORAGE_F32>(s, exact, c01, tile, d_grids, d_items, n, d_partial);
-    case MSBG_STORAGE_F16: return launch_st<BSX, MSBG_STORAGE_F16>(s, exact, c01, tile, d_grids, d_items, n, d_partial);
-    case MSBG_STORAGE_BF16: return launch_st<BSX, MSBG_STORAGE_BF16>(s, exact, c01, tile, d_grids, d_items, n, d_partial);
-    case MSBG_STORAGE_F64: return launch_st<BSX, MSBG_STORAGE_F64>(s, exact, c01, tile, d_grids, d_items, n, d_partial);
+    case MSBG_STORAGE_U16: return launch_st<BSX, MSBG_STORAGE_U16>(s, exact, c01, tile, d_grids, d_items, n, d_partial, iter);
+    case MSBG_STORAGE_F32: return launch_st<BSX, MSBG_STORAGE_F32>(s, exact, c01, tile, d_grids, d_items, n, d_partial, iter);
+    case MSBG_STORAGE_F16: return launch_st<BSX, MSBG_STORAGE_F16>(s, exact, c01, tile, d_grids, d_items, n, d_partial, iter);
+    case MSBG_STORAGE_BF16: return launch_st<BSX, MSBG_STORAGE_BF16>(s, exact, c01, tile, d_grids, d_items, n, d_partial, iter);
+    case MSBG_STORAGE_F64: return launch_st<BSX, MSBG_STORAGE_F64>(s, exact, c01, tile, d_grids, d_items, n, d_partial, iter);
+    case MSBG_STORAGE_U8: return launch_st<BSX, MSBG_STORAGE_U8>(s, exact, c01, tile, d_grids, d_items, n, d_partial, iter);
   }
   set_error("sweep: bad storage %d", storage);
   return MSBG_EINVAL;
@@ -119,6 +121,7 @@ int prepare_bsx(msbg_grid *g, Channel &ch) {
     case MSBG_STORAGE_F16: return prepare_t<BSX, MSBG_STORAGE_F16>(g, cl, 0);
     case MSBG_STORAGE_BF16: return prepare_t<BSX, MSBG_STORAGE_BF16>(g, cl, 0);
     case MSBG_STORAGE_F64: return prepare_t<BSX, MSBG_STORAGE_F64>(g, cl, 0);
+    case MSBG_STORAGE_U8: return prepare_t<BSX, MSBG_STORAGE_U8>(g, cl, 0);
   }
   set_error("sweep: bad storage %d", ch.storage);
   return MSBG_EINVAL;
@@ -148,6 +151,7 @@ int sweep_prepare_level(msbg_grid *g, ChanLevel &cl, int storage, int64_t fromPa
       case MSBG_STORAGE_F16: return prepare_t<B, MSBG_STORAGE_F16>(g, cl, fromPage);
       case MSBG_STORAGE_BF16: return prepare_t<B, MSBG_STORAGE_BF16>(g, cl, fromPage);
       case MSBG_STORAGE_F64: return prepare_t<B, MSBG_STORAGE_F64>(g, cl, fromPage);
+      case MSBG_STORAGE_U8: return prepare_t<B, MSBG_STORAGE_U8>(g, cl, fromPage);
     }
     set_error("sweep: bad storage %d", storage);
     return MSBG_EINVAL;
@@ -189,12 +193,12 @@ SweepGrid sweep_desc(Channel &ch, float dt, int doConstr01) {
 
 int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, bool c01, int tile,
                        const SweepGrid *d_grids, const int2 *d_items, int64_t n,
-                       double *d_partial) {
+                       double *d_partial, int iter) {
   if (tile < 0) tile = default_tile(bsx, storage, n);
   switch (bsx) {
-    case 8: return launch_bsx<8>(s, storage, exact, c01, tile, d_grids, d_items, n, d_partial);
-    case 16: return launch_bsx<16>(s, storage, exact, c01, tile, d_grids, d_items, n, d_partial);
-    case 32: return launch_bsx<32>(s, storage, exact, c01, tile, d_grids, d_items, n, d_partial);
+    case 8: return launch_bsx<8>(s, storage, exact, c01, tile, d_grids, d_items, n, d_partial, iter);
+    case 16: return launch_bsx<16>(s, storage, exact, c01, tile, d_grids, d_items, n, d_partial, iter);
+    case 32: return launch_bsx<32>(s, storage, exact, c01, tile, d_grids, d_items, n, d_partial, iter);
   }
   set_error("sweep: block size %d not supported by the sweep kernel (8, 16, 32)", bsx);
   return MSBG_EUNSUPP;

@@ -52,6 +52,8 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
   ChanLevel &cl = ch.lv[level];
   MSBG_CHECK(cl.bsx == 8 || cl.bsx == 16 || cl.bsx == 32, MSBG_EUNSUPP,
              "level %d block size %d: the sweep kernel runs 8, 16 and 32", level, cl.bsx);
+  MSBG_CHECK(ch.storage != MSBG_STORAGE_U8, MSBG_EUNSUPP,
+             "coarse-level sweeps: UNORM8 channels are swept by applyChannelPdeFast only");
   if (coarse) {
     MSBG_CHECK(level + 1 < g->nlevels && ch.lv[level + 1].defined, MSBG_ESTATE,
                "OPT_BC_COARSE_LEVEL needs level %d of channel %d", level + 1, chan);
@@ -261,7 +263,7 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
       MSBG_TRY(launch_sweep_items(g->stream, cl.bsx, ch.storage, exact, doConstrZeroOne != 0, -1,
                                   W.d_desc.p,
                                   W.d_items.p + off[pass], n,
-                                  residual ? d_partial.p + off[pass] : nullptr));
+                                  residual ? d_partial.p + off[pass] : nullptr, iter));
       npasses++;
     }
     if (residual) MSBG_TRY(launch_reduce_residual(g, d_partial.p, ntot, d_resid.p + iter));

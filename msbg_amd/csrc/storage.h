@@ -111,6 +111,45 @@ struct Stor<MSBG_STORAGE_BF16> {
   }
 };
 
+// UNORM8: renderDensToFloat_<uint8_t> u*(1.f/255) (sbg.h:228-231); the
+// conversion store is round_to_int(v*255) = round half to even
+// (renderDensFromFloat_storeSimd8<uint8_t>, sbg.h:319-323); the sweep's store
+// rounds stochastically (enc_sr).
+template <>
+struct Stor<MSBG_STORAGE_U8> {
+  using raw = uint8_t;
+  using ct = float;
+  __device__ static inline float dec(raw u) { return (float)u * (1.0f / 255.0f); }
+  __device__ static inline raw enc(float f) { return (raw)(int)__builtin_rintf(f * 255.0f); }
+};
+
+// Stochastic rounding of the sweep's UNORM8 store
+// (renderDensFromFloat_storeSimd8_SR<uint8_t>, sbg.h:324-340): with
+// v = 255*f, the result is floor(v) + 1 when the uniform number u in [0,1)
+// is below v - floor(v), else floor(v). The reference draws u from a
+// per-thread Lehmer stream (FMA_FastRand, fastmath.h:68-73) whose order
+// depends on its thread schedule; here u comes from a counter-based hash of
+// (sweep step, page, voxel), with the reference's own construction of a
+// float in [1,2) from the top 23 bits minus 1, so that one run is
+// reproducible and the oracle draws the same numbers.
+__host__ __device__ inline uint32_t sr_mix(uint32_t h) {  // murmur3's fmix32
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+__host__ __device__ inline float sr_uniform(uint32_t step, uint32_t page, uint32_t vid) {
+  const uint32_t h = sr_mix(sr_mix(sr_mix(step ^ 0x6a09e667u) + page) + vid);
+  return u2f((h >> 9) | 0x3f800000u) - 1.0f;
+}
+__device__ inline uint8_t u8_from_float_sr(float f, float u) {
+#pragma clang fp contract(off)  // v is rounded before the fraction, as in the reference
+  const float v = 255.0f * f, fl = floorf(v), fr = v - fl;
+  return (uint8_t)(int)(u < fr ? fl + 1.0f : fl);
+}
+
 template <>
 struct Stor<MSBG_STORAGE_F64> {
   using raw = double;
@@ -123,6 +162,7 @@ struct Stor<MSBG_STORAGE_F64> {
 template <int ST>
 inline typename Stor<ST>::raw raw_value(double raw) {
   if constexpr (ST == MSBG_STORAGE_U16) return (uint16_t)raw;
+  else if constexpr (ST == MSBG_STORAGE_U8) return (uint8_t)raw;
   else if constexpr (ST == MSBG_STORAGE_F32) return (float)raw;
   else if constexpr (ST == MSBG_STORAGE_F16) return f16_bits_from_float((float)raw);
   else if constexpr (ST == MSBG_STORAGE_BF16) return bf16_bits_from_float((float)raw);
@@ -132,6 +172,7 @@ inline typename Stor<ST>::raw raw_value(double raw) {
 // raw storage value (as double) -> float, for const block values on the host
 inline float raw_to_float_host(int storage, double raw) {
   if (storage == MSBG_STORAGE_U16) return (float)(uint16_t)raw * (1.0f / 65535.0f);
+  if (storage == MSBG_STORAGE_U8) return (float)(uint8_t)raw * (1.0f / 255.0f);
   return (float)raw;
 }
 

@@ -43,6 +43,13 @@ __device__ inline void load4(const uint16_t *p, uint16_t v[4]) {
   v[2] = (uint16_t)((u >> 32) & 0xffffu);
   v[3] = (uint16_t)(u >> 48);
 }
+__device__ inline void load4(const uint8_t *p, uint8_t v[4]) {
+  const uint32_t u = *(const MSBG_GLOBAL uint32_t *)p;
+  v[0] = (uint8_t)(u & 0xffu);
+  v[1] = (uint8_t)((u >> 8) & 0xffu);
+  v[2] = (uint8_t)((u >> 16) & 0xffu);
+  v[3] = (uint8_t)(u >> 24);
+}
 __device__ inline void load4(const float *p, float v[4]) {
   const f32x4_t u = *(const MSBG_GLOBAL f32x4_t *)p;
   v[0] = u.x;
@@ -59,6 +66,9 @@ __device__ inline void load4(const double *p, double v[4]) {
 }
 __device__ inline void store2(uint16_t *p, uint16_t a, uint16_t b) {
   *(MSBG_GLOBAL uint32_t *)p = (uint32_t)a | ((uint32_t)b << 16);
+}
+__device__ inline void store2(uint8_t *p, uint8_t a, uint8_t b) {
+  *(MSBG_GLOBAL uint16_t *)p = (uint16_t)((uint32_t)a | ((uint32_t)b << 8));
 }
 __device__ inline void store2(float *p, float a, float b) {
   *(MSBG_GLOBAL f32x2_t *)p = f32x2_t{a, b};
@@ -357,7 +367,8 @@ struct Pack {
 };
 
 // OP: the stencil (OP_MCF is the hot path); HARD: hard constraints around
-// the field at entry (G.phi0); iter: the sweep index (Taubin's parity);
+// the field at entry (G.phi0); iter: the sweep step (Taubin's parity; the
+// UNORM8 store's stochastic rounding draws from it);
 // nItems: the number of items of this launch (the last workgroup may hold
 // fewer than BPW); C01 (fast path only): every grid of the launch clamps to
 // [0,1] (doConstrZeroOne), which the last fma of the update does with its
@@ -411,7 +422,6 @@ __global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
   const uint8_t *constsRaw = G.consts;
   const int64_t npages = G.npages;
   const float dtf = G.dt, lo01 = G.lo, hi01 = G.hi;
-  (void)iter;
 
   __shared__ __align__(16) L bufs[BPW][2][PL];
   __shared__ int codesAll[BPW][27];
@@ -658,6 +668,14 @@ __global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
         e[k] = (R)pk;
         e[k + 1] = (R)(pk >> 16);
       }
+    } else if constexpr (ST == MSBG_STORAGE_U8) {
+#pragma unroll
+      for (int b = 0; b < PY; b++)
+#pragma unroll
+        for (int a = 0; a < PX; a++) {
+          const uint32_t vid = (uint32_t)((z * BSX + y0 + b) * BSX + x0 + a);
+          e[b * PX + a] = u8_from_float_sr(o[b * PX + a], sr_uniform((uint32_t)iter, (uint32_t)page, vid));
+        }
     } else {
 #pragma unroll
       for (int k = 0; k < PX * PY; k++) e[k] = S::enc(o[k]);

@@ -14,7 +14,7 @@ from . import _lib as L
 from ._lib import check, lib
 
 _NP_DTYPE = {L.U16: np.uint16, L.F32: np.float32, L.F16: np.float16, L.BF16: np.uint16,
-             L.F64: np.float64}
+             L.F64: np.float64, L.U8: np.uint8}
 
 
 def _ptr(a):

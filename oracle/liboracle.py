@@ -11,17 +11,18 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
-U16, F32, F16, BF16, F64 = 1, 2, 3, 4, 5
+U16, F32, F16, BF16, F64, U8 = 1, 2, 3, 4, 5, 6
 OPT_INTERPOLATE, OPT_BC_NEUMANN, OPT_BC_COARSE_LEVEL = 1 << 7, 1 << 10, 1 << 26
 NULL, EMPTY, FULL = -1, -2, -3
-_NP = {U16: np.uint16, F32: np.float32, F16: np.uint16, BF16: np.uint16, F64: np.float64}
+_NP = {U16: np.uint16, F32: np.float32, F16: np.uint16, BF16: np.uint16, F64: np.float64,
+       U8: np.uint8}
 
 
 class OrLevel(C.Structure):
     _fields_ = [("sx", C.c_int), ("sy", C.c_int), ("sz", C.c_int), ("bsx", C.c_int),
                 ("nbx", C.c_int), ("nby", C.c_int), ("nbz", C.c_int),
                 ("page", C.c_void_p), ("pages", C.c_void_p), ("storage", C.c_int),
-                ("empty_raw", C.c_double), ("full_raw", C.c_double)]
+                ("empty_raw", C.c_double), ("full_raw", C.c_double), ("sr_step", C.c_uint32)]
 
 
 _libs = {}
@@ -66,6 +67,9 @@ def load(kind="strict"):
             "or_ghost_downsample": (None, [P, I, P]),
             "or_prepare_ghost_blocks": (None, [P, I, P, P, P]),
             "or_interpolate_ghost": (C.c_double, [P, F, F, F]),
+            "or_u8_to_float": (F, [C.c_uint8]),
+            "or_sr_uniform": (F, [C.c_uint32, C.c_uint32, C.c_uint32]),
+            "or_u8_from_float_sr": (C.c_uint8, [F, F]),
             "or_interp_linear": (F, [P, P]),
             "or_bspline3_weights": (None, [F, P, P]),
             "or_interp_points": (None, [P, I, P, I64, P, P]),
@@ -95,12 +99,16 @@ class Level:
         self.pages = np.ascontiguousarray(pages, dtype=_NP[storage])
         self.storage = storage
         self.empty_raw = empty_raw
-        self.full_raw = (65535.0 if storage == U16 else 1.0) if full_raw is None else full_raw
+        self.full_raw = ({U16: 65535.0, U8: 255.0}.get(storage, 1.0) if full_raw is None
+                         else full_raw)
+        # UNORM8: the sweep step of the next call's first sweep (each call
+        # advances it by its sweep count, like a grid of the library)
+        self.sr_step = 0
 
     def struct(self):
         return OrLevel(self.sx, self.sy, self.sz, self.bsx, self.nbx, self.nby, self.nbz,
                        _p(self.page), _p(self.pages), self.storage, self.empty_raw,
-                       self.full_raw)
+                       self.full_raw, self.sr_step & 0xffffffff)
 
 
 def domain_bounds(sx, sy, sz):
@@ -212,6 +220,7 @@ def pde_sweeps(level, blocks, levels, numIter, op, mode=MODE_COLOR8, dt=0.1, doC
                               nthreads)
     if rc < 0:
         raise ValueError("or_pde_sweeps: unsupported combination")
+    level.sr_step += n
     return res[:n] if residual else None
 
 
@@ -233,6 +242,7 @@ def mcf_sweeps(level, blocks, levels, numIter, dt, doConstrZeroOne=1, residual=F
     s = level.struct()
     load(kind).or_mcf_sweeps(C.byref(s), _p(b), b.size, _p(lv), numIter, dt, doConstrZeroOne,
                              _p(res), nthreads)
+    level.sr_step += n
     return res[:n] if residual else None
 
 
@@ -252,6 +262,7 @@ def mcf_sweeps_level(level, lo, blocks, levels, sweepLevel, opt, numIter, dt,
                                     doConstrZeroOne, _p(res), nthreads)
     if rc < 0:
         raise ValueError("or_mcf_sweeps_level: unsupported combination")
+    level.sr_step += n
     return res[:n] if residual else None
 
 

@@ -105,8 +105,42 @@ float or_bf16_to_float(uint16_t h) {
   return f;
 }
 
+/* renderDensToFloat_<uint8_t>: u * (1.f/255) (sbg.h:228-231) */
+float or_u8_to_float(uint8_t u) {
+  const float s = 1.0f / 255.0f;
+  return (float)u * s;
+}
+
+/* renderDensFromFloat_storeSimd8_SR<uint8_t> (sbg.h:324-340): v = 255 f,
+ * floor(v) + 1 if u < v - floor(v), else floor(v), truncated to int */
+uint8_t or_u8_from_float_sr(float f, float u) {
+  const float v = 255.0f * f, fl = floorf(v), fr = v - fl;
+  return (uint8_t)(int)(u < fr ? fl + 1.0f : fl);
+}
+
+/* murmur3's fmix32 (the finalizer FMA_FastRandSeed mixes with,
+ * fastmath.h:92-106), chained over (step, page, voxel); the float in [0,1)
+ * is built like FMA_FastRand's (fastmath.h:68-73): the top 23 bits as the
+ * mantissa of a number in [1,2), minus 1 */
+static inline uint32_t sr_mix(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+float or_sr_uniform(uint32_t step, uint32_t page, uint32_t vid) {
+  const uint32_t h = sr_mix(sr_mix(sr_mix(step ^ 0x6a09e667u) + page) + vid);
+  const uint32_t b = (h >> 9) | 0x3f800000u;
+  float f;
+  memcpy(&f, &b, 4);
+  return f - 1.0f;
+}
+
 static size_t storage_size(int st) {
   switch (st) {
+    case OR_U8: return 1;
     case OR_U16: case OR_F16: case OR_BF16: return 2;
     case OR_F32: return 4;
     case OR_F64: return 8;
@@ -119,6 +153,7 @@ static size_t storage_size(int st) {
 static inline float raw_to_float(int st, double raw) {
   switch (st) {
     case OR_U16: return or_u16_to_float((uint16_t)raw);
+    case OR_U8: return or_u8_to_float((uint8_t)raw);
     case OR_F64: return (float)raw;
     /* a const value of a half-width float channel is what its pages hold */
     case OR_F16: return or_f16_to_float(or_f16_from_float((float)raw));
@@ -136,6 +171,7 @@ static inline double page_raw(const or_level *L, int64_t page, int vid) {
     case OR_F16: return or_f16_to_float(((const uint16_t *)L->pages)[i]);
     case OR_BF16: return or_bf16_to_float(((const uint16_t *)L->pages)[i]);
     case OR_F64: return ((const double *)L->pages)[i];
+    case OR_U8: return ((const uint8_t *)L->pages)[i];
   }
   return 0;
 }
@@ -707,9 +743,10 @@ double or_mcf_voxel_f64(const double *h, int hsx, int hid, double dt,
   return phi;
 }
 
-/* store of the new value: UNORM16 via round-half-even (sbg.h:343-351); the
- * other storages round to nearest even in their own format. */
-static inline void store_float(const or_level *L, int64_t page, int vid, float v) {
+/* store of the new value: UNORM16 via round-half-even (sbg.h:343-351),
+ * UNORM8 with stochastic rounding at sweep step `step` (msbg4.cpp:829-837);
+ * the other storages round to nearest even in their own format. */
+static inline void store_float(const or_level *L, int64_t page, int vid, float v, uint32_t step) {
   size_t nv = (size_t)L->bsx * L->bsx * L->bsx;
   size_t i = (size_t)page * nv + (size_t)vid;
   switch (L->storage) {
@@ -718,6 +755,10 @@ static inline void store_float(const or_level *L, int64_t page, int vid, float v
     case OR_F16: ((uint16_t *)L->pages)[i] = or_f16_from_float(v); break;
     case OR_BF16: ((uint16_t *)L->pages)[i] = or_bf16_from_float(v); break;
     case OR_F64: ((double *)L->pages)[i] = (double)v; break;
+    case OR_U8:
+      ((uint8_t *)L->pages)[i] =
+          or_u8_from_float_sr(v, or_sr_uniform(step, (uint32_t)page, (uint32_t)vid));
+      break;
   }
 }
 
@@ -785,7 +826,7 @@ int or_mcf_sweeps_level(const or_level *L, const or_level *Lo, const int32_t *bl
                 for (int vx = 0; vx < b; vx++, vid++) {
                   int hid = (vx + 1) + (vy + 1) * hsx + (vz + 1) * hsx * hsx;
                   float ph = or_mcf_voxel(hf, hsx, hid, dt, doConstrZeroOne);
-                  store_float(L, pg, vid, ph);
+                  store_float(L, pg, vid, ph, L->sr_step + (uint32_t)it);
                   if (residual) {
                     double nw = raw_to_float(L->storage, page_raw(L, pg, vid));
                     double d = nw - (double)hf[hid];
@@ -933,7 +974,7 @@ int or_pde_sweeps(const or_level *L, const int32_t *blocks, int n, const uint16_
   if (nMaxIter % 2 == 1) nMaxIter++; /* msbg4.cpp:150-152 */
   const int b = L->bsx, hw = or_pde_halo(op), hsx = b + 2 * hw, nh = hsx * hsx * hsx;
   const int64_t nvb = (int64_t)b * b * b, nb = (int64_t)L->nbx * L->nby * L->nbz;
-  const int esz = L->storage == OR_F32 ? 4 : (L->storage == OR_F64 ? 8 : 2);
+  const int esz = (int)storage_size(L->storage);
   const int hard = P->densThr > 1e-12f || P->densThrTol1 > 1e-12f; /* msbg4.cpp:192-193 */
   int64_t npages = 0;
   for (int64_t i = 0; i < nb; i++)
@@ -1042,7 +1083,7 @@ int or_pde_sweeps(const or_level *L, const int32_t *blocks, int n, const uint16_
                                 hi = vminf(1.0f, p0 + P->densThrTol2);
                     ph = vmaxf(vminf(ph, hi), lo);
                   }
-                  store_float(dst, pg, vid, ph);
+                  store_float(dst, pg, vid, ph, L->sr_step + (uint32_t)it);
                   if (residual) {
                     const double nw = raw_to_float(L->storage, page_raw(dst, pg, vid));
                     const double d = nw - (double)hf[hid];
@@ -1134,7 +1175,7 @@ void or_prepare_ghost_blocks(or_level *lvl, int nLevels, const int32_t *fc,
     free(halo);
     free(lo);
   }
-  (void)storage_size;
+
 }
 
 /* ------------------------------------------------------------------------ */

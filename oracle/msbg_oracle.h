@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-enum { OR_U16 = 1, OR_F32 = 2, OR_F16 = 3, OR_BF16 = 4, OR_F64 = 5 };
+enum { OR_U16 = 1, OR_F32 = 2, OR_F16 = 3, OR_BF16 = 4, OR_F64 = 5, OR_U8 = 6 };
 enum { OR_NULL = -1, OR_EMPTY = -2, OR_FULL = -3 };
 
 /* halo options, values of msbg.h:110,113,126 */
@@ -50,13 +50,21 @@ typedef struct {
   int nbx, nby, nbz;     /* block grid (identical on every level) */
   const int32_t *page;   /* [nbx*nby*nbz] block map codes */
   void *pages;           /* page pool of this level */
-  int storage;           /* OR_U16 .. OR_F64 */
+  int storage;           /* OR_U16 .. OR_U8 */
   double empty_raw;      /* raw storage value of the EMPTY block */
   double full_raw;       /* raw storage value of the FULL block */
+  uint32_t sr_step;      /* UNORM8: sweep step of the next call's first sweep */
 } or_level;
 
 /* ---- scalar conversions (sbg.h:225-366, vectorf256.h:1064-1068) ---- */
 float or_u16_to_float(uint16_t u);                 /* u * (1.f/65535) */
+float or_u8_to_float(uint8_t u);                   /* u * (1.f/255) */
+/* The UNORM8 sweep store's stochastic rounding (sbg.h:324-340): the uniform
+ * number of (sweep step, page, voxel), and floor(255 f) + (u < frac). The
+ * numbers are this build's counter-based stream (the reference draws from
+ * per-thread Lehmer streams, fastmath.h:68-73). */
+float or_sr_uniform(uint32_t step, uint32_t page, uint32_t vid);
+uint8_t or_u8_from_float_sr(float f, float u);
 uint16_t or_u16_from_float_roundf(float f);        /* roundf(f*65535), scalar path */
 uint16_t or_u16_from_float_rne(float f);           /* round-half-even, SIMD store path */
 uint16_t or_f16_from_float(float f);               /* IEEE binary16, RNE */

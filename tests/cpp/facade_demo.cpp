@@ -7,7 +7,9 @@
 // comparison with the Python binding. Given argv[3], it also copies the
 // field into a float channel, runs applyChannelPdeCnf's Taubin-smoothed
 // laplacian flow on it and writes those pages there, and checks that a
-// storage type other than the channel's is rejected.
+// storage type other than the channel's is rejected. Given argv[4], it
+// converts the field to UNORM8 and runs applyChannelPdeFast<uint8_t> (the
+// stochastically rounded store) on it, writing those pages there.
 #include <cstdio>
 #include <vector>
 
@@ -86,6 +88,20 @@ int main(int argc, char **argv) {
       } catch (const MSBG::Error &e) {
         if (e.code != MSBG_EINVAL) return 1;
       }
+    }
+    if (argc > 4) {  // applyChannelPdeFast<uint8_t> (msbg4.cpp:1008-1030)
+      msbg->copyChannel<uint8_t>(MSBG::CH_UINT16_1, MSBG::CH_UINT8_1);
+      msbg->applyChannelPdeFast<uint8_t>(MSBG::CH_UINT8_1, MSBG::CH_NULL, MSBG::CH_NULL,
+                                         &activeBlocks,
+                                         -(MSBG::PDE_MEAN_CURVATURE + MSBG::OPT_8_COLOR_SCHEME),
+                                         1, 0, 0, 0, 4, 1.0f, 0.1f);
+      FILE *f8 = std::fopen(argv[4], "wb");
+      std::vector<uint8_t> b8(bsx * bsx * bsx);
+      for (int bid : activeBlocks) {
+        msbg->getBlockData<uint8_t>(MSBG::CH_UINT8_1, bid, b8.data());
+        std::fwrite(b8.data(), 1, b8.size(), f8);
+      }
+      std::fclose(f8);
     }
     std::printf("facade_demo: %zu active blocks\n", activeBlocks.size());
     MSBG::MultiresSparseGrid::destroy(msbg);

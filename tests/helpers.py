@@ -79,9 +79,11 @@ def to_storage_level(S, bsx, pagemap, vals_f32, storage):
         pages = v.astype(np.float16).view(np.uint16)
     elif storage == O.BF16:
         pages = np.vectorize(L.or_bf16_from_float, otypes=[np.uint16])(v)
+    elif storage == O.U8:  # round_to_int(v * 255), half to even (sbg.h:319-323)
+        pages = np.rint(v * np.float32(255)).astype(np.uint8)
     else:
         raise ValueError(storage)
-    full = 1.0
+    full = 255.0 if storage == O.U8 else 1.0
     return O.Level(S, S, S, bsx, pagemap, pages, storage, 0.0, full)
 
 

@@ -279,7 +279,7 @@ def _np_view(pages, storage):
     return pages
 
 
-@pytest.mark.parametrize("storage", [M.U16, M.F32, M.F16, M.BF16, M.F64])
+@pytest.mark.parametrize("storage", [M.U16, M.F32, M.F16, M.BF16, M.F64, M.U8])
 @pytest.mark.parametrize("S,bsx", [(128, 16), (128, 32), (64, 8)])
 def test_sweep_exact_bitexact(S, bsx, storage):
     note("sweep exact", S, bsx, storage)
@@ -330,19 +330,20 @@ def test_sweep_fast_close_to_exact(S, bsx):
 # field's [0,1] range), UNORM16 2 LSB, FP16 and BF16 two units in the last
 # place at 1.0 (storage rounding of the fast result may land one step away,
 # and one neighbour's step can move a voxel by another)
-_FAST_MAX = {M.F32: 2e-5, M.U16: 2.0 / 65535.0, M.F16: 2.0 ** -9, M.BF16: 2.0 ** -6}
+_FAST_MAX = {M.F32: 2e-5, M.U16: 2.0 / 65535.0, M.F16: 2.0 ** -9, M.BF16: 2.0 ** -6,
+             M.U8: 2.0 / 255.0}
 
 
 @pytest.mark.parametrize("doC01", [1, 0])
-@pytest.mark.parametrize("storage", [M.U16, M.F32, M.F16, M.BF16])
+@pytest.mark.parametrize("storage", [M.U16, M.F32, M.F16, M.BF16, M.U8])
 @pytest.mark.parametrize("S,bsx", [(128, 16), (128, 32)])
 def test_sweep_fast_every_column_bounded(S, bsx, storage, doC01):
     """Every voxel-column variant of the fast kernel (2x2, 4x2, 4x4), with and
     without the [0,1] clamp, against the strict oracle after 2 sweeps: a max
     bound per storage (above) and an RMS bound. UNORM16 runs with the clamp
     only (unclamped values outside [0,1] wrap in the 16-bit store)."""
-    if storage == M.U16 and not doC01:
-        pytest.skip("UNORM16 wraps outside [0,1]; the unclamped mode is run on the float storages")
+    if storage in (M.U16, M.U8) and not doC01:
+        pytest.skip("UNORM wraps outside [0,1]; the unclamped mode is run on the float storages")
     xyz = _scene(S, seed=31 + bsx, nObj=6, nPts=3000)
     o = oracle_pipeline(S, bsx, xyz, do_sweep=False)
     init, L = _init_field(o, S, bsx, storage)
@@ -350,6 +351,8 @@ def test_sweep_fast_every_column_bounded(S, bsx, storage, doC01):
     want = L.pages
     if storage == M.U16:
         want = want.astype(np.float64) / 65535.0
+    elif storage == M.U8:
+        want = want.astype(np.float64) / 255.0
     elif storage == M.F16:
         want = want.view(np.float16).astype(np.float64)
     elif storage == M.BF16:
@@ -370,6 +373,8 @@ def test_sweep_fast_every_column_bounded(S, bsx, storage, doC01):
             got = g.download(M.CH_UINT16_1)
         if storage == M.U16:
             got = got.astype(np.float64) / 65535.0
+        elif storage == M.U8:
+            got = got.astype(np.float64) / 255.0
         elif storage == M.F16:
             got = got.view(np.float16).astype(np.float64)
         elif storage == M.BF16:
@@ -380,6 +385,35 @@ def test_sweep_fast_every_column_bounded(S, bsx, storage, doC01):
         note("max", d.max(), "rms", np.sqrt((d ** 2).mean()))
         assert d.max() <= _FAST_MAX[storage], (tile, d.max())
         assert np.sqrt((d ** 2).mean()) <= _FAST_MAX[storage] / 8
+
+
+def test_sweep_u8_stochastic_store_two_calls():
+    """applyChannelPdeFast<uint8_t> (msbg4.cpp:1008-1030) stores UNORM8 with
+    stochastic rounding (sbg.h:324-340). Exact arithmetic over two calls in a
+    row, bit for bit against the oracle: the grid's sweep step carries over
+    from call to call like the oracle level's. The entry points that have no
+    UNORM8 form say so."""
+    S, bsx = 128, 16
+    xyz = _scene(S, seed=41, nObj=6, nPts=3000)
+    o = oracle_pipeline(S, bsx, xyz, do_sweep=False)
+    init, L = _init_field(o, S, bsx, O.U8)
+    with M.MultiresSparseGrid.create("u8", S, S, S, bsx, options=M.OPT_SINGLE_LEVEL,
+                                     nLevels=3) as g:
+        g.set_arith(M.ARITH_EXACT)
+        g.setRefinementMap(o["lv"])
+        active = g.alloc_demo(M.CH_UINT8_1, M.U8)
+        g.upload(M.CH_UINT8_1, 0, init)
+        for call in range(2):
+            note("u8 call", call)
+            r_o = O.mcf_sweeps(L, o["active"], o["blv"], 2, 0.1, 1, residual=True)
+            r_g = g.applyChannelPdeFast(M.CH_UINT8_1, 0, 0, active, M.MCF_8COLOR, 1, 0, 0, 0, 2,
+                                        1.0, 0.1, residual=True)
+            assert np.array_equal(g.download(M.CH_UINT8_1), L.pages.reshape(-1, bsx, bsx, bsx))
+            np.testing.assert_allclose(r_g, r_o, rtol=1e-12)
+        with pytest.raises(M.MsbgError):
+            g.splat(M.CH_UINT8_1, 2.0, 2.0)
+        with pytest.raises(M.MsbgError):
+            g.finalize(M.CH_UINT8_1, 2.0, 2.0)
 
 
 def test_sweep_skips_non_level0_and_rejects_bad_lists():
@@ -517,8 +551,9 @@ def test_cpp_facade_demo_matches_binding(tmp_path):
     if not os.path.exists(exe):
         pytest.fail("tests/cpp/facade_demo not built (run the build step)")
     out, frame, cnf = tmp_path / "pages.bin", tmp_path / "frame.bin", tmp_path / "cnf.bin"
-    r = subprocess.run([exe, str(out), str(frame), str(cnf)], capture_output=True, text=True,
-                       timeout=120)
+    u8 = tmp_path / "u8.bin"
+    r = subprocess.run([exe, str(out), str(frame), str(cnf), str(u8)], capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     got = np.fromfile(out, dtype=np.uint16)
     S, bsx = 128, 16
@@ -539,6 +574,11 @@ def test_cpp_facade_demo_matches_binding(tmp_path):
                           1.0, 0.05, taubinStrength=0.5)
     got = np.fromfile(cnf, dtype=np.float32)
     assert np.array_equal(got, g.download(M.CH_FLOAT_1).reshape(-1))
+    # applyChannelPdeFast<uint8_t>: the same stochastic store at the same
+    # sweep steps of the grid
+    g.copyDataTo(30, M.CH_UINT8_1, M.U8)
+    g.applyChannelPdeFast(M.CH_UINT8_1, 0, 0, None, M.MCF_8COLOR, 1, 0, 0, 0, 4, 1.0, 0.1)
+    assert np.array_equal(np.fromfile(u8, dtype=np.uint8), g.download(M.CH_UINT8_1).reshape(-1))
     g.copyDataTo(30, 8, M.F32)  # without Taubin the result differs
     g.applyChannelPdeFast(8, 6, 7, None, -(1 + M.OPT_8_COLOR_SCHEME), 1, 0, 0, 0, 4, 1.0, 0.05)
     assert not np.array_equal(got, g.download(8).reshape(-1))

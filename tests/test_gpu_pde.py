@@ -28,8 +28,8 @@ def _grid(S=64, bsx=16, storage=M.F32, seed=1, nLevels=2):
     lv = np.where(rng.random(nb ** 3) < 0.15, 1, 0).astype(np.int32)
     g.setRefinementMap(lv)
     kind = rng.choice([3, 1, 2, 0], nb ** 3, p=[0.6, 0.15, 0.1, 0.15]).astype(np.uint8)
-    full = 65535.0 if storage == M.U16 else 1.0
-    empty = 2000.0 if storage == M.U16 else 0.03
+    full = {M.U16: 65535.0, M.U8: 255.0}.get(storage, 1.0)
+    empty = {M.U16: 2000.0, M.U8: 8.0}.get(storage, 0.03)
     g.channel_define(CH, storage, 0, kind, empty, full, 0.0)
     pm, npg = g.pagemap(CH)
     # a smooth field plus noise, in [0, 1]
@@ -42,12 +42,14 @@ def _grid(S=64, bsx=16, storage=M.F32, seed=1, nLevels=2):
                           0, 1)
     if storage == M.U16:
         raw = np.rint(vals * 65535).astype(np.uint16)
+    elif storage == M.U8:
+        raw = np.rint(vals * 255).astype(np.uint8)
     elif storage == M.F16:
         raw = vals.astype(np.float16)
     else:
         raw = vals
     g.upload(CH, 0, raw)
-    ost = {M.U16: O.U16, M.F32: O.F32, M.F16: O.F16}[storage]
+    ost = {M.U16: O.U16, M.F32: O.F32, M.F16: O.F16, M.U8: O.U8}[storage]
     levels, _ = g.getBlockInfo()
     return g, pm, ost, empty, full, levels
 
@@ -94,7 +96,9 @@ def test_operators_bitexact(op, alpha, taubin, mode):
                     "reads: the result depends on block order, in the reference as here")
     S, bsx = 64, 16
     lt = {"color8": -(op | FLAG8), "redblack": -op, "jacobi": op}[mode]
-    for storage in (M.F32, M.U16):
+    # UNORM8 (applyChannelPdeFast<uint8_t>): the stochastic store, same steps
+    # on both sides; the bi-laplacian has no UNORM8 form here
+    for storage in (M.F32, M.U16) if op == 2 else (M.F32, M.U16, M.U8):
         g, pm, ost, empty, full, levels = _grid(S, bsx, storage, seed=op + 7)
         with g:
             got, want, gr, wr = _run_both(g, pm, ost, empty, full, levels, S, bsx, lt, 3,
@@ -106,7 +110,7 @@ def test_operators_bitexact(op, alpha, taubin, mode):
                 g.pagemap(CH_T)
 
 
-@pytest.mark.parametrize("storage", [M.F32, M.U16, M.F16])
+@pytest.mark.parametrize("storage", [M.F32, M.U16, M.F16, M.U8])
 def test_mcf_jacobi_and_hard_constraints_bitexact(storage):
     note("mcf jacobi / hard", storage)
     S, bsx = 64, 16

@@ -763,3 +763,64 @@ def test_prepare_ghost_blocks_selects_fine_coarse_existing_blocks():
     assert np.all(got[17] == 0.125)  # in the list but without BLK_EXISTS
     assert np.all(got[41] == 0.125)  # only in the last level's list
     assert np.all(L0.pages == 0.5)  # the fine level is only read
+
+
+# --------------------------------------------------------------------------
+# UNORM8 (applyChannelPdeFast<uint8_t>): decode and the stochastic store
+# (sbg.h:228-231, 324-340; msbg4.cpp:829-837)
+# --------------------------------------------------------------------------
+def test_u8_decode_and_stochastic_rounding():
+    u = np.arange(256)
+    d = np.array([L.or_u8_to_float(int(x)) for x in u], np.float32)
+    assert np.array_equal(d, u.astype(np.float32) * np.float32(1 / 255))
+    # the uniform stream: [0, 1) on the 2^-23 lattice, mean 1/2, variance
+    # 1/12, and no correlation between neighbouring voxels or steps
+    r = np.array([[[L.or_sr_uniform(s, p, v) for v in range(256)] for p in range(8)]
+                  for s in range(8)], np.float64)
+    assert r.min() >= 0 and r.max() < 1
+    assert np.all(r * 2 ** 23 == np.floor(r * 2 ** 23))
+    assert abs(r.mean() - 0.5) < 0.01 and abs(r.var() - 1 / 12) < 0.004
+    c = lambda a, b: np.corrcoef(a.ravel(), b.ravel())[0, 1]  # noqa: E731
+    assert abs(c(r[:, :, 1:], r[:, :, :-1])) < 0.03
+    assert abs(c(r[1:], r[:-1])) < 0.03 and abs(c(r[:, 1:], r[:, :-1])) < 0.03
+    # the store: floor(255 f) or one above, up with probability frac(255 f),
+    # so its mean over the stream is 255 f (unbiased)
+    us = r.ravel()
+    for f in (0.0, 1.0, 0.5, 0.3, 0.123456, 0.9987, 1 / 510):
+        q = np.array([L.or_u8_from_float_sr(f, float(x)) for x in us], np.float64)
+        v = float(np.float32(255) * np.float32(f))
+        assert set(np.unique(q)) <= {np.floor(v), np.floor(v) + 1}
+        assert abs(q.mean() - v) < 4 * 0.5 / np.sqrt(us.size) + 1e-6, (f, q.mean(), v)
+    assert L.or_u8_from_float_sr(0.5, 0.0) == 128 and L.or_u8_from_float_sr(0.5, 0.5) == 127
+
+
+def test_u8_sweep_is_unbiased_against_fp32():
+    # The UNORM8 sweep tracks the FP32 sweep of the same start within a few
+    # steps of 1/255, and its mean error stays far below one step: stochastic
+    # rounding carries updates smaller than half a step on average, where
+    # round-to-nearest would drop them.
+    S, bsx = 32, 8
+    nb = S // bsx
+    z, y, x = np.meshgrid(*[np.arange(S)] * 3, indexing="ij")
+    f = (0.5 + 0.45 * np.sin(x / 4.0) * np.cos(y / 5.0 + z / 7.0)).astype(np.float32)
+    q = np.rint(f * np.float32(255)).astype(np.uint8)
+    f0 = q.astype(np.float32) * np.float32(1 / 255)  # the same start for both
+
+    def level(vals, st):
+        pages = np.stack([vals[(b // (nb * nb)) * bsx:(b // (nb * nb) + 1) * bsx,
+                               ((b // nb) % nb) * bsx:((b // nb) % nb + 1) * bsx,
+                               (b % nb) * bsx:(b % nb + 1) * bsx] for b in range(nb ** 3)])
+        return O.Level(S, S, S, bsx, np.arange(nb ** 3, dtype=np.int32), pages, st)
+
+    blocks = np.arange(nb ** 3, dtype=np.int32)
+    lv = np.zeros(nb ** 3, np.uint16)
+    L8, L32 = level(q, O.U8), level(f0, O.F32)
+    for _ in range(5):
+        O.mcf_sweeps(L8, blocks, lv, 2, 0.1, 1)
+        O.mcf_sweeps(L32, blocks, lv, 2, 0.1, 1)
+    assert L8.sr_step == 10  # the step advances with every sweep
+    e = L8.pages.astype(np.float64) / 255 - L32.pages.astype(np.float64)
+    assert np.abs(e).max() < 3 / 255
+    assert abs(e.mean()) < 0.05 / 255
+    moved = np.abs(L32.pages - level(f0, O.F32).pages) > 0.5 / 255
+    assert moved.sum() > 1000 and abs(e[moved].mean()) < 0.1 / 255
