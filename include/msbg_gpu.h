@@ -542,6 +542,106 @@ int msbg_cnn_conv3d_bwd_weights(const uint16_t *x, const uint16_t *dy, int n,
                                 int cin, int cout, int dout, int pad,
                                 float *dw, float *db);
 
+/* ======================================================================== */
+/* adaptive pressure relax (msbg3.cpp)                                      */
+/* ======================================================================== */
+
+/* A pressure hierarchy on a block grid of nbx*nby*nbz blocks of bsx0^3
+ * cells, each block at its own resolution level (edge bsx0 >> level). The
+ * reference keeps the solver's channels inside MultiresSparseGrid
+ * (CH_CELL_FLAGS, CH_DIAGONAL, CH_FACE_AREA, msbg.h:291-294) and fills them
+ * in code that is not part of its sources; this library computes them from
+ * the finest grid's cell types:
+ *   coarse cell: fluid if any child is fluid, else void if any is void, else
+ *   solid; face area 0 on faces touching a solid cell (outside the domain:
+ *   void where the border is open, solid where it is closed), 1 otherwise,
+ *   a coarse face the mean of its four children; CELL_PARTIAL_SOLID marks a
+ *   fluid cell with a face area below 1; the diagonal is 1 / the sum of the
+ *   coefficients the relax applies (0 on non-fluid cells).
+ * Multigrid level m holds block b at level max(level_b, m) (msbg.cpp:3374);
+ * block levels run down to 2^3 blocks, then denseLevels dense grids of
+ * nbx>>k x nby>>k x nbz>>k cells follow.
+ *
+ * Channel layout of level m: block levels hold the pages of all blocks in
+ * block-id order (x fastest), vid = (vz*bsx + vy)*bsx + vx; a dense level is
+ * one (sx+2)*(sy+2)*(sz+2) array with a zero ghost shell. */
+typedef struct msbg_ps msbg_ps;
+
+enum { MSBG_PS_FLUID = 0, MSBG_PS_SOLID = 1, MSBG_PS_VOID = 2 };
+/* domain boundary types (DBC_OPEN, DBC_SOLID, msbg.h:191-192) */
+enum { MSBG_DBC_OPEN = 0, MSBG_DBC_SOLID = 1 };
+/* channels of the hierarchy: unknown X, right-hand side B, residual R,
+ * Jacobi temporary T, diagonal D (read only) */
+enum { MSBG_PS_CH_X = 0, MSBG_PS_CH_B = 1, MSBG_PS_CH_R = 2, MSBG_PS_CH_T = 3,
+       MSBG_PS_CH_D = 4 };
+/* options with the reference's values (msbg.h:107,115,121) */
+#define MSBG_OPT_BOUNDZONE_ONLY (1u << 4)
+#define MSBG_OPT_CALC_RESIDUAL (1u << 12)
+#define MSBG_OPT_REVERSE_ORDER (1u << 18)
+/* cell and block flags (msbg.h:565-615) */
+#define MSBG_CELL_SOLID (1u << 0)
+#define MSBG_CELL_PARTIAL_SOLID (1u << 6)
+#define MSBG_CELL_VOID (1u << 12)
+#define MSBG_BLK_NO_FLUID (1u << 5)
+#define MSBG_BLK_CUTS_SOLID (1u << 15)
+
+typedef struct {
+  int nbx, nby, nbz; /* block grid */
+  int bsx0;          /* finest block edge: 4, 8 or 16 */
+  double dx0;        /* finest cell size (_dx0); level l has h = 2^l dx0 */
+  int domainBc[3][2];/* MSBG_DBC_* per axis, lower/upper (_domainBc) */
+  float omega;       /* relaxation factor (_mgSmOmega, 6/7, msbg.cpp:819) */
+  int blockIter;     /* sub-iterations per block (_mgSmBlockIter, 5, msbg.cpp:837) */
+  int denseLevels;   /* dense multigrid levels below the block levels */
+} msbg_ps_desc;
+
+/* Builds the hierarchy. blockLevels[nb] are the blocks' resolution levels
+ * (face neighbours may differ by one level at most, msbg3.cpp:690);
+ * cellType[(nbx*bsx0)*(nby*bsx0)*(nbz*bsx0)] holds MSBG_PS_* per finest
+ * cell, x fastest. All channels start at zero. */
+int msbg_ps_create(const msbg_ps_desc *desc, const int8_t *blockLevels,
+                   const uint8_t *cellType, msbg_ps **out);
+int msbg_ps_destroy(msbg_ps *ps);
+int msbg_ps_get_levels(const msbg_ps *ps, int *nLevelsMg, int *nLevelsBlock);
+/* floats in one channel of multigrid level levelMg */
+int msbg_ps_level_size(const msbg_ps *ps, int levelMg, int64_t *n);
+int msbg_ps_upload(msbg_ps *ps, int chan, int levelMg, const float *host);
+int msbg_ps_download(msbg_ps *ps, int chan, int levelMg, float *host);
+/* cell flags (one per channel float) and, on block levels, block flags */
+int msbg_ps_download_flags(msbg_ps *ps, int levelMg, uint16_t *cellFlags,
+                           uint16_t *blockFlags);
+int msbg_ps_fill(msbg_ps *ps, int chan, int levelMg, float v);
+
+/* MultiresSparseGrid::relax(boundaryZoneOnly, reverseOrder, levelMg, chanX,
+ * chanB, chanTmp, nIter) (msbg3.cpp:2244-2369). Block levels: nIter passes
+ * of red-black block Gauss-Seidel (relaxBlockList, msbg3.cpp:2170-2239),
+ * every block relaxed by IprocessBlockLaplacian's blockIter sub-iterations
+ * (msbg3.cpp:418-1600). Dense levels: nIter damped-Jacobi sweeps through
+ * chanTmp (ImultiplyLaplacianMatrixDenseLevel, msbg3.cpp:1917-2165).
+ * boundaryZoneOnly != 0 returns MSBG_EUNSUPP. */
+int msbg_ps_relax(msbg_ps *ps, int boundaryZoneOnly, int reverseOrder,
+                  int levelMg, int chanX, int chanB, int chanTmp, int nIter);
+/* MultiresSparseGrid::multiplyLaplacianMatrixOpt(options, levelMg, chanX,
+ * chanB, kap, chanDstY, pDotprod) (msbg3.cpp:1756-1912): Y = A X, or with
+ * MSBG_OPT_CALC_RESIDUAL Y = B - A X; without it *dotprod (if given)
+ * receives sum(Y * X) on a block level. kap is unused, as in the
+ * reference's body. Vectors of cells without fluid are not written. */
+int msbg_ps_multiply(msbg_ps *ps, unsigned options, int levelMg, int chanX,
+                     int chanB, float kap, int chanDstY, double *dotprod);
+/* Constant restriction from levelMg to levelMg+1 (a coarse cell gets the
+ * sum of its eight children) and constant prolongation from levelMg+1
+ * added into levelMg (MG_CONST_RESTRICTION, MG_CONST_PROLONGATION with
+ * alpha 1, msbgcell.h:99-101; the reference's operators are not part of
+ * its sources, these are this library's). */
+int msbg_ps_restrict(msbg_ps *ps, int levelMg, int chanSrc, int chanDst);
+int msbg_ps_prolongate(msbg_ps *ps, int levelMg, int chanSrc, int chanDst);
+/* One V-cycle on X/B of level 0 (the reference's multigrid driver is not
+ * part of its sources): nu1 relax passes, residual, restriction, recursion
+ * with zero start, prolongation, nu2 passes in reverse order; the coarsest
+ * level gets coarseIter passes. */
+int msbg_ps_vcycle(msbg_ps *ps, int nu1, int nu2, int coarseIter);
+int msbg_ps_synchronize(msbg_ps *ps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,3 +11,4 @@ from ._lib import (ARITH_EXACT, ARITH_FAST, BF16, CH_FLOAT_1, CH_FLOAT_2, CH_FLO
 from .msbg import (MCF_8COLOR, Batch, DeviceArray, MultiresSparseGrid, UNet, cnn_conv3d,  # noqa: F401
                    cnn_conv3d_bwd_data, cnn_conv3d_bwd_weights, copy_bandwidth, getBlockColor8,
                    scene_objects, scene_spheres)
+from .pressure import PressureGrid  # noqa: F401

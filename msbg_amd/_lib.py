@@ -125,7 +125,28 @@ SIGNATURES = {
     "msbg_cnn_conv3d_noalloc": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _I, _P]),
     "msbg_cnn_conv3d_bwd_data": (_I, [_P, _I, _I, _I, _I, _I, _P, _P]),
     "msbg_cnn_conv3d_bwd_weights": (_I, [_P, _P, _I, _I, _I, _I, _I, _P, _P]),
+    "msbg_ps_create": (_I, [_P, _P, _P, C.POINTER(_P)]),
+    "msbg_ps_destroy": (_I, [_P]),
+    "msbg_ps_get_levels": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
+    "msbg_ps_level_size": (_I, [_P, _I, C.POINTER(_I64)]),
+    "msbg_ps_upload": (_I, [_P, _I, _I, _P]),
+    "msbg_ps_download": (_I, [_P, _I, _I, _P]),
+    "msbg_ps_download_flags": (_I, [_P, _I, _P, _P]),
+    "msbg_ps_fill": (_I, [_P, _I, _I, _F]),
+    "msbg_ps_relax": (_I, [_P, _I, _I, _I, _I, _I, _I, _I]),
+    "msbg_ps_multiply": (_I, [_P, C.c_uint, _I, _I, _I, _F, _I, C.POINTER(_D)]),
+    "msbg_ps_restrict": (_I, [_P, _I, _I, _I]),
+    "msbg_ps_prolongate": (_I, [_P, _I, _I, _I]),
+    "msbg_ps_vcycle": (_I, [_P, _I, _I, _I]),
+    "msbg_ps_synchronize": (_I, [_P]),
 }
+
+
+class PsDesc(C.Structure):
+    """msbg_ps_desc"""
+    _fields_ = [("nbx", _I), ("nby", _I), ("nbz", _I), ("bsx0", _I), ("dx0", _D),
+                ("domainBc", (_I * 2) * 3), ("omega", _F), ("blockIter", _I),
+                ("denseLevels", _I)]
 
 _lib = None
 

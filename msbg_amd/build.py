@@ -61,7 +61,7 @@ def build_oracle(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {odir}/liboracle_strict.so, liboracle_fast.so, libcnn_oracle.so")
+        print(f"built {odir}/liboracle_strict.so, liboracle_fast.so, libcnn_oracle.so, libps_oracle.so")
 
 
 def build_facade_demo(verbose=True):
