@@ -494,15 +494,147 @@ def bench_levels(args):
     print(json.dumps(res))
 
 
+def pressure_cpu_baseline(G, budget_s):
+    """The oracle's relax (single-threaded C, ps_oracle.c) on the same tank
+    scene at 128^3 (8^3 blocks of 16^3): relax iterations timed within the
+    budget."""
+    from oracle import ps as P
+    nb, bsx0 = (8, 8, 8), 16
+    lev, typ = G.tank_scene(nb, bsx0, seed=7)
+    o = P.PsOracle(nb, bsx0, lev, typ, dbc=(0, 0, 1, 1, 1, 0), ndense=2)
+    fl = o.flags(0)
+    rng = np.random.default_rng(3)
+    x = np.where((fl & (P.CELL_SOLID | P.CELL_VOID)) == 0,
+                 rng.standard_normal(o.size(0)), 0).astype(np.float32)
+    o.set(P.CH_X, 0, x)
+    o.multiply(0, 0, chan_dst=P.CH_B)
+    o.fill(P.CH_X, 0, 0.0)
+    bf = o.block_flags(0)
+    L = P.block_levels_at(lev, 0)
+    cells = int(sum((bsx0 >> int(L[b])) ** 3 for b in range(L.size) if not bf[b] & P.BLK_NO_FLUID))
+    t0 = time.perf_counter()
+    o.relax(0, 1)
+    t1 = time.perf_counter() - t0
+    n = max(1, int(budget_s / max(t1, 1e-3)))
+    t0 = time.perf_counter()
+    for _ in range(n):
+        o.relax(0, 1)
+    t = time.perf_counter() - t0
+    o.close()
+    return {"value": cells * n / t, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"tank scene 128^3 (8^3 blocks of 16^3, levels {np.bincount(lev, minlength=4).tolist()}),"
+                      f" {cells} cells per relax iteration, {n} iterations in {t:.1f} s on 1 thread "
+                      f"(oracle built -O2 -ffp-contract=off)"}
+
+
+def bench_pressure(args):
+    """Adaptive pressure relax (msbg3.cpp relax()): one step = one relax
+    iteration at multigrid level 0 (red then black blocks, every block with
+    fluid relaxed by 5 lexicographic sub-iterations), on a 512^3 tank."""
+    from msbg_amd import pressure as G
+    nb, bsx0 = (32, 32, 32), 16
+    t0 = time.perf_counter()
+    lev, typ = G.tank_scene(nb, bsx0, seed=args.seed)
+    g = G.PressureGrid(nb, bsx0, lev, typ, dbc=(0, 0, 1, 1, 1, 0), dense_levels=3)
+    tsetup = time.perf_counter() - t0
+    fl = g.flags(0)
+    fluid = (fl & (G.CELL_SOLID | G.CELL_VOID)) == 0
+    rng = np.random.default_rng(args.seed)
+    xs = np.where(fluid, rng.standard_normal(g.size(0)), 0).astype(np.float32)
+    g.upload(G.CH_X, 0, xs)
+    g.fill(G.CH_B, 0, 0.0)
+    g.multiplyLaplacianMatrixOpt(0, 0, G.CH_X, G.CH_B, 0.0, G.CH_B)  # b = A x*
+    g.fill(G.CH_X, 0, 0.0)
+    cells = g.relaxed_cells(0)
+    log(f"pressure setup {tsetup:.1f}s, levels {np.bincount(lev, minlength=4).tolist()}, "
+        f"{cells} cells per relax iteration, {g.nmg} multigrid levels")
+    for _ in range(args.warmup):
+        g.relax(0, 0, 0, G.CH_X, G.CH_B, G.CH_T, 1)
+    g.synchronize()
+    kern = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.relax(0, 0, 0, G.CH_X, G.CH_B, G.CH_T, 1)
+        kern += g.last_timing()
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    relax_ms = kern / args.steps
+    # Laplacian multiply with residual, and V-cycles on the manufactured problem
+    mul = 0.0
+    for _ in range(5):
+        g.multiplyLaplacianMatrixOpt(G.OPT_CALC_RESIDUAL, 0, G.CH_X, G.CH_B, 0.0, G.CH_R)
+        mul += g.last_timing()
+    g.fill(G.CH_X, 0, 0.0)
+    b = g.download(G.CH_B, 0).astype(np.float64)
+    res, vms = [], 0.0
+    for _ in range(4):
+        g.vcycle(2, 2, 30)
+        vms += g.last_timing()
+        g.fill(G.CH_R, 0, 0.0)
+        g.multiplyLaplacianMatrixOpt(G.OPT_CALC_RESIDUAL, 0, G.CH_X, G.CH_B, 0.0, G.CH_R)
+        res.append(float(np.linalg.norm(g.download(G.CH_R, 0)) / np.linalg.norm(b)))
+    err = float(np.abs(g.download(G.CH_X, 0) - xs).max() / np.abs(xs).max())
+    # algorithmic HBM bytes of a relax iteration per cell: X read + write,
+    # B, D (4 bytes each), flags (2)
+    achieved = cells * 18 / (relax_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = pressure_cpu_baseline(G, args.cpu_seconds)
+        except OSError as e:
+            log(f"cpu baseline unavailable: {e}")
+    res_out = {
+        "metric": "cell-updates/s/relax-iteration",
+        "value": cells / (relax_ms * 1e-3),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "pressure relax: 512^3 tank (32^3 blocks of 16^3; fluid to 78% height, "
+                        "void above, seeded solid sphere r = 0.17*512), block levels 0-3 by "
+                        "distance to the fluid/solid/void band, + 3 dense multigrid levels; "
+                        "solid floor and y walls, open x ends and top; step = relax() at level 0 "
+                        "with 5 block sub-iterations, omega 6/7",
+            "levels": np.bincount(lev, minlength=4).tolist(),
+            "cells_per_iteration": cells,
+            "relax_ms": relax_ms,
+            "multiply_residual_ms": mul / 5,
+            "multiply_rate": cells / (mul / 5 * 1e-3),
+            "vcycle_ms": vms / 4,
+            # |b - A x| / |b| after each V-cycle (2+2 relax iterations, 30 on the
+            # coarsest level) from x = 0; FP32 storage floors it near 1e-5
+            "vcycle_residual": res,
+            "vcycle_max_err_after_4": err,
+            "setup_s": tsetup,
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_ps_block<16,relax> (+ the 8^3/4^3/2^3 launches of the same "
+                               "iteration): 18 B per cell per iteration; the 5 sub-iterations "
+                               "run in LDS, the bound is the 4x32-step wavefront",
+                     "avg_launch_ms": relax_ms},
+        "cpu_baseline": cpu,
+    }
+    g.close()
+    print(json.dumps(res_out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--path", default="sweep", choices=["sweep", "render", "levels"],
+    ap.add_argument("--path", default="sweep", choices=["sweep", "render", "levels", "pressure"],
                     help="sweep: the 8-colour MCF sweep (default); render: the demo's 4K "
                          "ray-march render of the smoothed field; levels: coarse-level sweeps "
-                         "of the multiresolution scene")
+                         "of the multiresolution scene; pressure: the adaptive pressure relax")
     ap.add_argument("--config", default=None, choices=sorted(CONFIGS),
                     help="default cfg2 (sweep), cfg1 (render)")
     ap.add_argument("--storage", default="f16", choices=sorted(STORAGES))
@@ -527,6 +659,9 @@ def main():
         return
     if args.path == "levels":
         bench_levels(args)
+        return
+    if args.path == "pressure":
+        bench_pressure(args)
         return
     args.config = args.config or "cfg2"
     import msbg_amd as M

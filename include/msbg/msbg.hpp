@@ -322,6 +322,83 @@ class MultiresSparseGrid {
   std::map<int64_t, std::unique_ptr<SBG::SparseGrid<float>>> f32_;
 };
 
+// The pressure solver's hierarchy and its relax / multiply
+// (MultiresSparseGrid::relax, multiplyLaplacianMatrixOpt, msbg3.cpp:1756,
+// 2244), over msbg_ps_*. Channels are PS_CH_X / _B / _R / _T.
+enum { PS_CH_X = MSBG_PS_CH_X, PS_CH_B = MSBG_PS_CH_B, PS_CH_R = MSBG_PS_CH_R,
+       PS_CH_T = MSBG_PS_CH_T, PS_CH_D = MSBG_PS_CH_D };
+enum { DBC_OPEN = MSBG_DBC_OPEN, DBC_SOLID = MSBG_DBC_SOLID };
+enum : unsigned {
+  OPT_BOUNDZONE_ONLY = MSBG_OPT_BOUNDZONE_ONLY,
+  OPT_CALC_RESIDUAL = MSBG_OPT_CALC_RESIDUAL,
+  OPT_REVERSE_ORDER = MSBG_OPT_REVERSE_ORDER,
+};
+
+class PressureGrid {
+ public:
+  // blockLevels[nbx*nby*nbz]; cellType (MSBG_PS_FLUID/SOLID/VOID) per finest
+  // cell, x fastest. omega and the sub-iteration count default to the
+  // reference's _mgSmOmega 6/7 and _mgSmBlockIter 5 (msbg.cpp:819, 837).
+  PressureGrid(int nbx, int nby, int nbz, int bsx0, double dx0, const int domainBc[3][2],
+               const int8_t *blockLevels, const uint8_t *cellType, int denseLevels = 0,
+               float omega = 6.f / 7.f, int blockIter = 5) {
+    msbg_ps_desc d{};
+    d.nbx = nbx, d.nby = nby, d.nbz = nbz, d.bsx0 = bsx0, d.dx0 = dx0;
+    for (int a = 0; a < 3; a++)
+      for (int s = 0; s < 2; s++) d.domainBc[a][s] = domainBc[a][s];
+    d.omega = omega, d.blockIter = blockIter, d.denseLevels = denseLevels;
+    check(msbg_ps_create(&d, blockLevels, cellType, &ps_));
+    check(msbg_ps_get_levels(ps_, &nmg_, &nlev_));
+  }
+  ~PressureGrid() { msbg_ps_destroy(ps_); }
+  PressureGrid(const PressureGrid &) = delete;
+  PressureGrid &operator=(const PressureGrid &) = delete;
+
+  int getNumMgLevels() const { return nmg_; }
+  int64_t channelSize(int levelMg) const {
+    int64_t n = 0;
+    check(msbg_ps_level_size(ps_, levelMg, &n));
+    return n;
+  }
+  void setChannel(int chan, int levelMg, const std::vector<float> &v) {
+    if ((int64_t)v.size() != channelSize(levelMg)) throw Error(MSBG_EINVAL, "channel size");
+    check(msbg_ps_upload(ps_, chan, levelMg, v.data()));
+  }
+  std::vector<float> getChannel(int chan, int levelMg) {
+    std::vector<float> v((size_t)channelSize(levelMg));
+    check(msbg_ps_download(ps_, chan, levelMg, v.data()));
+    return v;
+  }
+  void resetChannel(int chan, int levelMg) { check(msbg_ps_fill(ps_, chan, levelMg, 0.0f)); }
+
+  void relax(int boundaryZoneOnly, int reverseOrder, int levelMg, int chanX, int chanB,
+             int chanTmp, int nIter, unsigned options = 0) {
+    (void)options;
+    check(msbg_ps_relax(ps_, boundaryZoneOnly, reverseOrder, levelMg, chanX, chanB, chanTmp,
+                        nIter));
+  }
+  void multiplyLaplacianMatrixOpt(unsigned options, int levelMg, int chanX, int chanB, float kap,
+                                  int chanDstY, long double *pDotprod,
+                                  std::vector<int> *blockList = nullptr) {
+    if (blockList) throw Error(MSBG_EUNSUPP, "multiplyLaplacianMatrixOpt: block list");
+    double dot = 0;
+    check(msbg_ps_multiply(ps_, options, levelMg, chanX, chanB, kap, chanDstY,
+                           pDotprod ? &dot : nullptr));
+    if (pDotprod) *pDotprod = dot;
+  }
+  void restrictChannel(int levelMg, int chanSrc, int chanDst) {
+    check(msbg_ps_restrict(ps_, levelMg, chanSrc, chanDst));
+  }
+  void prolongateChannel(int levelMg, int chanSrc, int chanDst) {
+    check(msbg_ps_prolongate(ps_, levelMg, chanSrc, chanDst));
+  }
+  void vCycle(int nu1, int nu2, int coarseIter) { check(msbg_ps_vcycle(ps_, nu1, nu2, coarseIter)); }
+
+ private:
+  msbg_ps *ps_ = nullptr;
+  int nmg_ = 0, nlev_ = 0;
+};
+
 }  // namespace MSBG
 
 namespace RDR {

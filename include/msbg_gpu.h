@@ -641,6 +641,15 @@ int msbg_ps_prolongate(msbg_ps *ps, int levelMg, int chanSrc, int chanDst);
  * level gets coarseIter passes. */
 int msbg_ps_vcycle(msbg_ps *ps, int nu1, int nu2, int coarseIter);
 int msbg_ps_synchronize(msbg_ps *ps);
+/* Device time of the last msbg_ps_relax / _multiply / _vcycle call (HIP
+ * events on the hierarchy's stream around all of its launches); waits for
+ * it to finish. The reference's relax() reports its time the same way per
+ * call (msbg3.cpp:2349-2358). */
+int msbg_ps_last_timing(msbg_ps *ps, double *ms);
+/* Cells one relax iteration updates at levelMg: the cells of every block
+ * with fluid (relax()'s nTotalCells, msbg3.cpp:2228), or all cells of a
+ * dense level. */
+int msbg_ps_relaxed_cells(const msbg_ps *ps, int levelMg, int64_t *cells);
 
 #ifdef __cplusplus
 }

@@ -139,6 +139,8 @@ SIGNATURES = {
     "msbg_ps_prolongate": (_I, [_P, _I, _I, _I]),
     "msbg_ps_vcycle": (_I, [_P, _I, _I, _I]),
     "msbg_ps_synchronize": (_I, [_P]),
+    "msbg_ps_last_timing": (_I, [_P, C.POINTER(_D)]),
+    "msbg_ps_relaxed_cells": (_I, [_P, _I, C.POINTER(_I64)]),
 }
 
 

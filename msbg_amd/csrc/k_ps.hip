@@ -36,10 +36,9 @@ __device__ __forceinline__ int vox(int bsx, int x, int y, int z) { return x + bs
 // (msbg3.cpp:59-160), with a = 1/3 (finer neighbour, four cells) or 2/3
 // (coarser neighbour), times the face area when the block cuts a solid
 template <int BSX>
-__device__ double mixed_sum(const PsBlockArgs &a, int L, int L2, int b, int b2, int d, int s,
-                            int vx, int vy, int vz, bool cuts) {
+__device__ double mixed_sum(const PsBlockArgs &a, const float *X2, int L, int L2, int b, int b2,
+                            int d, int s, int vx, int vy, int vz, bool cuts) {
 #pragma clang fp contract(off)
-  const float *X2 = a.X + a.off[b2];
   double sum = 0;
   if (L2 < L) {
     constexpr int BH = 2 * BSX;
@@ -69,84 +68,170 @@ __device__ double mixed_sum(const PsBlockArgs &a, int L, int L2, int b, int b2, 
   return sum;
 }
 
-// face weight of face f of cell (x,y,z) (msbg3.cpp:606-633 and the border
-// slabs): own face areas inside the block, the neighbour's on the upper face
-// of the last cell, 1/0 for an open/solid upper domain border, 1 on a
-// mixed-resolution face (its value is premultiplied)
-template <int BSX>
-__device__ float face_w(const PsBlockArgs &a, int L, int b, int f, int kind, int b2, int x, int y,
-                        int z) {
-  constexpr int NC = BSX * BSX * BSX;
-  const int d = f >> 1, s = f & 1;
-  int c[3] = {x, y, z};
-  const float *own = a.fa[L][d] + (int64_t)b * NC;
-  if (!s) return (c[d] > 0 || kind != 2) ? own[vox(BSX, x, y, z)] : 1.0f;
-  if (c[d] < BSX - 1) {
-    c[d]++;
-    return own[vox(BSX, c[0], c[1], c[2])];
-  }
-  if (kind == 1) return a.openRight[d] ? 1.0f : 0.0f;
-  if (kind == 2) return 1.0f;
-  c[d] = 0;
-  return a.fa[L][d][(int64_t)b2 * NC + vox(BSX, c[0], c[1], c[2])];
+// N consecutive floats / uint16 as one aligned vector load
+template <int N>
+struct alignas(4 * N) F32v {
+  float v[N];
+};
+template <int N>
+struct alignas(2 * N) U16v {
+  uint16_t v[N];
+};
+template <int N>
+__device__ __forceinline__ F32v<N> load_f32v(const float *p) {
+  return *reinterpret_cast<const F32v<N> *>(__builtin_assume_aligned(p, 4 * N));
+}
+template <int N>
+__device__ __forceinline__ U16v<N> load_u16v(const uint16_t *p) {
+  return *reinterpret_cast<const U16v<N> *>(__builtin_assume_aligned(p, 2 * N));
 }
 
-template <int BSX, int MODE>
-__global__ __launch_bounds__(256) void k_ps_block(PsBlockArgs a, const int32_t *list, int n) {
+// the old value of the lane below / above inside a 16-lane row
+__device__ __forceinline__ float lane_below(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_above(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xf, 0xf, true));
+}
+
+// face weights of the six faces, stored per cell as k/64 in one byte each:
+// every face area of a block level (at most 3 halvings of a 0/1 face) is a
+// multiple of 1/64, so the encoding is exact
+__device__ __forceinline__ uint32_t w_code(float w) { return (uint32_t)(w * 64.0f); }
+__device__ __forceinline__ float w_decode(uint64_t c, int f) {
+  return (float)(uint32_t)((c >> (8 * f)) & 0xffu) * (1.0f / 64.0f);
+}
+
+template <int BSX, int MODE, bool CUTS>
+__global__ __launch_bounds__(256) void k_ps_block(PsBlockArgs a, const PsItem *items, int n) {
 #pragma clang fp contract(off)
   constexpr int SW = BSX >= 8 ? 8 : BSX;
-  constexpr int NV = BSX / SW;
+  constexpr int NV = BSX / SW;  // vectors per row = vectors per thread in the set-up
   constexpr int TPB = BSX * BSX;
   constexpr int NB = 256 / TPB;
   constexpr int HS = BSX + 2;
   constexpr int NC = BSX * BSX * BSX;
+  constexpr int NVEC = NC / SW;
   __shared__ float s_buf[NB][HS * HS * HS];
   __shared__ float s_D[NB][NC];
   __shared__ float s_B[NB][MODE == PS_MODE_MULTIPLY ? 1 : NC];
-  __shared__ uint8_t s_g[NB][NC / SW];
+  __shared__ uint8_t s_g[NB][NVEC];
+  __shared__ uint64_t s_w[CUTS ? NB : 1][CUTS ? NC : 1];
   __shared__ double s_red[MODE == PS_MODE_MULTIPLY ? 256 : 1];
 
   const int slot = threadIdx.x / TPB, t = threadIdx.x % TPB;
   const int x = t % BSX, y = t / BSX;
   const int item = blockIdx.x * NB + slot;
   const bool valid = item < n;
-  const int b = valid ? list[item] : 0;
-  const int L = valid ? a.L[b] : 0;
+  PsItem it0;
+  if (valid) {
+    it0 = items[item];
+  } else {
+    it0.off = 0, it0.b = 0, it0.L = 0;
+#pragma unroll
+    for (int f = 0; f < 6; f++) it0.nbr[f] = -1, it0.noff[f] = 0, it0.nL[f] = 0;
+  }
+  const int b = it0.b;
+  const int L = it0.L;
   const float h = (float)((double)(1 << L) * a.dx0);
   const float HR = 1.f / h;
-  const bool cuts = valid && (a.bflags[b] & kPsBlkCutsSolid);
-  const int64_t off = valid ? a.off[b] : 0;
+  const int64_t off = it0.off;
   float *buf = s_buf[slot];
   auto H = [](int xx, int yy, int zz) { return (xx + 1) + HS * ((yy + 1) + HS * (zz + 1)); };
 
-  // interior: gates, X*h (0 in vectors without fluid), D, B
-  for (int z = 0; z < BSX; z++) {
-    const int v = vox(BSX, x, y, z);
-    const uint16_t f = valid ? a.flags[off + v] : kPsCellSolid;
-    int any = (f & (kPsCellSolid | kPsCellPartialSolid)) != 0;
-    int fl = (f & (kPsCellSolid | kPsCellVoid)) == 0;
-#pragma unroll
-    for (int o = 1; o < SW; o <<= 1) {
-      any |= __shfl_xor(any, o);
-      fl |= __shfl_xor(fl, o);
-    }
-    const int gate = any | ((!fl) << 1);
-    if (x % SW == 0) s_g[slot][v / SW] = (uint8_t)gate;
-    buf[H(x, y, z)] = (valid && !(gate & 2)) ? a.X[off + v] * h : 0.0f;
-    s_D[slot][v] = valid ? a.D[off + v] : 0.0f;
-    if constexpr (MODE != PS_MODE_MULTIPLY) s_B[slot][v] = valid ? a.B[off + v] : 0.0f;
-  }
-
-  // border slabs: same level (X*h), outside the domain (0) or mixed
-  // resolution (procNeighMixedRes2 sum times h)
   int nb2[6], kind[6];
 #pragma unroll
   for (int f = 0; f < 6; f++) {
-    const int d = f >> 1, s = f & 1;
-    const int b2 = valid ? a.nbr[6 * b + f] : -1;
+    const int b2 = it0.nbr[f];
     nb2[f] = b2;
-    const int L2 = b2 >= 0 ? a.L[b2] : L;
-    kind[f] = b2 < 0 ? 1 : (L2 == L ? 0 : 2);
+    kind[f] = b2 < 0 ? 1 : (it0.nL[f] == L ? 0 : 2);
+  }
+
+  // interior, whole vectors per thread: gates, X*h (0 in vectors without
+  // fluid), D, B and, for blocks that cut a solid, the face weights
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    const int vec = t + k * TPB;
+    const int vid0 = vec * SW;
+    const int x0 = vid0 % BSX, yy = (vid0 / BSX) % BSX, zz = vid0 / (BSX * BSX);
+    U16v<SW> f;
+    F32v<SW> X, D, B;
+    if (valid) {
+      f = load_u16v<SW>(a.flags + off + vid0);
+      X = load_f32v<SW>(a.X + off + vid0);
+      D = load_f32v<SW>(a.D + off + vid0);
+      if constexpr (MODE != PS_MODE_MULTIPLY) B = load_f32v<SW>(a.B + off + vid0);
+    } else {
+#pragma unroll
+      for (int l = 0; l < SW; l++) f.v[l] = kPsCellSolid, X.v[l] = D.v[l] = B.v[l] = 0.0f;
+    }
+    int any = 0, fl = 0;
+#pragma unroll
+    for (int l = 0; l < SW; l++) {
+      any |= (f.v[l] & (kPsCellSolid | kPsCellPartialSolid)) != 0;
+      fl |= (f.v[l] & (kPsCellSolid | kPsCellVoid)) == 0;
+    }
+    const int gate = any | ((!fl) << 1);
+    s_g[slot][vec] = (uint8_t)gate;
+#pragma unroll
+    for (int l = 0; l < SW; l++) {
+      buf[H(x0 + l, yy, zz)] = (gate & 2) ? 0.0f : X.v[l] * h;
+      s_D[slot][vid0 + l] = D.v[l];
+      if constexpr (MODE != PS_MODE_MULTIPLY) s_B[slot][vid0 + l] = B.v[l];
+    }
+    if constexpr (CUTS) {
+      if (valid) {
+        const float *fu = a.fa[L][0] + (int64_t)b * NC, *fv = a.fa[L][1] + (int64_t)b * NC,
+                    *fw = a.fa[L][2] + (int64_t)b * NC;
+        const F32v<SW> U = load_f32v<SW>(fu + vid0), V = load_f32v<SW>(fv + vid0),
+                       W = load_f32v<SW>(fw + vid0);
+        const float unext = x0 + SW < BSX ? fu[vid0 + SW] : 0.0f;
+        F32v<SW> V1, W1;
+        if (yy < BSX - 1) V1 = load_f32v<SW>(fv + vid0 + BSX);
+        if (zz < BSX - 1) W1 = load_f32v<SW>(fw + vid0 + BSX * BSX);
+        // upper-border weights: the neighbour's lower face, 1/0 for an
+        // open/solid domain border, 1 on a mixed-resolution face
+        auto border = [&](int d, int p, int q) -> float {
+          const int kd = kind[2 * d + 1];
+          if (kd == 1) return a.openRight[d] ? 1.0f : 0.0f;
+          if (kd == 2) return 1.0f;
+          const int c = d == 0 ? vox(BSX, 0, p, q) : d == 1 ? vox(BSX, p, 0, q) : vox(BSX, p, q, 0);
+          return a.fa[L][d][(int64_t)nb2[2 * d + 1] * NC + c];
+        };
+#pragma unroll
+        for (int l = 0; l < SW; l++) {
+          const int xl = x0 + l;
+          const float w0 = (xl == 0 && kind[0] == 2) ? 1.0f : U.v[l];
+          const float w1 = xl < BSX - 1 ? (l < SW - 1 ? U.v[l + 1] : unext) : border(0, yy, zz);
+          const float w2 = (yy == 0 && kind[2] == 2) ? 1.0f : V.v[l];
+          const float w3 = yy < BSX - 1 ? V1.v[l] : border(1, xl, zz);
+          const float w4 = (zz == 0 && kind[4] == 2) ? 1.0f : W.v[l];
+          const float w5 = zz < BSX - 1 ? W1.v[l] : border(2, xl, yy);
+          s_w[slot][vid0 + l] = (uint64_t)w_code(w0) | ((uint64_t)w_code(w1) << 8) |
+                                ((uint64_t)w_code(w2) << 16) | ((uint64_t)w_code(w3) << 24) |
+                                ((uint64_t)w_code(w4) << 32) | ((uint64_t)w_code(w5) << 40);
+        }
+      }
+    }
+  }
+
+  // border slabs: same level (X*h), outside the domain (0) or mixed
+  // resolution (procNeighMixedRes2 sum times h); the same-level loads are
+  // all issued before any is used
+  float fv[6];
+#pragma unroll
+  for (int f = 0; f < 6; f++) {
+    const int d = f >> 1, s = f & 1;
+    int q[3];
+    q[d] = s ? 0 : BSX - 1;
+    if (d == 0) { q[1] = x; q[2] = y; }
+    else if (d == 1) { q[0] = x; q[2] = y; }
+    else { q[0] = x; q[1] = y; }
+    fv[f] = kind[f] == 0 ? a.X[it0.noff[f] + vox(BSX, q[0], q[1], q[2])] : 0.0f;
+  }
+#pragma unroll
+  for (int f = 0; f < 6; f++) {
+    const int d = f >> 1, s = f & 1;
     int c[3];
     c[d] = s ? BSX - 1 : 0;
     if (d == 0) { c[1] = x; c[2] = y; }
@@ -155,15 +240,12 @@ __global__ __launch_bounds__(256) void k_ps_block(PsBlockArgs a, const int32_t *
     int o[3] = {c[0], c[1], c[2]};
     o[d] += s ? 1 : -1;
     float val = 0.0f;
-    if (b2 >= 0) {
-      if (L2 == L) {
-        int q[3] = {c[0], c[1], c[2]};
-        q[d] = s ? 0 : BSX - 1;
-        val = a.X[a.off[b2] + vox(BSX, q[0], q[1], q[2])] * h;
-      } else {
-        const double sum = mixed_sum<BSX>(a, L, L2, b, b2, d, s, c[0], c[1], c[2], cuts);
-        val = (float)(sum * (double)h);
-      }
+    if (kind[f] == 0) {
+      val = fv[f] * h;
+    } else if (kind[f] == 2) {
+      const double sum = mixed_sum<BSX>(a, a.X + it0.noff[f], L, it0.nL[f], b, nb2[f], d, s, c[0],
+                                        c[1], c[2], CUTS);
+      val = (float)(sum * (double)h);
     }
     buf[H(o[0], o[1], o[2])] = val;
   }
@@ -171,46 +253,73 @@ __global__ __launch_bounds__(256) void k_ps_block(PsBlockArgs a, const int32_t *
 
   const float omega = a.omega, om1 = 1.0f - omega;
   // stencil sum: plain, or with the face weights where the vector is gated
-  auto stencil = [&](int xx, int yy, int zz, int gate, float F1, float F2, float F3, float F4,
-                     float F5, float F6) -> float {
-    if (cuts && (gate & 1)) {
+  auto stencil = [&](int gate, uint64_t wc, float F1, float F2, float F3, float F4, float F5,
+                     float F6) -> float {
+    if (CUTS && (gate & 1)) {
       float S = 0.f;
-      S = __builtin_fmaf(F1, face_w<BSX>(a, L, b, 0, kind[0], nb2[0], xx, yy, zz), S);
-      S = __builtin_fmaf(F2, face_w<BSX>(a, L, b, 1, kind[1], nb2[1], xx, yy, zz), S);
-      S = __builtin_fmaf(F3, face_w<BSX>(a, L, b, 2, kind[2], nb2[2], xx, yy, zz), S);
-      S = __builtin_fmaf(F4, face_w<BSX>(a, L, b, 3, kind[3], nb2[3], xx, yy, zz), S);
-      S = __builtin_fmaf(F5, face_w<BSX>(a, L, b, 4, kind[4], nb2[4], xx, yy, zz), S);
-      S = __builtin_fmaf(F6, face_w<BSX>(a, L, b, 5, kind[5], nb2[5], xx, yy, zz), S);
+      S = __builtin_fmaf(F1, w_decode(wc, 0), S);
+      S = __builtin_fmaf(F2, w_decode(wc, 1), S);
+      S = __builtin_fmaf(F3, w_decode(wc, 2), S);
+      S = __builtin_fmaf(F4, w_decode(wc, 3), S);
+      S = __builtin_fmaf(F5, w_decode(wc, 4), S);
+      S = __builtin_fmaf(F6, w_decode(wc, 5), S);
       return S;
     }
     return F1 + F2 + F3 + F4 + F5 + F6;
   };
+  auto wcode_at = [&](int v) -> uint64_t {
+    if constexpr (CUTS) return s_w[slot][v];
+    else return 0;
+  };
 
   if constexpr (MODE == PS_MODE_RELAX) {
     const int i = x / SW, lane = x % SW;
-    const int lid = __lane_id();
     for (int it = 0; it < a.biter; it++) {
       const bool rev = a.rev == (it & 1);
       if (it < a.biter - 1) {
+        // wavefront over the column owned by this thread: the cell k along
+        // the sweep direction is updated at step st0 + k; the thread keeps
+        // the new value below, the old value and the old value above in
+        // registers
         constexpr int T = (NV - 1) + 2 * (BSX - 1) + 1;
+        const int dz = rev ? -1 : 1;
+        const int zs = rev ? BSX - 1 : 0;
+        const int st0 = rev ? (NV - 1 - i) + (BSX - 1 - y) : i + y;
+        float prv = buf[H(x, y, zs - dz)];
+        float cur = buf[H(x, y, zs)];
+        float nxt = buf[H(x, y, zs + dz)];
+        int v = vox(BSX, x, y, zs);
+        float Bc = s_B[slot][v], Dc = s_D[slot][v];
+        int gc = s_g[slot][v / SW];
+        uint64_t wc = wcode_at(v);
         for (int st = 0; st < T; st++) {
-          const int z = rev ? (BSX - 1) - (st - (NV - 1 - i) - (BSX - 1 - y)) : st - i - y;
-          const bool act = z >= 0 && z < BSX;
-          const int zz = act ? z : 0;
-          const float own = buf[H(x, y, zz)];
-          const float lsh = __shfl(own, (lid + 63) & 63);
-          const float rsh = __shfl(own, (lid + 1) & 63);
-          if (act) {
-            const int v = vox(BSX, x, y, z);
-            const int gate = s_g[slot][v / SW];
-            if (!(gate & 2)) {
-              const float F1 = lane > 0 ? lsh : buf[H(x - 1, y, z)];
-              const float F2 = lane < SW - 1 ? rsh : buf[H(x + 1, y, z)];
-              const float S = stencil(x, y, z, gate, F1, F2, buf[H(x, y - 1, z)],
-                                      buf[H(x, y + 1, z)], buf[H(x, y, z - 1)],
-                                      buf[H(x, y, z + 1)]);
-              buf[H(x, y, z)] = __builtin_fmaf(om1, own, (omega * (s_B[slot][v] + S)) * s_D[slot][v]);
+          const int k = st - st0;
+          const int z = zs + dz * k;
+          const float lsh = lane_below(cur), rsh = lane_above(cur);
+          if (k >= 0 && k < BSX) {
+            // every LDS read of the step is issued before the step's write:
+            // the neighbours, the old value two cells on and the next cell's
+            // coefficients, so that their latencies overlap
+            float *p = buf + H(x, y, z);
+            const float e1 = p[-1], e2 = p[1], F3 = p[-HS], F4 = p[HS];
+            const float nx2 = k + 2 <= BSX ? p[2 * dz * HS * HS] : 0.0f;
+            const int vn = k + 1 < BSX ? vox(BSX, x, y, z + dz) : v;
+            const float Bn = s_B[slot][vn], Dn = s_D[slot][vn];
+            const int gn = s_g[slot][vn / SW];
+            const uint64_t wn = wcode_at(vn);
+            float Yn = cur;
+            if (!(gc & 2)) {
+              const float F1 = lane > 0 ? lsh : e1;
+              const float F2 = lane < SW - 1 ? rsh : e2;
+              const float F5 = rev ? nxt : prv, F6 = rev ? prv : nxt;
+              const float S = stencil(gc, wc, F1, F2, F3, F4, F5, F6);
+              Yn = __builtin_fmaf(om1, cur, (omega * (Bc + S)) * Dc);
+              *p = Yn;
             }
+            prv = Yn;
+            cur = nxt;
+            nxt = nx2;
+            v = vn, Bc = Bn, Dc = Dn, gc = gn, wc = wn;
           }
           __syncthreads();
         }
@@ -220,7 +329,7 @@ __global__ __launch_bounds__(256) void k_ps_block(PsBlockArgs a, const int32_t *
           const int gate = s_g[slot][v / SW];
           if (gate & 2) continue;
           const float F0 = buf[H(x, y, z)];
-          const float S = stencil(x, y, z, gate, buf[H(x - 1, y, z)], buf[H(x + 1, y, z)],
+          const float S = stencil(gate, wcode_at(v), buf[H(x - 1, y, z)], buf[H(x + 1, y, z)],
                                   buf[H(x, y - 1, z)], buf[H(x, y + 1, z)], buf[H(x, y, z - 1)],
                                   buf[H(x, y, z + 1)]);
           const float Y = __builtin_fmaf(om1, F0, (omega * (s_B[slot][v] + S)) * s_D[slot][v]);
@@ -235,7 +344,7 @@ __global__ __launch_bounds__(256) void k_ps_block(PsBlockArgs a, const int32_t *
       const int gate = s_g[slot][v / SW];
       if (gate & 2) continue;
       const float F0 = buf[H(x, y, z)];
-      const float S = stencil(x, y, z, gate, buf[H(x - 1, y, z)], buf[H(x + 1, y, z)],
+      const float S = stencil(gate, wcode_at(v), buf[H(x - 1, y, z)], buf[H(x + 1, y, z)],
                               buf[H(x, y - 1, z)], buf[H(x, y + 1, z)], buf[H(x, y, z - 1)],
                               buf[H(x, y, z + 1)]);
       const float D = s_D[slot][v];
@@ -398,12 +507,15 @@ __global__ void k_ps_copy(const float *s, float *d, int64_t n) {
     d[i] = s[i];
 }
 
-template <int BSX>
-void launch_block_bsx(hipStream_t s, int mode, const PsBlockArgs &a, const int32_t *list, int n,
+template <int BSX, bool CUTS>
+void launch_block_bsx(hipStream_t s, int mode, const PsBlockArgs &a, const PsItem *list, int n,
                       int wg) {
-  if (mode == PS_MODE_RELAX) hipLaunchKernelGGL((k_ps_block<BSX, PS_MODE_RELAX>), dim3(wg), dim3(256), 0, s, a, list, n);
-  else if (mode == PS_MODE_MULTIPLY) hipLaunchKernelGGL((k_ps_block<BSX, PS_MODE_MULTIPLY>), dim3(wg), dim3(256), 0, s, a, list, n);
-  else hipLaunchKernelGGL((k_ps_block<BSX, PS_MODE_RESIDUAL>), dim3(wg), dim3(256), 0, s, a, list, n);
+  if (mode == PS_MODE_RELAX)
+    hipLaunchKernelGGL((k_ps_block<BSX, PS_MODE_RELAX, CUTS>), dim3(wg), dim3(256), 0, s, a, list, n);
+  else if (mode == PS_MODE_MULTIPLY)
+    hipLaunchKernelGGL((k_ps_block<BSX, PS_MODE_MULTIPLY, CUTS>), dim3(wg), dim3(256), 0, s, a, list, n);
+  else
+    hipLaunchKernelGGL((k_ps_block<BSX, PS_MODE_RESIDUAL, CUTS>), dim3(wg), dim3(256), 0, s, a, list, n);
 }
 
 int grid_1d(int64_t n) { return (int)((n + 255) / 256); }
@@ -416,16 +528,20 @@ int ps_block_workgroups(int bsx, int n) {
   return (n + nb - 1) / nb;
 }
 
-int launch_ps_block(hipStream_t s, int bsx, int mode, const PsBlockArgs &a, const int32_t *list,
-                    int n, int *nWorkgroups) {
+int launch_ps_block(hipStream_t s, int bsx, int mode, bool cuts, const PsBlockArgs &a,
+                    const PsItem *list, int n, int *nWorkgroups) {
   const int wg = ps_block_workgroups(bsx, n);
   if (nWorkgroups) *nWorkgroups = wg;
   if (n <= 0) return 0;
-  switch (bsx) {
-    case 16: launch_block_bsx<16>(s, mode, a, list, n, wg); break;
-    case 8: launch_block_bsx<8>(s, mode, a, list, n, wg); break;
-    case 4: launch_block_bsx<4>(s, mode, a, list, n, wg); break;
-    case 2: launch_block_bsx<2>(s, mode, a, list, n, wg); break;
+  switch (bsx * 2 + (cuts ? 1 : 0)) {
+    case 32: launch_block_bsx<16, false>(s, mode, a, list, n, wg); break;
+    case 33: launch_block_bsx<16, true>(s, mode, a, list, n, wg); break;
+    case 16: launch_block_bsx<8, false>(s, mode, a, list, n, wg); break;
+    case 17: launch_block_bsx<8, true>(s, mode, a, list, n, wg); break;
+    case 8: launch_block_bsx<4, false>(s, mode, a, list, n, wg); break;
+    case 9: launch_block_bsx<4, true>(s, mode, a, list, n, wg); break;
+    case 4: launch_block_bsx<2, false>(s, mode, a, list, n, wg); break;
+    case 5: launch_block_bsx<2, true>(s, mode, a, list, n, wg); break;
     default: set_error("pressure relax: block edge %d unsupported", bsx); return MSBG_EUNSUPP;
   }
   MSBG_HIP(hipGetLastError());

@@ -49,14 +49,15 @@ struct MgLevel {
   std::vector<int8_t> hL;
   std::vector<int64_t> hoff;
   std::vector<uint16_t> hflags, hbflags;
-  // lists: relax[parity][L], multiply[L]
+  // block lists per (coordinate-sum parity, level, BLK_CUTS_SOLID)
   std::vector<int32_t> hlist;
-  int listOff[2][kPsMaxBlockLevels] = {}, listLen[2][kPsMaxBlockLevels] = {};
+  std::vector<PsItem> hitems;  // the listed blocks' records, same order
+  int listOff[2][kPsMaxBlockLevels][2] = {}, listLen[2][kPsMaxBlockLevels][2] = {};
   DevBuf<int8_t> L;
   DevBuf<int64_t> off;
   DevBuf<float> ch[4], D, fa[3];
   DevBuf<uint16_t> flags, bflags;
-  DevBuf<int32_t> list;
+  DevBuf<PsItem> list;
 };
 
 }  // namespace
@@ -70,10 +71,13 @@ struct msbg_ps {
   std::vector<int32_t> hnbr;
   MgLevel mg[kMaxMg];
   DevBuf<float> dpfa[kPsMaxBlockLevels][3];
-  DevBuf<int32_t> nbr;
   DevBuf<double> dot;
   hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};  // start / end of the last timed call
+  bool timed = false;
   ~msbg_ps() {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -297,20 +301,37 @@ int build_block_level(msbg_ps &ps, int m) {
           D[g.hoff[b] + v] = sum > 0 ? (float)(1.0 / sum) : 0.0f;
         }
   }
-  // block lists per (coordinate-sum parity, level) of the blocks with fluid;
-  // the relax takes one parity per pass, the multiply both
+  // block lists per (coordinate-sum parity, level, cuts a solid) of the
+  // blocks with fluid; the relax takes one parity per pass, the multiply both
   g.hlist.clear();
   for (int par = 0; par < 2; par++)
-    for (int L = 0; L < kPsMaxBlockLevels; L++) {
-      g.listOff[par][L] = (int)g.hlist.size();
-      for (int b = 0; b < nb; b++) {
-        if (g.hL[b] != L || (g.hbflags[b] & kPsBlkNoFluid)) continue;
-        int bc[3];
-        block_xyz(ps, b, bc);
-        if (((bc[0] + bc[1] + bc[2]) & 1) == par) g.hlist.push_back(b);
+    for (int L = 0; L < kPsMaxBlockLevels; L++)
+      for (int cu = 0; cu < 2; cu++) {
+        g.listOff[par][L][cu] = (int)g.hlist.size();
+        for (int b = 0; b < nb; b++) {
+          if (g.hL[b] != L || (g.hbflags[b] & kPsBlkNoFluid)) continue;
+          if (((g.hbflags[b] & kPsBlkCutsSolid) != 0) != (cu != 0)) continue;
+          int bc[3];
+          block_xyz(ps, b, bc);
+          if (((bc[0] + bc[1] + bc[2]) & 1) == par) g.hlist.push_back(b);
+        }
+        g.listLen[par][L][cu] = (int)g.hlist.size() - g.listOff[par][L][cu];
       }
-      g.listLen[par][L] = (int)g.hlist.size() - g.listOff[par][L];
+  g.hitems.resize(g.hlist.size());
+  for (size_t k = 0; k < g.hlist.size(); k++) {
+    const int b = g.hlist[k];
+    PsItem &it = g.hitems[k];
+    std::memset(&it, 0, sizeof(it));
+    it.b = b;
+    it.off = g.hoff[b];
+    it.L = g.hL[b];
+    for (int f = 0; f < 6; f++) {
+      const int b2 = ps.hnbr[6 * b + f];
+      it.nbr[f] = b2;
+      it.noff[f] = b2 >= 0 ? g.hoff[b2] : 0;
+      it.nL[f] = b2 >= 0 ? g.hL[b2] : 0;
     }
+  }
   MSBG_TRY(g.D.alloc(n));
   MSBG_HIP(hipMemcpy(g.D.p, D.data(), n * sizeof(float), hipMemcpyHostToDevice));
   return 0;
@@ -378,9 +399,9 @@ int upload_level(msbg_ps &ps, int m) {
     MSBG_HIP(hipMemcpy(g.L.p, g.hL.data(), ps.nb, hipMemcpyHostToDevice));
     MSBG_HIP(hipMemcpy(g.off.p, g.hoff.data(), ps.nb * sizeof(int64_t), hipMemcpyHostToDevice));
     MSBG_HIP(hipMemcpy(g.bflags.p, g.hbflags.data(), ps.nb * sizeof(uint16_t), hipMemcpyHostToDevice));
-    if (!g.hlist.empty()) {
-      MSBG_TRY(g.list.alloc(g.hlist.size()));
-      MSBG_HIP(hipMemcpy(g.list.p, g.hlist.data(), g.hlist.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!g.hitems.empty()) {
+      MSBG_TRY(g.list.alloc(g.hitems.size()));
+      MSBG_HIP(hipMemcpy(g.list.p, g.hitems.data(), g.hitems.size() * sizeof(PsItem), hipMemcpyHostToDevice));
     }
   }
   return 0;
@@ -389,9 +410,6 @@ int upload_level(msbg_ps &ps, int m) {
 PsBlockArgs block_args(msbg_ps &ps, int m, int cx, int cb, int cdst) {
   MgLevel &g = ps.mg[m];
   PsBlockArgs a{};
-  a.L = g.L.p;
-  a.off = g.off.p;
-  a.nbr = ps.nbr.p;
   a.X = g.ch[cx].p;
   a.B = cb >= 0 ? g.ch[cb].p : nullptr;
   a.Y = g.ch[cdst].p;
@@ -455,12 +473,13 @@ int relax_impl(msbg_ps *ps, int reverseOrder, int m, int chX, int chB, int chTmp
       // relaxBlockList (msbg3.cpp:2187-2208): pass 0 takes the blocks whose
       // coordinate sum has parity 1 (parity 0 with OPT_REVERSE_ORDER)
       const int par = reverseOrder ? pass : 1 - pass;
-      for (int L = 0; L < kPsMaxBlockLevels; L++) {
-        const int n = g.listLen[par][L];
-        if (!n) continue;
-        MSBG_TRY(launch_ps_block(ps->stream, ps->desc.bsx0 >> L, PS_MODE_RELAX, a,
-                                 g.list.p + g.listOff[par][L], n, nullptr));
-      }
+      for (int L = 0; L < ps->nlev; L++)
+        for (int cu = 0; cu < 2; cu++) {
+          const int n = g.listLen[par][L][cu];
+          if (!n) continue;
+          MSBG_TRY(launch_ps_block(ps->stream, ps->desc.bsx0 >> L, PS_MODE_RELAX, cu != 0, a,
+                                   g.list.p + g.listOff[par][L][cu], n, nullptr));
+        }
     }
   return 0;
 }
@@ -477,19 +496,20 @@ int multiply_impl(msbg_ps *ps, unsigned options, int m, int chX, int chB, int ch
   int total = 0;
   for (int par = 0; par < 2; par++)
     for (int L = 0; L < ps->nlev; L++)
-      if (g.listLen[par][L]) total += ps_block_workgroups(ps->desc.bsx0 >> L, g.listLen[par][L]);
+      for (int cu = 0; cu < 2; cu++) total += ps_block_workgroups(ps->desc.bsx0 >> L, g.listLen[par][L][cu]);
   if (!resid) MSBG_TRY(ps->dot.ensure(std::max(total, 1)));
   int wgOff = 0;
-  for (int L = 0; L < kPsMaxBlockLevels; L++)
-    for (int par = 0; par < 2; par++) {
-      const int n = g.listLen[par][L];
-      if (!n) continue;
-      a.dot = resid ? nullptr : ps->dot.p + wgOff;
-      int wg = 0;
-      MSBG_TRY(launch_ps_block(ps->stream, ps->desc.bsx0 >> L, resid ? PS_MODE_RESIDUAL : PS_MODE_MULTIPLY,
-                               a, g.list.p + g.listOff[par][L], n, &wg));
-      wgOff += wg;
-    }
+  for (int L = 0; L < ps->nlev; L++)
+    for (int par = 0; par < 2; par++)
+      for (int cu = 0; cu < 2; cu++) {
+        const int n = g.listLen[par][L][cu];
+        if (!n) continue;
+        a.dot = resid ? nullptr : ps->dot.p + wgOff;
+        int wg = 0;
+        MSBG_TRY(launch_ps_block(ps->stream, ps->desc.bsx0 >> L, resid ? PS_MODE_RESIDUAL : PS_MODE_MULTIPLY,
+                                 cu != 0, a, g.list.p + g.listOff[par][L][cu], n, &wg));
+        wgOff += wg;
+      }
   if (dot) {
     double s = 0;
     if (!resid && wgOff) {
@@ -586,6 +606,8 @@ int msbg_ps_create(const msbg_ps_desc *d, const int8_t *blockLevels, const uint8
   for (size_t i = 0; i < n0; i++)
     MSBG_CHECK(cellType[i] <= MSBG_PS_VOID, MSBG_EINVAL, "msbg_ps_create: cell type %d", cellType[i]);
   MSBG_HIP(hipStreamCreate(&ps->stream));
+  MSBG_HIP(hipEventCreate(&ps->ev[0]));
+  MSBG_HIP(hipEventCreate(&ps->ev[1]));
   build_levels(*ps);
   for (int l = 0; l < nlev; l++) {
     const int bsx = bsx0 >> l;
@@ -606,8 +628,6 @@ int msbg_ps_create(const msbg_ps_desc *d, const int8_t *blockLevels, const uint8
       MSBG_HIP(hipMemcpy(ps->dpfa[l][a].p, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice));
     }
   }
-  MSBG_TRY(ps->nbr.alloc(ps->hnbr.size()));
-  MSBG_HIP(hipMemcpy(ps->nbr.p, ps->hnbr.data(), ps->hnbr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   for (int m = 0; m < ps->nmg; m++) {
     MSBG_TRY(m < nlev ? build_block_level(*ps, m) : build_dense_level(*ps, m));
     MSBG_TRY(upload_level(*ps, m));
@@ -683,7 +703,11 @@ int msbg_ps_relax(msbg_ps *ps, int boundaryZoneOnly, int reverseOrder, int level
     MSBG_CHECK(chan_ok(chanTmp) && chanTmp != chanX && chanTmp != chanB, MSBG_EINVAL,
                "msbg_ps_relax: a dense level needs a distinct chanTmp");
   if (!nIter) return 0;
-  return relax_impl(ps, reverseOrder, levelMg, chanX, chanB, chanTmp, nIter);
+  MSBG_HIP(hipEventRecord(ps->ev[0], ps->stream));
+  MSBG_TRY(relax_impl(ps, reverseOrder, levelMg, chanX, chanB, chanTmp, nIter));
+  MSBG_HIP(hipEventRecord(ps->ev[1], ps->stream));
+  ps->timed = true;
+  return 0;
 }
 
 int msbg_ps_multiply(msbg_ps *ps, unsigned options, int levelMg, int chanX, int chanB, float kap,
@@ -697,7 +721,11 @@ int msbg_ps_multiply(msbg_ps *ps, unsigned options, int levelMg, int chanX, int 
              "msbg_ps_multiply: no dot product on a dense level (msbg3.cpp:1804)");
   MSBG_CHECK(!(options & MSBG_OPT_BOUNDZONE_ONLY), MSBG_EUNSUPP, "msbg_ps_multiply: boundary zone only");
   if (!resid) chanB = chanX;  // unused
-  return multiply_impl(ps, options, levelMg, chanX, chanB, chanDstY, dotprod);
+  MSBG_HIP(hipEventRecord(ps->ev[0], ps->stream));
+  MSBG_TRY(multiply_impl(ps, options, levelMg, chanX, chanB, chanDstY, dotprod));
+  MSBG_HIP(hipEventRecord(ps->ev[1], ps->stream));
+  ps->timed = true;
+  return 0;
 }
 
 int msbg_ps_restrict(msbg_ps *ps, int levelMg, int chanSrc, int chanDst) {
@@ -716,7 +744,37 @@ int msbg_ps_prolongate(msbg_ps *ps, int levelMg, int chanSrc, int chanDst) {
 
 int msbg_ps_vcycle(msbg_ps *ps, int nu1, int nu2, int coarseIter) {
   MSBG_CHECK(ps && nu1 >= 0 && nu2 >= 0 && coarseIter >= 0, MSBG_EINVAL, "msbg_ps_vcycle: bad args");
-  return vcycle_level(ps, 0, nu1, nu2, coarseIter);
+  MSBG_HIP(hipEventRecord(ps->ev[0], ps->stream));
+  MSBG_TRY(vcycle_level(ps, 0, nu1, nu2, coarseIter));
+  MSBG_HIP(hipEventRecord(ps->ev[1], ps->stream));
+  ps->timed = true;
+  return 0;
+}
+
+int msbg_ps_last_timing(msbg_ps *ps, double *ms) {
+  MSBG_CHECK(ps && ms, MSBG_EINVAL, "msbg_ps_last_timing: bad args");
+  MSBG_CHECK(ps->timed, MSBG_ESTATE, "msbg_ps_last_timing: no relax/multiply/vcycle call yet");
+  MSBG_HIP(hipEventSynchronize(ps->ev[1]));
+  float t = 0;
+  MSBG_HIP(hipEventElapsedTime(&t, ps->ev[0], ps->ev[1]));
+  *ms = t;
+  return 0;
+}
+
+int msbg_ps_relaxed_cells(const msbg_ps *ps, int levelMg, int64_t *cells) {
+  MSBG_CHECK(ps && cells && levelMg >= 0 && levelMg < ps->nmg, MSBG_EINVAL, "msbg_ps_relaxed_cells: bad args");
+  const MgLevel &g = ps->mg[levelMg];
+  int64_t n = 0;
+  if (g.dense) {
+    n = (int64_t)g.sx * g.sy * g.sz;
+  } else {
+    for (const PsItem &it : g.hitems) {
+      const int e = ps->desc.bsx0 >> it.L;
+      n += (int64_t)e * e * e;
+    }
+  }
+  *cells = n;
+  return 0;
 }
 
 int msbg_ps_synchronize(msbg_ps *ps) {

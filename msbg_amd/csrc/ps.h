@@ -22,9 +22,6 @@ enum { PS_MODE_RELAX = 0, PS_MODE_MULTIPLY = 1, PS_MODE_RESIDUAL = 2 };
 
 // One block multigrid level as the block kernel reads it.
 struct PsBlockArgs {
-  const int8_t *L;          // effective level of every block at this mg level
-  const int64_t *off;       // page offset of every block (floats)
-  const int32_t *nbr;       // 6 face neighbours per block (-x,+x,-y,+y,-z,+z), -1 outside
   const float *X;           // values the halo is built from
   const float *B;           // right-hand side
   float *Y;                 // destination (the relax writes X in place)
@@ -38,6 +35,18 @@ struct PsBlockArgs {
   int rev;                  // OPT_REVERSE_ORDER given
   int biter;                // sub-iterations per block (_mgSmBlockIter)
   double *dot;              // multiply: one partial dot product per workgroup
+};
+
+// One listed block with everything its halo needs, so that the kernel reads
+// one record instead of chasing block tables.
+struct alignas(16) PsItem {
+  int64_t off;      // page offset of the block (floats)
+  int64_t noff[6];  // page offsets of the face neighbours (0 outside the domain)
+  int32_t b;        // block id
+  int32_t nbr[6];   // face neighbours (-x,+x,-y,+y,-z,+z), -1 outside the domain
+  int8_t L;         // the block's level at this multigrid level
+  int8_t nL[6];     // the neighbours' levels
+  int8_t pad;
 };
 
 // One dense multigrid level (padded by a one-cell zero shell).
@@ -62,8 +71,9 @@ struct PsXferArgs {
   const uint16_t *flagsF, *flagsC;
 };
 
-int launch_ps_block(hipStream_t s, int bsx, int mode, const PsBlockArgs &a, const int32_t *list,
-                    int n, int *nWorkgroups);
+// cuts: every listed block has BLK_CUTS_SOLID (face weights), or none has
+int launch_ps_block(hipStream_t s, int bsx, int mode, bool cuts, const PsBlockArgs &a,
+                    const PsItem *items, int n, int *nWorkgroups);
 int ps_block_workgroups(int bsx, int n);
 int launch_ps_dense(hipStream_t s, int mode, const PsDenseArgs &a);
 int launch_ps_restrict(hipStream_t s, const PsXferArgs &a, const float *src, float *dst);

@@ -131,3 +131,54 @@ class PressureGrid:
 
     def synchronize(self):
         _lib.check(_lib.lib().msbg_ps_synchronize(self.h))
+
+    def last_timing(self):
+        """Device milliseconds of the last relax / multiply / vcycle call."""
+        ms = C.c_double()
+        _lib.check(_lib.lib().msbg_ps_last_timing(self.h, C.byref(ms)))
+        return ms.value
+
+    def relaxed_cells(self, levelMg):
+        """Cells one relax iteration updates at levelMg (relax()'s nTotalCells)."""
+        n = C.c_int64()
+        _lib.check(_lib.lib().msbg_ps_relaxed_cells(self.h, levelMg, C.byref(n)))
+        return n.value
+
+
+def tank_scene(nb, bsx0, seed=1, fill=0.78, radius=0.17):
+    """Synthetic pressure workload: a tank filled with fluid up to fill * the
+    domain height (void above), a solid sphere of radius * the smallest
+    extent at a seeded position in the fluid. Block levels follow the
+    chessboard block distance d to the blocks that hold fluid next to solid
+    or void: level max(0, min(d - 1, 3)), so face neighbours differ by one
+    level at most. Returns (levels int8 [nb], cell types uint8 [z][y][x])."""
+    nbx, nby, nbz = nb
+    sx, sy, sz = nbx * bsx0, nby * bsx0, nbz * bsx0
+    rng = np.random.default_rng(seed)
+    t = np.zeros((sz, sy, sx), np.uint8)
+    t[int(fill * sz):] = VOID
+    c = np.array([rng.uniform(0.3, 0.7) * sx, rng.uniform(0.3, 0.7) * sy,
+                  rng.uniform(0.25, 0.5) * sz])
+    r = radius * min(sx, sy, sz)
+    zz = (np.arange(sz) + 0.5 - c[2]) ** 2
+    yy = (np.arange(sy) + 0.5 - c[1]) ** 2
+    xx = (np.arange(sx) + 0.5 - c[0]) ** 2
+    for z in range(sz):
+        if zz[z] < r * r:
+            m = (yy[:, None] + xx[None, :] + zz[z]) < r * r
+            t[z][m] = SOLID
+    tb = t.reshape(nbz, bsx0, nby, bsx0, nbx, bsx0)
+    fl = (tb == FLUID).any(axis=(1, 3, 5))
+    band = fl & ((tb == SOLID).any(axis=(1, 3, 5)) | (tb == VOID).any(axis=(1, 3, 5)))
+    nlev = int(np.log2(bsx0))
+    d = np.where(band, 0, nlev + 1).astype(np.int64)
+    for _ in range(nlev + 1):  # chessboard distance by repeated 3x3x3 minimum
+        p = np.pad(d, 1, constant_values=nlev + 1)
+        m = d.copy()
+        for dz in range(3):
+            for dy in range(3):
+                for dx in range(3):
+                    m = np.minimum(m, p[dz:dz + nbz, dy:dy + nby, dx:dx + nbx] + 1)
+        d = m
+    lev = np.clip(d - 1, 0, nlev - 1).astype(np.int8)
+    return lev.ravel(), t

@@ -9,7 +9,10 @@
 // laplacian flow on it and writes those pages there, and checks that a
 // storage type other than the channel's is rejected. Given argv[4], it
 // converts the field to UNORM8 and runs applyChannelPdeFast<uint8_t> (the
-// stochastically rounded store) on it, writing those pages there.
+// stochastically rounded store) on it, writing those pages there. Given
+// argv[5], it builds a small pressure hierarchy (a tank with a solid box,
+// two block levels), runs two V-cycles, one relax() and the residual
+// multiply, and writes the levels, cell types, X and R there.
 #include <cstdio>
 #include <vector>
 
@@ -102,6 +105,38 @@ int main(int argc, char **argv) {
         std::fwrite(b8.data(), 1, b8.size(), f8);
       }
       std::fclose(f8);
+    }
+    if (argc > 5) {  // relax / multiplyLaplacianMatrixOpt (msbg3.cpp:1756, 2244)
+      const int nbx = 4, nby = 4, nbz = 2, b0 = 16, sx = nbx * b0, sy = nby * b0, sz = nbz * b0;
+      std::vector<int8_t> lev(nbx * nby * nbz);
+      for (int b = 0; b < (int)lev.size(); b++) lev[b] = (b % nbx) < 2 ? 0 : 1;
+      std::vector<uint8_t> typ((size_t)sx * sy * sz, MSBG_PS_FLUID);
+      for (int z = 0; z < sz; z++)
+        for (int y = 0; y < sy; y++)
+          for (int x = 0; x < sx; x++) {
+            uint8_t &t = typ[x + (size_t)sx * (y + (size_t)sy * z)];
+            if (z >= 24) t = MSBG_PS_VOID;
+            if (x >= 20 && x < 28 && y >= 20 && y < 44 && z >= 4 && z < 14) t = MSBG_PS_SOLID;
+          }
+      const int dbc[3][2] = {{MSBG::DBC_OPEN, MSBG::DBC_OPEN},
+                             {MSBG::DBC_SOLID, MSBG::DBC_SOLID},
+                             {MSBG::DBC_SOLID, MSBG::DBC_OPEN}};
+      MSBG::PressureGrid ps(nbx, nby, nbz, b0, 1.0 / sx, dbc, lev.data(), typ.data(), 1);
+      std::vector<float> B((size_t)ps.channelSize(0));
+      for (size_t i = 0; i < B.size(); i++) B[i] = 1e-4f * (float)((int)(i % 13) - 6);
+      ps.setChannel(MSBG::PS_CH_B, 0, B);
+      ps.vCycle(2, 2, 20);
+      ps.vCycle(2, 2, 20);
+      ps.relax(0, 0, 0, MSBG::PS_CH_X, MSBG::PS_CH_B, MSBG::PS_CH_T, 1);
+      ps.multiplyLaplacianMatrixOpt(MSBG::OPT_CALC_RESIDUAL, 0, MSBG::PS_CH_X, MSBG::PS_CH_B, 0.0f,
+                                    MSBG::PS_CH_R, nullptr);
+      std::vector<float> X = ps.getChannel(MSBG::PS_CH_X, 0), R = ps.getChannel(MSBG::PS_CH_R, 0);
+      FILE *fp = std::fopen(argv[5], "wb");
+      std::fwrite(lev.data(), 1, lev.size(), fp);
+      std::fwrite(typ.data(), 1, typ.size(), fp);
+      std::fwrite(X.data(), 4, X.size(), fp);
+      std::fwrite(R.data(), 4, R.size(), fp);
+      std::fclose(fp);
     }
     std::printf("facade_demo: %zu active blocks\n", activeBlocks.size());
     MSBG::MultiresSparseGrid::destroy(msbg);

@@ -551,9 +551,9 @@ def test_cpp_facade_demo_matches_binding(tmp_path):
     if not os.path.exists(exe):
         pytest.fail("tests/cpp/facade_demo not built (run the build step)")
     out, frame, cnf = tmp_path / "pages.bin", tmp_path / "frame.bin", tmp_path / "cnf.bin"
-    u8 = tmp_path / "u8.bin"
-    r = subprocess.run([exe, str(out), str(frame), str(cnf), str(u8)], capture_output=True,
-                       text=True, timeout=120)
+    u8, pres = tmp_path / "u8.bin", tmp_path / "pressure.bin"
+    r = subprocess.run([exe, str(out), str(frame), str(cnf), str(u8), str(pres)],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     got = np.fromfile(out, dtype=np.uint16)
     S, bsx = 128, 16
@@ -582,6 +582,25 @@ def test_cpp_facade_demo_matches_binding(tmp_path):
     g.copyDataTo(30, 8, M.F32)  # without Taubin the result differs
     g.applyChannelPdeFast(8, 6, 7, None, -(1 + M.OPT_8_COLOR_SCHEME), 1, 0, 0, 0, 4, 1.0, 0.05)
     assert not np.array_equal(got, g.download(8).reshape(-1))
+    # MSBG::PressureGrid (relax, multiplyLaplacianMatrixOpt, V-cycles) through
+    # the facade = the Python binding on the same inputs
+    from msbg_amd import pressure as PR
+    raw = np.fromfile(pres, dtype=np.uint8)
+    nb, b0 = (4, 4, 2), 16
+    lev = raw[:32].view(np.int8)
+    typ = raw[32:32 + 64 * 64 * 32].reshape(32, 64, 64)
+    with PR.PressureGrid(nb, b0, lev, typ, dbc=(0, 0, 1, 1, 1, 0), dense_levels=1) as ps:
+        n = ps.size(0)
+        xr = raw[32 + typ.size:].view(np.float32)
+        assert xr.size == 2 * n
+        ps.upload(PR.CH_B, 0, np.float32(1e-4) * ((np.arange(n) % 13) - 6).astype(np.float32))
+        ps.vcycle(2, 2, 20)
+        ps.vcycle(2, 2, 20)
+        ps.relax(0, 0, 0, PR.CH_X, PR.CH_B, PR.CH_T, 1)
+        ps.multiplyLaplacianMatrixOpt(PR.OPT_CALC_RESIDUAL, 0, PR.CH_X, PR.CH_B, 0.0, PR.CH_R)
+        assert np.array_equal(xr[:n].view(np.uint32), ps.download(PR.CH_X, 0).view(np.uint32))
+        assert np.array_equal(xr[n:].view(np.uint32), ps.download(PR.CH_R, 0).view(np.uint32))
+        assert np.abs(xr[:n]).max() > 0
 
 
 # --------------------------------------------------------------------------
