@@ -502,9 +502,23 @@ int msbg_unet_forward(msbg_unet *u, const uint16_t *x, int n, float *y);
 int msbg_unet_backward(msbg_unet *u, const uint16_t *x, const float *dy,
                        int n);
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) on mean((y - target)^2);
- * *loss receives the loss before the step */
+ * *loss receives the loss before the step. With loss == NULL the step is
+ * queued without waiting and its loss goes to a running sum that
+ * msbg_unet_take_loss returns. */
 int msbg_unet_train_step(msbg_unet *u, const uint16_t *x,
                          const float *target, int n, float lr, double *loss);
+/* The same step on rows of a resident sample set: the windows X[rows[i]]
+ * ([N][36^3] BF16, device) and targets T[rows[i]] ([N][32^3] FP32, device),
+ * rows a host array of n sample indices, gathered on the device. */
+int msbg_unet_train_step_rows(msbg_unet *u, const uint16_t *X, const float *T,
+                              const int32_t *rows, int n, float lr,
+                              double *loss);
+/* forward of the rows X[rows[i]] into y [n][32^3] (device) */
+int msbg_unet_forward_rows(msbg_unet *u, const uint16_t *X,
+                           const int32_t *rows, int n, float *y);
+/* the summed mean losses of the steps queued with loss == NULL since the
+ * last call, and their count; waits for them and resets the sum */
+int msbg_unet_take_loss(msbg_unet *u, double *sum, int64_t *steps);
 /* 36^3 windows of level-0 value blocks (host block ids) of a 32^3-block
  * channel, under the sweep's halo rule, into x (device). */
 int msbg_unet_gather(msbg_grid *g, int chan, const int32_t *blocks, int n,

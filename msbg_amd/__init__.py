@@ -8,7 +8,7 @@ from ._lib import (ARITH_EXACT, ARITH_FAST, BF16, CH_FLOAT_1, CH_FLOAT_2, CH_FLO
                    CH_UINT8_1, CH_UINT16_1, F16, F32,
                    F64, IP_BSPLINE3, IP_LINEAR, OPT_8_COLOR_SCHEME, OPT_BC_COARSE_LEVEL, OPT_BC_NEUMANN, OPT_INTERPOLATE,
                    OPT_SINGLE_LEVEL, PDE_MEAN_CURVATURE, U8, U16, MsbgError)
-from .msbg import (MCF_8COLOR, Batch, DeviceArray, MultiresSparseGrid, UNet, cnn_conv3d,  # noqa: F401
+from .msbg import (MCF_8COLOR, Batch, DeviceArray, DeviceView, MultiresSparseGrid, UNet, cnn_conv3d,  # noqa: F401
                    cnn_conv3d_bwd_data, cnn_conv3d_bwd_weights, copy_bandwidth, getBlockColor8,
                    scene_objects, scene_spheres)
 from .pressure import PressureGrid  # noqa: F401

@@ -117,6 +117,9 @@ SIGNATURES = {
     "msbg_unet_forward": (_I, [_P, _P, _I, _P]),
     "msbg_unet_backward": (_I, [_P, _P, _P, _I]),
     "msbg_unet_train_step": (_I, [_P, _P, _P, _I, _F, _P]),
+    "msbg_unet_train_step_rows": (_I, [_P, _P, _P, _P, _I, _F, _P]),
+    "msbg_unet_forward_rows": (_I, [_P, _P, _P, _I, _P]),
+    "msbg_unet_take_loss": (_I, [_P, C.POINTER(_D), C.POINTER(_I64)]),
     "msbg_unet_gather": (_I, [_P, _I, _P, _I, _P]),
     "msbg_unet_scatter": (_I, [_P, _I, _P, _I, _P]),
     "msbg_unet_extract": (_I, [_P, _I, _P, _I, _P]),

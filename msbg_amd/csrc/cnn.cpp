@@ -18,6 +18,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -72,6 +73,8 @@ struct msbg_unet {
   DevBuf<uint16_t> ga[10], gc9, gc7, gp2, gp1, dzt, xtp;
   DevBuf<float> wpart, hpart, hsum, rsum;
   DevBuf<double> lpart, lsum;
+  DevBuf<double> lacc;     // running {loss sum, steps} of steps queued without a loss readback
+  DevBuf<int32_t> rows;    // sample rows of the current *_rows call
 };
 
 namespace {
@@ -210,6 +213,44 @@ int load_x(msbg_unet *u, const uint16_t *x, int n) {
   return 0;
 }
 
+// the windows (and targets) of sample rows `rows` (host) of the resident
+// sets X [N][36^3] and T [N][32^3], gathered on the device
+int load_rows(msbg_unet *u, const uint16_t *X, const float *T, const int32_t *rows, int n) {
+  MSBG_TRY(u->rows.ensure((size_t)std::max(n, u->cap)));
+  MSBG_HIP(hipMemcpyAsync(u->rows.p, rows, sizeof(int32_t) * n, hipMemcpyHostToDevice, u->stream));
+  MSBG_TRY(launch_gather_rows(u->stream, u->x0.p, X, u->rows.p, n, sizeof(uint16_t) * (int64_t)kVin));
+  if (T)
+    MSBG_TRY(launch_gather_rows(u->stream, u->tgt.p, T, u->rows.p, n, sizeof(float) * (int64_t)kVout));
+  return 0;
+}
+
+// forward, MSE, backward and Adam on the loaded batch; with loss == NULL the
+// step stays queued and its mean loss goes to the running accumulator
+int train_core(msbg_unet *u, int n, float lr, double *loss) {
+  MSBG_TRY(forward(u, n));
+  const int64_t cnt = (int64_t)n * kVout;
+  MSBG_TRY(launch_mse(u->stream, u->y.p, u->tgt.p, cnt, u->dy.p, u->lpart.p));
+  MSBG_TRY(launch_sum_f64(u->stream, u->lpart.p, mse_blocks(cnt), u->lsum.p));
+  MSBG_TRY(backward(u, n));
+  u->adam_t++;
+  const float b1 = 0.9f, b2 = 0.999f;
+  MSBG_TRY(launch_adam(u->stream, u->p.p, u->g.p, u->m.p, u->v.p, u->nparam, lr, b1, b2, 1e-8f,
+                       1.0f - std::pow(b1, (float)u->adam_t), 1.0f - std::pow(b2, (float)u->adam_t)));
+  u->swz_valid = false;
+  if (loss) {
+    double sum = 0;
+    MSBG_HIP(hipMemcpyAsync(&sum, u->lsum.p, sizeof(double), hipMemcpyDeviceToHost, u->stream));
+    MSBG_HIP(hipStreamSynchronize(u->stream));
+    *loss = sum / (double)cnt;
+    return 0;
+  }
+  if (!u->lacc.p) {
+    MSBG_TRY(u->lacc.alloc(2));
+    MSBG_HIP(hipMemsetAsync(u->lacc.p, 0, 2 * sizeof(double), u->stream));
+  }
+  return launch_loss_acc(u->stream, u->lsum.p, 1.0 / (double)cnt, u->lacc.p);
+}
+
 }  // namespace
 
 extern "C" {
@@ -339,21 +380,45 @@ int msbg_unet_train_step(msbg_unet *u, const uint16_t *x, const float *target, i
              "msbg_unet_train_step: batch %d must be 8 * 2^k", n);
   MSBG_TRY(ensure_cap(u, n, true));
   MSBG_TRY(load_x(u, x, n));
+  MSBG_HIP(hipMemcpyAsync(u->tgt.p, target, sizeof(float) * (int64_t)n * kVout, hipMemcpyDefault,
+                          u->stream));
+  return train_core(u, n, lr, loss);
+}
+
+int msbg_unet_train_step_rows(msbg_unet *u, const uint16_t *X, const float *T, const int32_t *rows,
+                              int n, float lr, double *loss) {
+  MSBG_CHECK(u && X && T && rows && lr >= 0.f, MSBG_EINVAL, "msbg_unet_train_step_rows: bad args");
+  MSBG_CHECK(n % 8 == 0 && (n & (n - 1)) == 0, MSBG_EINVAL,
+             "msbg_unet_train_step_rows: batch %d must be 8 * 2^k", n);
+  for (int i = 0; i < n; i++)
+    MSBG_CHECK(rows[i] >= 0, MSBG_EINVAL, "msbg_unet_train_step_rows: row %d negative", i);
+  MSBG_TRY(ensure_cap(u, n, true));
+  MSBG_TRY(load_rows(u, X, T, rows, n));
+  return train_core(u, n, lr, loss);
+}
+
+int msbg_unet_forward_rows(msbg_unet *u, const uint16_t *X, const int32_t *rows, int n, float *y) {
+  MSBG_CHECK(u && X && rows && y && n > 0, MSBG_EINVAL, "msbg_unet_forward_rows: bad args");
+  for (int i = 0; i < n; i++)
+    MSBG_CHECK(rows[i] >= 0, MSBG_EINVAL, "msbg_unet_forward_rows: row %d negative", i);
+  MSBG_TRY(ensure_cap(u, n, false));
+  MSBG_TRY(load_rows(u, X, nullptr, rows, n));
   MSBG_TRY(forward(u, n));
-  const int64_t cnt = (int64_t)n * kVout;
-  MSBG_HIP(hipMemcpyAsync(u->tgt.p, target, sizeof(float) * cnt, hipMemcpyDefault, u->stream));
-  MSBG_TRY(launch_mse(u->stream, u->y.p, u->tgt.p, cnt, u->dy.p, u->lpart.p));
-  MSBG_TRY(launch_sum_f64(u->stream, u->lpart.p, mse_blocks(cnt), u->lsum.p));
-  MSBG_TRY(backward(u, n));
-  u->adam_t++;
-  const float b1 = 0.9f, b2 = 0.999f;
-  MSBG_TRY(launch_adam(u->stream, u->p.p, u->g.p, u->m.p, u->v.p, u->nparam, lr, b1, b2, 1e-8f,
-                       1.0f - std::pow(b1, (float)u->adam_t), 1.0f - std::pow(b2, (float)u->adam_t)));
-  u->swz_valid = false;
-  double sum = 0;
-  MSBG_HIP(hipMemcpyAsync(&sum, u->lsum.p, sizeof(double), hipMemcpyDeviceToHost, u->stream));
+  MSBG_HIP(hipMemcpyAsync(y, u->y.p, sizeof(float) * n * kVout, hipMemcpyDefault, u->stream));
   MSBG_HIP(hipStreamSynchronize(u->stream));
-  if (loss) *loss = sum / (double)cnt;
+  return 0;
+}
+
+int msbg_unet_take_loss(msbg_unet *u, double *sum, int64_t *steps) {
+  MSBG_CHECK(u && sum, MSBG_EINVAL, "msbg_unet_take_loss: bad args");
+  double h[2] = {0, 0};
+  if (u->lacc.p) {
+    MSBG_HIP(hipMemcpyAsync(h, u->lacc.p, sizeof(h), hipMemcpyDeviceToHost, u->stream));
+    MSBG_HIP(hipStreamSynchronize(u->stream));
+    MSBG_HIP(hipMemsetAsync(u->lacc.p, 0, sizeof(h), u->stream));
+  }
+  *sum = h[0];
+  if (steps) *steps = (int64_t)h[1];
   return 0;
 }
 

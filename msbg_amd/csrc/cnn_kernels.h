@@ -65,6 +65,11 @@ int mse_blocks(int64_t count);
 int launch_mse(hipStream_t s, const float *y, const float *t, int64_t count, float *dy,
                double *partial);
 int launch_sum_f64(hipStream_t s, const double *partial, int n, double *out);
+// acc[0] += *sum * scale, acc[1] += 1
+int launch_loss_acc(hipStream_t s, const double *sum, double scale, double *acc);
+// dst[i] = src[rows[i]] for n rows of rowBytes (a multiple of 16); rows on the device
+int launch_gather_rows(hipStream_t s, void *dst, const void *src, const int32_t *rows, int n,
+                       int64_t rowBytes);
 int launch_adam(hipStream_t s, float *p, const float *g, float *m, float *v, int64_t n, float lr,
                 float b1, float b2, float eps, float bc1, float bc2);
 

@@ -21,6 +21,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cstdint>
 
 #include "cnn_kernels.h"
@@ -647,6 +649,27 @@ __global__ __launch_bounds__(256) void k_sum_f64(const double *__restrict__ part
   if (tid == 0) *out = red[0];
 }
 
+// acc[0] += *sum * scale, acc[1] += 1 (the running loss of queued steps)
+__global__ void k_loss_acc(const double *__restrict__ sum, double scale, double *__restrict__ acc) {
+  if (threadIdx.x == 0) {
+    acc[0] += *sum * scale;
+    acc[1] += 1.0;
+  }
+}
+
+// dst[i] = src[rows[i]], rows of `row16` 16-byte words
+__global__ __launch_bounds__(256) void k_gather_rows(uint4 *__restrict__ dst,
+                                                     const uint4 *__restrict__ src,
+                                                     const int32_t *__restrict__ rows,
+                                                     int64_t row16) {
+  const int64_t r = blockIdx.y;
+  const uint4 *s = src + (int64_t)rows[r] * row16;
+  uint4 *d = dst + r * row16;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < row16;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
+
 __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
                        float *__restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
                        float bc1, float bc2) {
@@ -899,6 +922,26 @@ int launch_mse(hipStream_t s, const float *y, const float *t, int64_t count, flo
 
 int launch_sum_f64(hipStream_t s, const double *partial, int n, double *out) {
   hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, s, partial, n, out);
+  MSBG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_loss_acc(hipStream_t s, const double *sum, double scale, double *acc) {
+  hipLaunchKernelGGL(k_loss_acc, dim3(1), dim3(64), 0, s, sum, scale, acc);
+  MSBG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_gather_rows(hipStream_t s, void *dst, const void *src, const int32_t *rows, int n,
+                       int64_t rowBytes) {
+  if (n <= 0) return 0;
+  if (rowBytes % 16) {
+    set_error("gather rows: row of %lld bytes is not a multiple of 16", (long long)rowBytes);
+    return MSBG_EINVAL;
+  }
+  const int64_t row16 = rowBytes / 16;
+  hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)std::min<int64_t>((row16 + 255) / 256, 64), n),
+                     dim3(256), 0, s, (uint4 *)dst, (const uint4 *)src, rows, row16);
   MSBG_HIP(hipGetLastError());
   return 0;
 }

@@ -322,20 +322,12 @@ struct Marcher {
 // COUNT: also accumulate {density samples taken, samples that interpolated,
 // hits} into
 // counts[0..2] (one atomic per wave and counter); off in the product launch
-// Waves per SIMD the render kernel is compiled for. Measured on MI355X (4K
-// frame of a 512^3 UNORM16 scene): 6 (80 VGPRs, the once-per-ray shading
-// spills a little) ran 10% faster than the unbounded 125 VGPRs at 4 waves;
-// 5 and 8 were in between.
-#ifndef MSBG_RENDER_WAVES
-#define MSBG_RENDER_WAVES 6
-#endif
-#if MSBG_RENDER_WAVES > 0
-#define MSBG_RENDER_LB __launch_bounds__(256, MSBG_RENDER_WAVES)
-#else
-#define MSBG_RENDER_LB __launch_bounds__(256)
-#endif
+// Compiled for 6 waves per SIMD. Measured on MI355X (4K frame of a 512^3
+// UNORM16 scene): 6 (80 VGPRs, the once-per-ray shading spills a little) ran
+// 10% faster than the unbounded 125 VGPRs at 4 waves; 5 and 8 were in between.
+constexpr int kRenderWavesPerSimd = 6;
 template <int ST, bool COUNT>
-__global__ MSBG_RENDER_LB void k_render(RenderVol V, RenderCam C, int W, int H,
+__global__ __launch_bounds__(256, kRenderWavesPerSimd) void k_render(RenderVol V, RenderCam C, int W, int H,
                                                 uint32_t *__restrict__ rgb,
                                                 unsigned long long *__restrict__ counts) {
   const int x = blockIdx.x * 16 + (threadIdx.x & 15);

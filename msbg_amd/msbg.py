@@ -69,6 +69,10 @@ class DeviceArray:
     def data_ptr(self):
         return self._p.value or 0
 
+    def rows(self, start, count=1):
+        """View of rows [start, start + count) along the first axis."""
+        return DeviceView(self, start, count)
+
     def upload(self, a):
         a = np.ascontiguousarray(a, dtype=self.dtype)
         assert a.nbytes == self.nbytes, (a.shape, self.shape)
@@ -97,22 +101,44 @@ class DeviceArray:
         self.free()
 
 
+class DeviceView:
+    """Rows [start, start + count) of a DeviceArray (no ownership)."""
+
+    def __init__(self, base, start, count):
+        assert 0 <= start and start + count <= base.shape[0]
+        self.base = base
+        self.shape = (int(count),) + base.shape[1:]
+        self.dtype = base.dtype
+        self._row = int(np.prod(base.shape[1:], dtype=np.int64)) * base.dtype.itemsize
+        self._off = int(start) * self._row
+        self.nbytes = int(count) * self._row
+
+    def data_ptr(self):
+        return self.base.data_ptr() + self._off
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.shape, self.shape)
+        check(lib().msbg_dev_upload(C.c_void_p(self.data_ptr()), _ptr(a), a.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        check(lib().msbg_dev_download(_ptr(out), C.c_void_p(self.data_ptr()), out.nbytes))
+        return out
+
+
 def _dptr(t):
-    """Device address of a tensor-like object with data_ptr() (e.g. a torch
-    CUDA tensor); the library only sees the address. Work already queued on
-    the tensor's own stream (torch's current stream) is waited for first:
-    the library runs on its own streams."""
-    if getattr(t, "is_cuda", False):
-        import torch
-        torch.cuda.current_stream(t.device).synchronize()
+    """Device address of a DeviceArray / DeviceView (or of any object with
+    data_ptr(), whose pending writes the caller has completed)."""
     return C.c_void_p(t.data_ptr())
 
 
 class UNet:
     """Learned warm start: a 3-stage 3D U-Net mapping 36^3 block windows to
     the block's change over K plain sweeps (no counterpart in the reference).
-    Tensor arguments are device tensors: windows BF16 [n,36,36,36],
-    predictions / targets / output gradients FP32 [n,32,32,32]."""
+    Tensor arguments are DeviceArray / DeviceView objects (library-owned
+    device memory): windows BF16 bits [n,36,36,36] (uint16), predictions /
+    targets / output gradients FP32 [n,32,32,32]."""
 
     def __init__(self, seed=0):
         h = C.c_void_p()
@@ -142,11 +168,34 @@ class UNet:
     def backward(self, x, dy):
         check(lib().msbg_unet_backward(self._h, _dptr(x), _dptr(dy), x.shape[0]))
 
-    def train_step(self, x, target, lr):
+    def train_step(self, x, target, lr, wait=True):
+        """One Adam step; returns the loss (wait=True) or queues the step and
+        adds its loss to the running sum of take_loss() (wait=False)."""
         loss = C.c_double()
         check(lib().msbg_unet_train_step(self._h, _dptr(x), _dptr(target), x.shape[0], lr,
-                                         C.byref(loss)))
-        return loss.value
+                                         C.byref(loss) if wait else None))
+        return loss.value if wait else None
+
+    def train_step_rows(self, X, T, rows, lr, wait=False):
+        """One Adam step on rows of resident sets X [N,36,36,36] (BF16 bits)
+        and T [N,32,32,32]; queued unless wait=True."""
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        assert r.size and r.max() < X.shape[0] and r.max() < T.shape[0]
+        loss = C.c_double()
+        check(lib().msbg_unet_train_step_rows(self._h, _dptr(X), _dptr(T), _ptr(r), r.size, lr,
+                                              C.byref(loss) if wait else None))
+        return loss.value if wait else None
+
+    def forward_rows(self, X, rows, y):
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        assert r.size == y.shape[0] and r.max() < X.shape[0]
+        check(lib().msbg_unet_forward_rows(self._h, _dptr(X), _ptr(r), r.size, _dptr(y)))
+
+    def take_loss(self):
+        """(sum of the mean losses, steps) of the queued steps since the last call."""
+        s, n = C.c_double(), C.c_int64()
+        check(lib().msbg_unet_take_loss(self._h, C.byref(s), C.byref(n)))
+        return s.value, n.value
 
     def warm_start(self, grid, chan, batch=64):
         """Add the predicted change to every level-0 value block; device ms."""

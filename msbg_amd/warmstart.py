@@ -17,8 +17,9 @@ training on the GPU, and time to tolerance of plain versus learned + plain.
   are compared (max and RMS difference) at the end.
 
 Run: python -m msbg_amd.warmstart [--scenes N ...]; prints JSON.
-Device tensors (windows, targets) are torch CUDA tensors; the library only
-sees their addresses.
+The sample sets (windows, targets) live in library-owned device memory
+(DeviceArray); training gathers each batch's rows on the device and queues
+its steps without waiting, reading the loss once per epoch.
 """
 import argparse
 import json
@@ -28,7 +29,7 @@ import time
 import numpy as np
 
 from . import _lib as L
-from .msbg import MCF_8COLOR, Batch, MultiresSparseGrid, UNet
+from .msbg import MCF_8COLOR, Batch, DeviceArray, MultiresSparseGrid, UNet
 
 CH_IN, CH_K, CH_RUN, CH_PLAIN = 60, 61, 62, 63
 RSCAN = 4.0
@@ -76,12 +77,13 @@ def level0_blocks(g, chan):
 
 def make_training_set(nScenes=10000, perScene=8, K=64, S=1024, seed=100, group=16,
                       objRange=(8, 40), dt=0.1, log=None):
-    """(X, T, stats): BF16 windows [N,36,36,36] and FP32 changes [N,32,32,32]."""
-    import torch
+    """(X, T, stats): device sets of BF16 windows [N,36,36,36] (uint16 bits)
+    and FP32 changes [N,32,32,32]."""
     N = nScenes * perScene
-    X = torch.empty((N, 36, 36, 36), dtype=torch.bfloat16, device="cuda")
-    T = torch.empty((N, 32, 32, 32), dtype=torch.float32, device="cuda")
-    base = torch.empty((perScene, 32, 32, 32), dtype=torch.float32, device="cuda")
+    X = DeviceArray((N, 36, 36, 36), np.uint16)
+    T = DeviceArray((N, 32, 32, 32), np.float32)
+    after = DeviceArray((perScene, 32, 32, 32), np.float32)
+    base = DeviceArray((perScene, 32, 32, 32), np.float32)
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
     n, blocks, sweep_ms = 0, 0, 0.0
@@ -100,62 +102,66 @@ def make_training_set(nScenes=10000, perScene=8, K=64, S=1024, seed=100, group=1
                 cand = level0_blocks(g, CH_IN)
                 blocks += cand.size
                 pick = np.sort(rng.choice(cand, perScene, replace=cand.size < perScene))
-                UNet.gather(g, CH_IN, pick, X[n:n + perScene])
-                UNet.extract(g, CH_K, pick, T[n:n + perScene])
+                UNet.gather(g, CH_IN, pick, X.rows(n, perScene))
+                UNet.extract(g, CH_K, pick, after)
                 UNet.extract(g, CH_IN, pick, base)
-                T[n:n + perScene] -= base
+                # the change over K sweeps, an FP32 difference
+                T.rows(n, perScene).upload(after.numpy() - base.numpy())
                 n += perScene
         finally:
             for g in grids:
                 g.destroy()
         if log and (s0 // group) % 16 == 0:
             log(json.dumps({"scenes": s0 + len(grids), "seconds": time.perf_counter() - t0}))
-    torch.cuda.synchronize()
+    after.free()
+    base.free()
     return X, T, {"scenes": nScenes, "samples": N, "K": K, "S": S,
                   "mean_level0_blocks": blocks / nScenes, "sweep_ms": sweep_ms,
                   "seconds": time.perf_counter() - t0}
 
 
 def train(net, X, T, epochs=4, batch=32, lr=1e-3, val_frac=0.05, seed=0, log=None):
-    import torch
+    """Adam on the resident sets: every batch's rows are gathered on the
+    device and its step is queued; the loss is read once per epoch."""
     N = X.shape[0]
     nval = max(batch, int(N * val_frac))
-    gen = torch.Generator(device="cpu").manual_seed(seed)
-    perm = torch.randperm(N, generator=gen).to("cuda")
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(N).astype(np.int32)
     val, tr = perm[:nval], perm[nval:]
-    ntr = (tr.numel() // batch) * batch
+    ntr = (tr.size // batch) * batch
     steps = epochs * (ntr // batch)
     hist, step = [], 0
+    nv = (nval // batch) * batch
+    tval = np.stack([T.rows(int(i)).numpy()[0] for i in val[:nv]])
+    y = DeviceArray((batch, 32, 32, 32), np.float32)
     t0 = time.perf_counter()
 
     def val_loss():
-        y = torch.empty((batch, 32, 32, 32), dtype=torch.float32, device="cuda")
         se, zero = 0.0, 0.0
-        for i in range(0, nval - batch + 1, batch):
-            idx = val[i:i + batch]
-            net.forward(X.index_select(0, idx).contiguous(), y)
-            t = T.index_select(0, idx)
-            se += float(((y - t) ** 2).sum())
-            zero += float((t ** 2).sum())
+        for i in range(0, nv, batch):
+            net.forward_rows(X, val[i:i + batch], y)
+            d = y.numpy().astype(np.float64) - tval[i:i + batch]
+            se += float((d * d).sum())
+            zero += float((tval[i:i + batch].astype(np.float64) ** 2).sum())
         return se / max(zero, 1e-30)
 
     rel0 = val_loss()
+    net.take_loss()
     for ep in range(epochs):
-        order = tr[torch.randperm(tr.numel(), generator=gen).to("cuda")][:ntr]
-        losses = []
+        order = rng.permutation(tr)[:ntr]
         for i in range(0, ntr, batch):
-            idx = order[i:i + batch]
             # cosine decay to lr/10
             a = lr * (0.55 + 0.45 * np.cos(np.pi * step / max(steps, 1)))
-            losses.append(net.train_step(X.index_select(0, idx).contiguous(),
-                                         T.index_select(0, idx).contiguous(), float(a)))
+            net.train_step_rows(X, T, order[i:i + batch], float(a))
             step += 1
+        lsum, nsteps = net.take_loss()
         rel = val_loss()
-        hist.append({"epoch": ep + 1, "train_mse": float(np.mean(losses)), "val_rel_mse": rel,
+        hist.append({"epoch": ep + 1, "train_mse": lsum / max(nsteps, 1), "val_rel_mse": rel,
                      "seconds": time.perf_counter() - t0})
         if log:
             log(json.dumps(hist[-1]))
     secs = time.perf_counter() - t0
+    y.free()
     return {"samples": N, "val": nval, "epochs": epochs, "batch": batch, "lr": lr, "steps": steps,
             "val_rel_mse_initial": rel0, "history": hist, "seconds": secs,
             "train_tflops": 3 * 2 * macs_per_block() * steps * batch / secs / 1e12}
@@ -244,6 +250,7 @@ def main(argv=None):
     ap.add_argument("--scenes", type=int, default=2000)
     ap.add_argument("--per-scene", type=int, default=8)
     ap.add_argument("--K", type=int, default=1024)
+    ap.add_argument("--train-size", type=int, default=1024, help="edge of the training scenes")
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -259,17 +266,17 @@ def main(argv=None):
     def log(s):
         print(s, file=sys.stderr, flush=True)
 
-    import torch
-    torch.cuda.init()  # torch takes the device before the library does
     ds = tr = None
     with UNet(seed=a.seed) as net:
         if a.load:
             net.set_params(np.load(a.load, allow_pickle=False))
         else:
-            X, T, ds = make_training_set(a.scenes, a.per_scene, a.K, seed=a.seed, log=log)
+            X, T, ds = make_training_set(a.scenes, a.per_scene, a.K, S=a.train_size, seed=a.seed,
+                                         log=log)
             log(json.dumps({"dataset": ds}))
             tr = train(net, X, T, a.epochs, a.batch, a.lr, seed=a.seed, log=log)
-            del X, T
+            X.free()
+            T.free()
             if a.save:
                 np.save(a.save, net.get_params())
         evals = [time_to_tolerance(net, a.eval_size, a.eval_objects, s, log=log)

@@ -14,32 +14,38 @@ from oracle import cnn as O
 from oracle import liboracle as OL
 
 pytestmark = pytest.mark.gpu
-torch = pytest.importorskip("torch")
 M = pytest.importorskip("msbg_amd")
-
-CUDA = "cuda"
+DA = M.DeviceArray
 
 
 def note(*a):
     print("[gpu-cnn]", *a, file=sys.stderr, flush=True)
 
 
+def bf16_bits(a):
+    """BF16 bits (round to nearest even) of finite float32 values."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
 def bf16_exact(a):
     """float32 array of BF16-representable values (round to nearest even)."""
-    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(torch.bfloat16).float().numpy()
+    return (bf16_bits(a).astype(np.uint32) << 16).view(np.float32)
 
 
 def dev_bf16(a):
-    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(torch.bfloat16).to(CUDA)
+    return DA.from_numpy(bf16_bits(a))
 
 
 def dev_f32(a):
-    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(CUDA)
+    return DA.from_numpy(np.ascontiguousarray(a, np.float32))
 
 
 def host(t):
-    torch.cuda.synchronize()
-    return t.float().cpu().numpy().astype(np.float64)
+    a = t.numpy()
+    if a.dtype == np.uint16:  # BF16 bits
+        a = (a.astype(np.uint32) << 16).view(np.float32)
+    return a.astype(np.float64)
 
 
 def assert_bf16_close(got, ref, what):
@@ -63,7 +69,7 @@ def test_conv_forward_matches_oracle(cin, cout, dout, pad):
     w = bf16_exact(rng.standard_normal((cout, cin, 27)) * np.sqrt(2.0 / (27 * cin)))
     b = (rng.standard_normal(cout) * 0.1).astype(np.float32)
     for relu in (True, False):
-        y = torch.empty((n, dout, dout, dout, cout), dtype=torch.bfloat16, device=CUDA)
+        y = DA((n, dout, dout, dout, cout), np.uint16)
         M.cnn_conv3d(dev_bf16(x), cin, cout, dout, pad, dev_f32(w), dev_f32(b), relu, y)
         ref = O.conv3d(x, w, b, pad, relu)
         assert_bf16_close(host(y), ref, ("fwd", relu))
@@ -79,7 +85,7 @@ def test_conv_backward_data_matches_oracle(cin, cout, din, pad):
     dout = din + 2 * pad - 2
     dy = bf16_exact(rng.standard_normal((n, dout, dout, dout, cout)))
     w = bf16_exact(rng.standard_normal((cout, cin, 27)) * 0.2)
-    dx = torch.empty((n, din, din, din, cin), dtype=torch.bfloat16, device=CUDA)
+    dx = DA((n, din, din, din, cin), np.uint16)
     M.cnn_conv3d_bwd_data(dev_bf16(dy), cin, cout, din, pad, dev_f32(w), dx)
     assert_bf16_close(host(dx), O.conv3d_bwd_data(dy, w, pad), "bwd data")
 
@@ -94,8 +100,8 @@ def test_conv_backward_weights_matches_oracle(cin, cout, dout, pad):
     din = dout + 2 - 2 * pad
     x = bf16_exact(rng.uniform(-1, 1, (n, din, din, din, cin)))
     dy = bf16_exact(rng.standard_normal((n, dout, dout, dout, cout)))
-    dw = torch.empty((cout, cin, 27), dtype=torch.float32, device=CUDA)
-    db = torch.empty(cout, dtype=torch.float32, device=CUDA)
+    dw = DA((cout, cin, 27), np.float32)
+    db = DA(cout, np.float32)
     M.cnn_conv3d_bwd_weights(dev_bf16(x), dev_bf16(dy), cin, cout, dout, pad, dw, db)
     rdw, rdb = O.conv3d_bwd_weights(x, dy, pad)
     # FP32 sums of n*dout^3 exact products: error grows like sqrt(terms) * ulp
@@ -139,7 +145,7 @@ def test_unet_forward_matches_oracle():
         net.set_params(p)
         assert np.array_equal(net.get_params(), p)
         x = _windows(rng, 2)
-        y = torch.empty((2, 32, 32, 32), dtype=torch.float32, device=CUDA)
+        y = DA((2, 32, 32, 32), np.float32)
         net.forward(dev_bf16(x), y)
         got = host(y)
     ref, _ = O.forward(p.astype(np.float64), x, round_bf16_=True)
@@ -205,6 +211,35 @@ def test_unet_training_reduces_loss_deterministically():
     assert runs[0][-1] < 0.75 * runs[0][0]
 
 
+def test_unet_training_on_resident_rows_matches_batches():
+    """train_step_rows gathers rows of resident sets on the device: the
+    same steps as train_step on the gathered batches, and the queued steps'
+    running loss equals the waited losses."""
+    note("unet training on rows")
+    rng = np.random.default_rng(52)
+    x = _windows(rng, 16)
+    c = x[:, 2:34, 2:34, 2:34]
+    t = ((np.roll(c, 1, 1) + np.roll(c, -1, 2)) * 0.5 - c).astype(np.float32)
+    X, T = dev_bf16(x), dev_f32(t)
+    batches = [rng.permutation(16)[:8].astype(np.int32) for _ in range(6)]
+    with M.UNet(seed=11) as a, M.UNet(seed=11) as b, M.UNet(seed=11) as q:
+        la, lb = [], []
+        for r in batches:
+            la.append(a.train_step(dev_bf16(x[r]), dev_f32(t[r]), 2e-3))
+            lb.append(b.train_step_rows(X, T, r, 2e-3, wait=True))
+            q.train_step_rows(X, T, r, 2e-3)  # queued, no wait
+        assert la == lb
+        s, n = q.take_loss()
+        assert n == len(batches) and abs(s - sum(la)) <= 1e-12 * abs(s)
+        assert np.array_equal(a.get_params(), b.get_params())
+        assert np.array_equal(a.get_params(), q.get_params())
+        assert q.take_loss() == (0.0, 0)
+        y1, y2 = DA((8, 32, 32, 32), np.float32), DA((8, 32, 32, 32), np.float32)
+        a.forward(dev_bf16(x[batches[0]]), y1)
+        a.forward_rows(X, batches[0], y2)
+        assert np.array_equal(y1.numpy(), y2.numpy())
+
+
 def _gather_grid(storage, seed=2):
     S, bsx = 128, 32
     nb = S // bsx
@@ -236,13 +271,12 @@ def test_gather_windows_match_halo_oracle(storage):
     g, pm, L = _gather_grid(st)
     with g:
         bids = np.nonzero(pm >= 0)[0].astype(np.int32)
-        x = torch.empty((bids.size, 36, 36, 36), dtype=torch.bfloat16, device=CUDA)
+        x = DA((bids.size, 36, 36, 36), np.uint16)
         M.UNet.gather(g, 40, bids, x)
-        got = x.view(torch.int16).cpu().numpy().view(np.uint16)
+        got = x.numpy()
         for i, bid in enumerate(bids):
             h = OL.halo_fill(L, int(bid), 2, OL.OPT_BC_NEUMANN)
-            want = torch.from_numpy(h).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
-            assert np.array_equal(got[i], want), bid
+            assert np.array_equal(got[i], bf16_bits(h)), bid
 
 
 @pytest.mark.parametrize("storage", ["f32", "u16"])
@@ -253,7 +287,7 @@ def test_scatter_adds_clamped(storage):
     with g:
         bids = np.nonzero(pm >= 0)[0][::2].astype(np.int32)
         before = g.download(40).copy()
-        ext = torch.empty((bids.size, 32, 32, 32), dtype=torch.float32, device=CUDA)
+        ext = DA((bids.size, 32, 32, 32), np.float32)
         M.UNet.extract(g, 40, bids, ext)
         dec = before[pm[bids]].astype(np.float32)
         if st == M.U16:
@@ -286,9 +320,9 @@ def test_warm_start_matches_gather_forward_scatter():
         net.set_params(_params(rng, net.nparam))
         net.warm_start(g1, 40, batch=5)
         bids = np.nonzero(pm >= 0)[0].astype(np.int32)
-        x = torch.empty((bids.size, 36, 36, 36), dtype=torch.bfloat16, device=CUDA)
+        x = DA((bids.size, 36, 36, 36), np.uint16)
         M.UNet.gather(g2, 40, bids, x)
-        y = torch.empty((bids.size, 32, 32, 32), dtype=torch.float32, device=CUDA)
+        y = DA((bids.size, 32, 32, 32), np.float32)
         net.forward(x, y)
         M.UNet.scatter(g2, 40, bids, y)
         assert np.array_equal(g1.download(40), g2.download(40))
