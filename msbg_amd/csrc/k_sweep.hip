@@ -68,9 +68,14 @@ int launch_one(hipStream_t s, const SweepGrid *d_grids, const int2 *d_items, int
   constexpr int B = Pack<NT>::BPW;
   for (int64_t off = 0; off < n; off += 0x40000000) {
     const int cnt = (int)(n - off < 0x40000000 ? n - off : 0x40000000);
-    hipLaunchKernelGGL((k_sweep<BSX, ST, EXACT, PX, PY, OP_MCF, false, B, C01>),
-                       dim3((unsigned)((cnt + B - 1) / B)), dim3(NT * B), 0, s, d_grids,
-                       d_items + off, d_partial ? d_partial + off : nullptr, iter, cnt);
+    if (d_partial)
+      hipLaunchKernelGGL((k_sweep<BSX, ST, EXACT, PX, PY, OP_MCF, false, B, C01, true>),
+                         dim3((unsigned)((cnt + B - 1) / B)), dim3(NT * B), 0, s, d_grids,
+                         d_items + off, d_partial + off, iter, cnt);
+    else
+      hipLaunchKernelGGL((k_sweep<BSX, ST, EXACT, PX, PY, OP_MCF, false, B, C01, false>),
+                         dim3((unsigned)((cnt + B - 1) / B)), dim3(NT * B), 0, s, d_grids,
+                         d_items + off, nullptr, iter, cnt);
     MSBG_HIP(hipGetLastError());
   }
   return 0;

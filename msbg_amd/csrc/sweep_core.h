@@ -136,21 +136,107 @@ struct Patch<_Float16, PX, PY> {
 };
 
 // a + b and a - b of two patch values, one rounding in FP32. FP16 operands
-// go into v_fma_mix_f32 as a * one + b, each widened exactly inside the
-// instruction; `one` is 1.0 in a register the compiler does not look into,
-// so that it keeps the fma form instead of an add of two conversions.
-__device__ inline float vadd(float a, float b, float) { return a + b; }
-__device__ inline float vsub(float a, float b, float) { return a - b; }
-__device__ inline float vadd(_Float16 a, _Float16 b, float one) {
-  return __builtin_fmaf((float)a, one, (float)b);
+// go straight into v_fma_mix_f32 as a * 1.0 + (+-b), each widened exactly
+// inside the instruction. The instruction is written out because the
+// compiler, seeing one FP16 value used by several sums, converts it once and
+// then spends a separate v_cvt_f32_f16 on every value of the patch.
+__device__ inline float vadd(float a, float b) { return a + b; }
+__device__ inline float vsub(float a, float b) { return a - b; }
+
+// a + s * b for the FP16 halves ha / hb of the words a / b (s = +-1)
+#define MSBG_MIX(SIGN, OPSEL)                                                        \
+  asm("v_fma_mix_f32 %0, %1, 1.0, " SIGN "%2 " OPSEL " op_sel_hi:[1,0,1]" : "=v"(r) \
+      : "v"(a), "v"(b))
+__device__ inline float mix_add(uint32_t a, int ha, uint32_t b, int hb) {
+  float r;
+  if (ha == 0 && hb == 0) MSBG_MIX("", "");
+  else if (ha == 1 && hb == 0) MSBG_MIX("", "op_sel:[1,0,0]");
+  else if (ha == 0 && hb == 1) MSBG_MIX("", "op_sel:[0,0,1]");
+  else MSBG_MIX("", "op_sel:[1,0,1]");
+  return r;
 }
-__device__ inline float vsub(_Float16 a, _Float16 b, float one) {
-  return __builtin_fmaf((float)a, one, -(float)b);
+__device__ inline float mix_sub(uint32_t a, int ha, uint32_t b, int hb) {
+  float r;
+  if (ha == 0 && hb == 0) MSBG_MIX("-", "");
+  else if (ha == 1 && hb == 0) MSBG_MIX("-", "op_sel:[1,0,0]");
+  else if (ha == 0 && hb == 1) MSBG_MIX("-", "op_sel:[0,0,1]");
+  else MSBG_MIX("-", "op_sel:[1,0,1]");
+  return r;
 }
-__device__ inline float opaque_one() {
-  float one;
-  asm("s_mov_b32 %0, 1.0" : "=s"(one));
-  return one;
+#undef MSBG_MIX
+
+// f0 * -4 + w and dt * m + f0 (clamped to [0, 1] when C01) with f0 the FP16
+// half h of the word a
+__device__ inline float mix_m4(uint32_t a, int h, float w) {
+  float r;
+  if (h == 0)
+    asm("v_fma_mix_f32 %0, %1, -4.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(a), "v"(w));
+  else
+    asm("v_fma_mix_f32 %0, %1, -4.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r)
+        : "v"(a), "v"(w));
+  return r;
+}
+template <bool C01>
+__device__ inline float mix_step(float dt, float m, uint32_t a, int h) {
+  float r;
+  if (C01) {
+    if (h == 0)
+      asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,0,1] clamp" : "=v"(r)
+          : "v"(dt), "v"(m), "v"(a));
+    else
+      asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[0,0,1] op_sel_hi:[0,0,1] clamp" : "=v"(r)
+          : "v"(dt), "v"(m), "v"(a));
+  } else {
+    if (h == 0)
+      asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,0,1]" : "=v"(r) : "v"(dt), "v"(m), "v"(a));
+    else
+      asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "=v"(r)
+          : "v"(dt), "v"(m), "v"(a));
+  }
+  return r;
+}
+
+// sums and differences of patch values A(ja, ia) and B(jb, ib)
+template <typename L, int PX, int PY>
+__device__ inline float pdiff(const Patch<L, PX, PY> &A, int ja, int ia,
+                              const Patch<L, PX, PY> &B, int jb, int ib) {
+  return vsub((float)A.at(ja, ia), (float)B.at(jb, ib));
+}
+template <typename L, int PX, int PY>
+__device__ inline float psum(const Patch<L, PX, PY> &A, int ja, int ia,
+                             const Patch<L, PX, PY> &B, int jb, int ib) {
+  return vadd((float)A.at(ja, ia), (float)B.at(jb, ib));
+}
+template <int PX, int PY>
+__device__ inline float pdiff(const Patch<_Float16, PX, PY> &A, int ja, int ia,
+                              const Patch<_Float16, PX, PY> &B, int jb, int ib) {
+  return mix_sub(A.w[ja][ia >> 1], ia & 1, B.w[jb][ib >> 1], ib & 1);
+}
+template <int PX, int PY>
+__device__ inline float psum(const Patch<_Float16, PX, PY> &A, int ja, int ia,
+                             const Patch<_Float16, PX, PY> &B, int jb, int ib) {
+  return mix_add(A.w[ja][ia >> 1], ia & 1, B.w[jb][ib >> 1], ib & 1);
+}
+// w - 4 f0 and the limited step dt * m + f0 for f0 = M(j, i)
+template <typename L, int PX, int PY>
+__device__ inline float pm4(const Patch<L, PX, PY> &M, int j, int i, float w) {
+  return __builtin_fmaf((float)M.at(j, i), -4.0f, w);
+}
+template <int PX, int PY>
+__device__ inline float pm4(const Patch<_Float16, PX, PY> &M, int j, int i, float w) {
+  return mix_m4(M.w[j][i >> 1], i & 1, w);
+}
+template <bool C01, typename L, int PX, int PY>
+__device__ inline float pstep(const Patch<L, PX, PY> &M, int j, int i, float dt, float m,
+                              float lo, float hi) {
+  const float phi = __builtin_fmaf(dt, m, (float)M.at(j, i));
+  return C01 ? __builtin_amdgcn_fmed3f(phi, 0.0f, 1.0f) : __builtin_amdgcn_fmed3f(phi, lo, hi);
+}
+template <bool C01, int PX, int PY>
+__device__ inline float pstep(const Patch<_Float16, PX, PY> &M, int j, int i, float dt, float m,
+                              float lo, float hi) {
+  const float phi = mix_step<C01>(dt, m, M.w[j][i >> 1], i & 1);
+  return C01 ? phi : __builtin_amdgcn_fmed3f(phi, lo, hi);
 }
 
 // Fast evaluation of the reference's update (msbg4.cpp:559-637) for a PX x PY
@@ -166,35 +252,35 @@ __device__ inline float opaque_one() {
 // dt * clamp(u, +-0.1/dt) (lim = 0.1/dt), so that the last step is one fma
 // onto f0 whose [0,1] clamp is the instruction's clamp bit.
 template <int PX, int PY, bool C01, typename PT>
-__device__ inline void mcf_fast(const PT &D, const PT &M, const PT &U, float one, float dt,
-                                    float lim, float lo, float hi, float *out) {
+__device__ inline void mcf_fast(const PT &D, const PT &M, const PT &U, float dt, float lim,
+                                float lo, float hi, float *out) {
   constexpr int W = PX + 2;
   float colY[PY][W], dzc[PY + 2][W];
 #pragma unroll
   for (int b = 0; b < PY; b++)
 #pragma unroll
-    for (int i = 0; i < W; i++) colY[b][i] = vsub(M.at(b + 2, i), M.at(b, i), one);
+    for (int i = 0; i < W; i++) colY[b][i] = pdiff(M, b + 2, i, M, b, i);
 #pragma unroll
   for (int j = 0; j < PY + 2; j++)
 #pragma unroll
     for (int i = 0; i < W; i++)
-      if ((j >= 1 && j <= PY) || (i >= 1 && i <= PX)) dzc[j][i] = vsub(U.at(j, i), D.at(j, i), one);
+      if ((j >= 1 && j <= PY) || (i >= 1 && i <= PX)) dzc[j][i] = pdiff(U, j, i, D, j, i);
 #pragma unroll
   for (int b = 0; b < PY; b++)
 #pragma unroll
     for (int a = 0; a < PX; a++) {
       const int i = a + 1, j = b + 1;
-      const auto f0 = M.at(j, i);
-      const float sxx = vadd(M.at(j, i - 1), M.at(j, i + 1), one);
-      const float syy = vadd(M.at(j - 1, i), M.at(j + 1, i), one);
-      const float szz = vadd(D.at(j, i), U.at(j, i), one);
-      const float dx = vsub(M.at(j, i + 1), M.at(j, i - 1), one), dy = colY[b][i], dz = dzc[j][i];
+      const float sxx = psum(M, j, i - 1, M, j, i + 1);
+      const float syy = psum(M, j - 1, i, M, j + 1, i);
+      const float szz = psum(D, j, i, U, j, i);
+      const float dx = pdiff(M, j, i + 1, M, j, i - 1), dy = colY[b][i], dz = dzc[j][i];
       const float cyx = colY[b][i + 1] - colY[b][i - 1];
       const float czx = dzc[j][i + 1] - dzc[j][i - 1];
       const float czy = dzc[j + 1][i] - dzc[j - 1][i];
-      const float dz2 = dz * dz, dx2 = dx * dx;
-      const float A = __builtin_fmaf(dy, dy, dz2), B = dx2 + dz2, Cq = __builtin_fmaf(dy, dy, dx2);
-      const float G = A + dx2;
+      // five operations for A, B, C and G: C = G - dz^2
+      const float dz2 = dz * dz;
+      const float A = __builtin_fmaf(dy, dy, dz2), B = __builtin_fmaf(dx, dx, dz2);
+      const float G = __builtin_fmaf(dx, dx, A), Cq = G - dz2;
       float p1 = dy * cyx;
       p1 = __builtin_fmaf(dz, czx, p1);
       float t = (dy * dz) * czy;
@@ -204,12 +290,10 @@ __device__ inline void mcf_fast(const PT &D, const PT &M, const PT &U, float one
       Np = __builtin_fmaf(Cq, szz, Np);
       Np = __builtin_fmaf(-0.5f, t, Np);
       const float r = __builtin_amdgcn_rcpf(G);
-      float u = __builtin_fmaf((float)f0, -4.0f, Np * r);
+      float u = pm4(M, j, i, Np * r);
       u = G > 4.0f * 1e-7f ? u : 0.0f;
       const float m = __builtin_amdgcn_fmed3f(u, -lim, lim);
-      const float phi = __builtin_fmaf(dt, m, (float)f0);
-      out[b * PX + a] = C01 ? __builtin_amdgcn_fmed3f(phi, 0.0f, 1.0f)
-                            : __builtin_amdgcn_fmed3f(phi, lo, hi);
+      out[b * PX + a] = pstep<C01>(M, j, i, dt, m, lo, hi);
     }
 }
 
@@ -340,9 +424,6 @@ template <typename R>
 struct Src {
   const R *m, *p0, *p;
   int stride;
-  __device__ inline const R *at(int zz, int n) const {
-    return zz < 0 ? m : (zz >= n ? p : p0 + zz * stride);
-  }
 };
 
 // Voxel column per thread (PX x PY): the fast default is 4 x 2 for 32^3
@@ -372,7 +453,9 @@ struct Pack {
 // nItems: the number of items of this launch (the last workgroup may hold
 // fewer than BPW); C01 (fast path only): every grid of the launch clamps to
 // [0,1] (doConstrZeroOne), which the last fma of the update does with its
-// clamp bit.
+// clamp bit; RES: the launch may accumulate the FP64 residual (partial !=
+// nullptr). Launches without a residual take RES = false, which leaves the
+// residual's conversions and branches out of the z-loop.
 //
 // The fast MCF path (OP_MCF, not EXACT, not HARD, FP32 arithmetic) on FP16
 // pages keeps the planes in LDS and the patches in registers as FP16 bits
@@ -381,10 +464,11 @@ struct Pack {
 // decoded patches, which keeps three waves per SIMD, at the price of sums and
 // differences in v_fma_mix_f32 instead of plain FP32 adds.
 template <int BSX, int ST, bool EXACT, int PX, int PY, int OP = OP_MCF, bool HARD = false,
-          int BPW = 1, bool C01 = false>
+          int BPW = 1, bool C01 = false, bool RES = true>
 __global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
     const SweepGrid *__restrict__ grids, const int2 *__restrict__ items,
-    double *__restrict__ partial, int iter, int nItems) {
+    double *__restrict__ partialArg, int iter, int nItems) {
+  double *__restrict__ const partial = RES ? partialArg : nullptr;
   using S = Stor<ST>;
   using R = typename S::raw;
   using C = typename S::ct;
@@ -423,10 +507,10 @@ __global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
   const int64_t npages = G.npages;
   const float dtf = G.dt, lo01 = G.lo, hi01 = G.hi;
 
-  __shared__ __align__(16) L bufs[BPW][2][PL];
+  __shared__ __align__(16) L bufs[BPW][3][PL];
   __shared__ int codesAll[BPW][27];
   __shared__ double red[(NT + 63) / 64];
-  L(&buf)[2][PL] = bufs[sub];
+  L(&buf)[3][PL] = bufs[sub];
   int *codes = codesAll[sub];
 
   const int page = item.y;
@@ -562,6 +646,10 @@ __global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
     rs[q].p = one(2, BSX, &sdummy);
     rs[q].stride = st0;
   }
+  // ring sources of the next interior plane, advanced by one plane per load
+  const R *rcur[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; q++) rcur[q] = rs[q].p0;
 
   // one plane's values held between its load and its LDS store: decoded,
   // or the FP16 bits (two per register) on the FP16 fast path
@@ -574,12 +662,12 @@ __global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
   // chunk k of this thread: 4 consecutive x of one row, e = y * BSX + x
   auto chunk_off = [&](int k) -> int { return (tid + k * NT) * 4; };
 
-  auto load_plane = [&](int zz, Staged &st) {
+  // plane zz of the block (0 <= zz < BSX, loaded in order): page rows from
+  // the block's own page at a uniform base, ring cells at their cursors
+  auto load_inner = [&](int zz, Staged &st) {
 #pragma unroll
     for (int k = 0; k < LPT; k++) {
-      const int e = chunk_off(k);
-      const R *src = zz < 0 ? zsrcM + (constM ? 0 : e)
-                            : (zz >= BSX ? zsrcP + (constP ? 0 : e) : own + zz * PS + e);
+      const R *src = own + zz * PS + chunk_off(k);
       if constexpr (H16) {
         const uint64_t u = *(const MSBG_GLOBAL uint64_t *)src;
         st.in[2 * k] = (uint32_t)u;
@@ -594,9 +682,37 @@ __global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
       if constexpr (H16)
-        st.ring[q] = ldg(rs[q].at(zz, BSX));
+        st.ring[q] = ldg(rcur[q]);
       else
-        st.ring[q] = dec(ldg(rs[q].at(zz, BSX)));
+        st.ring[q] = dec(ldg(rcur[q]));
+      rcur[q] += rs[q].stride;
+    }
+  };
+  // the plane below (top = false) or above (top = true) the block
+  auto load_outer = [&](bool top, Staged &st) {
+    const R *base = top ? zsrcP : zsrcM;
+    const bool cst = top ? constP : constM;
+#pragma unroll
+    for (int k = 0; k < LPT; k++) {
+      const R *src = base + (cst ? 0 : chunk_off(k));
+      if constexpr (H16) {
+        const uint64_t u = *(const MSBG_GLOBAL uint64_t *)src;
+        st.in[2 * k] = (uint32_t)u;
+        st.in[2 * k + 1] = (uint32_t)(u >> 32);
+      } else {
+        R r[4];
+        load4(src, r);
+#pragma unroll
+        for (int u = 0; u < 4; u++) st.in[k * 4 + u] = dec(r[u]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const R *src = top ? rs[q].p : rs[q].m;
+      if constexpr (H16)
+        st.ring[q] = ldg(src);
+      else
+        st.ring[q] = dec(ldg(src));
     }
   };
 
@@ -653,7 +769,6 @@ __global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
 
   double acc = 0.0;
   const bool wFace0 = x0 == 0, wFace1 = x0 == BSX - PX, wY0 = y0 == 0, wY1 = y0 == BSX - PY;
-  const float fone = FAST ? opaque_one() : 1.0f;
   const float lim = FAST ? 0.1f / dtf : 0.0f;  // limiter bound of the fast path, in units of dt
 
   // encode and store the new values of plane z, keep the side-cars equal to
@@ -713,7 +828,7 @@ __global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
     if (BPW > 1 && !active) return;
     C o[PX * PY];
     if constexpr (FAST) {
-      mcf_fast<PX, PY, C01>(Dp, Mp, Up, fone, dtf, lim, lo01, hi01, o);
+      mcf_fast<PX, PY, C01>(Dp, Mp, Up, dtf, lim, lo01, hi01, o);
     } else if constexpr (OP == OP_MCF) {
       const bool c01x = !HARD && lo01 == 0.0f;
 #pragma unroll
@@ -758,30 +873,50 @@ __global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
   PT pA, pB, pC;
   Staged s0;
 
+  // Plane p of the march (-1..BSX) is staged in buf[(p + 3) % 3], so that
+  // every step of the three-step loop below addresses a fixed buffer.
   // prologue: planes -1 and 0 into pA, pB; plane 1 prefetched
-  load_plane(-1, s0);
-  put_plane(buf[1], s0);
-  load_plane(0, s0);
+  load_outer(false, s0);
+  put_plane(buf[2], s0);
+  load_inner(0, s0);
   __syncthreads();
-  get_patch(buf[1], pA);
+  get_patch(buf[2], pA);
   put_plane(buf[0], s0);
-  load_plane(1, s0);
+  load_inner(1, s0);
   __syncthreads();
   get_patch(buf[0], pB);
 
-  // step z: stage plane z+1 (prefetched), prefetch z+2, compute plane z
-  auto step = [&](int z, const PT &Dp, const PT &Mp, PT &Up) {
-    put_plane(buf[(z + 1) & 1], s0);
-    if (z + 2 <= BSX) load_plane(z + 2, s0);
+  // step z: stage plane z+1 (prefetched) into buf[nb], prefetch plane z+2
+  // (ld 1: a plane of the block, 2: the plane above it, 0: none), compute
+  // plane z. Plane z+1 overwrites plane z-2, whose patch was read two
+  // barriers earlier.
+  auto step = [&](int z, int nb, int ld, const PT &Dp, const PT &Mp, PT &Up) {
+    put_plane(buf[nb], s0);
+    if (ld == 1)
+      load_inner(z + 2, s0);
+    else if (ld == 2)
+      load_outer(true, s0);
     __syncthreads();
-    get_patch(buf[(z + 1) & 1], Up);
+    get_patch(buf[nb], Up);
     compute(z, Dp, Mp, Up);
   };
 
-  for (int z = 0; z < BSX; z += 3) {
-    step(z, pA, pB, pC);
-    if (z + 1 < BSX) step(z + 1, pB, pC, pA);
-    if (z + 2 < BSX) step(z + 2, pC, pA, pB);
+  constexpr int ZM = ((BSX - 2) / 3) * 3;  // steps whose prefetch is a plane of the block
+  for (int z = 0; z < ZM; z += 3) {
+    step(z, 1, 1, pA, pB, pC);
+    step(z + 1, 2, 1, pB, pC, pA);
+    step(z + 2, 0, 1, pC, pA, pB);
+  }
+#pragma unroll
+  for (int t = 0; t < BSX - ZM; t++) {  // the last 2..4 steps
+    const int z = ZM + t;
+    const int ld = z + 2 < BSX ? 1 : (z + 2 == BSX ? 2 : 0);
+    if (t % 3 == 0)
+      step(z, 1, ld, pA, pB, pC);
+    else if (t % 3 == 1)
+      step(z, 2, ld, pB, pC, pA);
+    else
+      step(z, 0, ld, pC, pA, pB);
   }
 
   if (partial) {
