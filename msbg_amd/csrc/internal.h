@@ -179,6 +179,13 @@ struct Channel {
 // grid's topology version and the block list are those of `key`.
 struct LevelWork {
   uint64_t key = 0;
+  // what the key was computed from: the topology version and the block list
+  // (has_list false: every value block of the level), compared in full
+  // before the set-up is reused, so that a hash collision cannot hand one
+  // list's set-up to another
+  uint64_t topo = 0;
+  bool has_list = false;
+  std::vector<int32_t> list;
   ChanLevel wl;
   DevBuf<int32_t> d_ghost;
   DevBuf<uint32_t> d_gmask;  // per ghost page: bit (dz+1)*9+(dy+1)*3+dx+1 = swept block at +d

@@ -72,14 +72,29 @@ int msbg_apply_channel_pde_level(msbg_grid *g, int chan, int level, unsigned hal
     for (int64_t i = 0; i < nList; i++) mix((uint32_t)blockList[i]);
   const uint64_t slot = ((uint64_t)(uint32_t)chan << 8) | ((uint64_t)level << 4) |
                         (coarse ? ((haloOpt & MSBG_OPT_INTERPOLATE) ? 2u : 1u) : 0u);
+  // the cached set-up is reused only if its topology version and block
+  // list equal this call's, not just their hash
+  auto same_inputs = [&](const LevelWork &w) -> bool {
+    if (w.key != key || w.topo != g->topo_version || w.has_list != (blockList != nullptr))
+      return false;
+    if (!blockList) return true;
+    if ((int64_t)w.list.size() != nList) return false;
+    for (int64_t i = 0; i < nList; i++)
+      if (w.list[(size_t)i] != blockList[i]) return false;
+    return true;
+  };
   {
     // a stale set-up is destroyed in place (its device buffers freed)
     auto itw = g->level_work.find(slot);
-    if (itw != g->level_work.end() && itw->second.key != key) g->level_work.erase(itw);
+    if (itw != g->level_work.end() && !same_inputs(itw->second)) g->level_work.erase(itw);
   }
   LevelWork &W = g->level_work[slot];
   ChanLevel &wl = W.wl;
-  if (W.key != key) {
+  if (!same_inputs(W)) {
+    W.topo = g->topo_version;
+    W.has_list = blockList != nullptr;
+    W.list.clear();
+    if (blockList) W.list.assign(blockList, blockList + nList);
     // swept blocks: the list's (or every) value block at this level, by colour
     std::vector<int32_t> per[8];
     std::vector<int32_t> swept;

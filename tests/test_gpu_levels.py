@@ -21,6 +21,13 @@ def note(*a):
     sys.stdout.flush()
 
 
+def _raw(storage, vals):
+    """float32 values in [0, 1] as the channel's raw elements"""
+    return {M.U16: lambda v: np.rint(v * 65535).astype(np.uint16),
+            M.F32: lambda v: v, M.F16: lambda v: v.astype(np.float16),
+            M.BF16: lambda v: (v.view(np.uint32) >> 16).astype(np.uint16)}[storage](vals)
+
+
 def _grid(S, bsx, storage, seed, ghosts):
     """Three levels; every level holds its own blocks as value pages and
     NULL elsewhere, then (float storages) prepareGhostBlocks adds the ghost
@@ -46,10 +53,7 @@ def _grid(S, bsx, storage, seed, ghosts):
             c = rng.uniform(0, b, 3)
             r = np.sqrt((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2)
             vals[p] = np.clip(0.5 + 0.45 * np.cos(r / rng.uniform(1.5, 4)), 0, 1)
-        raw = {M.U16: lambda v: np.rint(v * 65535).astype(np.uint16),
-               M.F32: lambda v: v, M.F16: lambda v: v.astype(np.float16),
-               M.BF16: lambda v: (v.view(np.uint32) >> 16).astype(np.uint16)}[storage](vals)
-        g.upload(CH, level, raw)
+        g.upload(CH, level, _raw(storage, vals))
     if ghosts:
         g.prepareGhostBlocks(CH)
     return g, levels, empty, full
@@ -187,3 +191,35 @@ def test_level_sweep_cached_setup_follows_changes():
         O.mcf_sweeps_level(L[1], L[2], b1, levels, 1, opt, 2, 0.1, 1)
         g.applyChannelPdeLevel(CH, 1, opt, None, 2, 0.1, 1)
         assert np.array_equal(g.download(CH, 1), L[1].pages)
+
+
+@pytest.mark.parametrize("S,bsx", [(128, 32), (64, 16)])
+@pytest.mark.parametrize("storage", [M.F32, M.F16, M.BF16, M.U16])
+def test_level_sweep_carried_sidecars_interpolate(storage, S, bsx):
+    """Three COARSE_LEVEL|INTERPOLATE calls in a row on one cached set-up,
+    only the coarser level changed between them: from the second call on the
+    level's own side-cars carry over and the ghost fill rewrites the ghost
+    pages' side-car entries. Every call equals the oracle bit for bit, in
+    every storage, for 16^3 and 8^3 pages of level 1."""
+    note("levels carried side-cars", storage, S, bsx)
+    ghosts = storage != M.U16  # prepareGhostBlocks downsamples float storages
+    g, levels, empty, full = _grid(S, bsx, storage, seed=21, ghosts=ghosts)
+    with g:
+        g.set_arith(M.ARITH_EXACT)
+        opt = M.OPT_BC_COARSE_LEVEL | M.OPT_INTERPOLATE
+        b1 = np.nonzero(levels == 1)[0]
+        assert len(b1)
+        L = _olevels(g, S, bsx, storage, empty, full)
+        rng = np.random.default_rng(5)
+        for call in range(3):
+            if call:
+                shape = g.download(CH, 2).shape
+                new2 = _raw(storage, rng.random(shape).astype(np.float32))
+                g.upload(CH, 2, new2)
+                L[2].pages[...] = new2.view(np.uint16) if storage == M.F16 else new2
+            O.mcf_sweeps_level(L[1], L[2], b1, levels, 1, opt, 2, 0.1, 1)
+            g.applyChannelPdeLevel(CH, 1, opt, None, 2, 0.1, 1)
+            got = g.download(CH, 1)
+            if storage == M.F16:
+                got = got.view(np.uint16)
+            assert np.array_equal(got, L[1].pages), (storage, S, bsx, call)

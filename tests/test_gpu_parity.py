@@ -451,6 +451,31 @@ def test_sweep_deterministic_and_constant_field():
         assert np.all(g.download(M.CH_FLOAT_1) == np.float32(0.375))
 
 
+@pytest.mark.parametrize("storage", [M.F32, M.F16, M.U16])
+def test_sweep_repeatable_every_column_shape(storage):
+    """Fast sweeps of one field give the same bits every time, for every
+    voxel column (2x2, 4x2, 4x4: four, two and one waves per 32^3 block), so
+    that the LDS planes a workgroup's waves share are never overwritten
+    before every wave has read them (a race shows as run-to-run differences
+    after many sweeps)."""
+    S, bsx = 256, 32
+    xyz = _scene(S, seed=5, nObj=12, nPts=8000)
+    r = gpu_pipeline(S, bsx, xyz, storage=storage, do_sweep=False)
+    g, active, splat = r["grid"], r["active"], r["splat"]
+    try:
+        for tile in (0, 1, 2):
+            note("repeatable", storage, tile)
+            g.set_tile(tile)
+            runs = []
+            for _ in range(3):
+                g.upload(30, 0, splat)
+                g.applyChannelPdeFast(30, 0, 0, active, M.MCF_8COLOR, 1, 0, 0, 0, 40, 1.0, 0.1)
+                runs.append(g.download(30).view(np.uint8).copy())
+            assert np.array_equal(runs[0], runs[1]) and np.array_equal(runs[0], runs[2]), tile
+    finally:
+        g.destroy()
+
+
 # --------------------------------------------------------------------------
 # halo fill (test hook) and cross-level ghosts
 # --------------------------------------------------------------------------
