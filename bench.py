@@ -58,7 +58,7 @@ STORAGES = {"u16": 1, "f32": 2, "f16": 3, "bf16": 4, "f64": 5}
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE on this workload, each
 # scaled by the same counter's reading for k_copy16 (a copy of known size:
 # FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); profiles/k_sweep_cfg2_f16.md
-TRAFFIC_RATIO = {("cfg2", "f16", "fast"): 1.163}
+TRAFFIC_RATIO = {("cfg2", "f16", "fast"): 1.168}
 
 
 def log(*a):
