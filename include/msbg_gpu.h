@@ -419,29 +419,47 @@ int msbg_render_counts(msbg_grid *g, int chan, int width, int height,
 
 /* A batch runs applyChannelPdeFast's 8-colour MCF sweep (msbg4.cpp:60-957,
  * same per-grid semantics as msbg_apply_channel_pde_fast over every level-0
- * value block) on many independent grids at once: one launch per colour pass
- * and kernel group (block size x storage x arithmetic), the whole run
- * captured into one HIP graph. No counterpart in the reference, which
- * sweeps one grid per call. Member grids must outlive the batch and stay
- * idle while it runs; a change to a member's blocks or pages is picked up
- * by the next call. */
+ * value block) on many independent grids at once. Grids are grouped by
+ * kernel (block size x storage x arithmetic). A group runs either as one
+ * launch per colour pass covering every grid's blocks of that colour, or as
+ * one persistent launch that takes (sweep, colour, grid, block) items from a
+ * work queue and waits per grid, not per launch, for the grid's previous
+ * colour pass (block sizes 16 and 32). The whole run is captured into one
+ * HIP graph. No counterpart in the reference, which sweeps one grid per
+ * call. Member grids must outlive the batch and stay idle while it runs; a
+ * change to a member's blocks or pages is picked up by the next call. */
 typedef struct msbg_batch msbg_batch;
 int msbg_batch_create(msbg_batch **out);
 int msbg_batch_destroy(msbg_batch *b);
 int msbg_batch_add(msbg_batch *b, msbg_grid *g, int chan, float timeStep,
                    int doConstrZeroOne);
 /* Time each voxel-column variant of each kernel group on the group's own
- * blocks (`passes` launches of every colour; the pages are restored after)
- * and keep the fastest; member grids adopt it for their own sweeps. */
+ * blocks (`passes` launches of every colour), then the persistent launch
+ * with each column shape that fits one block per workgroup (`passes`
+ * sweeps), and keep the fastest mapping; the pages are restored after.
+ * Member grids adopt the per-pass winner for their own sweeps. */
 int msbg_batch_autotune(msbg_batch *b, int passes);
 /* Kernel groups of the batch and their variant (0 = 2x2, 1 = 4x2, 2 = 4x4
  * voxel columns). */
 int msbg_batch_get_tiles(msbg_batch *b, int *bsx, int *storage, int *tile,
                          int cap, int *n);
-/* numIter rounded up to even sweeps of every member grid; useGraph != 0
- * replays one captured HIP graph (captured on first use per sweep count).
- * *ms receives the device time of the run. */
+/* ... and per group whether the tuned run uses the persistent launch (1) or
+ * one launch per colour pass (0). */
+int msbg_batch_get_modes(msbg_batch *b, int *persistent, int cap, int *n);
+#define MSBG_BATCH_EAGER 0      /* one launch per colour pass and group, no graph */
+#define MSBG_BATCH_GRAPH 1      /* one graph launch; each group as tuned (per pass by default) */
+#define MSBG_BATCH_PERSISTENT 2 /* one graph launch; every group of block size 16 or 32 as
+                                   one persistent launch */
+/* numIter rounded up to even sweeps of every member grid, in mode useGraph
+ * (MSBG_BATCH_*). Graphs are captured on first use per sweep count and mode.
+ * *ms receives the device time of the run. A persistent run whose dependency
+ * waits ran past their bound fails with MSBG_EDEVICE (its results are then
+ * not to be used). */
 int msbg_batch_run(msbg_batch *b, int numIter, int useGraph, double *ms);
+/* Wall-clock bound of one dependency wait of a persistent run, in
+ * microseconds (default 2 000 000). A test hook: 0 makes every wait give up
+ * at once, which exercises the failure report without a long wait. */
+int msbg_batch_set_wait_limit(msbg_batch *b, unsigned microseconds);
 /* Voxel-column variant used by this grid's own sweeps (-1 = default). */
 int msbg_set_tile(msbg_grid *g, int tile);
 

@@ -101,6 +101,8 @@ SIGNATURES = {
     "msbg_batch_autotune": (_I, [_P, _I]),
     "msbg_batch_get_tiles": (_I, [_P, _P, _P, _P, _I, _P]),
     "msbg_batch_run": (_I, [_P, _I, _I, _P]),
+    "msbg_batch_get_modes": (_I, [_P, _P, _I, _P]),
+    "msbg_batch_set_wait_limit": (_I, [_P, C.c_uint]),
     "msbg_set_tile": (_I, [_P, _I]),
     "msbg_channel_diff": (_I, [_P, _I, _I, _I, _P, _P]),
     "msbg_interpolate": (_I, [_P, _I, _I, _I, _P, _I64, _P, _P]),

@@ -1,10 +1,12 @@
-// Batched sweeps of many independent grids: one launch per (colour pass,
-// kernel group) covers the blocks of that colour in every grid of the group,
-// and the whole run (sweeps x 8 colours x groups) is captured once into a
-// HIP graph and replayed as a single graph launch. Grids are grouped by
-// block size, storage format and arithmetic mode (one kernel instantiation
-// each); the autotuner picks the voxel-column variant of each group by
-// timing it on the group's own blocks.
+// Batched sweeps of many independent grids, grouped by block size, storage
+// format and arithmetic mode (one kernel instantiation each). A group runs
+// either as one launch per colour pass covering the blocks of that colour in
+// every grid of the group, or as one persistent launch whose workgroups take
+// (sweep, colour, grid, block) items from a work queue and wait per grid for
+// its previous colour pass (k_sweep_queue). The whole run is captured once
+// into a HIP graph and replayed as a single graph launch. The autotuner picks
+// each group's mapping (per-pass launches or the persistent launch) and its
+// voxel-column variant by timing them on the group's own blocks.
 //
 // Each grid keeps the reference's per-grid semantics (colours 0..7 in order,
 // level-0 blocks, Neumann halos, numIter rounded up to even, msbg4.cpp:60-957);
@@ -34,6 +36,7 @@ struct msbg_batch {
     bool exact = false;
     bool c01 = false;  // every member clamps to [0,1]
     int tile = -1;
+    bool persistent = false;  // tuned: the persistent launch
     std::vector<int> grids;  // entry indices
     DevBuf<int2> items;
     int64_t off[9] = {0};
@@ -43,11 +46,18 @@ struct msbg_batch {
   std::vector<SweepGrid> hdesc;   // descriptors as built (host copy)
   std::vector<uint64_t> topo;     // each grid's topology version at build
   DevBuf<SweepGrid> d_desc;
+  // persistent runs: per entry its item counts before each colour ([e][9],
+  // [e][8] = all), and the queue state: done counts per entry, the head of
+  // each group, the failure flag
+  DevBuf<int> d_pre;
+  DevBuf<unsigned> d_qstate;
+  uint64_t wait_us = 2000000;  // bound of one dependency wait
   hipStream_t stream = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_sweeps = -1;
+  int graph_mode = -1;
   bool built = false;
   double last_ms = 0;
 };
@@ -60,6 +70,16 @@ void drop_graph(msbg_batch *b) {
   b->exec = nullptr;
   b->graph = nullptr;
   b->graph_sweeps = -1;
+  b->graph_mode = -1;
+}
+
+bool queue_ok(const msbg_batch::Group &G) {
+  return (G.bsx == 16 || G.bsx == 32) && G.storage != MSBG_STORAGE_F64;
+}
+
+// queue state: done counts (one line per entry), a head per group, the flag
+size_t qstate_words(size_t nEntries, size_t nGroups) {
+  return nEntries * kDoneStride + nGroups + 1;
 }
 
 // a member grid changed its block map, page pool or page contents since
@@ -139,6 +159,18 @@ int build(msbg_batch *b) {
                          hipMemcpyHostToDevice));
     }
   }
+  // per entry: its items of each colour, as prefix counts
+  std::vector<int> pre(b->entries.size() * 9, 0);
+  for (size_t k = 0; k < groups.size(); k++)
+    for (int c = 0; c < 8; c++)
+      for (const int2 &it : per[k][c]) pre[(size_t)it.x * 9 + c + 1]++;
+  for (size_t e = 0; e < b->entries.size(); e++)
+    for (int c = 0; c < 8; c++) pre[e * 9 + c + 1] += pre[e * 9 + c];
+  if (!pre.empty()) {
+    MSBG_TRY(b->d_pre.alloc(pre.size()));
+    MSBG_HIP(hipMemcpy(b->d_pre.p, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice));
+  }
+  MSBG_TRY(b->d_qstate.ensure(qstate_words(b->entries.size(), groups.size())));
   b->groups.swap(groups);
   b->hdesc = desc;
   b->topo.clear();
@@ -152,15 +184,61 @@ int build(msbg_batch *b) {
   return 0;
 }
 
-int enqueue(msbg_batch *b, int sweeps) {
+// whether group G runs persistently in mode `mode`
+bool runs_persistent(const msbg_batch::Group &G, int mode) {
+  if (!queue_ok(G)) return false;
+  return mode == MSBG_BATCH_PERSISTENT || (mode == MSBG_BATCH_GRAPH && G.persistent);
+}
+
+// the queue of group k for a run of `sweeps` sweeps
+BatchQueue queue_of(msbg_batch *b, size_t k, int sweeps) {
+  const auto &G = b->groups[k];
+  const size_t ne = b->entries.size();
+  BatchQueue q{};
+  q.items = G.items.p;
+  q.pre = b->d_pre.p;
+  q.done = b->d_qstate.p;
+  q.head = b->d_qstate.p + ne * kDoneStride + k;
+  q.fail = b->d_qstate.p + ne * kDoneStride + b->groups.size();
+  for (int c = 0; c < 9; c++) q.off[c] = (int)G.off[c];
+  q.sweeps = sweeps;
+  q.waitTicks = wall_ticks(b->wait_us);
+  return q;
+}
+
+int enqueue(msbg_batch *b, int sweeps, int mode) {
+  bool anyQ = false;
+  for (auto &G : b->groups) anyQ |= runs_persistent(G, mode) && G.off[8] > 0;
+  if (anyQ)
+    MSBG_TRY(launch_queue_reset(b->stream, b->d_qstate.p,
+                                (int)qstate_words(b->entries.size(), b->groups.size())));
+  for (size_t k = 0; k < b->groups.size(); k++) {
+    auto &G = b->groups[k];
+    if (!runs_persistent(G, mode) || G.off[8] == 0) continue;
+    MSBG_TRY(launch_sweep_queue(b->stream, G.bsx, G.storage, G.exact, G.c01, b->d_desc.p,
+                                queue_of(b, k, sweeps)));
+  }
   for (int it = 0; it < sweeps; it++)
     for (int pass = 0; pass < 8; pass++)
       for (auto &G : b->groups) {
+        if (runs_persistent(G, mode)) continue;
         int64_t n = G.off[pass + 1] - G.off[pass];
         if (!n) continue;
         MSBG_TRY(launch_sweep_items(b->stream, G.bsx, G.storage, G.exact, G.c01, G.tile, b->d_desc.p,
                                     G.items.p + G.off[pass], n, nullptr, it));
       }
+  return 0;
+}
+
+// after a run that held persistent launches: did a dependency wait give up?
+int check_queue(msbg_batch *b) {
+  unsigned fail = 0;
+  MSBG_HIP(hipMemcpy(&fail, b->d_qstate.p + b->entries.size() * kDoneStride + b->groups.size(),
+                     sizeof fail, hipMemcpyDeviceToHost));
+  MSBG_CHECK(!fail, MSBG_EDEVICE,
+             "persistent batch run: a dependency wait ran past its bound of %llu us; the "
+             "results of this run are not valid",
+             (unsigned long long)b->wait_us);
   return 0;
 }
 
@@ -259,10 +337,29 @@ int msbg_batch_autotune(msbg_batch *b, int passes) {
         bestMs = ms;
       }
     }
+    // the persistent launch over `passes` sweeps (one wave per block)
+    bool bestQ = false;
+    if (queue_ok(G)) {
+      const size_t k = (size_t)(&G - b->groups.data());
+      for (int rep = 0; rep < 2; rep++) {  // untimed, then timed
+        MSBG_TRY(launch_queue_reset(b->stream, b->d_qstate.p,
+                                    (int)qstate_words(b->entries.size(), b->groups.size())));
+        if (rep == 1) MSBG_HIP(hipEventRecord(b->ev[0], b->stream));
+        MSBG_TRY(launch_sweep_queue(b->stream, G.bsx, G.storage, false, G.c01, b->d_desc.p,
+                                    queue_of(b, k, passes)));
+      }
+      MSBG_HIP(hipEventRecord(b->ev[1], b->stream));
+      MSBG_HIP(hipEventSynchronize(b->ev[1]));
+      MSBG_TRY(check_queue(b));
+      float ms = 0;
+      MSBG_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+      bestQ = ms < bestMs;
+    }
     for (size_t i = 0; i < spans.size(); i++)
       MSBG_HIP(hipMemcpyAsync(spans[i].p, backup[i].p, spans[i].n, hipMemcpyDeviceToDevice,
                               b->stream));
     MSBG_HIP(hipStreamSynchronize(b->stream));
+    G.persistent = bestQ;
     G.tile = best;
     for (int e : G.grids) b->entries[e].g->tile[b->entries[e].g->lg0] = best;
   }
@@ -292,15 +389,20 @@ int msbg_batch_get_tiles(msbg_batch *b, int *bsx, int *storage, int *tile, int c
 }
 
 int msbg_batch_run(msbg_batch *b, int numIter, int useGraph, double *ms) {
-  MSBG_CHECK(b && numIter >= 0, MSBG_EINVAL, "msbg_batch_run: bad args");
+  MSBG_CHECK(b && numIter >= 0 && useGraph >= MSBG_BATCH_EAGER &&
+                 useGraph <= MSBG_BATCH_PERSISTENT,
+             MSBG_EINVAL, "msbg_batch_run: bad args");
   MSBG_TRY(build(b));
   int sweeps = numIter + (numIter % 2);  // msbg4.cpp:150-152
   for (auto &E : b->entries) MSBG_HIP(hipStreamSynchronize(E.g->stream));
-  if (useGraph && sweeps > 0) {
-    if (b->graph_sweeps != sweeps) {
+  const int mode = useGraph;
+  bool anyQ = false;
+  for (auto &G : b->groups) anyQ |= runs_persistent(G, mode) && G.off[8] > 0 && sweeps > 0;
+  if (mode != MSBG_BATCH_EAGER && sweeps > 0) {
+    if (b->graph_sweeps != sweeps || b->graph_mode != mode) {
       drop_graph(b);
       MSBG_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue(b, sweeps);
+      int rc = enqueue(b, sweeps, mode);
       hipGraph_t graph = nullptr;
       hipError_t e = hipStreamEndCapture(b->stream, &graph);
       if (rc) {
@@ -311,13 +413,14 @@ int msbg_batch_run(msbg_batch *b, int numIter, int useGraph, double *ms) {
       b->graph = graph;
       MSBG_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
       b->graph_sweeps = sweeps;
+      b->graph_mode = mode;
     }
     MSBG_HIP(hipEventRecord(b->ev[0], b->stream));
     MSBG_HIP(hipGraphLaunch(b->exec, b->stream));
     MSBG_HIP(hipEventRecord(b->ev[1], b->stream));
   } else {
     MSBG_HIP(hipEventRecord(b->ev[0], b->stream));
-    MSBG_TRY(enqueue(b, sweeps));
+    MSBG_TRY(enqueue(b, sweeps, mode));
     MSBG_HIP(hipEventRecord(b->ev[1], b->stream));
   }
   MSBG_HIP(hipEventSynchronize(b->ev[1]));
@@ -325,6 +428,26 @@ int msbg_batch_run(msbg_batch *b, int numIter, int useGraph, double *ms) {
   MSBG_HIP(hipEventElapsedTime(&t, b->ev[0], b->ev[1]));
   b->last_ms = t;
   if (ms) *ms = t;
+  if (anyQ) MSBG_TRY(check_queue(b));
+  return 0;
+}
+
+int msbg_batch_set_wait_limit(msbg_batch *b, unsigned microseconds) {
+  MSBG_CHECK(b, MSBG_EINVAL, "msbg_batch_set_wait_limit: batch is NULL");
+  if (b->wait_us != microseconds) drop_graph(b);  // the bound is a kernel argument
+  b->wait_us = microseconds;
+  return 0;
+}
+
+int msbg_batch_get_modes(msbg_batch *b, int *persistent, int cap, int *n) {
+  MSBG_CHECK(b, MSBG_EINVAL, "msbg_batch_get_modes: batch is NULL");
+  MSBG_TRY(build(b));
+  int k = 0;
+  for (auto &G : b->groups) {
+    if (k < cap && persistent) persistent[k] = G.persistent ? 1 : 0;
+    k++;
+  }
+  if (n) *n = k;
   return 0;
 }
 

@@ -297,6 +297,29 @@ SweepGrid sweep_desc(Channel &ch, float dt, int doConstr01);
 int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, bool c01, int tile,
                        const SweepGrid *d_grids, const int2 *d_items, int64_t n,
                        double *d_partial, int iter);
+// The persistent batched sweep of one group of a batch: one launch runs
+// every sweep and colour of the group's grids from a work queue, waiting per
+// grid for its previous colour pass (k_sweep_queue).
+constexpr int kDoneStride = 16;  // queue counters one 64-byte line apart
+struct BatchQueue {
+  const int2 *items;   // the group's items (entry, page), sorted by colour
+  const int *pre;      // per grid entry, 9 values: its items of colours < c; [8] = all
+  unsigned *done;      // per grid entry (stride kDoneStride): its items finished in this run
+  unsigned *head;      // next queue index of this group
+  unsigned *fail;      // set when a dependency wait ran past waitTicks
+  int off[9];          // colour offsets into items
+  int sweeps;
+  uint64_t waitTicks;  // wall-clock ticks a dependency wait may last
+};
+// one wave and one block per workgroup (4x4 voxel columns at 32^3, 2x2 at
+// 16^3; not FP64); nwg = 0: as many workgroups as fit on the device at
+// once, but no more than the largest colour pass holds blocks (workgroups
+// beyond it would only wait)
+int launch_sweep_queue(hipStream_t s, int bsx, int storage, bool exact, bool c01,
+                       const SweepGrid *d_grids, const BatchQueue &q, int nwg = 0);
+int launch_queue_reset(hipStream_t s, unsigned *d_state, int n);
+// wall-clock ticks (s_memrealtime) in `us` microseconds on the current device
+uint64_t wall_ticks(uint64_t us);
 // the other applyChannelPdeFast operators (1, 3, 14, 15, 17) and MCF with
 // hard constraints, exact arithmetic, on the same item lists
 int launch_pde_items(hipStream_t s, int bsx, int storage, int op, bool hard,

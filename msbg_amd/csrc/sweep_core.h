@@ -463,12 +463,14 @@ struct Pack {
 // 4-wide column is one 8-byte and one 4-byte read): half the registers of
 // decoded patches, which keeps three waves per SIMD, at the price of sums and
 // differences in v_fma_mix_f32 instead of plain FP32 adds.
-template <int BSX, int ST, bool EXACT, int PX, int PY, int OP = OP_MCF, bool HARD = false,
-          int BPW = 1, bool C01 = false, bool RES = true>
-__global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
-    const SweepGrid *__restrict__ grids, const int2 *__restrict__ items,
-    double *__restrict__ partialArg, int iter, int nItems) {
-  double *__restrict__ const partial = RES ? partialArg : nullptr;
+// One colour pass of one block (item: grid, page) by NT threads (tid), the
+// sub-th group of BPW in its workgroup; inactive groups (past the end of the
+// item list) work on a valid item but store nothing.
+template <int BSX, int ST, bool EXACT, int PX, int PY, int OP, bool HARD, int BPW, bool C01>
+__device__ __forceinline__ void sweep_block(const SweepGrid *__restrict__ grids, const int2 item,
+                                            double *__restrict__ const partial, const int iter,
+                                            const int itemIdx, const bool active, const int sub,
+                                            const int tid) {
   using S = Stor<ST>;
   using R = typename S::raw;
   using C = typename S::ct;
@@ -491,14 +493,6 @@ __global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
   static_assert(!H16 || PX % 2 == 0, "FP16 patches hold pairs of x");
 
   static_assert(BPW == 1 || BPW * NT <= 64, "packed blocks must share one wave");
-  // work item: which grid, which page of it. A group past the end of the
-  // list works on the last item without storing anything, so that it still
-  // reaches every barrier.
-  const int sub = BPW > 1 ? (int)threadIdx.x / NT : 0;
-  const int tid = BPW > 1 ? (int)threadIdx.x % NT : (int)threadIdx.x;
-  const int itemIdx = (int)blockIdx.x * BPW + sub;
-  const bool active = itemIdx < nItems;
-  const int2 item = items[active ? itemIdx : nItems - 1];
   const SweepGrid G = grids[item.x];
   const int32_t *__restrict__ nbrTab = G.nbr;
   uint8_t *pagesRaw = G.pages;
@@ -953,6 +947,92 @@ __global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
         partial[itemIdx] = sum;
       }
     }
+  }
+}
+
+template <int BSX, int ST, bool EXACT, int PX, int PY, int OP = OP_MCF, bool HARD = false,
+          int BPW = 1, bool C01 = false, bool RES = true>
+__global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
+    const SweepGrid *__restrict__ grids, const int2 *__restrict__ items,
+    double *__restrict__ partialArg, int iter, int nItems) {
+  constexpr int NT = Tile<BSX, PX, PY>::NT;
+  // work item: which grid, which page of it. A group past the end of the
+  // list works on the last item without storing anything, so that it still
+  // reaches every barrier.
+  const int sub = BPW > 1 ? (int)threadIdx.x / NT : 0;
+  const int tid = BPW > 1 ? (int)threadIdx.x % NT : (int)threadIdx.x;
+  const int itemIdx = (int)blockIdx.x * BPW + sub;
+  const bool active = itemIdx < nItems;
+  const int2 item = items[active ? itemIdx : nItems - 1];
+  sweep_block<BSX, ST, EXACT, PX, PY, OP, HARD, BPW, C01>(grids, item, RES ? partialArg : nullptr,
+                                                          iter, itemIdx, active, sub, tid);
+}
+
+// ---- the persistent batched pass -------------------------------------------
+// The work queue of a batch group (BatchQueue) holds every (sweep, colour,
+// grid, block) item of the run in that order. Item i is sweep s = i / N8
+// (N8 = items per sweep), entry r = i - s N8 of the colour-sorted item list,
+// whose colour c satisfies off[c] <= r < off[c+1].
+
+// Workgroups (one wave each) take items in queue order. Before sweeping a
+// block of pass (s, c) of grid g, a workgroup waits until every block of g's
+// earlier passes is done: done[g] >= s * pre[g][8] + pre[g][c] (the
+// reference's colour order per grid, msbg4.cpp:384-900; grids are
+// independent, so one grid's next pass overlaps the others' current one).
+// Every item a wait depends on sits earlier in the queue and was taken by a
+// running workgroup, whose own waits point further back, so the waits end;
+// each is still bounded by waitTicks of wall-clock time, past which it sets
+// *fail and goes on (the run is then reported as failed, and no workgroup
+// can hang). Waits poll with a sleep between polls, one 64-byte line per
+// grid's counter. Stores are published with an agent-scope release before
+// the count and read after an agent-scope acquire.
+//
+// The loop has no lane-dependent branch: every lane executes each atomic,
+// adding 1 on lane 0 and 0 elsewhere, and all control (the item taken, the
+// loop's exit, the wait) goes through readfirstlane. A lane-0-only region
+// here lets the compiler thread lane 0's path from one iteration's count to
+// the next iteration's fetch past the other lanes, which then loop on a
+// stale item.
+template <int BSX, int ST, bool EXACT, int PX, int PY, bool C01>
+__global__ __launch_bounds__(64) void k_sweep_queue(const SweepGrid *__restrict__ grids,
+                                                    const BatchQueue q) {
+  static_assert(Tile<BSX, PX, PY>::NT == 64, "one wave per workgroup");
+  const bool lane0 = threadIdx.x == 0;
+  const unsigned n8 = (unsigned)q.off[8];
+  const unsigned total = n8 * (unsigned)q.sweeps;
+  for (;;) {
+    const unsigned i = __builtin_amdgcn_readfirstlane(atomicAdd(q.head, lane0 ? 1u : 0u));
+    if (i >= total) break;
+    const int s = (int)(i / n8), r = (int)(i - (unsigned)s * n8);
+    int c = 0;
+    while (c < 7 && r >= q.off[c + 1]) c++;
+    const int e = __builtin_amdgcn_readfirstlane(q.items[r].x);
+    const int page = __builtin_amdgcn_readfirstlane(q.items[r].y);
+    const int *pre = q.pre + (int64_t)e * 9;
+    const unsigned need = __builtin_amdgcn_readfirstlane((unsigned)s * (unsigned)pre[8] +
+                                                         (unsigned)pre[c]);
+    unsigned *const done = q.done + (int64_t)e * kDoneStride;
+    const uint64_t t0 = wall_clock64();
+    for (;;) {
+      const unsigned d = __builtin_amdgcn_readfirstlane(
+          __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (d >= need) break;
+      if ((uint64_t)wall_clock64() - t0 > q.waitTicks) {
+        atomicOr(q.fail, 1u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(16);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // the lane index through an opaque move, so that the block's per-lane
+    // set-up is computed per item instead of being hoisted out of the loop
+    // and held across it (which costs registers the z-march needs)
+    int lane;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"((int)threadIdx.x));
+    sweep_block<BSX, ST, EXACT, PX, PY, OP_MCF, false, 1, C01>(grids, make_int2(e, page), nullptr,
+                                                               s, 0, true, 0, lane);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(done, lane0 ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

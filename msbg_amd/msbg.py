@@ -548,7 +548,14 @@ class MultiresSparseGrid:
 
 
 class Batch:
-    """Batched 8-colour MCF sweeps of independent grids (one graph launch)."""
+    """Batched 8-colour MCF sweeps of independent grids (one graph launch).
+
+    run(mode): EAGER (one launch per colour pass and kernel group), GRAPH
+    (the same captured into one graph, or each group as the autotuner chose)
+    or PERSISTENT (one graph launch, every group of block size 16 or 32 as one
+    persistent launch with per-grid dependency waits)."""
+
+    EAGER, GRAPH, PERSISTENT = 0, 1, 2
 
     def __init__(self):
         h = C.c_void_p()
@@ -570,9 +577,23 @@ class Batch:
         check(lib().msbg_batch_get_tiles(self._h, _ptr(b), _ptr(s), _ptr(t), n.value, None))
         return list(zip(b.tolist(), s.tolist(), t.tolist()))
 
+    def modes(self):
+        """Per kernel group: 1 if the tuned run uses the persistent launch."""
+        n = C.c_int()
+        check(lib().msbg_batch_get_modes(self._h, None, 0, C.byref(n)))
+        m = np.zeros(n.value, np.int32)
+        check(lib().msbg_batch_get_modes(self._h, _ptr(m), n.value, None))
+        return m.tolist()
+
+    def set_wait_limit(self, microseconds):
+        """Wall-clock bound of one dependency wait of a persistent run."""
+        check(lib().msbg_batch_set_wait_limit(self._h, microseconds))
+
     def run(self, numIter, useGraph=True):
+        """useGraph: True/False (GRAPH/EAGER) or one of EAGER, GRAPH, PERSISTENT."""
+        mode = int(useGraph) if isinstance(useGraph, bool) else useGraph
         ms = C.c_double()
-        check(lib().msbg_batch_run(self._h, numIter, 1 if useGraph else 0, C.byref(ms)))
+        check(lib().msbg_batch_run(self._h, numIter, mode, C.byref(ms)))
         return ms.value
 
     def destroy(self):

@@ -205,14 +205,23 @@ def batched_study(nGrids=256, sizes=(128, 256, 512, 1024), blockSizes=(16, 32), 
             res["batch_graph_ms"] = timed(lambda: ba.run(sweeps, useGraph=True))
             bb.run(sweeps, useGraph=False)
             same("graph replay")
+            # one persistent launch per kernel group, per-grid dependency waits
+            res["batch_persistent_first_ms"] = timed(
+                lambda: ba.run(sweeps, useGraph=Batch.PERSISTENT))
+            bb.run(sweeps, useGraph=False)
+            same("persistent capture")
+            res["batch_persistent_ms"] = timed(lambda: ba.run(sweeps, useGraph=Batch.PERSISTENT))
+            bb.run(sweeps, useGraph=False)
+            same("persistent replay")
             t = time.perf_counter()
             ba.autotune(passes=4)
             res["autotune_seconds"] = time.perf_counter() - t
             res["tiles"] = ba.tiles()
+            res["persistent_by_group"] = ba.modes()
             res["batch_graph_tuned_ms"] = timed(lambda: ba.run(sweeps, useGraph=True))
             res["sequential_tuned_ms"] = timed(lambda: sequential(CH))
-            for key in ("sequential_ms", "batch_eager_ms", "batch_graph_ms", "batch_graph_tuned_ms",
-                        "sequential_tuned_ms"):
+            for key in ("sequential_ms", "batch_eager_ms", "batch_graph_ms", "batch_persistent_ms",
+                        "batch_graph_tuned_ms", "sequential_tuned_ms"):
                 res[key.replace("_ms", "_updates_per_s")] = rate(res[key])
             res["speedup_graph_vs_sequential"] = res["sequential_ms"] / res["batch_graph_ms"]
             res["speedup_tuned_vs_sequential"] = res["sequential_ms"] / res["batch_graph_tuned_ms"]

@@ -724,12 +724,13 @@ def test_batch_graph_replays_match_eager():
 def test_batch_cfg4_grids_against_oracle():
     """cfg4's generator at test sizes: 16 grids of 128^3-256^3, 16^3 or 32^3
     blocks, 1-3 levels, 1-20 % active, FP16 and FP32. Exact arithmetic
-    through one batched launch, as a captured graph and eagerly: every grid
-    bitwise equal to the oracle's own sweeps (msbg4.cpp:384-900). Fast
-    arithmetic through the batch: within the per-storage sweep bound."""
+    through one batched launch per pass as a captured graph and eagerly, and
+    through the persistent launch: every grid bitwise equal to the oracle's
+    own sweeps (msbg4.cpp:384-900). Fast arithmetic through the batch: within
+    the per-storage sweep bound."""
     from msbg_amd.studies import _scene_grid
     rng = np.random.default_rng(2024)
-    CH, CHE, CHF = 50, 51, 52
+    CH, CHE, CHF, CHP = 50, 51, 52, 53
     grids, want, want_fast, st = [], [], [], []
     try:
         for k in range(16):
@@ -745,6 +746,7 @@ def test_batch_cfg4_grids_against_oracle():
             st.append(storage)
             g.copyDataTo(CH, CHE, storage)
             g.copyDataTo(CH, CHF, storage)
+            g.copyDataTo(CH, CHP, storage)
             pm, _ = g.pagemap(CH)
             pages = g.download(CH)
             if storage == M.F16:
@@ -760,14 +762,16 @@ def test_batch_cfg4_grids_against_oracle():
             want_fast.append(Lf.pages)
         for g in grids:
             g.set_arith(M.ARITH_EXACT)
-        with M.Batch() as bg, M.Batch() as be:
+        with M.Batch() as bg, M.Batch() as be, M.Batch() as bp:
             for g in grids:
                 bg.add(g, CH, 0.1, 1)
                 be.add(g, CHE, 0.1, 1)
+                bp.add(g, CHP, 0.1, 1)
             bg.run(3, useGraph=True)
             be.run(3, useGraph=False)
+            bp.run(3, useGraph=M.Batch.PERSISTENT)
         for k, g in enumerate(grids):
-            for ch in (CH, CHE):
+            for ch in (CH, CHE, CHP):
                 got = g.download(ch)
                 if st[k] == M.F16:
                     got = got.view(np.uint16)
@@ -789,6 +793,56 @@ def test_batch_cfg4_grids_against_oracle():
             g.destroy()
 
 
+def test_batch_persistent_matches_per_pass_launches():
+    """The persistent launch (work queue, per-grid waits) gives the bits of
+    one launch per colour pass, in fast arithmetic, for 32^3 and 16^3 blocks
+    of FP16 grids, on three replays of its captured graph with eager work
+    between them."""
+    note("batch persistent")
+    specs = [(128, 32), (256, 32), (128, 16), (64, 16), (96, 32)]
+    a = _batch_grids(specs, seed=31)
+    b = _batch_grids(specs, seed=31)
+    with M.Batch() as bq, M.Batch() as bl:
+        for g in a:
+            bq.add(g, BCH, 0.1, 1)
+        for g in b:
+            bl.add(g, BCH, 0.1, 1)
+        for rep in range(3):
+            bq.run(4, useGraph=M.Batch.PERSISTENT)
+            bl.run(4, useGraph=False)
+            for g in a:
+                g.synchronize()
+                g.download(BCH)
+            for ga, gb in zip(a, b):
+                assert np.array_equal(ga.download(BCH).view(np.uint16),
+                                      gb.download(BCH).view(np.uint16)), rep
+
+
+def test_batch_persistent_reports_a_wait_past_its_bound():
+    """A dependency wait that runs past its poll bound ends the wait and is
+    reported as a failed run (forced here with a bound of zero time, which
+    lets blocks run before their grid's previous pass: no hang, an error);
+    with the default bound the same batch runs correctly again."""
+    note("batch persistent wait bound")
+    specs = [(128, 32), (128, 32), (128, 16), (128, 16)]
+    a = _batch_grids(specs, seed=41)
+    b = _batch_grids(specs, seed=41)
+    with M.Batch() as bq:
+        for g in a:
+            bq.add(g, BCH, 0.1, 1)
+        bq.set_wait_limit(0)
+        with pytest.raises(M.MsbgError):
+            bq.run(8, useGraph=M.Batch.PERSISTENT)
+        bq.set_wait_limit(2000000)
+        for ga, gb in zip(a, b):  # start again from the same field
+            ga.upload(BCH, 0, gb.download(BCH))
+        bq.run(2, useGraph=M.Batch.PERSISTENT)
+    for g in b:
+        g.applyChannelPdeFast(BCH, 0, 0, None, M.MCF_8COLOR, 1, 0, 0, 0, 2, 1.0, 0.1)
+    for ga, gb in zip(a, b):
+        assert np.array_equal(ga.download(BCH).view(np.uint16), gb.download(BCH).view(np.uint16))
+
+
 def test_batch_autotune_keeps_results():
     note("batch autotune")
     specs = [(128, 32), (128, 16)]
@@ -801,6 +855,7 @@ def test_batch_autotune_keeps_results():
         bt.autotune(passes=2)
         tiles = bt.tiles()
         assert len(tiles) == 2 and all(t in (0, 1, 2) for _, _, t in tiles)
+        assert all(m in (0, 1) for m in bt.modes())
         # tuning restores the pages it sweeps
         for g, x in zip(a, before):
             assert np.array_equal(g.download(BCH).view(np.uint16), x.view(np.uint16))
