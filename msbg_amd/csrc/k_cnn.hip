@@ -359,55 +359,99 @@ __global__ void k_wgrad_reduce(const float *__restrict__ partial, int chunks, in
   dw[i] = s;
 }
 
-__global__ void k_xpose_pad(const uint16_t *__restrict__ x, int lgNG, int din, int c, int pad,
-                            int cpad, uint16_t *__restrict__ xtp) {
+// The weight-gradient layout: for channel ch and voxel v, the 8 samples of
+// group g side by side ([ch][v][g][8]), from activations laid out
+// [n][v][c]. One workgroup per row of XR voxels, slice of 16 channels and
+// slice of up to 32 samples: the row is read with consecutive threads on
+// consecutive channels of a voxel and consecutive voxels of a row
+// (coalesced), staged in LDS, and written as 16-byte groups of 8 samples,
+// consecutive threads on consecutive (voxel, group) pairs of one channel.
+constexpr int kXC = 16;  // channels per workgroup
+constexpr int kXS = 32;  // samples per workgroup
+constexpr int kXR = 36;  // voxels of a row, at most
+
+// activations x [n][din^3][c] into xtp [cpad][dp^3][n/8][8], dp = din + 2 pad,
+// zero outside the input and for channels c..cpad-1
+__global__ __launch_bounds__(256) void k_xpose_pad(const uint16_t *__restrict__ x, int n, int din,
+                                                   int c, int pad, int cpad,
+                                                   uint16_t *__restrict__ xtp) {
+  __shared__ uint16_t T[kXS][kXR][kXC + 2];
   const int dp = din + 2 * pad;
-  const int64_t vp = (int64_t)dp * dp * dp, vin = (int64_t)din * din * din;
-  const int NG = 1 << lgNG;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)cpad * vp * NG) return;
-  const int ng = (int)(i & (NG - 1));
-  const int64_t pv = (i >> lgNG) % vp;
-  const int ch = (int)((i >> lgNG) / vp);
-  const int px = (int)(pv % dp), py = (int)((pv / dp) % dp), pz = (int)(pv / ((int64_t)dp * dp));
-  const int ix = px - pad, iy = py - pad, iz = pz - pad;
-  uint32_t w[4] = {0, 0, 0, 0};
-  if (ch < c && ix >= 0 && iy >= 0 && iz >= 0 && ix < din && iy < din && iz < din) {
-    const int64_t v = ((int64_t)iz * din + iy) * din + ix;
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int64_t n = (int64_t)ng * 8 + j;
-      const uint32_t h = x[(n * vin + v) * c + ch];
-      w[j >> 1] |= h << (16 * (j & 1));
-    }
+  const int pz = (int)blockIdx.x / dp, py = (int)blockIdx.x % dp;
+  const int c0 = (int)blockIdx.y * kXC, s0 = (int)blockIdx.z * kXS;
+  const int ns = n - s0 < kXS ? n - s0 : kXS;
+  const int iz = pz - pad, iy = py - pad;
+  const bool rowIn = iz >= 0 && iy >= 0 && iz < din && iy < din;
+  const int64_t vin = (int64_t)din * din * din;
+  const int tid = (int)threadIdx.x;
+  // load: (sample, voxel, channel), channel fastest
+  const int nload = ns * din * kXC;
+  for (int k = tid; k < nload; k += 256) {
+    const int ch = k % kXC, xi = (k / kXC) % din, j = k / (kXC * din);
+    uint16_t v = 0;
+    if (rowIn && c0 + ch < c)
+      v = x[((int64_t)(s0 + j) * vin + ((int64_t)iz * din + iy) * din + xi) * c + c0 + ch];
+    T[j][xi][ch] = v;
   }
-  *(uint4 *)(xtp + (((int64_t)ch * vp + pv) << (lgNG + 3)) + ng * 8) =
-      make_uint4(w[0], w[1], w[2], w[3]);
+  __syncthreads();
+  // store: (channel, voxel of the padded row, group), group fastest
+  const int NG = n / 8, gs = ns / 8;
+  const int64_t vp = (int64_t)dp * dp * dp;
+  const int nstore = kXC * dp * gs;
+  for (int k = tid; k < nstore; k += 256) {
+    const int g = k % gs, px = (k / gs) % dp, ch = k / (gs * dp);
+    if (c0 + ch >= cpad) continue;
+    const int xi = px - pad;
+    uint32_t w[4] = {0, 0, 0, 0};
+    if (xi >= 0 && xi < din) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) w[j >> 1] |= (uint32_t)T[g * 8 + j][xi][ch] << (16 * (j & 1));
+    }
+    const int64_t pv = ((int64_t)pz * dp + py) * dp + px;
+    *(uint4 *)(xtp + ((int64_t)(c0 + ch) * vp + pv) * (NG * 8) + (s0 / 8 + g) * 8) =
+        make_uint4(w[0], w[1], w[2], w[3]);
+  }
 }
 
-__global__ void k_relu_bwd_xpose(uint16_t *__restrict__ dy, const uint16_t *__restrict__ y,
-                                 int lgNG, int64_t v, int c, int relu,
-                                 uint16_t *__restrict__ dzt) {
-  const int NG = 1 << lgNG;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)c * v * NG) return;
-  const int ng = (int)(i & (NG - 1));
-  const int64_t vv = (i >> lgNG) % v;
-  const int ch = (int)((i >> lgNG) / v);
-  uint32_t w[4] = {0, 0, 0, 0};
-#pragma unroll
-  for (int j = 0; j < 8; j++) {
-    const int64_t n = (int64_t)ng * 8 + j;
-    const int64_t idx = (n * v + vv) * c + ch;
-    uint32_t g = dy[idx];
-    if (relu && !(bf2f(y[idx]) > 0.f)) {
-      g = 0;
-      dy[idx] = 0;
+// ReLU backward in place (dy = 0 where y <= 0, when relu) and the same
+// transpose of the masked gradient into dzt [c][v][n/8][8]; rows of d voxels
+__global__ __launch_bounds__(256) void k_relu_bwd_xpose(uint16_t *__restrict__ dy,
+                                                        const uint16_t *__restrict__ y, int n,
+                                                        int d, int c, int relu,
+                                                        uint16_t *__restrict__ dzt) {
+  __shared__ uint16_t T[kXS][kXR][kXC + 2];
+  const int rz = (int)blockIdx.x / d, ry = (int)blockIdx.x % d;
+  const int c0 = (int)blockIdx.y * kXC, s0 = (int)blockIdx.z * kXS;
+  const int ns = n - s0 < kXS ? n - s0 : kXS;
+  const int64_t v = (int64_t)d * d * d;
+  const int tid = (int)threadIdx.x;
+  const int nload = ns * d * kXC;
+  for (int k = tid; k < nload; k += 256) {
+    const int ch = k % kXC, xi = (k / kXC) % d, j = k / (kXC * d);
+    uint16_t g = 0;
+    if (c0 + ch < c) {
+      const int64_t idx = ((int64_t)(s0 + j) * v + ((int64_t)rz * d + ry) * d + xi) * c + c0 + ch;
+      g = dy[idx];
+      if (relu && !(bf2f(y[idx]) > 0.f)) {
+        g = 0;
+        dy[idx] = 0;
+      }
     }
-    w[j >> 1] |= g << (16 * (j & 1));
+    T[j][xi][ch] = g;
   }
-  *(uint4 *)(dzt + (((int64_t)ch * v + vv) << (lgNG + 3)) + ng * 8) =
-      make_uint4(w[0], w[1], w[2], w[3]);
+  __syncthreads();
+  const int NG = n / 8, gs = ns / 8;
+  const int nstore = kXC * d * gs;
+  for (int k = tid; k < nstore; k += 256) {
+    const int g = k % gs, xi = (k / gs) % d, ch = k / (gs * d);
+    if (c0 + ch >= c) continue;
+    uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 8; j++) w[j >> 1] |= (uint32_t)T[g * 8 + j][xi][ch] << (16 * (j & 1));
+    const int64_t vv = ((int64_t)rz * d + ry) * d + xi;
+    *(uint4 *)(dzt + ((int64_t)(c0 + ch) * v + vv) * (NG * 8) + (s0 / 8 + g) * 8) =
+        make_uint4(w[0], w[1], w[2], w[3]);
+  }
 }
 
 constexpr int kRowChunks = 64;
@@ -820,11 +864,12 @@ int launch_wgrad_reduce(hipStream_t s, const float *partial, int chunks, int cin
 int launch_xpose_pad(hipStream_t s, const uint16_t *x, int n, int din, int c, int pad, int cpad,
                      uint16_t *x_tp) {
   const int lg = lg_groups(n);
-  MSBG_CHECK(lg >= 0 && cpad >= c, MSBG_EINVAL, "xpose: batch %d is not 8 * 2^k", n);
+  MSBG_CHECK(lg >= 0 && cpad >= c && din <= kXR, MSBG_EINVAL,
+             "xpose: batch %d is not 8 * 2^k, or rows of %d voxels", n, din);
   const int dp = din + 2 * pad;
-  const int64_t total = (int64_t)cpad * dp * dp * dp * (n / 8);
-  hipLaunchKernelGGL(k_xpose_pad, dim3(blocks_for(total)), dim3(256), 0, s, x, lg, din, c, pad,
-                     cpad, x_tp);
+  const dim3 grid((unsigned)(dp * dp), (unsigned)((cpad + kXC - 1) / kXC),
+                  (unsigned)((n + kXS - 1) / kXS));
+  hipLaunchKernelGGL(k_xpose_pad, grid, dim3(256), 0, s, x, n, din, c, pad, cpad, x_tp);
   MSBG_HIP(hipGetLastError());
   return 0;
 }
@@ -832,10 +877,14 @@ int launch_xpose_pad(hipStream_t s, const uint16_t *x, int n, int din, int c, in
 int launch_relu_bwd_xpose(hipStream_t s, uint16_t *dy, const uint16_t *y, int n, int64_t v, int c,
                           bool relu, uint16_t *dz_t) {
   const int lg = lg_groups(n);
-  MSBG_CHECK(lg >= 0, MSBG_EINVAL, "relu_bwd: batch %d is not 8 * 2^k", n);
-  const int64_t total = (int64_t)c * v * (n / 8);
-  hipLaunchKernelGGL(k_relu_bwd_xpose, dim3(blocks_for(total)), dim3(256), 0, s, dy, y, lg, v, c,
-                     relu ? 1 : 0, dz_t);
+  int d = 1;
+  while ((int64_t)d * d * d < v) d++;
+  MSBG_CHECK(lg >= 0 && (int64_t)d * d * d == v && d <= kXR, MSBG_EINVAL,
+             "relu_bwd: batch %d is not 8 * 2^k, or volume %lld is not a cube of <= %d",
+             n, (long long)v, kXR);
+  const dim3 grid((unsigned)(d * d), (unsigned)((c + kXC - 1) / kXC),
+                  (unsigned)((n + kXS - 1) / kXS));
+  hipLaunchKernelGGL(k_relu_bwd_xpose, grid, dim3(256), 0, s, dy, y, n, d, c, relu ? 1 : 0, dz_t);
   MSBG_HIP(hipGetLastError());
   return 0;
 }
