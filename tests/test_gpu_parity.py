@@ -155,11 +155,13 @@ def test_halo_table_matches_pagemap():
 # --------------------------------------------------------------------------
 # splat / finalize
 # --------------------------------------------------------------------------
-@pytest.mark.parametrize("S,bsx", [(128, 16), (128, 32), (64, 8)])
+# 8^3 blocks are left out: the splat's footprint, 2 (rScan + 0.5) = 9 voxels,
+# is wider than such a block, which the reference's splat does not allow
+# (msbg_demo.cpp:541-636); the argument check rejects it
+@pytest.mark.parametrize("S,bsx", [(128, 16), (128, 32)])
 def test_splat_finalize_u16_bitexact(S, bsx):
     note("splat", S, bsx)
-    if 2 * (RSCAN + 0.5) > bsx:
-        pytest.skip("footprint wider than a block")
+    assert 2 * (RSCAN + 0.5) <= bsx
     xyz = _scene(S, seed=S + bsx, nObj=8, nPts=4000)
     o = oracle_pipeline(S, bsx, xyz, do_sweep=False)
     gp = gpu_pipeline(S, bsx, xyz, do_sweep=False)
@@ -334,16 +336,16 @@ _FAST_MAX = {M.F32: 2e-5, M.U16: 2.0 / 65535.0, M.F16: 2.0 ** -9, M.BF16: 2.0 **
              M.U8: 2.0 / 255.0}
 
 
-@pytest.mark.parametrize("doC01", [1, 0])
-@pytest.mark.parametrize("storage", [M.U16, M.F32, M.F16, M.BF16, M.U8])
+# the unclamped mode on the float storages only: UNORM values outside [0,1]
+# wrap in the integer store
+@pytest.mark.parametrize("storage,doC01", [(M.U16, 1), (M.F32, 1), (M.F16, 1), (M.BF16, 1),
+                                           (M.U8, 1), (M.F32, 0), (M.F16, 0), (M.BF16, 0)])
 @pytest.mark.parametrize("S,bsx", [(128, 16), (128, 32)])
 def test_sweep_fast_every_column_bounded(S, bsx, storage, doC01):
     """Every voxel-column variant of the fast kernel (2x2, 4x2, 4x4), with and
     without the [0,1] clamp, against the strict oracle after 2 sweeps: a max
     bound per storage (above) and an RMS bound. UNORM16 runs with the clamp
     only (unclamped values outside [0,1] wrap in the 16-bit store)."""
-    if storage in (M.U16, M.U8) and not doC01:
-        pytest.skip("UNORM wraps outside [0,1]; the unclamped mode is run on the float storages")
     xyz = _scene(S, seed=31 + bsx, nObj=6, nPts=3000)
     o = oracle_pipeline(S, bsx, xyz, do_sweep=False)
     init, L = _init_field(o, S, bsx, storage)
