@@ -14,6 +14,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 #include "internal.h"
@@ -96,19 +97,23 @@ __global__ void k_splat(const float *__restrict__ xyz, int64_t n, float rScan, f
 
 // ---- binned splat: LDS tiles, scattered colour by colour ------------------
 // Particles are binned into cells of kCell^3 voxels (counting sort). A
-// workgroup takes one cell: every footprint of its particles lies in the
-// cell grown by kHalo voxels per side (a kTile^3 tile of u32 codes in LDS),
-// so the particles min their codes into LDS with ds_min_u32, one wave per
-// particle and one lane per footprint voxel. The tile is then merged into
-// the pages with plain loads and stores: the cells of one colour (cell
-// coordinate parities) are 2 kCell apart and kCell >= 2 kHalo, so their
-// tiles are disjoint, and the 8 colours run as 8 launches in stream order.
+// workgroup takes a cell: every footprint of its particles lies in the cell
+// grown by H voxels per side (H from rScan, a tile of u32 codes in LDS), so
+// the particles min their codes into the tile with ds_min_u32, one wave per
+// particle and one lane per (y, x) position of its footprint box. The tile
+// is then merged into the pages with plain loads and stores: the cells of
+// one colour (cell coordinate parities) are 2 kCell apart and the tile is at
+// most 2 kCell wide, so the tiles of one colour are disjoint, and the 8
+// colours run as 8 launches in stream order. Workgroups stay resident and
+// walk their colour's cells in turn, fetching the next cell's block codes
+// and particles while they merge the current one.
 // The particle's walk in z,y,x order with its early abandon
-// (msbg_demo.cpp:615-632) becomes a ballot over the footprint in the same
-// order: lanes before the first in-range voxel of a non-value block apply,
-// and the particle ends there.
-constexpr int kCell = 16, kHalo = 5, kTile = kCell + 2 * kHalo, kTileVox = kTile * kTile * kTile;
-constexpr int kTileBlk = 125;  // blocks a tile can touch: (kTile/8 + 1)^3 at 8^3 blocks
+// (msbg_demo.cpp:615-632): a footprint whose blocks all hold a page never
+// abandons; any other becomes a ballot over the footprint in the same
+// order: lanes before the first in-range voxel of a block without a page
+// apply, and the particle ends there.
+constexpr int kCell = 16, kMaxHalo = 4;
+constexpr int kTileBlk = 125;  // blocks a tile can touch: (32/8 + 1)^3 at 8^3 blocks
 constexpr uint32_t kNoCode = 0xffffu;
 
 // Cells per axis (ncx..) and the cell keys: cells are numbered in 4x4x4
@@ -174,172 +179,412 @@ struct CellOfColour {
   }
 };
 
-constexpr int kSplatThreads = 1024, kMergeBatch = 9;
-// footprint boxes of the tile path have at most 9 voxels a side
-// (2 rScan + 1 with rScan + 0.5 <= kHalo - 0.25): 81 <= 2 * 64 per face
+constexpr int kSplatThreads = 1024;
+// the merge takes a pair of voxels in x over kMergeZ planes per thread
+constexpr int kMergeZ = 8;
+// footprint faces of at most (2 kMaxHalo + 1)^2 voxels: kFaceSets sets of 64 lanes
 constexpr int kFaceSets = 2;
-static_assert((2 * kHalo - 1) * (2 * kHalo - 1) <= kFaceSets * 64, "face sets");
+static_assert((2 * kMaxHalo + 1) * (2 * kMaxHalo + 1) <= kFaceSets * 64, "face sets");
 
-template <int ESZ>
-__global__ __launch_bounds__(kSplatThreads) void k_splat_tile(
-    const float *__restrict__ sorted, const uint32_t *__restrict__ off,
-    const uint32_t *__restrict__ cnt, const int32_t *__restrict__ list, CellGrid G, float rScan,
-    float distSqMax, float distSqMaxInv, int lg, int nbx, int nby,
+// The tile of halo H: x grown by HX (H rounded up to even, so that voxel
+// pairs in x stay aligned to pairs of page elements), y and z by H. Footprint
+// boxes have at most 2H + 1 voxels a side.
+template <int H>
+struct TileDims {
+  static constexpr int HX = (H + 1) & ~1;
+  static constexpr int TX = kCell + 2 * HX, TY = kCell + 2 * H, TZ = kCell + 2 * H;
+  static constexpr int PLANE = TX * TY, NV = PLANE * TZ, MAXPL = 2 * H + 1;
+  static constexpr int PARTS = (TZ + kMergeZ - 1) / kMergeZ, ITEMS = (TX / 2) * TY * PARTS;
+  static_assert(TX <= 2 * kCell && TY <= 2 * kCell, "tiles of one colour overlap");
+  static_assert((TX / 8 + 1) * (TY / 8 + 1) * (TZ / 8 + 1) <= kTileBlk, "tile blocks");
+  static_assert(ITEMS <= kSplatThreads, "one merge item per thread");
+  static_assert((MAXPL - 1) * PLANE * 4 < 65536, "plane offsets");
+  static_assert(NV % 4 == 0 && TX % 2 == 0, "tile fill, voxel pairs");
+};
+
+// one cell of a colour: its key, first particle and particle count
+struct CellMeta {
+  int key;
+  uint32_t off, cnt;
+  int pad;
+};
+
+__global__ void k_cell_meta(const int32_t *__restrict__ list, int n, const uint32_t *__restrict__ off,
+                            const uint32_t *__restrict__ cnt, CellMeta *__restrict__ meta) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  const int k = list[i];
+  meta[i] = CellMeta{k, off[k], cnt[k], 0};
+}
+
+__device__ inline int lane_i(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+__device__ inline float lane_f(float v, int k) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), k));
+}
+
+// The UNORM16 distance code of an in-range voxel, (uint16_t)roundf(f * 65535)
+// with f = d2 * distSqMaxInv (renderDensFromFloat_, sbg.h:272-315). For
+// x = f * 65535 in [0, 65536], trunc(x + 0.49999997f) in float is exactly
+// roundf(x); out-of-range voxels (d2 > distSqMax, or +inf) give 65535 or more
+// and are clamped to 65535, which is the tile's empty code and changes
+// nothing. codeScale = 65535 * distSqMaxInv when distSqMaxInv is a power of
+// two: then d2 * distSqMaxInv is exact and one multiply rounds as two do.
+template <bool ONE_MUL>
+__device__ inline uint32_t splat_code(float d2, float distSqMaxInv, float codeScale) {
+#pragma clang fp contract(off)
+  const float x = ONE_MUL ? d2 * codeScale : (d2 * distSqMaxInv) * 65535.0f;
+  return (uint32_t)__builtin_fminf(x + 0.49999997f, 65535.5f);
+}
+
+// where a cell's tile lies and which blocks lie under it
+struct TileGeo {
+  int tx0, ty0, tz0, bxa, bya, bza, nbtx, nbty, nbt;
+};
+
+template <int H>
+__device__ inline TileGeo tile_geo(const CellGrid &G, int key, int lg) {
+  using T = TileDims<H>;
+  int cx, cy, cz;
+  key_cell(G, key, cx, cy, cz);
+  TileGeo t;
+  t.tx0 = cx * kCell - T::HX;
+  t.ty0 = cy * kCell - H;
+  t.tz0 = cz * kCell - H;
+  t.bxa = (t.tx0 > 0 ? t.tx0 : 0) >> lg;
+  t.bya = (t.ty0 > 0 ? t.ty0 : 0) >> lg;
+  t.bza = (t.tz0 > 0 ? t.tz0 : 0) >> lg;
+  const int bxb = (t.tx0 + T::TX - 1 < G.sx - 1 ? t.tx0 + T::TX - 1 : G.sx - 1) >> lg;
+  const int byb = (t.ty0 + T::TY - 1 < G.sy - 1 ? t.ty0 + T::TY - 1 : G.sy - 1) >> lg;
+  const int bzb = (t.tz0 + T::TZ - 1 < G.sz - 1 ? t.tz0 + T::TZ - 1 : G.sz - 1) >> lg;
+  t.nbtx = bxb - t.bxa + 1;
+  t.nbty = byb - t.bya + 1;
+  t.nbt = t.nbtx * t.nbty * (bzb - t.bza + 1);
+  return t;
+}
+
+// what a thread fetches for a cell ahead of its walk: the page code of one
+// block under the tile and the first particle of its lane
+struct CellFetch {
+  int bval;
+  float x, y, z;
+};
+
+template <int ESZ, int H>
+__global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
+    const float *__restrict__ sorted, const CellMeta *__restrict__ meta, int ncells, CellGrid G,
+    float rScan, float distSqMax, float distSqMaxInv, float codeScale, int lg, int nbx, int nby,
     const int32_t *__restrict__ pagemap, uint8_t *__restrict__ pages) {
 #pragma clang fp contract(off)
-  __shared__ uint32_t tile[kTileVox];
-  __shared__ int32_t bcode[kTileBlk];
-  const int cell = list[blockIdx.x];
-  int cx, cy, cz;
-  key_cell(G, cell, cx, cy, cz);
-  const int tx0 = cx * kCell - kHalo, ty0 = cy * kCell - kHalo, tz0 = cz * kCell - kHalo;
-  // blocks under the in-domain part of the tile
-  const int bxa = (tx0 > 0 ? tx0 : 0) >> lg, bya = (ty0 > 0 ? ty0 : 0) >> lg,
-            bza = (tz0 > 0 ? tz0 : 0) >> lg;
-  const int bxb = (tx0 + kTile - 1 < G.sx - 1 ? tx0 + kTile - 1 : G.sx - 1) >> lg;
-  const int byb = (ty0 + kTile - 1 < G.sy - 1 ? ty0 + kTile - 1 : G.sy - 1) >> lg;
-  const int bzb = (tz0 + kTile - 1 < G.sz - 1 ? tz0 + kTile - 1 : G.sz - 1) >> lg;
-  const int nbtx = bxb - bxa + 1, nbty = byb - bya + 1, nbtz = bzb - bza + 1;
-  for (int e = threadIdx.x; e < kTileVox; e += blockDim.x) tile[e] = kNoCode;
-  for (int e = threadIdx.x; e < nbtx * nbty * nbtz; e += blockDim.x) {
-    const int ex = e % nbtx, ey = (e / nbtx) % nbty, ez = e / (nbtx * nbty);
-    bcode[e] = pagemap[((int64_t)(bza + ez) * nby + (bya + ey)) * nbx + (bxa + ex)];
-  }
-  __syncthreads();
+  using T = TileDims<H>;
+  __shared__ __align__(16) uint32_t tile[T::NV];
+  __shared__ int32_t bcode[2][kTileBlk];
+  const int stride = (int)gridDim.x;
+  int ci = (int)blockIdx.x;
+  if (ci >= ncells) return;
+  const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nwaves = (int)(blockDim.x >> 6);
+  for (int e = tid; e < T::NV / 4; e += (int)blockDim.x)
+    ((uint4 *)tile)[e] = make_uint4(kNoCode, kNoCode, kNoCode, kNoCode);
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  const uint32_t p0 = off[cell], np = cnt[cell];
-  const int smax[3] = {G.sx - 1, G.sy - 1, G.sz - 1};
-  // the wave's particles are wave, wave + nwaves, ...: each lane loads one
-  // of the next 64, then the wave takes them one by one from its lanes
-  auto lane_f = [](float v, int k) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), k));
-  };
-  for (uint32_t qb = wave; qb < np; qb += 64u * nwaves) {
-    const uint32_t ql = qb + (uint32_t)lane * nwaves;
-    float lx = 0.0f, ly = 0.0f, lz = 0.0f;
-    if (ql < np) {
-      const float *pp = sorted + 3 * (int64_t)(p0 + ql);
-      lx = pp[0];
-      ly = pp[1];
-      lz = pp[2];
+  auto fetch = [&](const CellMeta &m, const TileGeo &t) {
+    CellFetch f{0, 0.0f, 0.0f, 0.0f};
+    if (tid < t.nbt) {
+      const int ex = tid % t.nbtx, ey = (tid / t.nbtx) % t.nbty, ez = tid / (t.nbtx * t.nbty);
+      f.bval = pagemap[((int64_t)(t.bza + ez) * nby + (t.bya + ey)) * nbx + (t.bxa + ex)];
     }
-    const uint32_t left = (np - qb + nwaves - 1) / nwaves;
-    const int nq = left < 64u ? (int)left : 64;
-    for (int kq = 0; kq < nq; kq++) {
-      const float p[3] = {lane_f(lx, kq), lane_f(ly, kq), lane_f(lz, kq)};
+    const uint32_t ql = (uint32_t)(wave + lane * nwaves);
+    if (ql < m.cnt) {
+      const float *pp = sorted + 3 * (int64_t)(m.off + ql);
+      f.x = pp[0];
+      f.y = pp[1];
+      f.z = pp[2];
+    }
+    return f;
+  };
+  // the meta of cell i: a vector load (odd lanes read cell i + stride), so
+  // that it does not hold up the LDS waits of the merge, taken from lane 0
+  // an iteration later
+  auto meta_raw = [&](int i) {
+    const int j = (lane & 1) && i + stride < ncells ? i + stride : i;
+    return meta[j];
+  };
+  auto uni = [&](const CellMeta &v) {
+    return CellMeta{lane_i(v.key, 0), (uint32_t)lane_i((int)v.off, 0),
+                    (uint32_t)lane_i((int)v.cnt, 0), 0};
+  };
+
+  CellMeta mc = uni(meta_raw(ci));
+  CellMeta mnRaw = ci + stride < ncells ? meta_raw(ci + stride) : CellMeta{0, 0, 0, 0};
+  TileGeo tg = tile_geo<H>(G, mc.key, lg);
+  CellFetch fc = fetch(mc, tg);
+  const int smax[3] = {G.sx - 1, G.sy - 1, G.sz - 1};
+  const bool oneMul = codeScale != 0.0f;
+  int buf = 0;
+  for (;;) {
+    int32_t *const bc = bcode[buf];
+    if (tid < tg.nbt) bc[tid] = fc.bval;
+    // the next cell's meta, loaded an iteration ago
+    const CellMeta mn = ci + stride < ncells ? uni(mnRaw) : CellMeta{0, 0, 0, 0};
+    __syncthreads();  // block codes in, the previous merge has emptied the tile
+
+    // ---- walk: the wave's particles are wave, wave + nwaves, ...; each lane
+    // holds one of the next 64 and sets it up, then the wave walks them
+    for (uint32_t qb = (uint32_t)wave; qb < mc.cnt; qb += 64u * nwaves) {
+      int lane = tid & 63;
+      __asm__ volatile("" : "+v"(lane));
+      const uint32_t ql = qb + (uint32_t)lane * nwaves;
+      float lx = fc.x, ly = fc.y, lz = fc.z;
+      if (qb != (uint32_t)wave) {
+        lx = ly = lz = 0.0f;
+        if (ql < mc.cnt) {
+          const float *pp = sorted + 3 * (int64_t)(mc.off + ql);
+          lx = pp[0];
+          ly = pp[1];
+          lz = pp[2];
+        }
+      }
+      // this lane's particle: its footprint box (msbg_demo.cpp:571-614) and
+      // whether the walk can skip the block checks: a box of at most 64 face
+      // voxels whose blocks (at most two per axis) all hold a page never
+      // reaches the early abandon
+      const float lp[3] = {lx, ly, lz};
       int lo[3], hi[3];
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        int a = (int)__builtin_ceilf(p[k] - rScan - 0.5f);
-        int b = (int)__builtin_floorf(p[k] + rScan - 0.5f);
+        int a = (int)__builtin_ceilf(lp[k] - rScan - 0.5f);
+        int b = (int)__builtin_floorf(lp[k] + rScan - 0.5f);
         lo[k] = a > 0 ? a : 0;
         hi[k] = b < smax[k] ? b : smax[k];
       }
       const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
-      if (nx <= 0 || ny <= 0 || nz <= 0) continue;
-      const int nxy = nx * ny;
-      const float x0 = p[0] - 0.5f, y0 = p[1] - 0.5f, z0 = p[2] - 0.5f;
-      // lanes over the box's (y, x) face in y,x order (at most kFaceSets
-      // sets of 64), the planes in z order: z,y,x order is lane order
-      // within a plane. Per lane and set, what does not depend on z.
-      // t -> (t / nx, t % nx) through (t + 0.5) / nx in float: exact for
-      // t < 2^15 and nx <= kTile.
-      const float rx = 1.0f / (float)nx;
-      bool fv[kFaceSets];
-      float fdy2[kFaceSets], fdx2[kFaceSets];
-      int fblk[kFaceSets], ftile[kFaceSets];
+      const bool live = ql < mc.cnt && nx > 0 && ny > 0 && nz > 0;
+      bool fast = live && nx * ny <= 64 && nz <= T::MAXPL && (hi[0] >> lg) - (lo[0] >> lg) <= 1 &&
+                  (hi[1] >> lg) - (lo[1] >> lg) <= 1 && (hi[2] >> lg) - (lo[2] >> lg) <= 1;
+      if (fast) {
 #pragma unroll
-      for (int u = 0; u < kFaceSets; u++) {
-        const int t = u * 64 + lane;
-        const int qy = (int)(((float)t + 0.5f) * rx);
-        const int iy = lo[1] + qy, ix = lo[0] + (t - qy * nx);
-        fv[u] = t < nxy;
-        const float dy = (float)iy - y0, dx = (float)ix - x0;
-        fdy2[u] = dy * dy;
-        fdx2[u] = dx * dx;
-        fblk[u] = ((iy >> lg) - bya) * nbtx + ((ix >> lg) - bxa);
-        ftile[u] = (iy - ty0) * kTile + (ix - tx0);
+        for (int e = 0; e < 8; e++) {
+          const int bx = ((e & 1) ? hi[0] : lo[0]) >> lg, by = ((e & 2) ? hi[1] : lo[1]) >> lg,
+                    bz = ((e & 4) ? hi[2] : lo[2]) >> lg;
+          fast = fast && bc[((bz - tg.bza) * tg.nbty + (by - tg.bya)) * tg.nbtx + (bx - tg.bxa)] >= 0;
+        }
       }
-      const int nsets = (nxy + 63) >> 6;
-      // the box spans at most two blocks in z: both codes per lane, read once
-      const int bzlo = lo[2] >> lg;
-      const int zb0 = (bzlo - bza) * (nbty * nbtx), zb1 = ((hi[2] >> lg) - bza) * (nbty * nbtx);
-      int fc0[kFaceSets], fc1[kFaceSets];
+      const float z0l = lz - 0.5f;
+      float dz2[T::MAXPL];
 #pragma unroll
-      for (int u = 0; u < kFaceSets; u++) {
-        fc0[u] = fv[u] ? bcode[zb0 + fblk[u]] : 0;
-        fc1[u] = fv[u] ? bcode[zb1 + fblk[u]] : 0;
+      for (int k = 0; k < T::MAXPL; k++) {
+        const float dz = (float)(lo[2] + k) - z0l;
+        dz2[k] = dz * dz;
       }
-      bool done = false;
-      for (int iz = lo[2]; iz <= hi[2] && !done; iz++) {
-        const float dz = (float)iz - z0, dz2 = dz * dz;
-        const bool zlo = (iz >> lg) == bzlo;
-        const int tz = (iz - tz0) * (kTile * kTile);
+      // tile index of the box origin, and t -> t / nx as (t * magic) >> 16,
+      // exact for t < 64 and nx <= 64
+      const int org = ((lo[2] - tg.tz0) * T::TY + (lo[1] - tg.ty0)) * T::TX + (lo[0] - tg.tx0);
+      const uint32_t magic = fast ? (65535u + (uint32_t)nx) / (uint32_t)nx : 0u;
+      const int info = (live ? 1 : 0) | (fast ? 2 : 0) | ((nx & 63) << 2) | ((ny & 63) << 8) |
+                       ((nz & 63) << 14);
+
+      const uint32_t left = (mc.cnt - qb + nwaves - 1) / nwaves;
+      const int nq = left < 64u ? (int)left : 64;
+      for (int kq = 0; kq < nq; kq++) {
+        const int inf = lane_i(info, kq);
+        if (!(inf & 1)) continue;
+        if (inf & 2) {
+          // lanes over the box's (y, x) face, the planes in z order; lanes
+          // past the face take +inf distances, whose codes change nothing
+          const int fnx = (inf >> 2) & 63, fny = (inf >> 8) & 63, fnz = (inf >> 14) & 63;
+          const int qy = (int)(((uint32_t)lane * (uint32_t)lane_i((int)magic, kq)) >> 16);
+          const int rx = lane - qy * fnx;
+          const bool on = lane < fnx * fny;
+          const float dx = (float)(lane_i(lo[0], kq) + rx) - lane_f(lx - 0.5f, kq);
+          const float dy = (float)(lane_i(lo[1], kq) + qy) - lane_f(ly - 0.5f, kq);
+          const float dx2 = dx * dx, dy2 = on ? dy * dy : __builtin_inff();
+          uint32_t *const t = tile + (on ? lane_i(org, kq) + qy * T::TX + rx : 0);
+          auto planes = [&](auto one) {
+#pragma unroll
+            for (int k = 0; k < T::MAXPL; k++) {
+              if (k >= fnz) break;
+              const float dzy = lane_f(dz2[k], kq) + dy2;  // dz*dz + dy*dy, then + dx*dx
+              atomicMin(t + k * T::PLANE,
+                        splat_code<decltype(one)::value>(dzy + dx2, distSqMaxInv, codeScale));
+            }
+          };
+          if (oneMul)
+            planes(std::true_type{});
+          else
+            planes(std::false_type{});
+          continue;
+        }
+        // a box that may touch a block without a page: lanes over the face
+        // in y,x order (at most kFaceSets sets of 64), a ballot per plane
+        const float p[3] = {lane_f(lx, kq), lane_f(ly, kq), lane_f(lz, kq)};
+        int blo[3], bhi[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          blo[k] = lane_i(lo[k], kq);
+          bhi[k] = lane_i(hi[k], kq);
+        }
+        const int bnx = bhi[0] - blo[0] + 1, bny = bhi[1] - blo[1] + 1;
+        const int nxy = bnx * bny;
+        const float x0 = p[0] - 0.5f, y0 = p[1] - 0.5f, z0 = p[2] - 0.5f;
+        // t -> (t / nx, t % nx) through (t + 0.5) / nx in float: exact for
+        // t < 2^15 and nx <= 64
+        const float rx = 1.0f / (float)bnx;
+        bool fv[kFaceSets];
+        float fdy2[kFaceSets], fdx2[kFaceSets];
+        int fblk[kFaceSets], ftile[kFaceSets];
 #pragma unroll
         for (int u = 0; u < kFaceSets; u++) {
-          if (u >= nsets || done) break;
-          const float dzy = dz2 + fdy2[u];  // dz*dz + dy*dy, then + dx*dx
-          const float d2 = dzy + fdx2[u];
-          const bool inR = fv[u] && !(d2 > distSqMax);
-          const int c = zlo ? fc0[u] : fc1[u];
-          const uint64_t bad = __ballot(inR && c < 0);
-          const int lim = bad ? __builtin_ctzll(bad) : 64;
-          if (inR && lane < lim) {
-            const float f = d2 * distSqMaxInv;
-            const uint32_t v = (uint32_t)(uint16_t)(int)__builtin_roundf(f * 65535.0f);
-            atomicMin(&tile[tz + ftile[u]], v);
+          const int t = u * 64 + lane;
+          const int qy = (int)(((float)t + 0.5f) * rx);
+          const int iy = blo[1] + qy, ix = blo[0] + (t - qy * bnx);
+          fv[u] = t < nxy;
+          const float dy = (float)iy - y0, dx = (float)ix - x0;
+          fdy2[u] = dy * dy;
+          fdx2[u] = dx * dx;
+          fblk[u] = ((iy >> lg) - tg.bya) * tg.nbtx + ((ix >> lg) - tg.bxa);
+          ftile[u] = (iy - tg.ty0) * T::TX + (ix - tg.tx0);
+        }
+        const int nsets = (nxy + 63) >> 6;
+        // the box spans at most two blocks in z: both codes per lane, read once
+        const int bzlo = blo[2] >> lg;
+        const int zb0 = (bzlo - tg.bza) * (tg.nbty * tg.nbtx);
+        const int zb1 = ((bhi[2] >> lg) - tg.bza) * (tg.nbty * tg.nbtx);
+        int fc0[kFaceSets], fc1[kFaceSets];
+#pragma unroll
+        for (int u = 0; u < kFaceSets; u++) {
+          fc0[u] = fv[u] ? bc[zb0 + fblk[u]] : 0;
+          fc1[u] = fv[u] ? bc[zb1 + fblk[u]] : 0;
+        }
+        bool done = false;
+        for (int iz = blo[2]; iz <= bhi[2] && !done; iz++) {
+          const float dz = (float)iz - z0, dz2s = dz * dz;
+          const bool zlo = (iz >> lg) == bzlo;
+          const int tz = (iz - tg.tz0) * T::PLANE;
+#pragma unroll
+          for (int u = 0; u < kFaceSets; u++) {
+            if (u >= nsets || done) break;
+            const float dzy = dz2s + fdy2[u];  // dz*dz + dy*dy, then + dx*dx
+            const float d2 = dzy + fdx2[u];
+            const bool inR = fv[u] && !(d2 > distSqMax);
+            const int c = zlo ? fc0[u] : fc1[u];
+            const uint64_t bad = __ballot(inR && c < 0);
+            const int lim = bad ? __builtin_ctzll(bad) : 64;
+            if (inR && lane < lim)
+              atomicMin(&tile[tz + ftile[u]], splat_code<false>(d2, distSqMaxInv, 0.0f));
+            done = bad != 0;  // !isValueBlock -> goto loop_exit
           }
-          done = bad != 0;  // !isValueBlock -> goto loop_exit
         }
       }
     }
-  }
-  __syncthreads();
 
-  // merge the touched voxels (only value blocks were touched), the loads of
-  // kMergeBatch voxels in flight together
-  const int64_t pageElems = (int64_t)1 << (3 * lg);
-  const int msk = (1 << lg) - 1;
-  for (int e0 = threadIdx.x; e0 < kTileVox; e0 += kMergeBatch * kSplatThreads) {
-    uint32_t v[kMergeBatch], cur[kMergeBatch];
-    int64_t el[kMergeBatch];
+    // fetch the next cell while this one merges
+    const int cn = ci + stride;
+    const bool more = cn < ncells;
+    CellMeta mnnRaw{0, 0, 0, 0};
+    TileGeo tn = tg;
+    CellFetch fn{0, 0.0f, 0.0f, 0.0f};
+    if (more) {
+      if (cn + stride < ncells) mnnRaw = meta_raw(cn + stride);
+      tn = tile_geo<H>(G, mn.key, lg);
+      fn = fetch(mn, tn);
+    }
+    __syncthreads();  // every particle is in the tile
+
+    // ---- merge: a thread per pair of voxels in x over kMergeZ planes of
+    // the tile (only value blocks were touched). The tile's values are read
+    // and the tile emptied first, then the pages' values of the touched pairs
+    // are loaded together, then the smaller codes stored.
+    // (the thread's item is derived anew each cell, not kept live across the walk)
+    int mt = tid;
+    __asm__ volatile("" : "+v"(mt));
+    if (mt < T::ITEMS) {
+      constexpr int NP = T::TX / 2;
+      const int part = mt / (NP * T::TY), pc = mt - part * (NP * T::TY);
+      const int ty = pc / NP, jx = pc - ty * NP;
+      const int ix = tg.tx0 + 2 * jx, iy = tg.ty0 + ty, tzb = part * kMergeZ;
+      const int tbase = ty * T::TX + 2 * jx;
+      // the pair's two codes packed in one word (codes are at most 0xffff)
+      // planes past the tile (the last run only) read the tile's last plane,
+      // which is this thread's own, and count as untouched
+      uint32_t v[kMergeZ];
 #pragma unroll
-    for (int u = 0; u < kMergeBatch; u++) {
-      const int e = e0 + u * kSplatThreads;
-      v[u] = e < kTileVox ? tile[e] : kNoCode;
-      el[u] = 0;
-      if (v[u] != kNoCode) {
-        const int tz = e / (kTile * kTile), r = e - tz * (kTile * kTile), ty = r / kTile;
-        const int ix = tx0 + (r - ty * kTile), iy = ty0 + ty, iz = tz0 + tz;
-        const int c =
-            bcode[(((iz >> lg) - bza) * nbty + ((iy >> lg) - bya)) * nbtx + ((ix >> lg) - bxa)];
-        el[u] = c * pageElems + ((((iz & msk) << lg) + (iy & msk)) << lg) + (ix & msk);
+      for (int u = 0; u < kMergeZ; u++) {
+        const int tz = tzb + u < T::TZ ? tzb + u : T::TZ - 1;
+        const uint2 w = *(const uint2 *)&tile[tz * T::PLANE + tbase];
+        v[u] = tzb + u < T::TZ ? w.x | (w.y << 16) : 0xffffffffu;
+      }
+#pragma unroll
+      for (int u = 0; u < kMergeZ; u++) {
+        const int tz = tzb + u < T::TZ ? tzb + u : T::TZ - 1;
+        *(uint2 *)&tile[tz * T::PLANE + tbase] = make_uint2(kNoCode, kNoCode);
+      }
+      // voxels outside the domain are never touched; ix is even and the
+      // domain's x size a multiple of the block size, so ix + 1 is inside too
+      if (ix >= 0 && iy >= 0 && ix < G.sx && iy < G.sy) {
+        const int msk = (1 << lg) - 1;
+        const int colBlk = ((iy >> lg) - tg.bya) * tg.nbtx + ((ix >> lg) - tg.bxa);
+        const int64_t colEl = ((int64_t)(iy & msk) << lg) + (ix & msk);
+        // the run's planes in the domain lie in at most two blocks in z
+        // (kMergeZ <= the block edge)
+        int za = tg.tz0 + tzb, zb = za + kMergeZ - 1;
+        za = za > 0 ? za : 0;
+        zb = zb < G.sz - 1 ? zb : G.sz - 1;
+        const int zbl = za >> lg;
+        uint8_t *pb[2] = {pages, pages};
+        if (za <= zb) {
+          const int c0 = bc[(zbl - tg.bza) * (tg.nbty * tg.nbtx) + colBlk];
+          const int c1 = bc[((zb >> lg) - tg.bza) * (tg.nbty * tg.nbtx) + colBlk];
+          pb[0] = pages + ((int64_t)c0 * ((int64_t)1 << (3 * lg)) + colEl) * ESZ;
+          pb[1] = pages + ((int64_t)c1 * ((int64_t)1 << (3 * lg)) + colEl) * ESZ;
+        }
+        auto at = [&](int u) {
+          const int iz = tg.tz0 + tzb + u;
+          return pb[(iz >> lg) != zbl] + ((int64_t)(iz & msk) << (2 * lg)) * ESZ;
+        };
+        // the pages' codes of the touched pairs, packed the same way
+        uint32_t cur[kMergeZ];
+#pragma unroll
+        for (int u = 0; u < kMergeZ; u++) {
+          cur[u] = 0xffffffffu;
+          if (v[u] != 0xffffffffu) {
+            const uint8_t *q = at(u);
+            if constexpr (ESZ == 2) {
+              cur[u] = *(const uint32_t *)q;
+            } else if constexpr (ESZ == 4) {
+              const uint2 w = *(const uint2 *)q;  // codes: the splat starts them at 65535
+              cur[u] = w.x | (w.y << 16);
+            } else {  // FP64 pages hold the code in the low 32-bit word of the element
+              cur[u] = *(const uint32_t *)q | (*(const uint32_t *)(q + 8) << 16);
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kMergeZ; u++) {
+          const uint32_t lo = (v[u] & 0xffffu) < (cur[u] & 0xffffu) ? v[u] & 0xffffu : cur[u] & 0xffffu;
+          const uint32_t hi = (v[u] >> 16) < (cur[u] >> 16) ? v[u] >> 16 : cur[u] >> 16;
+          const uint32_t w = lo | (hi << 16);
+          if (w == cur[u]) continue;
+          uint8_t *q = at(u);
+          if constexpr (ESZ == 2) {
+            *(uint32_t *)q = w;
+          } else if constexpr (ESZ == 4) {
+            *(uint2 *)q = make_uint2(lo, hi);
+          } else {
+            *(uint32_t *)q = lo;
+            *(uint32_t *)(q + 8) = hi;
+          }
+        }
       }
     }
-#pragma unroll
-    for (int u = 0; u < kMergeBatch; u++) {
-      cur[u] = kNoCode;
-      if (v[u] != kNoCode) {
-        if constexpr (ESZ == 2)
-          cur[u] = ((const uint16_t *)pages)[el[u]];
-        else  // FP32 / FP64 pages hold the code in the low 32-bit word of the element
-          cur[u] = *(const uint32_t *)(pages + el[u] * ESZ);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kMergeBatch; u++) {
-      if (v[u] == kNoCode || v[u] >= cur[u]) continue;
-      if constexpr (ESZ == 2)
-        ((uint16_t *)pages)[el[u]] = (uint16_t)v[u];
-      else
-        *(uint32_t *)(pages + el[u] * ESZ) = v[u];
-    }
+    if (!more) break;
+    ci = cn;
+    mc = mn;
+    mnRaw = mnnRaw;
+    tg = tn;
+    fc = fn;
+    buf ^= 1;
   }
 }
 
-// distance codes start at 65535 (UNORM 1.0, msbg_demo.cpp:494-496) in the
-// pages of the non-UNORM16 storages
 template <typename W>
 __global__ void k_init_codes(W *__restrict__ p, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -432,18 +677,79 @@ CellGrid cell_grid(const msbg_grid *g) {
   return G;
 }
 
-int splat_binned(msbg_grid *g, ChanLevel &cl, int esz, float rScan, float distSqMax,
+// The tile halo for a footprint radius: every box lies within H voxels of
+// its particle's cell when H > rScan - 0.5 plus the float rounding of the
+// box bounds (two roundings at coordinates below 2^e: 2^(e-23)).
+int splat_halo(const msbg_grid *g, float rScan) {
+  const double m = (double)std::max(std::max(g->sx, g->sy), g->sz) + rScan + 1.0;
+  const double slack = std::ldexp(1.0, (int)std::ceil(std::log2(m)) - 23);
+  const int h = (int)std::floor((double)rScan - 0.5 + slack) + 1;
+  return h > 1 ? h : 1;
+}
+
+int device_cu_count() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+  }
+  return cus;
+}
+
+template <int ESZ, int H>
+int launch_splat_tile(hipStream_t s, int ncells, const float *sorted, const CellMeta *meta,
+                      const CellGrid &G, float rScan, float distSqMax, float distSqMaxInv,
+                      float codeScale, const msbg_grid *g, const ChanLevel &cl) {
+  auto kern = k_splat_tile<ESZ, H>;
+  int per = 0;
+  MSBG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, kSplatThreads, 0));
+  const int nwg = std::max(1, std::min(ncells, (per > 0 ? per : 1) * device_cu_count()));
+  hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(kSplatThreads), 0, s, sorted, meta, ncells, G,
+                     rScan, distSqMax, distSqMaxInv, codeScale, g->lg0, g->nbx, g->nby,
+                     cl.d_page.p, cl.d_pages.p);
+  MSBG_HIP(hipGetLastError());
+  return 0;
+}
+
+template <int ESZ>
+int launch_splat_tile_h(int h, hipStream_t s, int ncells, const float *sorted,
+                        const CellMeta *meta, const CellGrid &G, float rScan, float distSqMax,
+                        float distSqMaxInv, float codeScale, const msbg_grid *g,
+                        const ChanLevel &cl) {
+  switch (h) {
+    case 1: return launch_splat_tile<ESZ, 1>(s, ncells, sorted, meta, G, rScan, distSqMax, distSqMaxInv, codeScale, g, cl);
+    case 2: return launch_splat_tile<ESZ, 2>(s, ncells, sorted, meta, G, rScan, distSqMax, distSqMaxInv, codeScale, g, cl);
+    case 3: return launch_splat_tile<ESZ, 3>(s, ncells, sorted, meta, G, rScan, distSqMax, distSqMaxInv, codeScale, g, cl);
+    case 4: return launch_splat_tile<ESZ, 4>(s, ncells, sorted, meta, G, rScan, distSqMax, distSqMaxInv, codeScale, g, cl);
+  }
+  set_error("splat: tile halo %d", h);
+  return MSBG_EINVAL;
+}
+
+int splat_binned(msbg_grid *g, ChanLevel &cl, int esz, int halo, float rScan, float distSqMax,
                  float distSqMaxInv) {
   hipStream_t s = g->stream;
   MSBG_TRY(ensure_binned(g));
   const CellGrid G = cell_grid(g);
+  // one multiply for the code when distSqMaxInv is a power of two: the
+  // squared distances are 0 or at least 2^-50 (differences of floats from
+  // whole voxel positions), so d2 * distSqMaxInv stays exact and normal
+  int ex = 0;
+  const float codeScale = std::frexp(distSqMaxInv, &ex) == 0.5f && ex > -60 && ex < 60
+                              ? 65535.0f * distSqMaxInv
+                              : 0.0f;
   const int64_t nc = G.nkeys;
   // the even cells (colour 0) are the most numerous colour
   const int64_t cap = (int64_t)((G.ncx + 1) / 2) * ((G.ncy + 1) / 2) * ((G.ncz + 1) / 2);
   DevBuf<uint32_t> ncol;
   DevBuf<int32_t> lists;
+  DevBuf<CellMeta> meta;
   MSBG_TRY(ncol.alloc(8));
   MSBG_TRY(lists.alloc(8 * cap));
+  MSBG_TRY(meta.alloc(cap));
   MSBG_HIP(hipMemsetAsync(ncol.p, 0, sizeof(uint32_t) * 8, s));
   const uint32_t *cnt = g->bin_cnt.p, *off = g->bin_off.p;
   const float *sorted = g->bin_xyz.p;
@@ -461,19 +767,20 @@ int splat_binned(msbg_grid *g, ChanLevel &cl, int esz, float rScan, float distSq
   MSBG_HIP(hipMemcpyAsync(h, ncol.p, sizeof(h), hipMemcpyDeviceToHost, s));
   MSBG_HIP(hipStreamSynchronize(s));
   for (int col = 0; col < 8; col++) {
-    if (!h[col]) continue;
-    auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(h[col]), dim3(kSplatThreads), 0, s, sorted, off, cnt,
-                         lists.p + col * cap, G, rScan, distSqMax, distSqMaxInv, g->lg0, g->nbx,
-                         g->nby, cl.d_page.p, cl.d_pages.p);
-    };
-    if (esz == 2)
-      go(k_splat_tile<2>);
-    else if (esz == 4)
-      go(k_splat_tile<4>);
-    else
-      go(k_splat_tile<8>);
+    const int n = (int)h[col];
+    if (!n) continue;
+    hipLaunchKernelGGL(k_cell_meta, dim3((unsigned)((n + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, s, lists.p + col * cap, n, off, cnt, meta.p);
     MSBG_HIP(hipGetLastError());
+    if (esz == 2)
+      MSBG_TRY(launch_splat_tile_h<2>(halo, s, n, sorted, meta.p, G, rScan, distSqMax,
+                                      distSqMaxInv, codeScale, g, cl));
+    else if (esz == 4)
+      MSBG_TRY(launch_splat_tile_h<4>(halo, s, n, sorted, meta.p, G, rScan, distSqMax,
+                                      distSqMaxInv, codeScale, g, cl));
+    else
+      MSBG_TRY(launch_splat_tile_h<8>(halo, s, n, sorted, meta.p, G, rScan, distSqMax,
+                                      distSqMaxInv, codeScale, g, cl));
   }
   // the work buffers are freed on return
   MSBG_HIP(hipStreamSynchronize(s));
@@ -550,10 +857,10 @@ int launch_splat(msbg_grid *g, Channel &ch, float rScan) {
   if (!g->npoints) return 0;
   const float distSqMax = rScan * rScan, distSqMaxInv = 1.0f / distSqMax;
   int esz = storage_bytes(ch.storage);
-  // the binned splat needs every footprint inside its cell's tile, with a
-  // quarter voxel to spare for the float rounding of the footprint bounds
-  if (rScan + 0.5f <= (float)kHalo - 0.25f && g->npoints < ((int64_t)1 << 31) && esz != 1)
-    return splat_binned(g, cl, esz, rScan, distSqMax, distSqMaxInv);
+  // the binned splat needs every footprint inside its cell's tile
+  const int halo = splat_halo(g, rScan);
+  if (halo <= kMaxHalo && g->npoints < ((int64_t)1 << 31) && esz != 1)
+    return splat_binned(g, cl, esz, halo, rScan, distSqMax, distSqMaxInv);
   int64_t nb = (g->npoints + kThreads - 1) / kThreads;
   auto args = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(kThreads), 0, g->stream, g->d_xyz.p,
