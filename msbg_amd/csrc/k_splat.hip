@@ -282,7 +282,13 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
   int ci = (int)blockIdx.x;
   if (ci >= ncells) return;
   const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nwaves = (int)(blockDim.x >> 6);
+  constexpr uint32_t kWaves = kSplatThreads / 64;
+  // particles per wave and round: the cell's particles in kWaves contiguous
+  // chunks of at most 64, so that no wave sets up an empty chunk
+  auto chunk = [](uint32_t np) {
+    const uint32_t c = (np + kWaves - 1) / kWaves;
+    return c < 1u ? 1u : (c > 64u ? 64u : c);
+  };
   for (int e = tid; e < T::NV / 4; e += (int)blockDim.x)
     ((uint4 *)tile)[e] = make_uint4(kNoCode, kNoCode, kNoCode, kNoCode);
 
@@ -292,8 +298,8 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
       const int ex = tid % t.nbtx, ey = (tid / t.nbtx) % t.nbty, ez = tid / (t.nbtx * t.nbty);
       f.bval = pagemap[((int64_t)(t.bza + ez) * nby + (t.bya + ey)) * nbx + (t.bxa + ex)];
     }
-    const uint32_t ql = (uint32_t)(wave + lane * nwaves);
-    if (ql < m.cnt) {
+    const uint32_t C = chunk(m.cnt), ql = (uint32_t)wave * C + (uint32_t)lane;
+    if ((uint32_t)lane < C && ql < m.cnt) {
       const float *pp = sorted + 3 * (int64_t)(m.off + ql);
       f.x = pp[0];
       f.y = pp[1];
@@ -327,16 +333,19 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
     const CellMeta mn = ci + stride < ncells ? uni(mnRaw) : CellMeta{0, 0, 0, 0};
     __syncthreads();  // block codes in, the previous merge has emptied the tile
 
-    // ---- walk: the wave's particles are wave, wave + nwaves, ...; each lane
-    // holds one of the next 64 and sets it up, then the wave walks them
-    for (uint32_t qb = (uint32_t)wave; qb < mc.cnt; qb += 64u * nwaves) {
+    // ---- walk: the wave's particles are the chunks wave, wave + kWaves, ...;
+    // each lane holds one particle of the chunk and sets it up, then the
+    // wave walks them
+    const uint32_t C = chunk(mc.cnt);
+    for (uint32_t qb = (uint32_t)wave * C; qb < mc.cnt; qb += kWaves * C) {
       int lane = tid & 63;
       __asm__ volatile("" : "+v"(lane));
-      const uint32_t ql = qb + (uint32_t)lane * nwaves;
+      const uint32_t ql = qb + (uint32_t)lane;
+      const bool mine = (uint32_t)lane < C && ql < mc.cnt;
       float lx = fc.x, ly = fc.y, lz = fc.z;
-      if (qb != (uint32_t)wave) {
+      if (qb != (uint32_t)wave * C) {
         lx = ly = lz = 0.0f;
-        if (ql < mc.cnt) {
+        if (mine) {
           const float *pp = sorted + 3 * (int64_t)(mc.off + ql);
           lx = pp[0];
           ly = pp[1];
@@ -357,7 +366,7 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
         hi[k] = b < smax[k] ? b : smax[k];
       }
       const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
-      const bool live = ql < mc.cnt && nx > 0 && ny > 0 && nz > 0;
+      const bool live = mine && nx > 0 && ny > 0 && nz > 0;
       bool fast = live && nx * ny <= 64 && nz <= T::MAXPL && (hi[0] >> lg) - (lo[0] >> lg) <= 1 &&
                   (hi[1] >> lg) - (lo[1] >> lg) <= 1 && (hi[2] >> lg) - (lo[2] >> lg) <= 1;
       if (fast) {
@@ -378,12 +387,15 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
       // tile index of the box origin, and t -> t / nx as (t * magic) >> 16,
       // exact for t < 64 and nx <= 64
       const int org = ((lo[2] - tg.tz0) * T::TY + (lo[1] - tg.ty0)) * T::TX + (lo[0] - tg.tx0);
-      const uint32_t magic = fast ? (65535u + (uint32_t)nx) / (uint32_t)nx : 0u;
+      // (65536 / nx through the reciprocal: within 0.008 of the quotient,
+      // whose fraction is 0 or at least 1/64, so the floor is exact)
+      const uint32_t magic =
+          fast ? (uint32_t)(65536.0f * __builtin_amdgcn_rcpf((float)nx)) + 1u : 0u;
       const int info = (live ? 1 : 0) | (fast ? 2 : 0) | ((nx & 63) << 2) | ((ny & 63) << 8) |
                        ((nz & 63) << 14);
 
-      const uint32_t left = (mc.cnt - qb + nwaves - 1) / nwaves;
-      const int nq = left < 64u ? (int)left : 64;
+      const uint32_t left = mc.cnt - qb;
+      const int nq = left < C ? (int)left : (int)C;
       for (int kq = 0; kq < nq; kq++) {
         const int inf = lane_i(info, kq);
         if (!(inf & 1)) continue;
