@@ -367,15 +367,18 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
       }
       const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
       const bool live = mine && nx > 0 && ny > 0 && nz > 0;
-      bool fast = live && nx * ny <= 64 && nz <= T::MAXPL && (hi[0] >> lg) - (lo[0] >> lg) <= 1 &&
+      bool fast = live && __mul24(nx, ny) <= 64 && nz <= T::MAXPL && (hi[0] >> lg) - (lo[0] >> lg) <= 1 &&
                   (hi[1] >> lg) - (lo[1] >> lg) <= 1 && (hi[2] >> lg) - (lo[2] >> lg) <= 1;
       if (fast) {
+        // (block indices under a tile are below 2^24: 24-bit products)
+        const int xl = (lo[0] >> lg) - tg.bxa, xh = (hi[0] >> lg) - tg.bxa;
+        const int yl = (lo[1] >> lg) - tg.bya, yh = (hi[1] >> lg) - tg.bya;
+        const int zl = __mul24((lo[2] >> lg) - tg.bza, tg.nbty);
+        const int zh = __mul24((hi[2] >> lg) - tg.bza, tg.nbty);
+        const int r[4] = {__mul24(zl + yl, tg.nbtx), __mul24(zl + yh, tg.nbtx),
+                          __mul24(zh + yl, tg.nbtx), __mul24(zh + yh, tg.nbtx)};
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-          const int bx = ((e & 1) ? hi[0] : lo[0]) >> lg, by = ((e & 2) ? hi[1] : lo[1]) >> lg,
-                    bz = ((e & 4) ? hi[2] : lo[2]) >> lg;
-          fast = fast && bc[((bz - tg.bza) * tg.nbty + (by - tg.bya)) * tg.nbtx + (bx - tg.bxa)] >= 0;
-        }
+        for (int e = 0; e < 4; e++) fast = fast && bc[r[e] + xl] >= 0 && bc[r[e] + xh] >= 0;
       }
       const float z0l = lz - 0.5f;
       float dz2[T::MAXPL];
@@ -403,9 +406,9 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
           // lanes over the box's (y, x) face, the planes in z order; lanes
           // past the face take +inf distances, whose codes change nothing
           const int fnx = (inf >> 2) & 63, fny = (inf >> 8) & 63, fnz = (inf >> 14) & 63;
-          const int qy = (int)(((uint32_t)lane * (uint32_t)lane_i((int)magic, kq)) >> 16);
-          const int rx = lane - qy * fnx;
-          const bool on = lane < fnx * fny;
+          const int qy = (int)(__umul24((uint32_t)lane, (uint32_t)lane_i((int)magic, kq)) >> 16);
+          const int rx = lane - __mul24(qy, fnx);
+          const bool on = lane < __mul24(fnx, fny);
           const float dx = (float)(lane_i(lo[0], kq) + rx) - lane_f(lx - 0.5f, kq);
           const float dy = (float)(lane_i(lo[1], kq) + qy) - lane_f(ly - 0.5f, kq);
           const float dx2 = dx * dx, dy2 = on ? dy * dy : __builtin_inff();
@@ -548,17 +551,19 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
           pb[0] = pages + ((int64_t)c0 * ((int64_t)1 << (3 * lg)) + colEl) * ESZ;
           pb[1] = pages + ((int64_t)c1 * ((int64_t)1 << (3 * lg)) + colEl) * ESZ;
         }
-        auto at = [&](int u) {
-          const int iz = tg.tz0 + tzb + u;
-          return pb[(iz >> lg) != zbl] + ((int64_t)(iz & msk) << (2 * lg)) * ESZ;
-        };
         // the pages' codes of the touched pairs, packed the same way
+        uint8_t *qa[kMergeZ];
+#pragma unroll
+        for (int u = 0; u < kMergeZ; u++) {
+          const int iz = tg.tz0 + tzb + u;
+          qa[u] = pb[(iz >> lg) != zbl] + ((int64_t)(iz & msk) << (2 * lg)) * ESZ;
+        }
         uint32_t cur[kMergeZ];
 #pragma unroll
         for (int u = 0; u < kMergeZ; u++) {
           cur[u] = 0xffffffffu;
           if (v[u] != 0xffffffffu) {
-            const uint8_t *q = at(u);
+            const uint8_t *q = qa[u];
             if constexpr (ESZ == 2) {
               cur[u] = *(const uint32_t *)q;
             } else if constexpr (ESZ == 4) {
@@ -575,7 +580,7 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
           const uint32_t hi = (v[u] >> 16) < (cur[u] >> 16) ? v[u] >> 16 : cur[u] >> 16;
           const uint32_t w = lo | (hi << 16);
           if (w == cur[u]) continue;
-          uint8_t *q = at(u);
+          uint8_t *q = qa[u];
           if constexpr (ESZ == 2) {
             *(uint32_t *)q = w;
           } else if constexpr (ESZ == 4) {
