@@ -501,10 +501,14 @@ __device__ __forceinline__ void sweep_block(const SweepGrid *__restrict__ grids,
   const int64_t npages = G.npages;
   const float dtf = G.dt, lo01 = G.lo, hi01 = G.hi;
 
-  __shared__ __align__(16) L bufs[BPW][3][PL];
+  // 8^3 blocks stage all BSX + 2 planes at once (WHOLE); larger blocks march
+  // z through a ring of three planes
+  constexpr bool WHOLE = BSX <= 8;
+  constexpr int NBUF = WHOLE ? BSX + 2 : 3;
+  __shared__ __align__(16) L bufs[BPW][NBUF][PL];
   __shared__ int codesAll[BPW][27];
   __shared__ double red[(NT + 63) / 64];
-  L(&buf)[3][PL] = bufs[sub];
+  L(&buf)[NBUF][PL] = bufs[sub];
   int *codes = codesAll[sub];
 
   const int page = item.y;
@@ -865,71 +869,101 @@ __device__ __forceinline__ void sweep_block(const SweepGrid *__restrict__ grids,
   };
 
   PT pA, pB, pC;
-  // Plane p of the march (-1..BSX) is loaded PFD steps before its LDS store
-  // and staged one step before its patch is read: the patch reads of a step
-  // go to a plane stored by the previous step and are issued before this
-  // step's stores, so they do not queue behind them. PFD is three for the
-  // FP16 kernels of one wave per workgroup (registers to spare at two waves
-  // per SIMD; a step is about as long as a loaded HBM round trip), one
-  // otherwise. Plane p is held in stg[(p + 3) % 3] between its load and its
-  // LDS store and staged in buf[(p + 3) % 3], so that every step of the
-  // three-step loop below has fixed registers and LDS offsets.
-  constexpr int PFD = (H16 && NT * BPW >= 64) ? 3 : 1;
-  static_assert(PFD >= 1 && PFD <= 3, "prefetch distance");
-  Staged stg[3];
-
-  // prologue: planes -1, 0, 1 staged, 2..1+PFD in flight, patches of -1, 0
-  load_outer(false, stg[2]);
-  load_inner(0, stg[0]);
-  load_inner(1, stg[1]);
-  put_plane(buf[2], stg[2]);
-  load_inner(2, stg[2]);
-  put_plane(buf[0], stg[0]);
-  if (PFD >= 2) load_inner(3, stg[0]);
-  put_plane(buf[1], stg[1]);
-  if (PFD >= 3) load_inner(4, stg[1]);
-  __syncthreads();
-  get_patch(buf[2], pA);
-  get_patch(buf[0], pB);
-  // step 0 stages plane 2 over plane -1: every wave's patch of plane -1
-  // must be read first (later steps overwrite planes read two barriers back)
-  __syncthreads();
-
-  // step z: read the patch of plane z+1 (staged in buf[nb]), stage plane
-  // z+2 (ps) and prefetch plane z+2+PFD into its registers (ld 1: a plane of
-  // the block, 2: the plane above it, 0: none), compute plane z. Plane z+2
-  // overwrites plane z-1, whose patch was read two barriers earlier.
-  auto step = [&](int z, int nb, bool ps, int ld, const PT &Dp, const PT &Mp, PT &Up) {
-    const int ns = (nb + 1) % 3;  // slot and buffer of plane z+2
-    get_patch(buf[nb], Up);
-    if (ps) put_plane(buf[ns], stg[ns]);
-    if (ld == 1)
-      load_inner(z + 2 + PFD, stg[(ns + PFD) % 3]);
-    else if (ld == 2)
-      load_outer(true, stg[(ns + PFD) % 3]);
-    __syncthreads();
-    compute(z, Dp, Mp, Up);
-  };
-
-  // steps whose prefetch is a plane of the block
-  constexpr int ZM = ((BSX - 2 - PFD) / 3) * 3;
-  for (int z = 0; z < ZM; z += 3) {
-    step(z, 1, true, 1, pA, pB, pC);
-    step(z + 1, 2, true, 1, pB, pC, pA);
-    step(z + 2, 0, true, 1, pC, pA, pB);
-  }
+  if constexpr (WHOLE) {
+    // An 8^3 block's planes -1..BSX are loaded together, so that the block
+    // waits for one load latency instead of one per few planes, then stored
+    // to LDS; every plane is computed from LDS after one barrier (the same
+    // values as the ring: the block's planes as they were at the start)
+    Staged st[BSX + 2];
+    load_outer(false, st[0]);
 #pragma unroll
-  for (int t = 0; t < BSX - ZM; t++) {  // the last 3..6 steps
-    const int z = ZM + t;
-    const int pl = z + 2 + PFD;
-    const int ld = pl < BSX ? 1 : (pl == BSX ? 2 : 0);
-    const bool ps = z + 2 <= BSX;
-    if (t % 3 == 0)
-      step(z, 1, ps, ld, pA, pB, pC);
-    else if (t % 3 == 1)
-      step(z, 2, ps, ld, pB, pC, pA);
-    else
-      step(z, 0, ps, ld, pC, pA, pB);
+    for (int z = 0; z < BSX; z++) load_inner(z, st[z + 1]);
+    load_outer(true, st[BSX + 1]);
+#pragma unroll
+    for (int q = 0; q < BSX + 2; q++) put_plane(buf[q], st[q]);
+    __syncthreads();
+    get_patch(buf[0], pA);
+    get_patch(buf[1], pB);
+#pragma unroll
+    for (int z = 0; z < BSX; z += 3) {
+      get_patch(buf[z + 2], pC);
+      compute(z, pA, pB, pC);
+      if (z + 1 < BSX) {
+        get_patch(buf[z + 3], pA);
+        compute(z + 1, pB, pC, pA);
+      }
+      if (z + 2 < BSX) {
+        get_patch(buf[z + 4], pB);
+        compute(z + 2, pC, pA, pB);
+      }
+    }
+  } else {
+    // Plane p of the march (-1..BSX) is loaded PFD steps before its LDS store
+    // and staged one step before its patch is read: the patch reads of a step
+    // go to a plane stored by the previous step and are issued before this
+    // step's stores, so they do not queue behind them. PFD is three for the
+    // FP16 kernels of one wave per workgroup (registers to spare at two waves
+    // per SIMD; a step is about as long as a loaded HBM round trip), one
+    // otherwise. Plane p is held in stg[(p + 3) % 3] between its load and its
+    // LDS store and staged in buf[(p + 3) % 3], so that every step of the
+    // three-step loop below has fixed registers and LDS offsets.
+    constexpr int PFD = (H16 && NT * BPW >= 64) ? 3 : 1;
+    static_assert(PFD >= 1 && PFD <= 3, "prefetch distance");
+    Staged stg[3];
+
+    // prologue: planes -1, 0, 1 staged, 2..1+PFD in flight, patches of -1, 0
+    load_outer(false, stg[2]);
+    load_inner(0, stg[0]);
+    load_inner(1, stg[1]);
+    put_plane(buf[2], stg[2]);
+    load_inner(2, stg[2]);
+    put_plane(buf[0], stg[0]);
+    if (PFD >= 2) load_inner(3, stg[0]);
+    put_plane(buf[1], stg[1]);
+    if (PFD >= 3) load_inner(4, stg[1]);
+    __syncthreads();
+    get_patch(buf[2], pA);
+    get_patch(buf[0], pB);
+    // step 0 stages plane 2 over plane -1: every wave's patch of plane -1
+    // must be read first (later steps overwrite planes read two barriers back)
+    __syncthreads();
+
+    // step z: read the patch of plane z+1 (staged in buf[nb]), stage plane
+    // z+2 (ps) and prefetch plane z+2+PFD into its registers (ld 1: a plane of
+    // the block, 2: the plane above it, 0: none), compute plane z. Plane z+2
+    // overwrites plane z-1, whose patch was read two barriers earlier.
+    auto step = [&](int z, int nb, bool ps, int ld, const PT &Dp, const PT &Mp, PT &Up) {
+      const int ns = (nb + 1) % 3;  // slot and buffer of plane z+2
+      get_patch(buf[nb], Up);
+      if (ps) put_plane(buf[ns], stg[ns]);
+      if (ld == 1)
+        load_inner(z + 2 + PFD, stg[(ns + PFD) % 3]);
+      else if (ld == 2)
+        load_outer(true, stg[(ns + PFD) % 3]);
+      __syncthreads();
+      compute(z, Dp, Mp, Up);
+    };
+
+    // steps whose prefetch is a plane of the block
+    constexpr int ZM = ((BSX - 2 - PFD) / 3) * 3;
+    for (int z = 0; z < ZM; z += 3) {
+      step(z, 1, true, 1, pA, pB, pC);
+      step(z + 1, 2, true, 1, pB, pC, pA);
+      step(z + 2, 0, true, 1, pC, pA, pB);
+    }
+  #pragma unroll
+    for (int t = 0; t < BSX - ZM; t++) {  // the last 3..6 steps
+      const int z = ZM + t;
+      const int pl = z + 2 + PFD;
+      const int ld = pl < BSX ? 1 : (pl == BSX ? 2 : 0);
+      const bool ps = z + 2 <= BSX;
+      if (t % 3 == 0)
+        step(z, 1, ps, ld, pA, pB, pC);
+      else if (t % 3 == 1)
+        step(z, 2, ps, ld, pB, pC, pA);
+      else
+        step(z, 0, ps, ld, pC, pA, pB);
+    }
   }
 
   if (partial) {
