@@ -186,14 +186,15 @@ def test_splat_abandon_quirk_gpu():
         assert np.array_equal(got, L.pages)
 
 
-# The binned LDS splat (rScan + 0.5 <= 4.75) and the per-particle atomic one
-# (wider footprints) against the oracle's colour-ordered splat: particles up
-# to the domain's faces and integer positions, value/EMPTY/FULL/NULL blocks
-# mixed (the early abandon), the code in every storage (the low 32 bits of
-# FP32/FP64).
+# The binned LDS splat (tile halos 1 to 4: rScan below 4.5) and the
+# per-particle atomic one (wider footprints) against the oracle's
+# colour-ordered splat: particles up to the domain's faces and integer
+# positions, value/EMPTY/FULL/NULL blocks mixed (the early abandon), the code
+# in every storage (the low 32 bits of FP32/FP64).
 @pytest.mark.parametrize("S,bsx,rscan,storage", [
     (64, 16, 4.0, M.U16), (48, 8, 1.5, M.U16), (64, 32, 4.25, M.F32), (64, 16, 6.0, M.U16),
-    (96, 32, 3.0, M.F64), (64, 16, 2.0, M.BF16), (64, 32, 4.5, M.F16), (128, 64, 4.0, M.U16)])
+    (96, 32, 3.0, M.F64), (64, 16, 2.0, M.BF16), (64, 32, 4.5, M.F16), (128, 64, 4.0, M.U16),
+    (48, 8, 1.0, M.F32)])
 def test_splat_paths_bitexact(S, bsx, rscan, storage):
     note("splat paths", S, bsx, rscan, storage)
     nb = S // bsx
