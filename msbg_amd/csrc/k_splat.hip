@@ -180,6 +180,11 @@ struct CellOfColour {
 };
 
 constexpr int kSplatThreads = 1024;
+// particles set up per round of the walk, and their set-up words in LDS:
+// info, magic, tile origin, x and y origin (ints), x0, y0, dz^2 of up to 9
+// planes, the position (floats)
+constexpr uint32_t kRound = 64;
+constexpr int kParWords = 5;
 // the merge takes a pair of voxels in x over kMergeZ planes per thread
 constexpr int kMergeZ = 8;
 // footprint faces of at most (2 kMaxHalo + 1)^2 voxels: kFaceSets sets of 64 lanes
@@ -199,6 +204,7 @@ struct TileDims {
   static_assert((TX / 8 + 1) * (TY / 8 + 1) * (TZ / 8 + 1) <= kTileBlk, "tile blocks");
   static_assert(ITEMS <= kSplatThreads, "one merge item per thread");
   static_assert((MAXPL - 1) * PLANE * 4 < 65536, "plane offsets");
+  static_assert(MAXPL <= 9, "dz^2 words of a particle's set-up");
   static_assert(NV % 4 == 0 && TX % 2 == 0, "tile fill, voxel pairs");
 };
 
@@ -263,7 +269,7 @@ __device__ inline TileGeo tile_geo(const CellGrid &G, int key, int lg) {
 }
 
 // what a thread fetches for a cell ahead of its walk: the page code of one
-// block under the tile and the first particle of its lane
+// block under the tile and, for the first kRound threads, one particle
 struct CellFetch {
   int bval;
   float x, y, z;
@@ -278,17 +284,12 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
   using T = TileDims<H>;
   __shared__ __align__(16) uint32_t tile[T::NV];
   __shared__ int32_t bcode[2][kTileBlk];
+  __shared__ uint4 par[kRound][kParWords];  // a round's particle set-up
   const int stride = (int)gridDim.x;
   int ci = (int)blockIdx.x;
   if (ci >= ncells) return;
   const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
   constexpr uint32_t kWaves = kSplatThreads / 64;
-  // particles per wave and round: the cell's particles in kWaves contiguous
-  // chunks of at most 64, so that no wave sets up an empty chunk
-  auto chunk = [](uint32_t np) {
-    const uint32_t c = (np + kWaves - 1) / kWaves;
-    return c < 1u ? 1u : (c > 64u ? 64u : c);
-  };
   for (int e = tid; e < T::NV / 4; e += (int)blockDim.x)
     ((uint4 *)tile)[e] = make_uint4(kNoCode, kNoCode, kNoCode, kNoCode);
 
@@ -298,9 +299,8 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
       const int ex = tid % t.nbtx, ey = (tid / t.nbtx) % t.nbty, ez = tid / (t.nbtx * t.nbty);
       f.bval = pagemap[((int64_t)(t.bza + ez) * nby + (t.bya + ey)) * nbx + (t.bxa + ex)];
     }
-    const uint32_t C = chunk(m.cnt), ql = (uint32_t)wave * C + (uint32_t)lane;
-    if ((uint32_t)lane < C && ql < m.cnt) {
-      const float *pp = sorted + 3 * (int64_t)(m.off + ql);
+    if ((uint32_t)tid < kRound && (uint32_t)tid < m.cnt) {
+      const float *pp = sorted + 3 * (int64_t)(m.off + tid);
       f.x = pp[0];
       f.y = pp[1];
       f.z = pp[2];
@@ -333,91 +333,110 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
     const CellMeta mn = ci + stride < ncells ? uni(mnRaw) : CellMeta{0, 0, 0, 0};
     __syncthreads();  // block codes in, the previous merge has emptied the tile
 
-    // ---- walk: the wave's particles are the chunks wave, wave + kWaves, ...;
-    // each lane holds one particle of the chunk and sets it up, then the
-    // wave walks them
-    const uint32_t C = chunk(mc.cnt);
-    for (uint32_t qb = (uint32_t)wave * C; qb < mc.cnt; qb += kWaves * C) {
-      int lane = tid & 63;
-      __asm__ volatile("" : "+v"(lane));
-      const uint32_t ql = qb + (uint32_t)lane;
-      const bool mine = (uint32_t)lane < C && ql < mc.cnt;
-      float lx = fc.x, ly = fc.y, lz = fc.z;
-      if (qb != (uint32_t)wave * C) {
-        lx = ly = lz = 0.0f;
-        if (mine) {
-          const float *pp = sorted + 3 * (int64_t)(mc.off + ql);
-          lx = pp[0];
-          ly = pp[1];
-          lz = pp[2];
+    // ---- walk, in rounds of kRound particles: the first kRound threads set
+    // up one particle each (its footprint box, dz^2 of each plane, whether
+    // its blocks all hold a page: msbg_demo.cpp:571-614) into LDS, then the
+    // waves walk particles wave, wave + kWaves, ... of the round from there
+    for (uint32_t rb = 0; rb < mc.cnt; rb += kRound) {
+      if (rb) __syncthreads();  // the previous round is walked
+      int st = tid;
+      __asm__ volatile("" : "+v"(st));
+      if (st < (int)kRound) {
+        const uint32_t ql = rb + (uint32_t)st;
+        const bool mine = ql < mc.cnt;
+        float lx = fc.x, ly = fc.y, lz = fc.z;
+        if (rb) {
+          lx = ly = lz = 0.0f;
+          if (mine) {
+            const float *pp = sorted + 3 * (int64_t)(mc.off + ql);
+            lx = pp[0];
+            ly = pp[1];
+            lz = pp[2];
+          }
         }
-      }
-      // this lane's particle: its footprint box (msbg_demo.cpp:571-614) and
-      // whether the walk can skip the block checks: a box of at most 64 face
-      // voxels whose blocks (at most two per axis) all hold a page never
-      // reaches the early abandon
-      const float lp[3] = {lx, ly, lz};
-      int lo[3], hi[3];
+        const float lp[3] = {lx, ly, lz};
+        int lo[3], hi[3];
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        int a = (int)__builtin_ceilf(lp[k] - rScan - 0.5f);
-        int b = (int)__builtin_floorf(lp[k] + rScan - 0.5f);
-        lo[k] = a > 0 ? a : 0;
-        hi[k] = b < smax[k] ? b : smax[k];
-      }
-      const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
-      const bool live = mine && nx > 0 && ny > 0 && nz > 0;
-      bool fast = live && __mul24(nx, ny) <= 64 && nz <= T::MAXPL && (hi[0] >> lg) - (lo[0] >> lg) <= 1 &&
-                  (hi[1] >> lg) - (lo[1] >> lg) <= 1 && (hi[2] >> lg) - (lo[2] >> lg) <= 1;
-      if (fast) {
-        // (block indices under a tile are below 2^24: 24-bit products)
-        const int xl = (lo[0] >> lg) - tg.bxa, xh = (hi[0] >> lg) - tg.bxa;
-        const int yl = (lo[1] >> lg) - tg.bya, yh = (hi[1] >> lg) - tg.bya;
-        const int zl = __mul24((lo[2] >> lg) - tg.bza, tg.nbty);
-        const int zh = __mul24((hi[2] >> lg) - tg.bza, tg.nbty);
-        const int r[4] = {__mul24(zl + yl, tg.nbtx), __mul24(zl + yh, tg.nbtx),
-                          __mul24(zh + yl, tg.nbtx), __mul24(zh + yh, tg.nbtx)};
+        for (int k = 0; k < 3; k++) {
+          int a = (int)__builtin_ceilf(lp[k] - rScan - 0.5f);
+          int b = (int)__builtin_floorf(lp[k] + rScan - 0.5f);
+          lo[k] = a > 0 ? a : 0;
+          hi[k] = b < smax[k] ? b : smax[k];
+        }
+        const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+        const bool live = mine && nx > 0 && ny > 0 && nz > 0;
+        // a box of at most 64 face voxels whose blocks (at most two per axis)
+        // all hold a page never reaches the early abandon
+        bool fast = live && __mul24(nx, ny) <= 64 && nz <= T::MAXPL &&
+                    (hi[0] >> lg) - (lo[0] >> lg) <= 1 && (hi[1] >> lg) - (lo[1] >> lg) <= 1 &&
+                    (hi[2] >> lg) - (lo[2] >> lg) <= 1;
+        if (fast) {
+          // (block indices under a tile are below 2^24: 24-bit products)
+          const int xl = (lo[0] >> lg) - tg.bxa, xh = (hi[0] >> lg) - tg.bxa;
+          const int yl = (lo[1] >> lg) - tg.bya, yh = (hi[1] >> lg) - tg.bya;
+          const int zl = __mul24((lo[2] >> lg) - tg.bza, tg.nbty);
+          const int zh = __mul24((hi[2] >> lg) - tg.bza, tg.nbty);
+          const int r[4] = {__mul24(zl + yl, tg.nbtx), __mul24(zl + yh, tg.nbtx),
+                            __mul24(zh + yl, tg.nbtx), __mul24(zh + yh, tg.nbtx)};
 #pragma unroll
-        for (int e = 0; e < 4; e++) fast = fast && bc[r[e] + xl] >= 0 && bc[r[e] + xh] >= 0;
-      }
-      const float z0l = lz - 0.5f;
-      float dz2[T::MAXPL];
+          for (int e = 0; e < 4; e++) fast = fast && bc[r[e] + xl] >= 0 && bc[r[e] + xh] >= 0;
+        }
+        uint32_t w[4 * kParWords];
 #pragma unroll
-      for (int k = 0; k < T::MAXPL; k++) {
-        const float dz = (float)(lo[2] + k) - z0l;
-        dz2[k] = dz * dz;
+        for (int k = 0; k < 4 * kParWords; k++) w[k] = 0u;
+        // info, t -> t / nx as (t * magic) >> 16 (exact for t < 64 and nx <=
+        // 64; 65536 / nx through the reciprocal is within 0.008 of the
+        // quotient, whose fraction is 0 or at least 1/64, so the floor is
+        // exact), the tile index of the box origin, the box's x and y origin
+        w[0] = (uint32_t)((live ? 1 : 0) | (fast ? 2 : 0) | ((nx & 63) << 2) | ((ny & 63) << 8) |
+                          ((nz & 63) << 14));
+        w[1] = fast ? (uint32_t)(65536.0f * __builtin_amdgcn_rcpf((float)nx)) + 1u : 0u;
+        w[2] = (uint32_t)(((lo[2] - tg.tz0) * T::TY + (lo[1] - tg.ty0)) * T::TX + (lo[0] - tg.tx0));
+        w[3] = (uint32_t)lo[0];
+        w[4] = (uint32_t)lo[1];
+        w[5] = __builtin_bit_cast(uint32_t, lx - 0.5f);
+        w[6] = __builtin_bit_cast(uint32_t, ly - 0.5f);
+        const float z0l = lz - 0.5f;
+#pragma unroll
+        for (int k = 0; k < T::MAXPL; k++) {
+          const float dz = (float)(lo[2] + k) - z0l;
+          w[7 + k] = __builtin_bit_cast(uint32_t, dz * dz);
+        }
+        w[16] = __builtin_bit_cast(uint32_t, lx);
+        w[17] = __builtin_bit_cast(uint32_t, ly);
+        w[18] = __builtin_bit_cast(uint32_t, lz);
+#pragma unroll
+        for (int k = 0; k < kParWords; k++)
+          par[st][k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
       }
-      // tile index of the box origin, and t -> t / nx as (t * magic) >> 16,
-      // exact for t < 64 and nx <= 64
-      const int org = ((lo[2] - tg.tz0) * T::TY + (lo[1] - tg.ty0)) * T::TX + (lo[0] - tg.tx0);
-      // (65536 / nx through the reciprocal: within 0.008 of the quotient,
-      // whose fraction is 0 or at least 1/64, so the floor is exact)
-      const uint32_t magic =
-          fast ? (uint32_t)(65536.0f * __builtin_amdgcn_rcpf((float)nx)) + 1u : 0u;
-      const int info = (live ? 1 : 0) | (fast ? 2 : 0) | ((nx & 63) << 2) | ((ny & 63) << 8) |
-                       ((nz & 63) << 14);
+      __syncthreads();  // the round's set-up is in LDS
 
-      const uint32_t left = mc.cnt - qb;
-      const int nq = left < C ? (int)left : (int)C;
-      for (int kq = 0; kq < nq; kq++) {
-        const int inf = lane_i(info, kq);
+      const uint32_t left = mc.cnt - rb;
+      const int nr = left < kRound ? (int)left : (int)kRound;
+      for (int j = wave; j < nr; j += (int)kWaves) {
+        int lane = tid & 63;
+        __asm__ volatile("" : "+v"(lane));
+        const uint4 q0 = par[j][0];
+        const int inf = __builtin_amdgcn_readfirstlane((int)q0.x);
         if (!(inf & 1)) continue;
         if (inf & 2) {
           // lanes over the box's (y, x) face, the planes in z order; lanes
           // past the face take +inf distances, whose codes change nothing
+          const uint4 q1 = par[j][1], q2 = par[j][2], q3 = par[j][3];
+          const uint32_t dzw[12] = {q1.w, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w, 0, 0, 0};
           const int fnx = (inf >> 2) & 63, fny = (inf >> 8) & 63, fnz = (inf >> 14) & 63;
-          const int qy = (int)(__umul24((uint32_t)lane, (uint32_t)lane_i((int)magic, kq)) >> 16);
+          const int qy = (int)(__umul24((uint32_t)lane, q0.y) >> 16);
           const int rx = lane - __mul24(qy, fnx);
           const bool on = lane < __mul24(fnx, fny);
-          const float dx = (float)(lane_i(lo[0], kq) + rx) - lane_f(lx - 0.5f, kq);
-          const float dy = (float)(lane_i(lo[1], kq) + qy) - lane_f(ly - 0.5f, kq);
+          const float dx = (float)((int)q0.w + rx) - __builtin_bit_cast(float, q1.y);
+          const float dy = (float)((int)q1.x + qy) - __builtin_bit_cast(float, q1.z);
           const float dx2 = dx * dx, dy2 = on ? dy * dy : __builtin_inff();
-          uint32_t *const t = tile + (on ? lane_i(org, kq) + qy * T::TX + rx : 0);
+          uint32_t *const t = tile + (on ? (int)q0.z + qy * T::TX + rx : 0);
           auto planes = [&](auto one) {
 #pragma unroll
             for (int k = 0; k < T::MAXPL; k++) {
               if (k >= fnz) break;
-              const float dzy = lane_f(dz2[k], kq) + dy2;  // dz*dz + dy*dy, then + dx*dx
+              const float dzy = __builtin_bit_cast(float, dzw[k]) + dy2;  // dz*dz + dy*dy, then + dx*dx
               atomicMin(t + k * T::PLANE,
                         splat_code<decltype(one)::value>(dzy + dx2, distSqMaxInv, codeScale));
             }
@@ -430,12 +449,16 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
         }
         // a box that may touch a block without a page: lanes over the face
         // in y,x order (at most kFaceSets sets of 64), a ballot per plane
-        const float p[3] = {lane_f(lx, kq), lane_f(ly, kq), lane_f(lz, kq)};
+        const uint4 q4 = par[j][4];
+        const float p[3] = {__builtin_bit_cast(float, q4.x), __builtin_bit_cast(float, q4.y),
+                            __builtin_bit_cast(float, q4.z)};
         int blo[3], bhi[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-          blo[k] = lane_i(lo[k], kq);
-          bhi[k] = lane_i(hi[k], kq);
+          int a = (int)__builtin_ceilf(p[k] - rScan - 0.5f);
+          int b = (int)__builtin_floorf(p[k] + rScan - 0.5f);
+          blo[k] = a > 0 ? a : 0;
+          bhi[k] = b < smax[k] ? b : smax[k];
         }
         const int bnx = bhi[0] - blo[0] + 1, bny = bhi[1] - blo[1] + 1;
         const int nxy = bnx * bny;
