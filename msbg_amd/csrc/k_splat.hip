@@ -223,10 +223,9 @@ __global__ void k_cell_meta(const int32_t *__restrict__ list, int n, const uint3
   meta[i] = CellMeta{k, off[k], cnt[k], 0};
 }
 
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+
 __device__ inline int lane_i(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
-__device__ inline float lane_f(float v, int k) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), k));
-}
 
 // The UNORM16 distance code of an in-range voxel, (uint16_t)roundf(f * 65535)
 // with f = d2 * distSqMaxInv (renderDensFromFloat_, sbg.h:272-315). For
@@ -599,18 +598,19 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
         }
 #pragma unroll
         for (int u = 0; u < kMergeZ; u++) {
-          const uint32_t lo = (v[u] & 0xffffu) < (cur[u] & 0xffffu) ? v[u] & 0xffffu : cur[u] & 0xffffu;
-          const uint32_t hi = (v[u] >> 16) < (cur[u] >> 16) ? v[u] >> 16 : cur[u] >> 16;
-          const uint32_t w = lo | (hi << 16);
+          // both codes' minima in one packed 16-bit min
+          const uint32_t w = __builtin_bit_cast(
+              uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, v[u]),
+                                                  __builtin_bit_cast(u16x2, cur[u])));
           if (w == cur[u]) continue;
           uint8_t *q = qa[u];
           if constexpr (ESZ == 2) {
             *(uint32_t *)q = w;
           } else if constexpr (ESZ == 4) {
-            *(uint2 *)q = make_uint2(lo, hi);
+            *(uint2 *)q = make_uint2(w & 0xffffu, w >> 16);
           } else {
-            *(uint32_t *)q = lo;
-            *(uint32_t *)(q + 8) = hi;
+            *(uint32_t *)q = w & 0xffffu;
+            *(uint32_t *)(q + 8) = w >> 16;
           }
         }
       }
