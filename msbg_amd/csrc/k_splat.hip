@@ -185,6 +185,8 @@ constexpr int kSplatThreads = 1024;
 // planes, the position (floats)
 constexpr uint32_t kRound = 64;
 constexpr int kParWords = 5;
+// the wave that sets rounds up (it holds no merge item)
+constexpr int kSetupWave = kSplatThreads / 64 - 1;
 // the merge takes a pair of voxels in x over kMergeZ planes per thread
 constexpr int kMergeZ = 8;
 // footprint faces of at most (2 kMaxHalo + 1)^2 voxels: kFaceSets sets of 64 lanes
@@ -202,7 +204,7 @@ struct TileDims {
   static constexpr int PARTS = (TZ + kMergeZ - 1) / kMergeZ, ITEMS = (TX / 2) * TY * PARTS;
   static_assert(TX <= 2 * kCell && TY <= 2 * kCell, "tiles of one colour overlap");
   static_assert((TX / 8 + 1) * (TY / 8 + 1) * (TZ / 8 + 1) <= kTileBlk, "tile blocks");
-  static_assert(ITEMS <= kSplatThreads, "one merge item per thread");
+  static_assert(ITEMS <= kSplatThreads - 64, "one merge item per thread, none in the set-up wave");
   static_assert((MAXPL - 1) * PLANE * 4 < 65536, "plane offsets");
   static_assert(MAXPL <= 9, "dz^2 words of a particle's set-up");
   static_assert(NV % 4 == 0 && TX % 2 == 0, "tile fill, voxel pairs");
@@ -268,7 +270,7 @@ __device__ inline TileGeo tile_geo(const CellGrid &G, int key, int lg) {
 }
 
 // what a thread fetches for a cell ahead of its walk: the page code of one
-// block under the tile and, for the first kRound threads, one particle
+// block under the tile and, in the set-up wave, one particle of the first round
 struct CellFetch {
   int bval;
   float x, y, z;
@@ -298,8 +300,8 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
       const int ex = tid % t.nbtx, ey = (tid / t.nbtx) % t.nbty, ez = tid / (t.nbtx * t.nbty);
       f.bval = pagemap[((int64_t)(t.bza + ez) * nby + (t.bya + ey)) * nbx + (t.bxa + ex)];
     }
-    if ((uint32_t)tid < kRound && (uint32_t)tid < m.cnt) {
-      const float *pp = sorted + 3 * (int64_t)(m.off + tid);
+    if (wave == kSetupWave && (uint32_t)lane < m.cnt) {
+      const float *pp = sorted + 3 * (int64_t)(m.off + lane);
       f.x = pp[0];
       f.y = pp[1];
       f.z = pp[2];
@@ -318,11 +320,77 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
                     (uint32_t)lane_i((int)v.cnt, 0), 0};
   };
 
+  const int smax[3] = {G.sx - 1, G.sy - 1, G.sz - 1};
+  // A round's set-up, one particle per lane of the set-up wave: its
+  // footprint box (msbg_demo.cpp:571-614), whether its blocks (at most two
+  // per axis, codes from the page map) all hold a page, so that the walk can
+  // skip the early abandon, the tile origin and dz^2 of each plane.
+  auto setup = [&](bool mine, float lx, float ly, float lz, const TileGeo &t) {
+    const float lp[3] = {lx, ly, lz};
+    int lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      int a = (int)__builtin_ceilf(lp[k] - rScan - 0.5f);
+      int b = (int)__builtin_floorf(lp[k] + rScan - 0.5f);
+      lo[k] = a > 0 ? a : 0;
+      hi[k] = b < smax[k] ? b : smax[k];
+    }
+    const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+    const bool live = mine && nx > 0 && ny > 0 && nz > 0;
+    bool fast = live && __mul24(nx, ny) <= 64 && nz <= T::MAXPL &&
+                (hi[0] >> lg) - (lo[0] >> lg) <= 1 && (hi[1] >> lg) - (lo[1] >> lg) <= 1 &&
+                (hi[2] >> lg) - (lo[2] >> lg) <= 1;
+    if (fast) {
+      const int64_t xl = lo[0] >> lg, xh = hi[0] >> lg;
+      const int64_t r[4] = {((int64_t)(lo[2] >> lg) * nby + (lo[1] >> lg)) * nbx,
+                            ((int64_t)(lo[2] >> lg) * nby + (hi[1] >> lg)) * nbx,
+                            ((int64_t)(hi[2] >> lg) * nby + (lo[1] >> lg)) * nbx,
+                            ((int64_t)(hi[2] >> lg) * nby + (hi[1] >> lg)) * nbx};
+      int c[8];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        c[2 * e] = pagemap[r[e] + xl];
+        c[2 * e + 1] = pagemap[r[e] + xh];
+      }
+#pragma unroll
+      for (int e = 0; e < 8; e++) fast = fast && c[e] >= 0;
+    }
+    uint32_t w[4 * kParWords];
+#pragma unroll
+    for (int k = 0; k < 4 * kParWords; k++) w[k] = 0u;
+    // info, t -> t / nx as (t * magic) >> 16 (exact for t < 64 and nx <= 64;
+    // 65536 / nx through the reciprocal is within 0.008 of the quotient,
+    // whose fraction is 0 or at least 1/64, so the floor is exact), the tile
+    // index of the box origin, the box's x and y origin
+    w[0] = (uint32_t)((live ? 1 : 0) | (fast ? 2 : 0) | ((nx & 63) << 2) | ((ny & 63) << 8) |
+                      ((nz & 63) << 14));
+    w[1] = fast ? (uint32_t)(65536.0f * __builtin_amdgcn_rcpf((float)nx)) + 1u : 0u;
+    w[2] = (uint32_t)(((lo[2] - t.tz0) * T::TY + (lo[1] - t.ty0)) * T::TX + (lo[0] - t.tx0));
+    w[3] = (uint32_t)lo[0];
+    w[4] = (uint32_t)lo[1];
+    w[5] = __builtin_bit_cast(uint32_t, lx - 0.5f);
+    w[6] = __builtin_bit_cast(uint32_t, ly - 0.5f);
+    const float z0l = lz - 0.5f;
+#pragma unroll
+    for (int k = 0; k < T::MAXPL; k++) {
+      const float dz = (float)(lo[2] + k) - z0l;
+      w[7 + k] = __builtin_bit_cast(uint32_t, dz * dz);
+    }
+    w[16] = __builtin_bit_cast(uint32_t, lx);
+    w[17] = __builtin_bit_cast(uint32_t, ly);
+    w[18] = __builtin_bit_cast(uint32_t, lz);
+#pragma unroll
+    for (int k = 0; k < kParWords; k++)
+      par[lane][k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+  };
+
   CellMeta mc = uni(meta_raw(ci));
   CellMeta mnRaw = ci + stride < ncells ? meta_raw(ci + stride) : CellMeta{0, 0, 0, 0};
   TileGeo tg = tile_geo<H>(G, mc.key, lg);
   CellFetch fc = fetch(mc, tg);
-  const int smax[3] = {G.sx - 1, G.sy - 1, G.sz - 1};
+  // the first cell's first round; later cells' first rounds are set up
+  // while the cell before them merges
+  if (wave == kSetupWave) setup((uint32_t)lane < mc.cnt, fc.x, fc.y, fc.z, tg);
   const bool oneMul = codeScale != 0.0f;
   int buf = 0;
   for (;;) {
@@ -332,83 +400,26 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
     const CellMeta mn = ci + stride < ncells ? uni(mnRaw) : CellMeta{0, 0, 0, 0};
     __syncthreads();  // block codes in, the previous merge has emptied the tile
 
-    // ---- walk, in rounds of kRound particles: the first kRound threads set
-    // up one particle each (its footprint box, dz^2 of each plane, whether
-    // its blocks all hold a page: msbg_demo.cpp:571-614) into LDS, then the
-    // waves walk particles wave, wave + kWaves, ... of the round from there
+    // ---- walk, in rounds of kRound particles set up in LDS (the first
+    // round before this cell's walk began), each wave walking particles
+    // wave, wave + kWaves, ... of the round
     for (uint32_t rb = 0; rb < mc.cnt; rb += kRound) {
-      if (rb) __syncthreads();  // the previous round is walked
-      int st = tid;
-      __asm__ volatile("" : "+v"(st));
-      if (st < (int)kRound) {
-        const uint32_t ql = rb + (uint32_t)st;
-        const bool mine = ql < mc.cnt;
-        float lx = fc.x, ly = fc.y, lz = fc.z;
-        if (rb) {
-          lx = ly = lz = 0.0f;
+      if (rb) {
+        __syncthreads();  // the previous round is walked
+        if (wave == kSetupWave) {
+          const uint32_t ql = rb + (uint32_t)lane;
+          const bool mine = ql < mc.cnt;
+          float lx = 0.0f, ly = 0.0f, lz = 0.0f;
           if (mine) {
             const float *pp = sorted + 3 * (int64_t)(mc.off + ql);
             lx = pp[0];
             ly = pp[1];
             lz = pp[2];
           }
+          setup(mine, lx, ly, lz, tg);
         }
-        const float lp[3] = {lx, ly, lz};
-        int lo[3], hi[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          int a = (int)__builtin_ceilf(lp[k] - rScan - 0.5f);
-          int b = (int)__builtin_floorf(lp[k] + rScan - 0.5f);
-          lo[k] = a > 0 ? a : 0;
-          hi[k] = b < smax[k] ? b : smax[k];
-        }
-        const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
-        const bool live = mine && nx > 0 && ny > 0 && nz > 0;
-        // a box of at most 64 face voxels whose blocks (at most two per axis)
-        // all hold a page never reaches the early abandon
-        bool fast = live && __mul24(nx, ny) <= 64 && nz <= T::MAXPL &&
-                    (hi[0] >> lg) - (lo[0] >> lg) <= 1 && (hi[1] >> lg) - (lo[1] >> lg) <= 1 &&
-                    (hi[2] >> lg) - (lo[2] >> lg) <= 1;
-        if (fast) {
-          // (block indices under a tile are below 2^24: 24-bit products)
-          const int xl = (lo[0] >> lg) - tg.bxa, xh = (hi[0] >> lg) - tg.bxa;
-          const int yl = (lo[1] >> lg) - tg.bya, yh = (hi[1] >> lg) - tg.bya;
-          const int zl = __mul24((lo[2] >> lg) - tg.bza, tg.nbty);
-          const int zh = __mul24((hi[2] >> lg) - tg.bza, tg.nbty);
-          const int r[4] = {__mul24(zl + yl, tg.nbtx), __mul24(zl + yh, tg.nbtx),
-                            __mul24(zh + yl, tg.nbtx), __mul24(zh + yh, tg.nbtx)};
-#pragma unroll
-          for (int e = 0; e < 4; e++) fast = fast && bc[r[e] + xl] >= 0 && bc[r[e] + xh] >= 0;
-        }
-        uint32_t w[4 * kParWords];
-#pragma unroll
-        for (int k = 0; k < 4 * kParWords; k++) w[k] = 0u;
-        // info, t -> t / nx as (t * magic) >> 16 (exact for t < 64 and nx <=
-        // 64; 65536 / nx through the reciprocal is within 0.008 of the
-        // quotient, whose fraction is 0 or at least 1/64, so the floor is
-        // exact), the tile index of the box origin, the box's x and y origin
-        w[0] = (uint32_t)((live ? 1 : 0) | (fast ? 2 : 0) | ((nx & 63) << 2) | ((ny & 63) << 8) |
-                          ((nz & 63) << 14));
-        w[1] = fast ? (uint32_t)(65536.0f * __builtin_amdgcn_rcpf((float)nx)) + 1u : 0u;
-        w[2] = (uint32_t)(((lo[2] - tg.tz0) * T::TY + (lo[1] - tg.ty0)) * T::TX + (lo[0] - tg.tx0));
-        w[3] = (uint32_t)lo[0];
-        w[4] = (uint32_t)lo[1];
-        w[5] = __builtin_bit_cast(uint32_t, lx - 0.5f);
-        w[6] = __builtin_bit_cast(uint32_t, ly - 0.5f);
-        const float z0l = lz - 0.5f;
-#pragma unroll
-        for (int k = 0; k < T::MAXPL; k++) {
-          const float dz = (float)(lo[2] + k) - z0l;
-          w[7 + k] = __builtin_bit_cast(uint32_t, dz * dz);
-        }
-        w[16] = __builtin_bit_cast(uint32_t, lx);
-        w[17] = __builtin_bit_cast(uint32_t, ly);
-        w[18] = __builtin_bit_cast(uint32_t, lz);
-#pragma unroll
-        for (int k = 0; k < kParWords; k++)
-          par[st][k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+        __syncthreads();  // the round is set up
       }
-      __syncthreads();  // the round's set-up is in LDS
 
       const uint32_t left = mc.cnt - rb;
       const int nr = left < kRound ? (int)left : (int)kRound;
@@ -530,6 +541,8 @@ __global__ __launch_bounds__(kSplatThreads, 8) void k_splat_tile(
     // the tile (only value blocks were touched). The tile's values are read
     // and the tile emptied first, then the pages' values of the touched pairs
     // are loaded together, then the smaller codes stored.
+    // the set-up wave has no merge item: it sets up the next cell's first round
+    if (wave == kSetupWave && more) setup((uint32_t)lane < mn.cnt, fn.x, fn.y, fn.z, tn);
     // (the thread's item is derived anew each cell, not kept live across the walk)
     int mt = tid;
     __asm__ volatile("" : "+v"(mt));
