@@ -909,10 +909,14 @@ int launch_splat(msbg_grid *g, Channel &ch, float rScan) {
   }
   if (!g->npoints) return 0;
   const float distSqMax = rScan * rScan, distSqMaxInv = 1.0f / distSqMax;
-  int esz = storage_bytes(ch.storage);
+  const int esz = storage_bytes(ch.storage);
+  if (esz != 2 && esz != 4 && esz != 8) {  // codes need 16 bits (UNORM8 channels hold none)
+    set_error("splat: storage %d cannot hold the distance codes", ch.storage);
+    return MSBG_EUNSUPP;
+  }
   // the binned splat needs every footprint inside its cell's tile
   const int halo = splat_halo(g, rScan);
-  if (halo <= kMaxHalo && g->npoints < ((int64_t)1 << 31) && esz != 1)
+  if (halo <= kMaxHalo && g->npoints < ((int64_t)1 << 31))
     return splat_binned(g, cl, esz, halo, rScan, distSqMax, distSqMaxInv);
   int64_t nb = (g->npoints + kThreads - 1) / kThreads;
   auto args = [&](auto kern) {
