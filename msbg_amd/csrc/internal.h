@@ -301,6 +301,7 @@ int launch_sweep_items(hipStream_t s, int bsx, int storage, bool exact, bool c01
 // every sweep and colour of the group's grids from a work queue, waiting per
 // grid for its previous colour pass (k_sweep_queue).
 constexpr int kDoneStride = 16;  // queue counters one 64-byte line apart
+constexpr unsigned kQueueRun = 8;  // queue items a workgroup takes at a time
 struct BatchQueue {
   const int2 *items;   // the group's items (entry, page), sorted by colour
   const int *pre;      // per grid entry, 9 values: its items of colours < c; [8] = all

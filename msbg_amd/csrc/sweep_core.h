@@ -1008,13 +1008,14 @@ __global__ __launch_bounds__((Tile<BSX, PX, PY>::NT * BPW)) void k_sweep(
 // (N8 = items per sweep), entry r = i - s N8 of the colour-sorted item list,
 // whose colour c satisfies off[c] <= r < off[c+1].
 
-// Workgroups (one wave each) take items in queue order. Before sweeping a
-// block of pass (s, c) of grid g, a workgroup waits until every block of g's
-// earlier passes is done: done[g] >= s * pre[g][8] + pre[g][c] (the
-// reference's colour order per grid, msbg4.cpp:384-900; grids are
-// independent, so one grid's next pass overlaps the others' current one).
-// Every item a wait depends on sits earlier in the queue and was taken by a
-// running workgroup, whose own waits point further back, so the waits end;
+// Workgroups (one wave each) take runs of kQueueRun items in queue order.
+// Before sweeping a block of pass (s, c) of grid g, a workgroup waits until
+// every block of g's earlier passes is done: done[g] >= s * pre[g][8] +
+// pre[g][c] (the reference's colour order per grid, msbg4.cpp:384-900; grids
+// are independent, so one grid's next pass overlaps the others' current
+// one). Every item a wait depends on sits earlier in the queue, in an earlier
+// run or published before the waiting item within its own run, and was taken
+// by a running workgroup whose own waits point further back, so the waits end;
 // each is still bounded by waitTicks of wall-clock time, past which it sets
 // *fail and goes on (the run is then reported as failed, and no workgroup
 // can hang). Waits poll with a sleep between polls, one 64-byte line per
@@ -1034,39 +1035,66 @@ __global__ __launch_bounds__(64) void k_sweep_queue(const SweepGrid *__restrict_
   const bool lane0 = threadIdx.x == 0;
   const unsigned n8 = (unsigned)q.off[8];
   const unsigned total = n8 * (unsigned)q.sweeps;
+  // A workgroup takes kQueueRun consecutive items at a time. Its finished
+  // items of one grid and pass are published together (one release, one
+  // count), before an item of another grid or pass and at the run's end, so
+  // no item waits on an unpublished item of its own run. An item whose grid
+  // count this workgroup has already waited for and acquired skips the wait.
+  int ae = -1;
+  unsigned aneed = 0;
   for (;;) {
-    const unsigned i = __builtin_amdgcn_readfirstlane(atomicAdd(q.head, lane0 ? 1u : 0u));
-    if (i >= total) break;
-    const int s = (int)(i / n8), r = (int)(i - (unsigned)s * n8);
-    int c = 0;
-    while (c < 7 && r >= q.off[c + 1]) c++;
-    const int e = __builtin_amdgcn_readfirstlane(q.items[r].x);
-    const int page = __builtin_amdgcn_readfirstlane(q.items[r].y);
-    const int *pre = q.pre + (int64_t)e * 9;
-    const unsigned need = __builtin_amdgcn_readfirstlane((unsigned)s * (unsigned)pre[8] +
-                                                         (unsigned)pre[c]);
-    unsigned *const done = q.done + (int64_t)e * kDoneStride;
-    const uint64_t t0 = wall_clock64();
-    for (;;) {
-      const unsigned d = __builtin_amdgcn_readfirstlane(
-          __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      if (d >= need) break;
-      if ((uint64_t)wall_clock64() - t0 > q.waitTicks) {
-        atomicOr(q.fail, 1u);
-        break;
+    const unsigned base =
+        __builtin_amdgcn_readfirstlane(atomicAdd(q.head, lane0 ? kQueueRun : 0u));
+    if (base >= total) break;
+    const unsigned end = base + kQueueRun < total ? base + kQueueRun : total;
+    int pe = -1, ps = -1, pc = -1;
+    unsigned pn = 0;
+    auto publish = [&]() {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_fetch_add(q.done + (int64_t)pe * kDoneStride, lane0 ? pn : 0u,
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      pn = 0;
+    };
+    for (unsigned i = base; i < end; i++) {
+      const int s = (int)(i / n8), r = (int)(i - (unsigned)s * n8);
+      int c = 0;
+      while (c < 7 && r >= q.off[c + 1]) c++;
+      const int e = __builtin_amdgcn_readfirstlane(q.items[r].x);
+      const int page = __builtin_amdgcn_readfirstlane(q.items[r].y);
+      if (pn && (e != pe || s != ps || c != pc)) publish();
+      const int *pre = q.pre + (int64_t)e * 9;
+      const unsigned need = __builtin_amdgcn_readfirstlane((unsigned)s * (unsigned)pre[8] +
+                                                           (unsigned)pre[c]);
+      if (e != ae || need > aneed) {
+        unsigned *const done = q.done + (int64_t)e * kDoneStride;
+        const uint64_t t0 = wall_clock64();
+        for (;;) {
+          const unsigned d = __builtin_amdgcn_readfirstlane(
+              __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+          if (d >= need) break;
+          if ((uint64_t)wall_clock64() - t0 > q.waitTicks) {
+            atomicOr(q.fail, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(16);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        ae = e;
+        aneed = need;
       }
-      __builtin_amdgcn_s_sleep(16);
+      // the lane index through an opaque move, so that the block's per-lane
+      // set-up is computed per item instead of being hoisted out of the loop
+      // and held across it (which costs registers the z-march needs)
+      int lane;
+      asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"((int)threadIdx.x));
+      sweep_block<BSX, ST, EXACT, PX, PY, OP_MCF, false, 1, C01>(
+          grids, make_int2(e, page), nullptr, s, 0, true, 0, lane);
+      pe = e;
+      ps = s;
+      pc = c;
+      pn++;
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    // the lane index through an opaque move, so that the block's per-lane
-    // set-up is computed per item instead of being hoisted out of the loop
-    // and held across it (which costs registers the z-march needs)
-    int lane;
-    asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"((int)threadIdx.x));
-    sweep_block<BSX, ST, EXACT, PX, PY, OP_MCF, false, 1, C01>(grids, make_int2(e, page), nullptr,
-                                                               s, 0, true, 0, lane);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_fetch_add(done, lane0 ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (pn) publish();
   }
 }
 
