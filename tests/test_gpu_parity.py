@@ -821,6 +821,25 @@ def test_batch_persistent_matches_per_pass_launches():
                                       gb.download(BCH).view(np.uint16)), rep
 
 
+@pytest.mark.parametrize("spec", [(64, 16), (96, 32)])
+def test_batch_persistent_single_grid_runs_cross_passes(spec):
+    """One small grid alone in a persistent batch: its colour passes hold
+    fewer blocks than a workgroup's run of queue items, so runs cross from
+    one pass into the next of the same grid (the finished blocks are published
+    before the next pass's wait); the bits equal one launch per colour pass."""
+    note("batch persistent single grid", spec)
+    a = _batch_grids([spec], seed=51)
+    b = _batch_grids([spec], seed=51)
+    with M.Batch() as bq, M.Batch() as bl:
+        bq.add(a[0], BCH, 0.1, 1)
+        bl.add(b[0], BCH, 0.1, 1)
+        bq.run(6, useGraph=M.Batch.PERSISTENT)
+        bl.run(6, useGraph=False)
+        a[0].synchronize()
+        assert np.array_equal(a[0].download(BCH).view(np.uint16),
+                              b[0].download(BCH).view(np.uint16))
+
+
 def test_batch_persistent_reports_a_wait_past_its_bound():
     """A dependency wait that runs past its poll bound ends the wait and is
     reported as a failed run (forced here with a bound of zero time, which
