@@ -76,6 +76,13 @@ struct ConvShape {
   static constexpr int NTILE = TZ * TY * XP / 16;
   static constexpr int TN = NTILE / 4;
   static constexpr int SLAB = SZ * SY * SX * VS8;  // uint4 elements
+  // Whether a thread issues all its slab loads before its LDS stores. The
+  // decoder's first 32^3 layer (upsample + skip, 48 -> 16) and the 64 -> 64
+  // bottom layer stage one element at a time: batched, they lost a wave per
+  // SIMD and ran 9 % and 35 % slower on MI355X, while the other layers ran
+  // 6-28 % faster.
+  static constexpr bool BATCH = !(CIN == 48 && COUT == 16 && DOUT == 32) &&
+                                !(CIN == 64 && COUT == 64);
   static_assert(CIN % CC == 0 && CC % 8 == 0, "channel chunking");
   static_assert(COUT % 16 == 0, "cout tiles");
   static_assert(DOUT % TZ == 0 && DOUT % TY == 0, "spatial tiles");
@@ -133,8 +140,10 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ x,
       for (int s = 0; s < K::KS; s++)
         aw[s] = __builtin_bit_cast(bf16x8, wsw[(int64_t)(ch * K::KS + s) * 64 + lane]);
     }
-    constexpr int G8 = CC / 8, NG = K::SZ * SY * SX * G8;
-    for (int g = tid; g < NG; g += 256) {
+    // the slab element g of this chunk: (voxel, 8 channels), zero outside
+    // the input
+    constexpr int G8 = CC / 8, NG = K::SZ * SY * SX * G8, NIT = (NG + 255) / 256;
+    auto fetch = [&](int g) {
       const int c8 = g % G8, sv = g / G8;
       const int sx = sv % SX, sy = (sv / SX) % SY, sz = sv / (SX * SY);
       const int iz = z0 - PAD + sz, iy = y0 - PAD + sy, ix = sx - PAD;
@@ -154,7 +163,25 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ x,
                                ch * CC - CA + c8 * 8);
         }
       }
-      slab[sv * VS8 + c8] = v;
+      return v;
+    };
+    if constexpr (K::BATCH) {
+      // every load of the slab is issued before the first LDS store, so that
+      // a workgroup waits for one HBM round trip per chunk instead of one
+      // per element it stages
+      uint4 stage[NIT];
+#pragma unroll
+      for (int it = 0; it < NIT; it++) {
+        const int g = tid + it * 256;
+        stage[it] = g < NG ? fetch(g) : make_uint4(0, 0, 0, 0);
+      }
+#pragma unroll
+      for (int it = 0; it < NIT; it++) {
+        const int g = tid + it * 256;
+        if (g < NG) slab[(g / G8) * VS8 + g % G8] = stage[it];
+      }
+    } else {
+      for (int g = tid; g < NG; g += 256) slab[(g / G8) * VS8 + g % G8] = fetch(g);
     }
     __syncthreads();
 #pragma unroll
@@ -205,8 +232,10 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ x,
 
 // The single-channel first layer: k = tap (27 of 32 used), one channel, so
 // the slab holds one BF16 per voxel and each lane gathers its 8 taps. A
-// workgroup owns C1_TZ whole output planes; its waves stride over the
-// 16-voxel tiles.
+// workgroup owns C1_TZ whole output planes; its waves stride over 16-voxel
+// tiles of the planes' voxels in memory order, so that only the last tile
+// of a workgroup has idle lanes (tiles of 16 x-consecutive voxels left 14 of
+// every 48 lanes idle at the 34^3 edge).
 constexpr int C1_TZ = 2;
 
 template <int COUT, int DOUT, int PAD>
@@ -214,9 +243,9 @@ __global__ __launch_bounds__(256) void k_conv_c1(const uint16_t *__restrict__ x,
                                                  const uint4 *__restrict__ wsw,
                                                  const float *__restrict__ bias, int relu,
                                                  uint16_t *__restrict__ y) {
-  constexpr int DIN = DOUT + 2 - 2 * PAD, XP = (DOUT + 15) / 16 * 16, TM = COUT / 16;
-  constexpr int SZ = C1_TZ + 2, SY = DOUT + 2, SX = XP + 2;
-  constexpr int XT = XP / 16, NTILE = C1_TZ * DOUT * XT;
+  constexpr int DIN = DOUT + 2 - 2 * PAD, TM = COUT / 16;
+  constexpr int SZ = C1_TZ + 2, SY = DOUT + 2, SX = DOUT + 2;
+  constexpr int NV = C1_TZ * DOUT * DOUT, NTILE = (NV + 15) / 16;
   static_assert(DOUT % C1_TZ == 0, "z tiles");
   __shared__ uint16_t slab[SZ * SY * SX];
 
@@ -225,34 +254,54 @@ __global__ __launch_bounds__(256) void k_conv_c1(const uint16_t *__restrict__ x,
   const int n = blockIdx.y;
   const int z0 = (int)blockIdx.x * C1_TZ;
   const uint16_t *xs = x + (int64_t)n * DIN * DIN * DIN;
-  for (int g = tid; g < SZ * SY * SX; g += 256) {
+  // all loads of the slab are issued before the first LDS store (one HBM
+  // round trip per workgroup, not one per staged voxel)
+  constexpr int NG = SZ * SY * SX, NIT = (NG + 255) / 256;
+  uint16_t stage[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; it++) {
+    const int g = tid + it * 256;
     const int sx = g % SX, sy = (g / SX) % SY, sz = g / (SX * SY);
     const int iz = z0 - PAD + sz, iy = sy - PAD, ix = sx - PAD;
     uint16_t v = 0;
-    if (iz >= 0 && iz < DIN && iy >= 0 && iy < DIN && ix >= 0 && ix < DIN)
+    if (g < NG && iz >= 0 && iz < DIN && iy >= 0 && iy < DIN && ix >= 0 && ix < DIN)
       v = xs[((int64_t)iz * DIN + iy) * DIN + ix];
-    slab[g] = v;
+    stage[it] = v;
   }
+#pragma unroll
+  for (int it = 0; it < NIT; it++)
+    if (tid + it * 256 < NG) slab[tid + it * 256] = stage[it];
   bf16x8 a[TM];
 #pragma unroll
   for (int m = 0; m < TM; m++) a[m] = __builtin_bit_cast(bf16x8, wsw[m * 64 + lane]);
-  // this lane's 8 taps as slab offsets (taps 27..31 are zero)
+  float bv[TM][4];
+#pragma unroll
+  for (int m = 0; m < TM; m++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) bv[m][r] = bias ? bias[m * 16 + 4 * q + r] : 0.f;
+  // this lane's 8 taps as slab offsets; taps 27..31 read offset 0 and are
+  // masked to zero
   int toff[8];
+  uint32_t tmask[8];
 #pragma unroll
   for (int j = 0; j < 8; j++) {
     const int tap = 8 * q + j;
-    toff[j] = tap < 27 ? ((tap / 9) * SY + (tap / 3) % 3) * SX + tap % 3 : -1;
+    toff[j] = tap < 27 ? ((tap / 9) * SY + (tap / 3) % 3) * SX + tap % 3 : 0;
+    tmask[j] = tap < 27 ? 0xffffu : 0u;
   }
   __syncthreads();
 
+  uint16_t *yw = y + ((int64_t)n * DOUT * DOUT * DOUT + (int64_t)z0 * DOUT * DOUT) * COUT;
   for (int T = wave; T < NTILE; T += 4) {
-    const int tz = T / (DOUT * XT), ty = (T / XT) % DOUT, tx = (T % XT) * 16 + col;
+    const int v = T * 16 + col;          // voxel of the workgroup's planes
+    const int vc = v < NV ? v : NV - 1;  // idle lanes of the last tile read a valid voxel
+    const int tz = vc / (DOUT * DOUT), ty = (vc / DOUT) % DOUT, tx = vc % DOUT;
     const int base = (tz * SY + ty) * SX + tx;
     uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-      const uint32_t v = toff[j] >= 0 ? slab[base + toff[j]] : 0u;
-      w[j >> 1] |= v << (16 * (j & 1));
+      const uint32_t t = slab[base + toff[j]] & tmask[j];
+      w[j >> 1] |= t << (16 * (j & 1));
     }
     const bf16x8 b = __builtin_bit_cast(bf16x8, make_uint4(w[0], w[1], w[2], w[3]));
     f32x4 acc[TM];
@@ -260,16 +309,15 @@ __global__ __launch_bounds__(256) void k_conv_c1(const uint16_t *__restrict__ x,
     for (int m = 0; m < TM; m++)
       acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0,
                                                        0);
-    if (tx >= DOUT) continue;
-    uint16_t *o = y + ((int64_t)n * DOUT * DOUT * DOUT +
-                       ((int64_t)(z0 + tz) * DOUT + ty) * DOUT + tx) * COUT;
+    if (v >= NV) continue;
+    uint16_t *o = yw + (int64_t)v * COUT;
 #pragma unroll
     for (int m = 0; m < TM; m++) {
       const int co = m * 16 + 4 * q;
       uint32_t h[4];
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        float f = acc[m][r] + (bias ? bias[co + r] : 0.f);
+        float f = acc[m][r] + bv[m][r];
         if (relu) f = f > 0.f ? f : 0.f;
         h[r] = f2bf(f);
       }
