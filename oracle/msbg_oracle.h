@@ -61,7 +61,7 @@ float or_u16_to_float(uint16_t u);                 /* u * (1.f/65535) */
 float or_u8_to_float(uint8_t u);                   /* u * (1.f/255) */
 /* The UNORM8 sweep store's stochastic rounding (sbg.h:324-340): the uniform
  * number of (sweep step, page, voxel), and floor(255 f) + (u < frac). The
- * numbers are this build's counter-based stream (the reference draws from
+ * numbers are the library's counter-based stream (the reference draws from
  * per-thread Lehmer streams, fastmath.h:68-73). */
 float or_sr_uniform(uint32_t step, uint32_t page, uint32_t vid);
 uint8_t or_u8_from_float_sr(float f, float u);

@@ -3,7 +3,7 @@
 ctypes view of libps_oracle.so (ps_oracle.c): set-up of cell flags,
 diagonal and face areas on a block grid with per-block resolution levels,
 the reference's block relax, Laplacian multiply and dense-level Jacobi, and
-this build's constant restriction/prolongation and V-cycle.
+the library's own constant restriction/prolongation and V-cycle.
 """
 import ctypes as C
 import os

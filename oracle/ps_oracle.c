@@ -687,7 +687,7 @@ int or_ps_multiply(or_ps *ps, unsigned options, int m, int cx, int cb, int cdst,
 
 /* The reference names constant restriction and prolongation
  * (MG_CONST_RESTRICTION, MG_CONST_PROLONGATION with alpha 1, msbgcell.h:99-101)
- * but ships neither. This build's definition: a coarse cell receives the
+ * but ships neither. The library's definition: a coarse cell receives the
  * float sum of its eight children, children in x-fastest order; a fine fluid
  * cell adds its parent's value. Cells that do not coarsen at m+1 (blocks
  * already coarser than m) are copied or added one to one. Non-fluid target
@@ -788,7 +788,7 @@ int or_ps_prolongate(or_ps *ps, int m, int cs, int cd) {
   return 0;
 }
 
-/* V-cycle of this build (the reference's multigrid driver is not shipped):
+/* The library's V-cycle (the reference's multigrid driver is not shipped):
  * pre-relax nu1 (forward order), residual, restrict, recurse with zero
  * initial correction, prolongate, post-relax nu2 (reverse order); the
  * coarsest level is relaxed coarse_iter times. */
