@@ -147,8 +147,10 @@ int swizzle_all(msbg_unet *u) {
   return 0;
 }
 
-// x0 already in u->x0 (n blocks); y -> u->y
-int forward(msbg_unet *u, int n) {
+// x0 already in u->x0 (n blocks); y -> u->y. Without `train` (inference)
+// the last convolution applies the head in its store and its activation,
+// which only the backward pass reads, is not written.
+int forward(msbg_unet *u, int n, bool train) {
   hipStream_t s = u->stream;
   MSBG_TRY(swizzle_all(u));
   for (int l = 0; l < 10; l++) {
@@ -159,13 +161,17 @@ int forward(msbg_unet *u, int n) {
     else if (l == 8)  // decoder 1 reads [upsample(a7), a1] in place
       MSBG_TRY(launch_conv_cat(s, u->a[7].p, 32, u->a[1].p, 16, n, L.cout, L.dout, u->wf[l].p,
                                u->p.p + u->offB[l], true, u->a[l].p));
+    else if (l == 9 && !train)
+      MSBG_TRY(launch_conv_head(s, conv_input(u, l), n, cin_k(l), L.cout, L.dout, L.pad,
+                                u->wf[l].p, u->p.p + u->offB[l], true, nullptr,
+                                u->p.p + u->offH, u->y.p));
     else
       MSBG_TRY(launch_conv(s, conv_input(u, l), n, cin_k(l), L.cout, L.dout, L.pad, u->wf[l].p,
                            u->p.p + u->offB[l], true, u->a[l].p));
     if (l == 1) MSBG_TRY(launch_pool2(s, u->a[1].p, n, 32, 16, u->p1.p));
     if (l == 3) MSBG_TRY(launch_pool2(s, u->a[3].p, n, 16, 32, u->p2.p));
   }
-  return launch_head(s, u->a[9].p, n, u->p.p + u->offH, u->y.p);
+  return train ? launch_head(s, u->a[9].p, n, u->p.p + u->offH, u->y.p) : 0;
 }
 
 // gradients of sum(dy * y) into u->g; dy in u->dy; forward state present
@@ -227,7 +233,7 @@ int load_rows(msbg_unet *u, const uint16_t *X, const float *T, const int32_t *ro
 // forward, MSE, backward and Adam on the loaded batch; with loss == NULL the
 // step stays queued and its mean loss goes to the running accumulator
 int train_core(msbg_unet *u, int n, float lr, double *loss) {
-  MSBG_TRY(forward(u, n));
+  MSBG_TRY(forward(u, n, true));
   const int64_t cnt = (int64_t)n * kVout;
   MSBG_TRY(launch_mse(u->stream, u->y.p, u->tgt.p, cnt, u->dy.p, u->lpart.p));
   MSBG_TRY(launch_sum_f64(u->stream, u->lpart.p, mse_blocks(cnt), u->lsum.p));
@@ -354,7 +360,7 @@ int msbg_unet_forward(msbg_unet *u, const uint16_t *x, int n, float *y) {
   MSBG_CHECK(u && x && y, MSBG_EINVAL, "msbg_unet_forward: bad args");
   MSBG_TRY(ensure_cap(u, n, false));
   MSBG_TRY(load_x(u, x, n));
-  MSBG_TRY(forward(u, n));
+  MSBG_TRY(forward(u, n, false));
   MSBG_HIP(hipMemcpyAsync(y, u->y.p, sizeof(float) * n * kVout, hipMemcpyDefault, u->stream));
   MSBG_HIP(hipStreamSynchronize(u->stream));
   return 0;
@@ -366,7 +372,7 @@ int msbg_unet_backward(msbg_unet *u, const uint16_t *x, const float *dy, int n) 
              "msbg_unet_backward: batch %d must be 8 * 2^k", n);
   MSBG_TRY(ensure_cap(u, n, true));
   MSBG_TRY(load_x(u, x, n));
-  MSBG_TRY(forward(u, n));
+  MSBG_TRY(forward(u, n, true));
   MSBG_HIP(hipMemcpyAsync(u->dy.p, dy, sizeof(float) * n * kVout, hipMemcpyDefault, u->stream));
   MSBG_TRY(backward(u, n));
   MSBG_HIP(hipStreamSynchronize(u->stream));
@@ -403,7 +409,7 @@ int msbg_unet_forward_rows(msbg_unet *u, const uint16_t *X, const int32_t *rows,
     MSBG_CHECK(rows[i] >= 0, MSBG_EINVAL, "msbg_unet_forward_rows: row %d negative", i);
   MSBG_TRY(ensure_cap(u, n, false));
   MSBG_TRY(load_rows(u, X, nullptr, rows, n));
-  MSBG_TRY(forward(u, n));
+  MSBG_TRY(forward(u, n, false));
   MSBG_HIP(hipMemcpyAsync(y, u->y.p, sizeof(float) * n * kVout, hipMemcpyDefault, u->stream));
   MSBG_HIP(hipStreamSynchronize(u->stream));
   return 0;
@@ -555,7 +561,7 @@ int msbg_warm_start(msbg_grid *g, int chan, msbg_unet *u, int batch, double *ms)
       const int nb = std::min(batch, n - b0);
       rc = launch_gather36(u->stream, it->second.storage, cl.d_page.p, cl.d_pages.p, cl.empty_raw,
                            cl.full_raw, g->nbx, g->nby, g->nbz, d.p + b0, nb, u->x0.p);
-      if (!rc) rc = forward(u, nb);
+      if (!rc) rc = forward(u, nb, false);
       if (!rc && hipMemcpyAsync(pred.p + (int64_t)b0 * kVout, u->y.p, sizeof(float) * nb * kVout,
                                 hipMemcpyDeviceToDevice, u->stream) != hipSuccess) {
         set_error("msbg_warm_start: copy failed");

@@ -55,6 +55,12 @@ int launch_upcat_bwd(hipStream_t s, const uint16_t *dout, int ca, int cb, int n,
                      uint16_t *da, uint16_t *db);
 // y[n][32^3] = hb + sum_c hw[c] * d[n][v][c]; head = {hw[16], hb}
 int launch_head(hipStream_t s, const uint16_t *d, int n, const float *head, float *y);
+// the last 16 -> 16 convolution at 32^3 with the head applied in its store:
+// yh as launch_head on its output, bit for bit; y (the activation) may be
+// null when no backward pass needs it
+int launch_conv_head(hipStream_t s, const uint16_t *x, int n, int cin, int cout, int dout,
+                     int pad, const uint4 *wsw, const float *bias, bool relu, uint16_t *y,
+                     const float *head, float *yh);
 // dd = dy * hw (BF16); partials [blocks][17] of (sum dy*d_c, sum dy)
 int head_bwd_blocks(int n);
 int launch_head_bwd(hipStream_t s, const float *dy, const uint16_t *d, int n, const float *head,

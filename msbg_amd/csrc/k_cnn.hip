@@ -93,14 +93,21 @@ struct ConvShape {
 // first CA channels are the coarser activation x (edge DIN/2, nearest
 // upsampling), the other CIN - CA are the skip activation xb (edge DIN). A
 // channel chunk never straddles the two.
-template <int CIN, int COUT, int DOUT, int PAD, int CA = 0>
+// HEAD: the network's 1x1x1 head (16 -> 1, head = {hw[16], hb}) applied in
+// the store: yh[n][v] = hb + sum_c hw[c] * bf16(y[n][v][c]) in channel order,
+// the same fma chain as k_head, and y (the BF16 activation) is stored only
+// when not null.
+template <int CIN, int COUT, int DOUT, int PAD, int CA = 0, bool HEAD = false>
 __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ x,
                                               const uint16_t *__restrict__ xb,
                                               const uint4 *__restrict__ wsw,
                                               const float *__restrict__ bias, int relu,
-                                              uint16_t *__restrict__ y) {
+                                              uint16_t *__restrict__ y,
+                                              const float *__restrict__ head,
+                                              float *__restrict__ yh) {
   using K = ConvShape<CIN, COUT, DOUT, PAD>;
   static_assert(CA % K::CC == 0 && CA < CIN, "concat boundary inside a channel chunk");
+  static_assert(!HEAD || (COUT == 16 && K::XP == DOUT), "head: 16 channels, no idle lanes");
   constexpr int DIN = K::DIN, TZ = K::TZ, TY = K::TY, XP = K::XP, CC = K::CC, TM = K::TM,
                 TN = K::TN, SY = K::SY, SX = K::SX, VS8 = K::VS8;
   __shared__ uint4 slab[K::SLAB];
@@ -210,9 +217,15 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ x,
     }
   }
 
+  float hw[4] = {0.f, 0.f, 0.f, 0.f}, hb = 0.f;
+  if constexpr (HEAD) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) hw[r] = head[4 * q + r];
+    hb = head[16];
+  }
 #pragma unroll
   for (int t = 0; t < TN; t++) {
-    if (tx[t] >= DOUT) continue;
+    if (!HEAD && tx[t] >= DOUT) continue;
     const int64_t v = ((int64_t)(z0 + tz[t]) * DOUT + (y0 + ty[t])) * DOUT + tx[t];
     uint16_t *o = y + ((int64_t)n * DOUT * DOUT * DOUT + v) * COUT;
 #pragma unroll
@@ -225,7 +238,25 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ x,
         if (relu) f = f > 0.f ? f : 0.f;
         h[r] = f2bf(f);
       }
-      *(uint2 *)(o + co) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+      if (!HEAD || y) *(uint2 *)(o + co) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+      if constexpr (HEAD) {
+        // lane group q holds channels 4q..4q+3 of voxel v; the running sum
+        // passes from group q to group q+1 (16 lanes up), so that the chain
+        // runs over channels 0..15 in order and ends in group 3
+        float run = 0.f;
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+          float r2 = run;
+#pragma unroll
+          for (int r = 0; r < 4; r++) r2 = __builtin_fmaf(hw[r], bf2f((uint16_t)h[r]), r2);
+          run = q == g ? r2 : run;
+          if (g < 3) {
+            const float up = __shfl_up(run, 16, 64);
+            run = q == g + 1 ? up : run;
+          }
+        }
+        if (q == 3) yh[(int64_t)n * DOUT * DOUT * DOUT + v] = hb + run;
+      }
     }
   }
 }
@@ -799,7 +830,7 @@ int conv_t(hipStream_t s, const uint16_t *x, int n, const uint4 *wsw, const floa
   using K = ConvShape<CIN, COUT, DOUT, PAD>;
   dim3 grid((DOUT / K::TZ) * (DOUT / K::TY), (unsigned)n);
   hipLaunchKernelGGL((k_conv<CIN, COUT, DOUT, PAD, CA>), grid, dim3(256), 0, s, x, xb, wsw, bias,
-                     relu ? 1 : 0, y);
+                     relu ? 1 : 0, y, nullptr, nullptr);
   MSBG_HIP(hipGetLastError());
   return 0;
 }
@@ -887,6 +918,23 @@ int launch_conv_cat(hipStream_t s, const uint16_t *a, int ca, const uint16_t *b,
   if (ca == 32 && cb == 16 && cout == 16 && dout == 32)
     return conv_t<48, 16, 32, 1, 32>(s, a, n, wsw, bias, relu, y, b);
   set_error("conv: no concatenating kernel for %d+%d -> %d at %d^3", ca, cb, cout, dout);
+  return MSBG_EUNSUPP;
+}
+
+int launch_conv_head(hipStream_t s, const uint16_t *x, int n, int cin, int cout, int dout,
+                     int pad, const uint4 *wsw, const float *bias, bool relu, uint16_t *y,
+                     const float *head, float *yh) {
+  MSBG_CHECK(n >= 1 && n <= 65535, MSBG_EINVAL, "conv: batch %d outside [1, 65535]", n);
+  MSBG_CHECK(head && yh, MSBG_EINVAL, "conv head: null head or output");
+  if (cin == 16 && cout == 16 && dout == kOut && pad == 1) {
+    using K = ConvShape<16, 16, kOut, 1>;
+    dim3 grid((kOut / K::TZ) * (kOut / K::TY), (unsigned)n);
+    hipLaunchKernelGGL((k_conv<16, 16, kOut, 1, 0, true>), grid, dim3(256), 0, s, x, nullptr, wsw,
+                       bias, relu ? 1 : 0, y, head, yh);
+    MSBG_HIP(hipGetLastError());
+    return 0;
+  }
+  set_error("conv head: no kernel for cin %d cout %d dout %d pad %d", cin, cout, dout, pad);
   return MSBG_EUNSUPP;
 }
 
