@@ -80,7 +80,7 @@ struct ConvShape {
   // decoder's first 32^3 layer (upsample + skip, 48 -> 16) and the 64 -> 64
   // bottom layer stage one element at a time: batched, they lost a wave per
   // SIMD and ran 9 % and 35 % slower on MI355X, while the other layers ran
-  // 6-28 % faster.
+  // 6-24 % faster.
   static constexpr bool BATCH = !(CIN == 48 && COUT == 16 && DOUT == 32) &&
                                 !(CIN == 64 && COUT == 64);
   static_assert(CIN % CC == 0 && CC % 8 == 0, "channel chunking");
