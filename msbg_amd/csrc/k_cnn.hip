@@ -227,7 +227,6 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ x,
   for (int t = 0; t < TN; t++) {
     if (!HEAD && tx[t] >= DOUT) continue;
     const int64_t v = ((int64_t)(z0 + tz[t]) * DOUT + (y0 + ty[t])) * DOUT + tx[t];
-    uint16_t *o = y + ((int64_t)n * DOUT * DOUT * DOUT + v) * COUT;
 #pragma unroll
     for (int m = 0; m < TM; m++) {
       const int co = m * 16 + 4 * q;
@@ -238,7 +237,9 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ x,
         if (relu) f = f > 0.f ? f : 0.f;
         h[r] = f2bf(f);
       }
-      if (!HEAD || y) *(uint2 *)(o + co) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+      if (!HEAD || y)
+        *(uint2 *)(y + ((int64_t)n * DOUT * DOUT * DOUT + v) * COUT + co) =
+            make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
       if constexpr (HEAD) {
         // lane group q holds channels 4q..4q+3 of voxel v; the running sum
         // passes from group q to group q+1 (16 lanes up), so that the chain
