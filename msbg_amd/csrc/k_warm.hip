@@ -28,11 +28,13 @@ __global__ void k_gather36(const int32_t *__restrict__ page_map, const uint8_t *
                            const int32_t *__restrict__ bids, int n, uint16_t *__restrict__ out) {
   using S = Stor<ST>;
   using R = typename S::raw;
-  constexpr int64_t WV = (int64_t)W * W * W, PE = (int64_t)B * B * B;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)n * WV) return;
-  const int64_t k = i / WV;
-  const int w = (int)(i % WV);
+  constexpr int WV = W * W * W;
+  constexpr int64_t PE = (int64_t)B * B * B;
+  // window k = blockIdx.y, voxel w of it (no 64-bit division per voxel)
+  const int w = (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
+  const int k = (int)blockIdx.y;
+  if (w >= WV || k >= n) return;
+  const int64_t i = (int64_t)k * WV + w;
   const int wx = w % W, wy = (w / W) % W, wz = w / (W * W);
   const int bid = bids[k];
   const int bx = bid % nbx, by = (bid / nbx) % nby, bz = bid / (nbx * nby);
@@ -118,31 +120,37 @@ int launch_extract32(hipStream_t s, int storage, const int32_t *page_map, const 
 int launch_gather36(hipStream_t s, int storage, const int32_t *page_map, const uint8_t *pages,
                     double empty_raw, double full_raw, int nbx, int nby, int nbz,
                     const int32_t *bids, int n, uint16_t *out) {
-  const int64_t total = (int64_t)n * W * W * W;
-  const dim3 grid((unsigned)((total + 255) / 256)), blk(256);
+  MSBG_CHECK(n >= 0, MSBG_EINVAL, "gather: %d windows", n);
   const float ev = raw_to_float_host(storage, empty_raw), fv = raw_to_float_host(storage, full_raw);
-  switch (storage) {
-    case MSBG_STORAGE_U16:
-      hipLaunchKernelGGL(k_gather36<MSBG_STORAGE_U16>, grid, blk, 0, s, page_map, pages, ev, fv,
-                         nbx, nby, nbz, bids, n, out);
-      break;
-    case MSBG_STORAGE_F32:
-      hipLaunchKernelGGL(k_gather36<MSBG_STORAGE_F32>, grid, blk, 0, s, page_map, pages, ev, fv,
-                         nbx, nby, nbz, bids, n, out);
-      break;
-    case MSBG_STORAGE_F16:
-      hipLaunchKernelGGL(k_gather36<MSBG_STORAGE_F16>, grid, blk, 0, s, page_map, pages, ev, fv,
-                         nbx, nby, nbz, bids, n, out);
-      break;
-    case MSBG_STORAGE_BF16:
-      hipLaunchKernelGGL(k_gather36<MSBG_STORAGE_BF16>, grid, blk, 0, s, page_map, pages, ev, fv,
-                         nbx, nby, nbz, bids, n, out);
-      break;
-    default:
-      set_error("gather: storage %d not supported by the network input", storage);
-      return MSBG_EUNSUPP;
+  // one grid row per window, at most 65535 windows per launch
+  for (int off = 0; off < n; off += 65535) {
+    const int cnt = n - off < 65535 ? n - off : 65535;
+    const dim3 grid((unsigned)((W * W * W + 255) / 256), (unsigned)cnt), blk(256);
+    const int32_t *b = bids + off;
+    uint16_t *o = out + (int64_t)off * W * W * W;
+    switch (storage) {
+      case MSBG_STORAGE_U16:
+        hipLaunchKernelGGL(k_gather36<MSBG_STORAGE_U16>, grid, blk, 0, s, page_map, pages, ev, fv,
+                           nbx, nby, nbz, b, cnt, o);
+        break;
+      case MSBG_STORAGE_F32:
+        hipLaunchKernelGGL(k_gather36<MSBG_STORAGE_F32>, grid, blk, 0, s, page_map, pages, ev, fv,
+                           nbx, nby, nbz, b, cnt, o);
+        break;
+      case MSBG_STORAGE_F16:
+        hipLaunchKernelGGL(k_gather36<MSBG_STORAGE_F16>, grid, blk, 0, s, page_map, pages, ev, fv,
+                           nbx, nby, nbz, b, cnt, o);
+        break;
+      case MSBG_STORAGE_BF16:
+        hipLaunchKernelGGL(k_gather36<MSBG_STORAGE_BF16>, grid, blk, 0, s, page_map, pages, ev, fv,
+                           nbx, nby, nbz, b, cnt, o);
+        break;
+      default:
+        set_error("gather: storage %d not supported by the network input", storage);
+        return MSBG_EUNSUPP;
+    }
+    MSBG_HIP(hipGetLastError());
   }
-  MSBG_HIP(hipGetLastError());
   return 0;
 }
 
