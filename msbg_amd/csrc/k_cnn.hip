@@ -62,7 +62,7 @@ __device__ inline void unpack8(const uint4 &u, float (&f)[8]) {
 __device__ inline uint4 pack8(const float (&f)[8]) {
   uint32_t w[4];
 #pragma unroll
-  for (int i = 0; i < 4; i++) w[i] = (uint32_t)f2bf(f[2 * i]) | ((uint32_t)f2bf(f[2 * i + 1]) << 16);
+  for (int i = 0; i < 4; i++) w[i] = pk_bf16(f[2 * i], f[2 * i + 1]);
   return make_uint4(w[0], w[1], w[2], w[3]);
 }
 

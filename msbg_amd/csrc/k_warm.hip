@@ -58,7 +58,9 @@ __global__ void k_gather36(const int32_t *__restrict__ page_map, const uint8_t *
     v = (float)S::dec(P[(int64_t)code * PE + ((gz & (B - 1)) * B + (gy & (B - 1))) * B +
                          (gx & (B - 1))]);
   }
-  out[i] = bf16_bits_from_float(v);
+  // round to nearest even in hardware (v_cvt_pk_bf16_f32): the same bits as
+  // bf16_bits_from_float for every non-NaN value with FP32 denormals kept
+  out[i] = __builtin_bit_cast(uint16_t, (__bf16)v);
 }
 
 template <int ST>
